@@ -1,0 +1,233 @@
+/*
+ * tinyad_amd.h -- C ABI of the MI355X (gfx950) sparse second-order path.
+ *
+ * One shared library, libtinyad_amd.so, exports exactly the functions below.
+ * Signatures use plain pointers, sizes and enums only.  Every function returns
+ * a tad_status; on failure tad_last_error() holds a message for the calling
+ * thread (the reference's TINYAD_ERROR_throw / TINYAD_ASSERT, Utils/Out.hh:64-157,
+ * throws std::runtime_error instead; the C++ front-end in
+ * include/tinyad_amd/ScalarFunction.hh turns non-zero statuses back into
+ * std::runtime_error).
+ *
+ * Memory: every array argument is paired with the tad_mem of the call.  With
+ * TAD_MEM_DEVICE the pointers are HIP device pointers on the context's device
+ * and nothing crosses PCIe; with TAD_MEM_HOST they are host pointers.
+ *
+ * Variable layout matches the reference: x has n_vars = d * n_handles entries,
+ * entry d * handle + i is coordinate i of that handle (Detail/Element.hh:159-170).
+ *
+ * Hessians are returned as CSR values over a fixed pattern (tad_fn_pattern).
+ * The pattern is the union of all element (i, j) pairs in full symmetric
+ * storage, explicit zeros included, i.e. the pattern Eigen's setFromTriplets
+ * produces for the reference's triplets (ScalarFunctionImpl.hh:378-399).  For a
+ * symmetric matrix the CSR arrays equal the CSC arrays of Eigen's
+ * column-major SparseMatrix.
+ *
+ * Determinism: no floating-point atomics are used.  Element contributions are
+ * summed per output entry in element order (terms in insertion order), so
+ * repeated calls return identical bits.
+ */
+#ifndef TINYAD_AMD_H
+#define TINYAD_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum tad_status
+{
+    TAD_OK = 0,
+    TAD_ERR_INVALID = 1,     /* bad argument / size mismatch (TINYAD_ASSERT_*)            */
+    TAD_ERR_HIP = 2,         /* HIP runtime error                                         */
+    TAD_ERR_NONFINITE = 3,   /* non-finite element gradient or Hessian                   */
+                             /* (ScalarObjectiveTerm.hh:252-253)                          */
+    TAD_ERR_SOLVE = 4,       /* linear solve failed (NewtonDirection.hh:43-44)           */
+    TAD_ERR_NAN_ENERGY = 5,  /* NaN energy in the line search (LineSearch.hh:52)          */
+    TAD_ERR_UNSUPPORTED = 6
+} tad_status;
+
+typedef enum tad_mem
+{
+    TAD_MEM_HOST = 0,
+    TAD_MEM_DEVICE = 1
+} tad_mem;
+
+typedef enum tad_precision
+{
+    TAD_PREC_FP64 = 0,  /* everything in FP64                                              */
+    TAD_PREC_MIXED = 1  /* element gradient/Hessian/projection in FP32; energies, CSR      */
+                        /* reduction, gradient reduction and the solve in FP64            */
+} tad_precision;
+
+/* Energy catalog (the per-element functions add_elements can register).
+ * Parameter rows, one per element (row-major, params_per_elem doubles):
+ *   TAD_E_SYMDIR_TRI  d=2, N=3: [Mr^-1 (2x2 row-major), w]     5 doubles
+ *   TAD_E_SYMDIR_TET  d=3, N=4: [Mr^-1 (3x3 row-major), w]    10 doubles
+ *     E = +inf if det(M) <= 0 else w (||M Mr^-1||_F^2 + ||(M Mr^-1)^-1||_F^2),
+ *     M = col_mat(v1 - v0, v2 - v0[, v3 - v0])   (tests/NewtonTest.cc:28-44)
+ *   TAD_E_PENALTY     N=1, any d: [target (d), w]     E = w ||target - x_v||^2
+ *                                                       (tests/NewtonTest.cc:48-55)
+ *   TAD_E_QUADRATIC   any N, d with N*d <= 12: [A (KxK), b (K), c], K = N*d
+ *     E = 0.5 x_e^T A x_e + b^T x_e + c over the element's local variables
+ *     (tests/ScalarFunctionTest.cc, tests/DynamicElementsTest.cc:100-111)      */
+typedef enum tad_energy
+{
+    TAD_E_SYMDIR_TRI = 1,
+    TAD_E_SYMDIR_TET = 2,
+    TAD_E_PENALTY = 3,
+    TAD_E_QUADRATIC = 4
+} tad_energy;
+
+typedef struct tad_ctx_s* tad_ctx;
+typedef struct tad_fn_s* tad_fn;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* tad_last_error(void);
+
+/* Library version string. */
+const char* tad_version(void);
+
+/* --- context: one device, one HIP stream --------------------------------- */
+
+/* Binds device `device`; all work of the context's functions is queued on one
+ * HIP stream (created here, default priority) unless tad_ctx_set_stream is
+ * given an existing one.  Replaces the reference's OpenMP team
+ * (Detail/Parallel.hh:20-69, EvalSettings.hh:14-21). */
+tad_status tad_ctx_create(int device, tad_ctx* out);
+tad_status tad_ctx_destroy(tad_ctx ctx);
+/* hip_stream: a hipStream_t (NULL = the context's own stream). */
+tad_status tad_ctx_set_stream(tad_ctx ctx, void* hip_stream);
+tad_status tad_ctx_get_stream(tad_ctx ctx, void** hip_stream);
+tad_status tad_ctx_synchronize(tad_ctx ctx);
+/* When enabled, every kernel launch is bracketed by HIP events on the
+ * context's stream; tad_ctx_kernel_stats reports per-kernel-name totals. */
+tad_status tad_ctx_enable_timing(tad_ctx ctx, int enable);
+/* name: kernel family ("element", "assemble_hessian", "spmv", ...);
+ * returns launch count and summed milliseconds since the last reset. */
+tad_status tad_ctx_kernel_stats(tad_ctx ctx, const char* name, int64_t* launches, double* total_ms);
+tad_status tad_ctx_reset_stats(tad_ctx ctx);
+
+/* --- objective function ---------------------------------------------------- */
+
+/* scalar_function<d>(range(n_handles)) (ScalarFunctionImpl.hh:418-442, ctor
+ * :47-61).  d in 1..3; handles are 0..n_handles-1. */
+tad_status tad_fn_create(tad_ctx ctx, int d, int64_t n_handles, tad_precision precision, tad_fn* out);
+tad_status tad_fn_destroy(tad_fn fn);
+
+/* add_elements<N>(range(n_elem), energy) (ScalarFunctionImpl.hh:63-100).
+ * conn: n_elem x N handle indices (row-major, host memory), local variable order
+ * = connectivity order (first-access order of Element::variables, Element.hh:208-260).
+ * params: n_elem x params_per_elem (row-major, host memory).  Elements may not
+ * reference the same handle twice.  Invalidates the Hessian pattern. */
+tad_status tad_fn_add_elements(tad_fn fn, tad_energy energy, int valence, int64_t n_elem,
+                               const int32_t* conn, const double* params, int params_per_elem);
+
+tad_status tad_fn_n_vars(tad_fn fn, int64_t* n_vars);
+tad_status tad_fn_n_elements(tad_fn fn, int64_t* n_elements);
+
+/* Builds (once) and returns the Hessian CSR pattern: n_vars + 1 row offsets and
+ * nnz column indices, device pointers owned by fn, valid until the next
+ * add_elements or tad_fn_destroy.  Host copies are written to row_ptr_host /
+ * col_idx_host when those are non-NULL. */
+tad_status tad_fn_pattern(tad_fn fn, int64_t* nnz, const int32_t** row_ptr_dev, const int32_t** col_idx_dev,
+                          int32_t* row_ptr_host, int32_t* col_idx_host);
+
+/* eval(x) (ScalarFunctionImpl.hh:256-282): passive energy, FP64 sum. */
+tad_status tad_fn_eval(tad_fn fn, const double* x, tad_mem mem, double* f);
+
+/* eval_with_gradient(x, f, g) (ScalarFunctionImpl.hh:284-314). g: n_vars. */
+tad_status tad_fn_eval_with_gradient(tad_fn fn, const double* x, double* f, double* g, tad_mem mem);
+
+/* eval_with_derivatives(x, f, g, H) (ScalarFunctionImpl.hh:316-352): unprojected
+ * Hessian.  H_values: nnz values in the pattern order (may be NULL: the values
+ * stay in the function's device buffer, see tad_fn_hessian_values). */
+tad_status tad_fn_eval_with_derivatives(tad_fn fn, const double* x, double* f, double* g, double* H_values,
+                                        tad_mem mem);
+
+/* eval_with_hessian_proj(x, f, g, H_proj, eps) (ScalarFunctionImpl.hh:378-416,
+ * per-element projection HessianProjection.hh:48-101; eps >= 0 clamps
+ * eigenvalues to eps, eps < 0 takes absolute values; default 1e-9). */
+tad_status tad_fn_eval_with_hessian_proj(tad_fn fn, const double* x, double eps, double* f, double* g,
+                                         double* H_values, tad_mem mem);
+
+/* Device pointers of the function's own buffers holding the gradient and CSR
+ * values of the last second-order evaluation (valid until the next call). */
+tad_status tad_fn_device_buffers(tad_fn fn, const double** g_dev, const double** H_values_dev);
+
+/* --- solver ---------------------------------------------------------------- */
+
+typedef struct tad_solve_opts
+{
+    double w_identity;  /* regularization w in (w I + H) d = rhs (NewtonDirection.hh:32)  */
+    double tol;         /* relative residual ||r|| <= tol ||rhs||                        */
+    int max_iters;
+    int check_every;    /* host polls convergence every this many iterations             */
+} tad_solve_opts;
+
+typedef struct tad_solve_stats
+{
+    int iters;
+    int converged;
+    double rel_residual;
+} tad_solve_stats;
+
+tad_status tad_solve_default_opts(tad_solve_opts* opts);
+
+/* Solves (w I + H) sol = rhs by block-Jacobi preconditioned CG (FP64) on the
+ * function's pattern; replaces the SimplicialLDLT factorize/solve of
+ * newton_direction (NewtonDirection.hh:25-48, LinearSolver.hh:13-19).
+ * H_values == NULL uses the function's buffer from the last second-order eval. */
+tad_status tad_fn_solve(tad_fn fn, const double* H_values, const double* rhs, double* sol, tad_mem mem,
+                        const tad_solve_opts* opts, tad_solve_stats* stats);
+
+typedef struct tad_newton_opts
+{
+    double projection_eps;  /* eval_with_hessian_proj eps (default 1e-9)                */
+    tad_solve_opts solve;   /* Newton direction solve                                   */
+    double ls_s_max;        /* line_search defaults (LineSearch.hh:33-36):               */
+    double ls_shrink;       /*   s_max 1, shrink 0.8, 64 trials, Armijo c 1e-4          */
+    int ls_max_iters;
+    double ls_armijo;
+    int ls_batch;           /* step sizes evaluated per energy launch (1..8)            */
+} tad_newton_opts;
+
+typedef struct tad_newton_stats
+{
+    double f;               /* energy at the input x                                   */
+    double f_new;           /* energy at the returned x                                */
+    double step;            /* accepted step size (0 if the line search failed)         */
+    double newton_decrement;/* -0.5 d^T g (NewtonDecrement.hh:20-26)                     */
+    double grad_max;        /* max |g| at the input x                                  */
+    int cg_iters;
+    int ls_trials;          /* energies evaluated in the line search                    */
+    int ls_failed;          /* 1: no step accepted, x unchanged (LineSearch.hh:62-64)   */
+    int solve_converged;
+} tad_newton_stats;
+
+tad_status tad_newton_default_opts(tad_newton_opts* opts);
+
+/* One projected-Newton iteration on x (n_vars, updated in place):
+ * eval_with_hessian_proj -> newton_direction -> line_search
+ * (the loop body of tests/NewtonTest.cc:69-75). */
+tad_status tad_newton_step(tad_fn fn, double* x, tad_mem mem, const tad_newton_opts* opts, tad_newton_stats* stats);
+
+/* line_search(x0, d, f, g, eval) (LineSearch.hh:26-65) on device data. */
+tad_status tad_line_search(tad_fn fn, const double* x0, const double* dir, double f, const double* g,
+                           double* x_out, tad_mem mem, const tad_newton_opts* opts, double* f_out,
+                           double* step_out, int* trials_out);
+
+/* --- self test -------------------------------------------------------------- */
+
+/* Runs the device Dual<2> operators on a reference known-answer expression
+ * (the same list as the CPU oracle's or_dual_known_answer) and writes
+ * val, grad[2], Hess[2x2] to out[7] (host memory). */
+tad_status tad_selftest_dual(tad_ctx ctx, int which, double x, double y, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TINYAD_AMD_H */

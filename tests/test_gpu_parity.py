@@ -1,0 +1,347 @@
+"""GPU parity: the HIP library (through the C ABI) against the CPU oracle and
+against the reference's own known answers.
+
+Tolerances (FP64 mode): normwise relative error <= 1e-10 for g and H_proj
+(max |a - b| / max |b|), <= 1e-12 for f; identical sparsity patterns.  Mixed
+mode (FP32 element derivatives): f identical to the FP64 path (values are
+always FP64), g and H within 2e-5 normwise.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg  # noqa: F401
+
+import oracle
+import tinyad_amd as tad
+from tinyad_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+
+
+def nrel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = max(np.abs(b).max(initial=0.0), 1e-300)
+    return float(np.abs(a - b).max(initial=0.0) / den)
+
+
+def assert_same_pattern(Hg, Ho):
+    Hg = Hg.tocsr()
+    Ho = Ho.tocsr()
+    Hg.sort_indices()
+    Ho.sort_indices()
+    assert Hg.shape == Ho.shape
+    assert Hg.nnz == Ho.nnz
+    assert np.array_equal(Hg.indptr, Ho.indptr)
+    assert np.array_equal(Hg.indices, Ho.indices)
+
+
+def perturbed_cube(m=3, sigma=0.15, seed=11):
+    P = configs.twist_cube(m=m, seed=seed)
+    h = P.info["h"]
+    rest = P.info["rest"]
+    x = rest + sigma * h * configs.normal(seed + 1, rest.size, stream=9)
+    P.x0 = x
+    return P
+
+
+# ---------------------------------------------------------------------------
+# Reference known answers (tests/ScalarFunctionTest.cc)
+# ---------------------------------------------------------------------------
+
+def load_bunny():
+    import json
+    import os
+    d = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "bunny_closed.json")))
+    return np.array(d["V"], dtype=np.float64), np.array(d["F"], dtype=np.int32)
+
+
+def reference_laplace(n, F):
+    """tests/DynamicElementsTest.cc:31-52: L(v1,v1) += 1, L(v1,v2) -= 1 per directed face edge."""
+    L = sp.lil_matrix((n, n))
+    for f in F:
+        for i in range(3):
+            v1, v2 = f[i], f[(i + 1) % 3]
+            L[v1, v1] += 1.0
+            L[v1, v2] -= 1.0
+    return L.tocsr()
+
+
+def quad_params(A, b, c):
+    A = np.asarray(A, dtype=np.float64)
+    return np.concatenate([A.ravel(), np.asarray(b, dtype=np.float64).ravel(), [c]])
+
+
+def test_scalar_function_basic(ctx):
+    # x0^2 + x1^2 at (1, 2): f 5, g (2, 4), H diag(2, 2)   (ScalarFunctionTest.cc:10-42)
+    func = tad.scalar_function(2, 1, ctx)
+    func.add_elements(1, tad.QUADRATIC, [[0]], quad_params(2 * np.eye(2), [0, 0], 0))
+    x = np.array([1.0, 2.0])
+    assert func.eval(x) == 5.0
+    f, g = func.eval_with_gradient(x)
+    assert f == 5.0 and g[0] == 2.0 and g[1] == 4.0
+    f, g, H = func.eval_with_derivatives(x)
+    assert f == 5.0 and H[0, 0] == 2.0 and H[1, 1] == 2.0
+
+
+def test_scalar_function_1d(ctx):
+    # 2 x^2 + x + 1 at 1: f 4, g 5, H_proj 4   (ScalarFunctionTest.cc:44-66)
+    func = tad.scalar_function(1, 1, ctx)
+    func.add_elements(1, tad.QUADRATIC, [[0]], quad_params([[4.0]], [1.0], 1.0))
+    f, g, H = func.eval_with_hessian_proj(np.array([1.0]))
+    assert abs(f - 4.0) <= 1e-16 and abs(g[0] - 5.0) <= 1e-16 and abs(H[0, 0] - 4.0) <= 1e-16
+
+
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_scalar_function_2d(ctx, sign):
+    # +-(2 x0^2 + 2 x0 x1 + x1^2 + x0 + 1) at (1, 2)   (ScalarFunctionTest.cc:72-151)
+    func = tad.scalar_function(2, 1, ctx)
+    A = sign * np.array([[4.0, 2.0], [2.0, 2.0]])
+    func.add_elements(1, tad.QUADRATIC, [[0]], quad_params(A, sign * np.array([1.0, 0.0]), sign * 1.0))
+    x = np.array([1.0, 2.0])
+    f, g, H = func.eval_with_derivatives(x)
+    f2, g2, Hp = func.eval_with_hessian_proj(x)
+    assert abs(f - sign * 12) <= 1e-16 and abs(f2 - sign * 12) <= 1e-16
+    assert np.allclose(g, sign * np.array([9.0, 6.0]), atol=1e-16, rtol=0)
+    assert np.allclose(g2, sign * np.array([9.0, 6.0]), atol=1e-16, rtol=0)
+    assert np.allclose(H.toarray(), A, atol=1e-16, rtol=0)
+    if sign > 0:
+        assert np.array_equal(Hp.toarray(), A)  # PD: projection leaves it unchanged
+    else:
+        ev = np.linalg.eigvalsh(Hp.toarray())
+        assert ev.min() > 0.0
+
+
+def test_element_variables_unused_handle(ctx):
+    # 2 handles, the element touches only handle 0 (ScalarFunctionTest.cc:153-178)
+    func = tad.scalar_function(2, 2, ctx)
+    func.add_elements(1, tad.QUADRATIC, [[0]], quad_params(np.zeros((2, 2)), [0, 0], 0.0))
+    x = np.zeros(4)
+    f, g, H = func.eval_with_hessian_proj(x)
+    assert f == 0.0 and func.eval(x) == 0.0
+    assert H.shape == (4, 4) and H.nnz == 4
+
+
+def test_sum_of_variables(ctx):
+    # v.sum() at (1, 1) = 2   (ScalarFunctionTest.cc:181-215, Move1)
+    func = tad.scalar_function(2, 1, ctx)
+    func.add_elements(1, tad.QUADRATIC, [[0]], quad_params(np.zeros((2, 2)), [1, 1], 0.0))
+    assert func.eval(np.ones(2)) == 2.0
+
+
+@pytest.mark.parametrize("which,x,y", [(0, -1.0, -0.5), (1, 5.0, 0.0), (1, 1.0, 1.0), (1, -1.0, -1.0),
+                                       (2, 1.0, 1.5), (3, 0.7, -1.3)])
+def test_dual_ops_against_oracle(ctx, which, x, y):
+    fo, go, Ho = oracle.dual_known_answer(which, x, y)
+    fg, gg, Hg = tad.selftest_dual(which, x, y, ctx)
+    assert abs(fg - fo) <= 1e-13 * max(1, abs(fo))
+    assert np.allclose(gg, go, rtol=1e-12, atol=1e-12)
+    assert np.allclose(Hg, Ho, rtol=1e-12, atol=1e-12)
+    if which == 0:  # ScalarTestBinaryOperators.cc:623-642
+        assert fg == -2.0 and np.allclose(gg, [4, -4]) and np.allclose(Hg, [[-4, 8], [8, -16]])
+    if which == 2:  # ScalarTestBinaryOperators.cc:711-727
+        assert abs(fg - 3.0) < 1e-12 and abs(gg[0] - 4.5) < 1e-12 and abs(Hg[0, 0] - 3.0) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# Parity against the oracle
+# ---------------------------------------------------------------------------
+
+def problems():
+    return {
+        "grid_param_12": configs.grid_param(12, seed=5),
+        "twist_cube_3": configs.twist_cube(m=3, seed=5),
+        "cube_perturbed": perturbed_cube(3, 0.25, 13),
+        "newton_test": configs.newton_test_problem(),
+    }
+
+
+@pytest.mark.parametrize("name", ["grid_param_12", "twist_cube_3", "cube_perturbed", "newton_test"])
+@pytest.mark.parametrize("eps", [1e-9, -1.0, 0.05])
+def test_eval_with_hessian_proj_matches_oracle(ctx, name, eps):
+    P = problems()[name]
+    fo = oracle.Function.from_problem(P)
+    fg = tad.ScalarFunction.from_problem(P, ctx)
+    f1, g1, H1 = fo.eval_with_hessian_proj(P.x0, eps)
+    f2, g2, H2 = fg.eval_with_hessian_proj(P.x0, eps)
+    assert_same_pattern(H2, H1)
+    assert abs(f2 - f1) <= 1e-12 * abs(f1)
+    assert nrel(g2, g1) <= FP64_TOL
+    assert nrel(H2.toarray(), H1.toarray()) <= FP64_TOL
+    assert abs(fg.eval(P.x0) - f1) <= 1e-12 * abs(f1)
+
+
+@pytest.mark.parametrize("name", ["grid_param_12", "cube_perturbed", "newton_test"])
+def test_eval_with_derivatives_and_gradient_match_oracle(ctx, name):
+    P = problems()[name]
+    fo = oracle.Function.from_problem(P)
+    fg = tad.ScalarFunction.from_problem(P, ctx)
+    f1, g1, H1 = fo.eval_with_derivatives(P.x0)
+    f2, g2, H2 = fg.eval_with_derivatives(P.x0)
+    assert_same_pattern(H2, H1)
+    assert abs(f2 - f1) <= 1e-12 * abs(f1)
+    assert nrel(g2, g1) <= FP64_TOL
+    assert nrel(H2.toarray(), H1.toarray()) <= FP64_TOL
+    f3, g3 = fg.eval_with_gradient(P.x0)
+    assert f3 == f2 and nrel(g3, g1) <= FP64_TOL
+
+
+def test_inverted_elements(ctx):
+    # an inverted element contributes +inf, zero gradient and eps I (NewtonTest.cc:40-41)
+    P = configs.twist_cube(m=2, seed=3)
+    X = P.x0.reshape(-1, 3).copy()
+    T = P.terms[0].conn
+    a, b = T[0, 1], T[0, 2]
+    X[[a, b]] = X[[b, a]]
+    x = X.ravel()
+    fo = oracle.Function.from_problem(P)
+    fg = tad.ScalarFunction.from_problem(P, ctx)
+    f1, g1, H1 = fo.eval_with_hessian_proj(x)
+    f2, g2, H2 = fg.eval_with_hessian_proj(x)
+    assert f1 == np.inf and f2 == np.inf and fg.eval(x) == np.inf
+    assert nrel(g2, g1) <= FP64_TOL
+    assert nrel(H2.toarray(), H1.toarray()) <= FP64_TOL
+
+
+def test_deterministic_bits(ctx):
+    P = perturbed_cube(4, 0.2, 21)
+    fg = tad.ScalarFunction.from_problem(P, ctx)
+    r1 = fg.eval_with_hessian_proj(P.x0)
+    r2 = fg.eval_with_hessian_proj(P.x0)
+    assert r1[0] == r2[0]
+    assert np.array_equal(r1[1], r2[1])
+    assert np.array_equal(r1[2].data, r2[2].data)
+
+
+def test_device_arrays(ctx):
+    import torch
+    P = configs.twist_cube(m=3, seed=2)
+    fg = tad.ScalarFunction.from_problem(P, ctx)
+    fh, gh, Hh = fg.eval_with_hessian_proj(P.x0)
+    xd = torch.from_numpy(P.x0).cuda()
+    fd, gd, Hd = fg.eval_with_hessian_proj(xd)
+    assert fd == fh
+    assert np.array_equal(gd.cpu().numpy(), gh)
+    assert np.array_equal(Hd.cpu().numpy(), Hh.data)
+
+
+@pytest.mark.parametrize("name", ["grid_param_12", "cube_perturbed"])
+def test_mixed_precision_errors(ctx, name):
+    P = problems()[name]
+    fo = oracle.Function.from_problem(P)
+    fm = tad.ScalarFunction.from_problem(P, ctx, precision="mixed")
+    f1, g1, H1 = fo.eval_with_hessian_proj(P.x0)
+    f2, g2, H2 = fm.eval_with_hessian_proj(P.x0)
+    assert abs(f2 - f1) <= 1e-12 * abs(f1)
+    assert nrel(g2, g1) <= 2e-5
+    assert nrel(H2.toarray(), H1.toarray()) <= 2e-5
+
+
+def test_laplacian_known_answer(ctx):
+    # Hessian of 1/4 sum (edge differences)^2 per triangle on the closed bunny =
+    # graph Laplacian (DynamicElementsTest.cc:60-91, reference_laplace :31-52)
+    V, F = load_bunny()
+    n = V.shape[0]
+    A = 0.5 * (3 * np.eye(3) - np.ones((3, 3)))
+    func = tad.scalar_function(1, n, ctx)
+    func.add_elements(3, tad.QUADRATIC, F, quad_params(A, np.zeros(3), 0.0))
+    L = func.eval_hessian_of_quadratic()
+    Lr = reference_laplace(n, F)
+    assert L.shape == Lr.shape and L.nnz == Lr.nnz
+    assert sp.linalg.norm(L - Lr) <= 1e-12
+
+
+# ---------------------------------------------------------------------------
+# Errors (ExceptionTest.cc semantics as status codes)
+# ---------------------------------------------------------------------------
+
+def test_nonfinite_derivatives_raise_and_recover(ctx):
+    P = configs.twist_cube(m=1, seed=1)
+    fg = tad.ScalarFunction.from_problem(P, ctx)
+    X = P.info["rest"].reshape(-1, 3).copy()
+    T = P.terms[0].conn
+    # squash one tet to det(M) ~ 1e-200 > 0: its inverse-norm term overflows
+    v = T[0]
+    X[v[3]] = X[v[0]] + 1e-67 * (X[v[3]] - X[v[0]]) + 1e-67 * (X[v[1]] - X[v[0]])
+    x = X.ravel()
+    for _ in range(2):
+        with pytest.raises(tad.TadError) as ei:
+            fg.eval_with_hessian_proj(x)
+        assert ei.value.status == 3
+    f, g, H = fg.eval_with_hessian_proj(P.info["rest"])
+    assert np.isfinite(f)
+
+
+def test_size_mismatch_raises(ctx):
+    fg = tad.ScalarFunction.from_problem(configs.newton_test_problem(), ctx)
+    with pytest.raises(tad.TadError) as ei:
+        fg.eval(np.zeros(5))
+    assert ei.value.status == 1
+
+
+# ---------------------------------------------------------------------------
+# Solver and Newton
+# ---------------------------------------------------------------------------
+
+def test_pcg_matches_direct_solve(ctx):
+    import scipy.sparse.linalg as spla
+    P = configs.twist_cube(m=4, seed=3)
+    fg = tad.ScalarFunction.from_problem(P, ctx)
+    f, g, H = fg.eval_with_hessian_proj(P.x0)
+    d, st = fg.solve(g, tol=1e-13)
+    ref = spla.spsolve(H.tocsc(), g)
+    assert st.converged
+    assert nrel(d, ref) <= 1e-9
+    d2, it, rel = oracle.pcg(H, g, block=3, tol=1e-13)
+    assert nrel(d, d2) <= 1e-9
+    d3, st3 = fg.solve(g, tol=1e-13, w_identity=0.5)
+    ref3 = spla.spsolve((H + 0.5 * sp.identity(H.shape[0])).tocsc(), g)
+    assert nrel(d3, ref3) <= 1e-9
+
+
+def run_newton_test(ctx, iters=10):
+    P = configs.newton_test_problem()
+    fg = tad.ScalarFunction.from_problem(P, ctx)
+    x = P.x0.copy()
+    opts = tad.newton_default_opts(solve_tol=1e-14)
+    for _ in range(iters):
+        fg.newton_step(x, opts)
+    return fg, x
+
+
+def test_newton_test_converges(ctx):
+    # NewtonTest.cc:12-95: nnz = 4V + 8(V + F - 1) = 96; f -> 4, |g|_inf -> 0
+    fg, x = run_newton_test(ctx)
+    assert fg.nnz == 4 * 6 + 8 * (6 + 4 - 1)
+    f, g, H = fg.eval_with_hessian_proj(x)
+    assert abs(fg.eval(x) - f) == 0.0
+    assert abs(f - 4.0) <= 1e-12
+    assert np.abs(g).max() <= 1e-10
+
+
+def test_newton_deterministic(ctx):
+    # NewtonTest.cc:97-111: repeated optimizations give identical x
+    xs = [run_newton_test(ctx)[1] for _ in range(3)]
+    for x in xs[1:]:
+        assert np.array_equal(x, xs[0])
+
+
+def test_newton_matches_oracle_trajectory_grid(ctx):
+    # grid parametrization: 8 Newton iterations on GPU vs oracle + direct solves
+    import scipy.sparse.linalg as spla
+    P = configs.grid_param(20, seed=4)
+    fo = oracle.Function.from_problem(P)
+    fg = tad.ScalarFunction.from_problem(P, ctx)
+    xo = P.x0.copy()
+    xg = P.x0.copy()
+    opts = tad.newton_default_opts(solve_tol=1e-13)
+    for _ in range(8):
+        f, g, H = fo.eval_with_hessian_proj(xo)
+        d = spla.spsolve(H.tocsc(), -g)
+        xo, _, _, _ = fo.line_search(xo, d, f, g)
+        fg.newton_step(xg, opts)
+    fo_end = fo.eval(xo)
+    fg_end = fg.eval(xg)
+    assert abs(fg_end - fo_end) <= 1e-8 * abs(fo_end)

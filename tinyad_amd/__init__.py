@@ -1,0 +1,342 @@
+"""tinyad_amd -- MI355X (gfx950) sparse second-order path with TinyAD's
+ScalarFunction interface.
+
+Python mirror of the reference's C++ API (include/TinyAD/ScalarFunction.hh):
+
+    import tinyad_amd as tad
+    func = tad.scalar_function(2, n_handles)                 # scalar_function<2>(range(n))
+    func.add_elements(3, tad.SYMDIR_TRI, F, params)          # add_elements<3>(range(nF), lambda)
+    x = func.x_from_data(V)                                  # x_from_data
+    f, g, H = func.eval_with_hessian_proj(x)                 # eval_with_hessian_proj
+
+Element functions come from the energy catalog (tad.SYMDIR_TRI, SYMDIR_TET,
+PENALTY, QUADRATIC) with per-element parameter rows instead of a C++ lambda.
+Arrays may be numpy (host) or torch CUDA tensors (device-resident: nothing
+crosses PCIe).  All compute runs in libtinyad_amd.so; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import TadError, lib, check, NewtonOpts, NewtonStats, SolveOpts, SolveStats
+
+SYMDIR_TRI = 1
+SYMDIR_TET = 2
+PENALTY = 3
+QUADRATIC = 4
+
+DEFAULT_HESSIAN_PROJECTION_EPS = 1e-9  # HessianProjection.hh:16
+
+__all__ = ["Context", "ScalarFunction", "scalar_function", "TadError", "SYMDIR_TRI", "SYMDIR_TET", "PENALTY",
+           "QUADRATIC", "newton_default_opts", "solve_default_opts", "version"]
+
+
+def version() -> str:
+    return lib().tad_version().decode()
+
+
+def _is_torch(a) -> bool:
+    return type(a).__module__.startswith("torch")
+
+
+def _order_after_torch(ctx, device):
+    """Queue library work behind torch's producer kernels: share torch's
+    current stream, or (torch's legacy default stream) wait for it."""
+    import torch
+    st = torch.cuda.current_stream(device)
+    if st.cuda_stream:
+        ctx.set_stream(st.cuda_stream)
+    else:
+        st.synchronize()
+        ctx.set_stream(None)
+
+
+class Context:
+    """A device plus one HIP stream (tad_ctx)."""
+
+    _default = None
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        check(lib().tad_ctx_create(device, ctypes.byref(h)))
+        self._h = h
+        self.device = device
+
+    @classmethod
+    def default(cls):
+        if cls._default is None:
+            cls._default = Context(0)
+        return cls._default
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().tad_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        check(lib().tad_ctx_get_stream(self._h, ctypes.byref(s)))
+        return s.value or 0
+
+    def set_stream(self, stream_ptr: int | None):
+        check(lib().tad_ctx_set_stream(self._h, ctypes.c_void_p(stream_ptr or 0)))
+
+    def synchronize(self):
+        check(lib().tad_ctx_synchronize(self._h))
+
+    def enable_timing(self, on: bool = True):
+        check(lib().tad_ctx_enable_timing(self._h, 1 if on else 0))
+
+    def reset_stats(self):
+        check(lib().tad_ctx_reset_stats(self._h))
+
+    def kernel_stats(self, name: str):
+        n = ctypes.c_int64()
+        ms = ctypes.c_double()
+        check(lib().tad_ctx_kernel_stats(self._h, name.encode(), ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
+
+
+def solve_default_opts(**kw) -> SolveOpts:
+    o = SolveOpts()
+    check(lib().tad_solve_default_opts(ctypes.byref(o)))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def newton_default_opts(**kw) -> NewtonOpts:
+    o = NewtonOpts()
+    check(lib().tad_newton_default_opts(ctypes.byref(o)))
+    for k, v in kw.items():
+        if k.startswith("solve_"):
+            setattr(o.solve, k[6:], v)
+        else:
+            setattr(o, k, v)
+    return o
+
+
+class ScalarFunction:
+    """ScalarFunction<d, double, int> backed by the HIP library."""
+
+    def __init__(self, d: int, n_handles: int, ctx: Context | None = None, precision: str = "fp64"):
+        self.ctx = ctx or Context.default()
+        self.d = int(d)
+        self.n_handles = int(n_handles)
+        self.n_vars = self.d * self.n_handles
+        self.precision = precision
+        prec = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}[precision]
+        h = ctypes.c_void_p()
+        check(lib().tad_fn_create(self.ctx._h, self.d, self.n_handles, prec, ctypes.byref(h)))
+        self._h = h
+        self.n_elements = 0
+        self._pattern = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().tad_fn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- construction ------------------------------------------------------
+    def add_elements(self, valence: int, energy: int, conn, params):
+        conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1, valence)
+        params = np.ascontiguousarray(params, dtype=np.float64)
+        if params.ndim == 1:
+            params = np.ascontiguousarray(np.broadcast_to(params, (conn.shape[0], params.size)))
+        check(lib().tad_fn_add_elements(self._h, energy, valence, conn.shape[0], conn.ctypes.data,
+                                        params.ctypes.data, params.shape[1]))
+        self.n_elements += conn.shape[0]
+        self._pattern = None
+
+    @classmethod
+    def from_problem(cls, problem, ctx=None, precision="fp64"):
+        f = cls(problem.d, problem.n_handles, ctx, precision)
+        for t in problem.terms:
+            f.add_elements(t.valence, t.energy, t.conn, t.params)
+        return f
+
+    def x_from_data(self, data) -> np.ndarray:
+        """x_from_data (ScalarFunctionImpl.hh:216-235): rows of `data` are the handle values."""
+        x = np.ascontiguousarray(data, dtype=np.float64).reshape(self.n_handles, self.d).ravel().copy()
+        if not np.all(np.isfinite(x)):
+            raise TadError(1, "x_from_data: non-finite entries")
+        return x
+
+    def x_to_data(self, x) -> np.ndarray:
+        """x_to_data (ScalarFunctionImpl.hh:237-254)."""
+        x = np.asarray(x, dtype=np.float64)
+        if x.size != self.n_vars:
+            raise TadError(1, "x_to_data: size mismatch")
+        return x.reshape(self.n_handles, self.d).copy()
+
+    # -- pattern -----------------------------------------------------------
+    def pattern(self):
+        """(row_ptr, col_idx) host copies of the Hessian CSR pattern."""
+        if self._pattern is None:
+            nnz = ctypes.c_int64()
+            check(lib().tad_fn_pattern(self._h, ctypes.byref(nnz), None, None, None, None))
+            rp = np.zeros(self.n_vars + 1, dtype=np.int32)
+            ci = np.zeros(nnz.value, dtype=np.int32)
+            check(lib().tad_fn_pattern(self._h, None, None, None, rp.ctypes.data, ci.ctypes.data))
+            self._pattern = (rp, ci)
+        return self._pattern
+
+    @property
+    def nnz(self) -> int:
+        return int(self.pattern()[1].size)
+
+    def pattern_device(self):
+        """(nnz, row_ptr_dev, col_idx_dev) raw device pointers."""
+        nnz = ctypes.c_int64()
+        rp = ctypes.c_void_p()
+        ci = ctypes.c_void_p()
+        check(lib().tad_fn_pattern(self._h, ctypes.byref(nnz), ctypes.byref(rp), ctypes.byref(ci), None, None))
+        return nnz.value, rp.value, ci.value
+
+    # -- evaluation --------------------------------------------------------
+    def _x(self, x):
+        if _is_torch(x):
+            import torch
+            if x.device.type != "cuda" or x.dtype != torch.float64 or not x.is_contiguous():
+                raise TadError(1, "device x must be a contiguous float64 CUDA tensor")
+            if x.numel() != self.n_vars:
+                raise TadError(1, f"x has {x.numel()} entries, expected {self.n_vars}")
+            _order_after_torch(self.ctx, x.device)
+            return x.data_ptr(), _lib.MEM_DEVICE
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.size != self.n_vars:
+            raise TadError(1, f"x has {x.size} entries, expected {self.n_vars}")
+        self._xkeep = x
+        return x.ctypes.data, _lib.MEM_HOST
+
+    def eval(self, x) -> float:
+        ptr, mem = self._x(x)
+        f = ctypes.c_double()
+        check(lib().tad_fn_eval(self._h, ptr, mem, ctypes.byref(f)))
+        return f.value
+
+    __call__ = eval
+
+    def _outputs(self, mem, with_h):
+        if mem == _lib.MEM_DEVICE:
+            import torch
+            dev = torch.device("cuda", self.ctx.device)
+            g = torch.empty(self.n_vars, dtype=torch.float64, device=dev)
+            H = torch.empty(self.nnz, dtype=torch.float64, device=dev) if with_h else None
+            return g, H, g.data_ptr(), (H.data_ptr() if with_h else None)
+        g = np.zeros(self.n_vars)
+        H = np.zeros(self.nnz) if with_h else None
+        return g, H, g.ctypes.data, (H.ctypes.data if with_h else None)
+
+    def _wrap_h(self, H, mem):
+        if mem == _lib.MEM_DEVICE:
+            return H
+        import scipy.sparse as sp
+        rp, ci = self.pattern()
+        return sp.csr_matrix((H, ci, rp), shape=(self.n_vars, self.n_vars))
+
+    def eval_with_gradient(self, x):
+        ptr, mem = self._x(x)
+        g, _, gp, _ = self._outputs(mem, False)
+        f = ctypes.c_double()
+        check(lib().tad_fn_eval_with_gradient(self._h, ptr, ctypes.byref(f), gp, mem))
+        return f.value, g
+
+    def eval_with_derivatives(self, x):
+        ptr, mem = self._x(x)
+        g, H, gp, hp = self._outputs(mem, True)
+        f = ctypes.c_double()
+        check(lib().tad_fn_eval_with_derivatives(self._h, ptr, ctypes.byref(f), gp, hp, mem))
+        return f.value, g, self._wrap_h(H, mem)
+
+    def eval_hessian(self, x):
+        return self.eval_with_derivatives(x)[2]
+
+    def eval_hessian_of_quadratic(self):
+        """Hessian at x = 0 (ScalarFunctionImpl.hh:369-376)."""
+        return self.eval_hessian(np.zeros(self.n_vars))
+
+    def eval_with_hessian_proj(self, x, eps: float = DEFAULT_HESSIAN_PROJECTION_EPS):
+        ptr, mem = self._x(x)
+        g, H, gp, hp = self._outputs(mem, True)
+        f = ctypes.c_double()
+        check(lib().tad_fn_eval_with_hessian_proj(self._h, ptr, float(eps), ctypes.byref(f), gp, hp, mem))
+        return f.value, g, self._wrap_h(H, mem)
+
+    # -- solver ------------------------------------------------------------
+    def solve(self, rhs, H=None, **opts):
+        """Solves (w I + H) sol = rhs with the library's PCG; H=None uses the last evaluated Hessian."""
+        o = solve_default_opts(**opts)
+        st = SolveStats()
+        if _is_torch(rhs):
+            import torch
+            _order_after_torch(self.ctx, rhs.device)
+            sol = torch.empty_like(rhs)
+            hp = H.data_ptr() if H is not None else None
+            check(lib().tad_fn_solve(self._h, hp, rhs.data_ptr(), sol.data_ptr(), _lib.MEM_DEVICE,
+                                     ctypes.byref(o), ctypes.byref(st)))
+            return sol, st
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        sol = np.zeros_like(rhs)
+        hv = None
+        if H is not None:
+            hv = np.ascontiguousarray(H.data if hasattr(H, "data") and not isinstance(H, np.ndarray) else H,
+                                      dtype=np.float64)
+        check(lib().tad_fn_solve(self._h, hv.ctypes.data if hv is not None else None, rhs.ctypes.data,
+                                 sol.ctypes.data, _lib.MEM_HOST, ctypes.byref(o), ctypes.byref(st)))
+        return sol, st
+
+    def newton_step(self, x, opts: NewtonOpts | None = None) -> dict:
+        """One projected-Newton iteration in place on x (NewtonTest.cc:69-75 loop body)."""
+        o = opts or newton_default_opts()
+        st = NewtonStats()
+        if _is_torch(x):
+            self._x(x)
+            ptr, mem = x.data_ptr(), _lib.MEM_DEVICE
+        else:
+            if not (isinstance(x, np.ndarray) and x.dtype == np.float64 and x.flags.c_contiguous):
+                raise TadError(1, "host x must be a contiguous float64 numpy array (updated in place)")
+            ptr, mem = x.ctypes.data, _lib.MEM_HOST
+        check(lib().tad_newton_step(self._h, ptr, mem, ctypes.byref(o), ctypes.byref(st)))
+        return st.as_dict()
+
+    def line_search(self, x0, d, f, g, out=None, opts: NewtonOpts | None = None):
+        """line_search (LineSearch.hh:26-65) on device tensors; returns (x_new, f_new, step, trials)."""
+        import torch
+        o = opts or newton_default_opts()
+        self._x(x0)
+        out = torch.empty_like(x0) if out is None else out
+        fo, so, tr = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        check(lib().tad_line_search(self._h, x0.data_ptr(), d.data_ptr(), float(f), g.data_ptr(), out.data_ptr(),
+                                    _lib.MEM_DEVICE, ctypes.byref(o), ctypes.byref(fo), ctypes.byref(so),
+                                    ctypes.byref(tr)))
+        return out, fo.value, so.value, tr.value
+
+
+def scalar_function(d: int, n_handles: int, ctx: Context | None = None, precision: str = "fp64") -> ScalarFunction:
+    """scalar_function<d>(range(n_handles)) (ScalarFunctionImpl.hh:418-442)."""
+    return ScalarFunction(d, n_handles, ctx, precision)
+
+
+def selftest_dual(which: int, x: float, y: float, ctx: Context | None = None):
+    ctx = ctx or Context.default()
+    out = (ctypes.c_double * 7)()
+    check(lib().tad_selftest_dual(ctx._h, which, x, y, out))
+    v = np.array(out[:])
+    return v[0], v[1:3], v[3:].reshape(2, 2)

@@ -1,0 +1,1034 @@
+// Host side of libtinyad_amd.so: the C ABI declared in include/tinyad_amd.h.
+//
+// Object model (mirrors include/TinyAD/ScalarFunction.hh:36-236):
+//   tad_ctx  -- a device and one HIP stream (the reference's OpenMP team,
+//               Detail/Parallel.hh:20-69).
+//   tad_fn   -- ScalarFunction: variable dimension d, handle count, objective
+//               terms added by tad_fn_add_elements (ScalarObjectiveTerm), plus
+//               the device-resident state: connectivity and parameters (SoA),
+//               the CSR pattern, the element-order gather maps, the element
+//               record arena and the solver work vectors.
+//
+// Evaluation = element kernel per term (records into the arena) -> gather of
+// g and CSR values in element order -> deterministic FP64 energy sum.
+#include "tinyad_amd.h"
+#include "tad_internal.hpp"
+
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+using namespace tad;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+tad_status fail(tad_status st, const std::string& msg)
+{
+    g_last_error = msg;
+    return st;
+}
+
+#define TAD_HIP(call)                                                                              \
+    do                                                                                             \
+    {                                                                                              \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(TAD_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+#define TAD_TRY(call)                                                                              \
+    do                                                                                             \
+    {                                                                                              \
+        tad_status s_ = (call);                                                                    \
+        if (s_ != TAD_OK) return s_;                                                               \
+    } while (0)
+
+int energy_param_count(int energy, int d, int N)
+{
+    const int K = d * N;
+    switch (energy)
+    {
+        case TAD_E_SYMDIR_TRI: return 5;
+        case TAD_E_SYMDIR_TET: return 10;
+        case TAD_E_PENALTY: return d + 1;
+        case TAD_E_QUADRATIC: return K * K + K + 1;
+        default: return -1;
+    }
+}
+
+constexpr int rec_size(int N, int d) { return N * d + (N * (N + 1) / 2) * d * d; }
+
+template <class T>
+struct DevBuf
+{
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t count)
+    {
+        if (count <= n && p) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct TimedPair
+{
+    std::string name;
+    hipEvent_t a, b;
+};
+
+struct KernelTimer : Timer
+{
+    bool enabled = false;
+    std::vector<TimedPair> pending;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, std::pair<int64_t, double>> stats;
+    std::map<std::string, hipEvent_t> open;
+
+    hipEvent_t get()
+    {
+        if (!pool.empty())
+        {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    void begin(const char* name, hipStream_t s) override
+    {
+        if (!enabled) return;
+        hipEvent_t e = get();
+        (void)hipEventRecord(e, s);
+        open[name] = e;
+    }
+    void end(const char* name, hipStream_t s) override
+    {
+        if (!enabled) return;
+        auto it = open.find(name);
+        if (it == open.end()) return;
+        hipEvent_t e = get();
+        (void)hipEventRecord(e, s);
+        pending.push_back({ name, it->second, e });
+        open.erase(it);
+        if (pending.size() > 4096) collect();
+    }
+    void collect()
+    {
+        for (auto& p : pending)
+        {
+            (void)hipEventSynchronize(p.b);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, p.a, p.b);
+            auto& st = stats[p.name];
+            st.first += 1;
+            st.second += ms;
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    ~KernelTimer() override
+    {
+        collect();
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+} // namespace
+
+struct tad_ctx_s
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    KernelTimer timer;
+    double* pinned = nullptr;  // small pinned staging buffer (64 doubles)
+};
+
+struct HostTerm
+{
+    int energy, N, n_params;
+    int64_t n_elem;
+    std::vector<int32_t> conn;  // row-major n_elem x N
+};
+
+struct tad_fn_s
+{
+    tad_ctx ctx = nullptr;
+    int d = 0;
+    int64_t n_handles = 0, n_vars = 0, n_elem_total = 0;
+    int precision = 0;
+    std::vector<TermDev> terms;
+    std::vector<HostTerm> hterms;
+    int64_t arena_size = 0;  // entries of the record arena
+
+    // pattern (block rows = handles)
+    bool pattern_ready = false;
+    int64_t n_blocks = 0, nnz = 0;
+    DevBuf<int32_t> brow_ptr, bcol, block_row, diag_block, row_ptr, col_idx;
+    DevBuf<int64_t> hmap_ptr, gmap_ptr;
+    DevBuf<uint32_t> hmap, gmap;
+
+    // evaluation buffers
+    DevBuf<unsigned char> arena;
+    DevBuf<double> elem_f, x_buf, g_buf, H_buf, partials, results, steps;
+    DevBuf<int> err;
+    // line search
+    DevBuf<double> ls_f, x_tmp;
+    // solver
+    DevBuf<double> Dinv, r, z, p, q, sol;
+    DevBuf<PcgScalars> scal;
+    bool h_valid = false;
+
+    hipStream_t s() const { return ctx->stream; }
+};
+
+namespace {
+
+tad_status check_fn(tad_fn fn)
+{
+    if (!fn) return fail(TAD_ERR_INVALID, "null function handle");
+    return TAD_OK;
+}
+
+void free_pattern(tad_fn fn)
+{
+    fn->brow_ptr.release(); fn->bcol.release(); fn->block_row.release(); fn->diag_block.release();
+    fn->row_ptr.release(); fn->col_idx.release(); fn->hmap_ptr.release(); fn->gmap_ptr.release();
+    fn->hmap.release(); fn->gmap.release();
+    fn->pattern_ready = false;
+}
+
+// Builds the block CSR pattern, the scalar CSR pattern and the element-order
+// gather maps.  Runs once per function (after the last add_elements).
+tad_status build_pattern(tad_fn fn)
+{
+    const int d = fn->d;
+    const int64_t nh = fn->n_handles;
+    // 1. adjacency (with duplicates) per handle
+    std::vector<int64_t> cnt(nh + 1, 0);
+    for (const HostTerm& t : fn->hterms)
+        for (int64_t e = 0; e < t.n_elem; ++e)
+            for (int a = 0; a < t.N; ++a) cnt[t.conn[e * t.N + a] + 1] += t.N;
+    for (int64_t u = 0; u < nh; ++u) cnt[u + 1] += cnt[u];
+    std::vector<int32_t> adj(cnt[nh]);
+    {
+        std::vector<int64_t> cur(cnt.begin(), cnt.end() - 1);
+        for (const HostTerm& t : fn->hterms)
+            for (int64_t e = 0; e < t.n_elem; ++e)
+                for (int a = 0; a < t.N; ++a)
+                {
+                    const int32_t u = t.conn[e * t.N + a];
+                    for (int b = 0; b < t.N; ++b) adj[cur[u]++] = t.conn[e * t.N + b];
+                }
+    }
+    // handles used by no element keep empty rows, as in the reference's
+    // triplet assembly; their diagonal block index is -1
+    std::vector<int32_t> brow_ptr(nh + 1, 0), bcol, diag(nh, -1);
+    bcol.reserve(adj.size() / 3 + nh);
+    for (int64_t u = 0; u < nh; ++u)
+    {
+        auto b = adj.begin() + cnt[u], e = adj.begin() + cnt[u + 1];
+        std::sort(b, e);
+        e = std::unique(b, e);
+        for (auto it = b; it != e; ++it) bcol.push_back(*it);
+        brow_ptr[u + 1] = (int32_t)bcol.size();
+        for (int32_t k = brow_ptr[u]; k < brow_ptr[u + 1]; ++k)
+            if (bcol[k] == u) diag[u] = k - brow_ptr[u];
+    }
+    const int64_t nb = (int64_t)bcol.size();
+    if ((double)nb * d * d >= 2147483647.0)
+        return fail(TAD_ERR_UNSUPPORTED, "Hessian has more than 2^31 non-zeros");
+    std::vector<int32_t> block_row(nb);
+    for (int64_t u = 0; u < nh; ++u)
+        for (int32_t k = brow_ptr[u]; k < brow_ptr[u + 1]; ++k) block_row[k] = (int32_t)u;
+
+    auto find_block = [&](int32_t u, int32_t v) -> int64_t {
+        const int32_t* b = bcol.data() + brow_ptr[u];
+        const int32_t* e = bcol.data() + brow_ptr[u + 1];
+        return (int64_t)(std::lower_bound(b, e, v) - bcol.data());
+    };
+
+    // 2. Hessian gather map (element order within each block)
+    std::vector<int64_t> hptr(nb + 1, 0);
+    std::vector<int64_t> gptr(nh + 1, 0);
+    for (const HostTerm& t : fn->hterms)
+        for (int64_t e = 0; e < t.n_elem; ++e)
+            for (int a = 0; a < t.N; ++a)
+            {
+                const int32_t u = t.conn[e * t.N + a];
+                gptr[u + 1] += 1;
+                for (int b = 0; b < t.N; ++b) hptr[find_block(u, t.conn[e * t.N + b]) + 1] += 1;
+            }
+    for (int64_t k = 0; k < nb; ++k) hptr[k + 1] += hptr[k];
+    for (int64_t u = 0; u < nh; ++u) gptr[u + 1] += gptr[u];
+    std::vector<uint32_t> hmap(hptr[nb]), gmap(gptr[nh]);
+    {
+        std::vector<int64_t> hc(hptr.begin(), hptr.end() - 1), gc(gptr.begin(), gptr.end() - 1);
+        for (size_t ti = 0; ti < fn->hterms.size(); ++ti)
+        {
+            const HostTerm& t = fn->hterms[ti];
+            const TermDev& td = fn->terms[ti];
+            const int RS = rec_size(t.N, d);
+            for (int64_t e = 0; e < t.n_elem; ++e)
+            {
+                const int64_t rec = td.rec_base + e * RS;
+                for (int a = 0; a < t.N; ++a)
+                {
+                    const int32_t u = t.conn[e * t.N + a];
+                    gmap[gc[u]++] = (uint32_t)(rec + a * d);
+                    for (int b = 0; b < t.N; ++b)
+                    {
+                        const int lo = std::min(a, b), hi = std::max(a, b);
+                        const int bi = lo * t.N - (lo * (lo - 1)) / 2 + (hi - lo);
+                        const int64_t off = rec + t.N * d + (int64_t)bi * d * d;
+                        const uint32_t flag = (a > b) ? 0x80000000u : 0u;
+                        hmap[hc[find_block(u, t.conn[e * t.N + b])]++] = (uint32_t)off | flag;
+                    }
+                }
+            }
+        }
+    }
+    // 3. scalar CSR
+    const int64_t n = fn->n_vars;
+    std::vector<int32_t> row_ptr(n + 1, 0);
+    for (int64_t u = 0; u < nh; ++u)
+    {
+        const int nbu = brow_ptr[u + 1] - brow_ptr[u];
+        for (int i = 0; i < d; ++i) row_ptr[d * u + i + 1] = row_ptr[d * u + i] + d * nbu;
+    }
+    const int64_t nnz = row_ptr[n];
+    std::vector<int32_t> col_idx(nnz);
+    for (int64_t u = 0; u < nh; ++u)
+    {
+        const int nbu = brow_ptr[u + 1] - brow_ptr[u];
+        for (int i = 0; i < d; ++i)
+        {
+            int64_t o = row_ptr[d * u + i];
+            for (int k = 0; k < nbu; ++k)
+                for (int j = 0; j < d; ++j) col_idx[o++] = d * bcol[brow_ptr[u] + k] + j;
+        }
+    }
+    hipStream_t s = fn->s();
+    auto up = [&](auto& buf, const auto& vec) -> hipError_t {
+        hipError_t e = buf.ensure(vec.size());
+        if (e != hipSuccess) return e;
+        if (vec.empty()) return hipSuccess;
+        return hipMemcpyAsync(buf.p, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice, s);
+    };
+    TAD_HIP(up(fn->brow_ptr, brow_ptr));
+    TAD_HIP(up(fn->bcol, bcol));
+    TAD_HIP(up(fn->block_row, block_row));
+    TAD_HIP(up(fn->diag_block, diag));
+    TAD_HIP(up(fn->row_ptr, row_ptr));
+    TAD_HIP(up(fn->col_idx, col_idx));
+    TAD_HIP(up(fn->hmap_ptr, hptr));
+    TAD_HIP(up(fn->hmap, hmap));
+    TAD_HIP(up(fn->gmap_ptr, gptr));
+    TAD_HIP(up(fn->gmap, gmap));
+    TAD_HIP(hipStreamSynchronize(s));
+    fn->n_blocks = nb;
+    fn->nnz = nnz;
+    fn->pattern_ready = true;
+    return TAD_OK;
+}
+
+tad_status ensure_ready(tad_fn fn)
+{
+    if (fn->terms.empty() && fn->n_elem_total == 0)
+    {
+        // no elements: f = 0, g = 0, empty H (ScalarFunctionTest.cc Move1 default case)
+    }
+    if (!fn->pattern_ready && !fn->terms.empty()) TAD_TRY(build_pattern(fn));
+    const size_t tsz = fn->precision == TAD_PREC_FP64 ? sizeof(double) : sizeof(float);
+    TAD_HIP(fn->arena.ensure((size_t)fn->arena_size * tsz));
+    TAD_HIP(fn->elem_f.ensure((size_t)fn->n_elem_total));
+    TAD_HIP(fn->x_buf.ensure((size_t)fn->n_vars));
+    TAD_HIP(fn->g_buf.ensure((size_t)fn->n_vars));
+    TAD_HIP(fn->H_buf.ensure((size_t)fn->nnz));
+    TAD_HIP(fn->partials.ensure(8 * kReduceBlocks + 4096));
+    TAD_HIP(fn->results.ensure(16));
+    TAD_HIP(fn->steps.ensure(16));
+    TAD_HIP(fn->err.ensure(1));
+    return TAD_OK;
+}
+
+const double* stage_x(tad_fn fn, const double* x, tad_mem mem)
+{
+    if (mem == TAD_MEM_DEVICE) return x;
+    if (hipMemcpyAsync(fn->x_buf.p, x, fn->n_vars * sizeof(double), hipMemcpyHostToDevice, fn->s()) != hipSuccess)
+        return nullptr;
+    return fn->x_buf.p;
+}
+
+// Runs element kernels + gathers.  mode: MODE_GRAD / MODE_HESS / MODE_HESS_PROJ.
+// g_dev / H_dev: device destinations.  Returns f via *f (synchronizes).
+tad_status eval_core(tad_fn fn, const double* x_dev, int mode, double eps, double* g_dev, double* H_dev, double* f)
+{
+    hipStream_t s = fn->s();
+    KernelTimer& tm = fn->ctx->timer;
+    TAD_HIP(hipMemsetAsync(fn->err.p, 0, sizeof(int), s));
+    tm.begin("element", s);
+    for (const TermDev& t : fn->terms)
+        TAD_HIP(launch_element(t, fn->d, fn->precision, mode, x_dev, eps, fn->elem_f.p, fn->arena.p, fn->err.p, s));
+    tm.end("element", s);
+    if (g_dev)
+    {
+        tm.begin("assemble_gradient", s);
+        TAD_HIP(launch_assemble_gradient(fn->d, fn->precision, fn->n_handles, fn->gmap_ptr.p, fn->gmap.p, fn->arena.p,
+                                         g_dev, s));
+        tm.end("assemble_gradient", s);
+    }
+    if (H_dev && mode >= MODE_HESS)
+    {
+        tm.begin("assemble_hessian", s);
+        TAD_HIP(launch_assemble_hessian(fn->d, fn->precision, fn->n_blocks, fn->block_row.p, fn->brow_ptr.p,
+                                        fn->hmap_ptr.p, fn->hmap.p, fn->arena.p, fn->row_ptr.p, H_dev, s));
+        tm.end("assemble_hessian", s);
+    }
+    tm.begin("energy_sum", s);
+    TAD_HIP(launch_sum(fn->elem_f.p, fn->n_elem_total, 1, fn->n_elem_total, fn->partials.p, fn->results.p, s));
+    tm.end("energy_sum", s);
+    double* pin = fn->ctx->pinned;
+    TAD_HIP(hipMemcpyAsync(pin, fn->results.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    TAD_HIP(hipMemcpyAsync(pin + 1, fn->err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    TAD_HIP(hipStreamSynchronize(s));
+    int errv = 0;
+    std::memcpy(&errv, pin + 1, sizeof(int));
+    *f = pin[0];
+    if (errv) return fail(TAD_ERR_NONFINITE, "non-finite element gradient or Hessian");
+    return TAD_OK;
+}
+
+tad_status eval_energy_batch(tad_fn fn, const double* x_dev, const double* dir_dev, const double* steps_host, int ncand,
+                             double* f_out)
+{
+    hipStream_t s = fn->s();
+    KernelTimer& tm = fn->ctx->timer;
+    const int64_t ne = fn->n_elem_total;
+    if (dir_dev)
+    {
+        TAD_HIP(fn->ls_f.ensure((size_t)ne * 8));
+        double* pin = fn->ctx->pinned;
+        std::memcpy(pin + 16, steps_host, ncand * sizeof(double));
+        TAD_HIP(hipMemcpyAsync(fn->steps.p, pin + 16, ncand * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    double* dst = dir_dev ? fn->ls_f.p : fn->elem_f.p;
+    const int nc = dir_dev ? ncand : 1;
+    tm.begin("energy", s);
+    for (const TermDev& t : fn->terms)
+        TAD_HIP(launch_energy(t, fn->d, x_dev, dir_dev, fn->steps.p, nc, dst, ne, s));
+    tm.end("energy", s);
+    tm.begin("energy_sum", s);
+    TAD_HIP(launch_sum(dst, ne, nc, ne, fn->partials.p, fn->results.p, s));
+    tm.end("energy_sum", s);
+    double* pin = fn->ctx->pinned;
+    TAD_HIP(hipMemcpyAsync(pin + 32, fn->results.p, nc * sizeof(double), hipMemcpyDeviceToHost, s));
+    TAD_HIP(hipStreamSynchronize(s));
+    for (int c = 0; c < nc; ++c) f_out[c] = pin[32 + c];
+    return TAD_OK;
+}
+
+tad_status dot_dev(tad_fn fn, const double* a, const double* b, double* out)
+{
+    hipStream_t s = fn->s();
+    TAD_HIP(launch_dot(a, b, fn->n_vars, fn->partials.p, fn->results.p + 8, s));
+    double* pin = fn->ctx->pinned;
+    TAD_HIP(hipMemcpyAsync(pin + 48, fn->results.p + 8, sizeof(double), hipMemcpyDeviceToHost, s));
+    TAD_HIP(hipStreamSynchronize(s));
+    *out = pin[48];
+    return TAD_OK;
+}
+
+tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, double* sol_dev,
+                      const tad_solve_opts& o, tad_solve_stats* st)
+{
+    const int d = fn->d;
+    const int64_t nh = fn->n_handles, n = fn->n_vars;
+    hipStream_t s = fn->s();
+    KernelTimer& tm = fn->ctx->timer;
+    TAD_HIP(fn->Dinv.ensure((size_t)nh * d * d));
+    TAD_HIP(fn->r.ensure(n));
+    TAD_HIP(fn->z.ensure(n));
+    TAD_HIP(fn->p.ensure(n));
+    TAD_HIP(fn->q.ensure(n));
+    TAD_HIP(fn->scal.ensure(1));
+    // per-block partials of the PCG kernels: the SpMV grid has nh * 16 / 256
+    // blocks (16 lanes per block row), the vector kernels nh / 256; two
+    // partial arrays each
+    const int64_t spmv_blocks = (nh * 16 + 255) / 256;
+    const int64_t vec_blocks = (nh + 255) / 256;
+    TAD_HIP(fn->partials.ensure((size_t)(2 * std::max(spmv_blocks, vec_blocks) + 8 * kReduceBlocks + 64)));
+    tm.begin("pcg_setup", s);
+    TAD_HIP(launch_pcg_setup(d, nh, fn->brow_ptr.p, fn->diag_block.p, fn->row_ptr.p, H_dev, o.w_identity, fn->Dinv.p, s));
+    tm.end("pcg_setup", s);
+    PcgScalars hs;
+    std::memset(&hs, 0, sizeof(hs));
+    hs.tol2 = o.tol * o.tol;
+    hs.max_iters = o.max_iters;
+    PcgScalars* pin = (PcgScalars*)(fn->ctx->pinned + 56);
+    std::memcpy(pin, &hs, sizeof(hs));
+    TAD_HIP(hipMemcpyAsync(fn->scal.p, pin, sizeof(hs), hipMemcpyHostToDevice, s));
+    tm.begin("pcg_init", s);
+    TAD_HIP(launch_pcg_init(d, nh, rhs_dev, fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->p.p, fn->partials.p, fn->scal.p, s));
+    tm.end("pcg_init", s);
+    const int every = std::max(1, o.check_every);
+    Timer* timer = fn->ctx->timer.enabled ? &fn->ctx->timer : nullptr;
+    int64_t launched = 0;
+    for (;;)
+    {
+        TAD_HIP(hipMemcpyAsync(pin, fn->scal.p, sizeof(hs), hipMemcpyDeviceToHost, s));
+        TAD_HIP(hipStreamSynchronize(s));
+        if (pin->done) break;
+        // the device sets done at max_iters; this host bound only guards
+        // against corrupted solver state
+        if (launched > (int64_t)o.max_iters + every)
+            return fail(TAD_ERR_SOLVE, "PCG did not report termination");
+        launched += every;
+        for (int k = 0; k < every; ++k)
+            TAD_HIP(launch_pcg_iteration(d, nh, fn->brow_ptr.p, fn->bcol.p, fn->row_ptr.p, H_dev, o.w_identity,
+                                         fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->p.p, fn->q.p, fn->partials.p,
+                                         fn->scal.p, s, timer));
+    }
+    std::memcpy(&hs, pin, sizeof(hs));
+    const double rel = hs.bb > 0 ? std::sqrt(hs.rr / hs.bb) : 0.0;
+    if (st)
+    {
+        st->iters = hs.iters;
+        st->rel_residual = rel;
+        st->converged = (!hs.breakdown && hs.rr <= hs.tol2 * hs.bb) ? 1 : 0;
+    }
+    if (hs.breakdown) return fail(TAD_ERR_SOLVE, "Linear solve failed: matrix not positive definite");
+    if (!std::isfinite(rel)) return fail(TAD_ERR_SOLVE, "Linear solve failed: non-finite residual");
+    return TAD_OK;
+}
+
+// Armijo backtracking on device data (LineSearch.hh:26-65).  The direction is
+// dir_scale * dir (dir_scale = -1 lets the Newton step pass the solution of
+// H d' = g without negating it).  Writes the accepted point to x_out.
+tad_status line_search_core(tad_fn fn, const double* x0, const double* dir, double dir_scale, double f,
+                            double dir_dot_g, double* x_out, const tad_newton_opts& o, double* f_out, double* s_out,
+                            int* trials, bool* failed)
+{
+    if (!(o.ls_s_max > 0.0)) return fail(TAD_ERR_INVALID, "Max step size not positive.");
+    const bool try_one = o.ls_s_max > 1.0;
+    const int batch = std::min(8, std::max(1, o.ls_batch));
+    double s = o.ls_s_max;
+    int it = 0;
+    *failed = false;
+    while (it < o.ls_max_iters)
+    {
+        const int nc = std::min(batch, o.ls_max_iters - it);
+        double svals[8], steps[8], fv[8];
+        for (int c = 0; c < nc; ++c)
+        {
+            svals[c] = s;
+            steps[c] = dir_scale * s;
+            if (try_one && s > 1.0 && s * o.ls_shrink < 1.0) s = 1.0;
+            else s *= o.ls_shrink;
+        }
+        TAD_TRY(eval_energy_batch(fn, x0, dir, steps, nc, fv));
+        for (int c = 0; c < nc; ++c)
+        {
+            ++it;
+            if (fv[c] != fv[c]) return fail(TAD_ERR_NAN_ENERGY, "energy is NaN in line search");
+            if (fv[c] <= f + o.ls_armijo * svals[c] * dir_dot_g)
+            {
+                double* pin = fn->ctx->pinned;
+                pin[24] = steps[c];
+                TAD_HIP(hipMemcpyAsync(fn->steps.p + 8, pin + 24, sizeof(double), hipMemcpyHostToDevice, fn->s()));
+                TAD_HIP(launch_axpy_point(x0, dir, fn->steps.p + 8, fn->n_vars, x_out, fn->s()));
+                TAD_HIP(hipStreamSynchronize(fn->s()));
+                *f_out = fv[c];
+                *s_out = svals[c];
+                *trials = it;
+                return TAD_OK;
+            }
+        }
+    }
+    if (x_out != x0)
+        TAD_HIP(hipMemcpyAsync(x_out, x0, fn->n_vars * sizeof(double), hipMemcpyDeviceToDevice, fn->s()));
+    TAD_HIP(hipStreamSynchronize(fn->s()));
+    *f_out = f;
+    *s_out = 0.0;
+    *trials = it;
+    *failed = true;
+    return TAD_OK;
+}
+
+} // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* tad_last_error(void) { return g_last_error.c_str(); }
+
+const char* tad_version(void) { return "tinyad_amd 0.1 (gfx950)"; }
+
+tad_status tad_ctx_create(int device, tad_ctx* out)
+{
+    if (!out) return fail(TAD_ERR_INVALID, "null output pointer");
+    int count = 0;
+    TAD_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(TAD_ERR_INVALID, "invalid device index");
+    TAD_HIP(hipSetDevice(device));
+    std::unique_ptr<tad_ctx_s> c(new tad_ctx_s());
+    c->device = device;
+    TAD_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+    c->stream = c->own_stream;
+    TAD_HIP(hipHostMalloc((void**)&c->pinned, 256 * sizeof(double), hipHostMallocDefault));
+    *out = c.release();
+    return TAD_OK;
+}
+
+tad_status tad_ctx_destroy(tad_ctx ctx)
+{
+    if (!ctx) return TAD_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->timer.collect();
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return TAD_OK;
+}
+
+tad_status tad_ctx_set_stream(tad_ctx ctx, void* hip_stream)
+{
+    if (!ctx) return fail(TAD_ERR_INVALID, "null context");
+    ctx->stream = hip_stream ? (hipStream_t)hip_stream : ctx->own_stream;
+    return TAD_OK;
+}
+
+tad_status tad_ctx_get_stream(tad_ctx ctx, void** hip_stream)
+{
+    if (!ctx || !hip_stream) return fail(TAD_ERR_INVALID, "null argument");
+    *hip_stream = (void*)ctx->stream;
+    return TAD_OK;
+}
+
+tad_status tad_ctx_synchronize(tad_ctx ctx)
+{
+    if (!ctx) return fail(TAD_ERR_INVALID, "null context");
+    TAD_HIP(hipStreamSynchronize(ctx->stream));
+    return TAD_OK;
+}
+
+tad_status tad_ctx_enable_timing(tad_ctx ctx, int enable)
+{
+    if (!ctx) return fail(TAD_ERR_INVALID, "null context");
+    ctx->timer.enabled = enable != 0;
+    return TAD_OK;
+}
+
+tad_status tad_ctx_kernel_stats(tad_ctx ctx, const char* name, int64_t* launches, double* total_ms)
+{
+    if (!ctx || !name) return fail(TAD_ERR_INVALID, "null argument");
+    ctx->timer.collect();
+    auto it = ctx->timer.stats.find(name);
+    if (launches) *launches = it == ctx->timer.stats.end() ? 0 : it->second.first;
+    if (total_ms) *total_ms = it == ctx->timer.stats.end() ? 0.0 : it->second.second;
+    return TAD_OK;
+}
+
+tad_status tad_ctx_reset_stats(tad_ctx ctx)
+{
+    if (!ctx) return fail(TAD_ERR_INVALID, "null context");
+    ctx->timer.collect();
+    ctx->timer.stats.clear();
+    return TAD_OK;
+}
+
+tad_status tad_fn_create(tad_ctx ctx, int d, int64_t n_handles, tad_precision precision, tad_fn* out)
+{
+    if (!ctx || !out) return fail(TAD_ERR_INVALID, "null argument");
+    if (d < 1 || d > 3) return fail(TAD_ERR_INVALID, "variable dimension must be 1, 2 or 3");
+    if (n_handles < 1) return fail(TAD_ERR_INVALID, "at least one variable handle is required");
+    if ((double)d * n_handles >= 2147483647.0) return fail(TAD_ERR_UNSUPPORTED, "too many variables");
+    if (precision != TAD_PREC_FP64 && precision != TAD_PREC_MIXED) return fail(TAD_ERR_INVALID, "bad precision");
+    tad_fn f = new tad_fn_s();
+    f->ctx = ctx;
+    f->d = d;
+    f->n_handles = n_handles;
+    f->n_vars = d * n_handles;
+    f->precision = precision;
+    *out = f;
+    return TAD_OK;
+}
+
+tad_status tad_fn_destroy(tad_fn fn)
+{
+    if (!fn) return TAD_OK;
+    (void)hipSetDevice(fn->ctx->device);
+    (void)hipStreamSynchronize(fn->s());
+    for (TermDev& t : fn->terms)
+    {
+        if (t.conn) (void)hipFree(t.conn);
+        if (t.params) (void)hipFree(t.params);
+    }
+    free_pattern(fn);
+    fn->arena.release(); fn->elem_f.release(); fn->x_buf.release(); fn->g_buf.release(); fn->H_buf.release();
+    fn->partials.release(); fn->results.release(); fn->steps.release(); fn->err.release();
+    fn->ls_f.release(); fn->x_tmp.release();
+    fn->Dinv.release(); fn->r.release(); fn->z.release(); fn->p.release(); fn->q.release(); fn->sol.release();
+    fn->scal.release();
+    delete fn;
+    return TAD_OK;
+}
+
+tad_status tad_fn_add_elements(tad_fn fn, tad_energy energy, int valence, int64_t n_elem, const int32_t* conn,
+                               const double* params, int params_per_elem)
+{
+    TAD_TRY(check_fn(fn));
+    const int d = fn->d, N = valence;
+    if (N < 1 || d * N > 12) return fail(TAD_ERR_INVALID, "element size d * valence must be in 1..12");
+    if (n_elem < 0) return fail(TAD_ERR_INVALID, "negative element count");
+    if (energy == TAD_E_SYMDIR_TRI && !(d == 2 && N == 3))
+        return fail(TAD_ERR_INVALID, "symmetric Dirichlet triangles need d = 2 and valence 3");
+    if (energy == TAD_E_SYMDIR_TET && !(d == 3 && N == 4))
+        return fail(TAD_ERR_INVALID, "symmetric Dirichlet tets need d = 3 and valence 4");
+    if (energy == TAD_E_PENALTY && N != 1) return fail(TAD_ERR_INVALID, "penalty elements have valence 1");
+    const int np = energy_param_count(energy, d, N);
+    if (np < 0) return fail(TAD_ERR_INVALID, "unknown energy");
+    if (np != params_per_elem)
+        return fail(TAD_ERR_INVALID, "energy expects " + std::to_string(np) + " parameters per element, got " +
+                                         std::to_string(params_per_elem));
+    if (n_elem > 0 && (!conn || !params)) return fail(TAD_ERR_INVALID, "null connectivity or parameters");
+    for (int64_t e = 0; e < n_elem; ++e)
+        for (int a = 0; a < N; ++a)
+        {
+            const int32_t v = conn[e * N + a];
+            if (v < 0 || v >= fn->n_handles)
+                return fail(TAD_ERR_INVALID, "element " + std::to_string(e) + " references handle out of range");
+            for (int b = 0; b < a; ++b)
+                if (conn[e * N + b] == v)
+                    return fail(TAD_ERR_INVALID, "element " + std::to_string(e) + " references a handle twice");
+        }
+    for (int64_t i = 0; i < n_elem * (int64_t)np; ++i)
+        if (!std::isfinite(params[i])) return fail(TAD_ERR_INVALID, "non-finite element parameter");
+    const int RS = rec_size(N, d);
+    if ((double)(fn->arena_size + n_elem * (int64_t)RS) >= 2147483647.0)
+        return fail(TAD_ERR_UNSUPPORTED, "element record arena exceeds 2^31 entries");
+
+    TermDev t;
+    t.energy = energy;
+    t.N = N;
+    t.n_params = np;
+    t.n_elem = n_elem;
+    t.rec_base = fn->arena_size;
+    t.f_base = fn->n_elem_total;
+    if (n_elem > 0)
+    {
+        std::vector<int32_t> cs((size_t)n_elem * N);
+        for (int64_t e = 0; e < n_elem; ++e)
+            for (int a = 0; a < N; ++a) cs[a * n_elem + e] = conn[e * N + a];
+        std::vector<double> ps((size_t)n_elem * np);
+        for (int64_t e = 0; e < n_elem; ++e)
+            for (int k = 0; k < np; ++k) ps[k * n_elem + e] = params[e * np + k];
+        TAD_HIP(hipMalloc((void**)&t.conn, cs.size() * sizeof(int32_t)));
+        TAD_HIP(hipMalloc((void**)&t.params, ps.size() * sizeof(double)));
+        TAD_HIP(hipMemcpy(t.conn, cs.data(), cs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        TAD_HIP(hipMemcpy(t.params, ps.data(), ps.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    HostTerm h;
+    h.energy = energy;
+    h.N = N;
+    h.n_params = np;
+    h.n_elem = n_elem;
+    h.conn.assign(conn, conn + n_elem * N);
+    fn->terms.push_back(t);
+    fn->hterms.push_back(std::move(h));
+    fn->arena_size += n_elem * RS;
+    fn->n_elem_total += n_elem;
+    free_pattern(fn);
+    fn->h_valid = false;
+    return TAD_OK;
+}
+
+tad_status tad_fn_n_vars(tad_fn fn, int64_t* n_vars)
+{
+    TAD_TRY(check_fn(fn));
+    *n_vars = fn->n_vars;
+    return TAD_OK;
+}
+
+tad_status tad_fn_n_elements(tad_fn fn, int64_t* n_elements)
+{
+    TAD_TRY(check_fn(fn));
+    *n_elements = fn->n_elem_total;
+    return TAD_OK;
+}
+
+tad_status tad_fn_pattern(tad_fn fn, int64_t* nnz, const int32_t** row_ptr_dev, const int32_t** col_idx_dev,
+                          int32_t* row_ptr_host, int32_t* col_idx_host)
+{
+    TAD_TRY(check_fn(fn));
+    TAD_TRY(ensure_ready(fn));
+    if (nnz) *nnz = fn->nnz;
+    if (row_ptr_dev) *row_ptr_dev = fn->row_ptr.p;
+    if (col_idx_dev) *col_idx_dev = fn->col_idx.p;
+    if (row_ptr_host)
+        TAD_HIP(hipMemcpy(row_ptr_host, fn->row_ptr.p, (fn->n_vars + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (col_idx_host && fn->nnz)
+        TAD_HIP(hipMemcpy(col_idx_host, fn->col_idx.p, fn->nnz * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return TAD_OK;
+}
+
+tad_status tad_fn_eval(tad_fn fn, const double* x, tad_mem mem, double* f)
+{
+    TAD_TRY(check_fn(fn));
+    if (!x || !f) return fail(TAD_ERR_INVALID, "null argument");
+    TAD_HIP(hipSetDevice(fn->ctx->device));
+    TAD_TRY(ensure_ready(fn));
+    if (fn->n_elem_total == 0) { *f = 0.0; return TAD_OK; }
+    const double* xd = stage_x(fn, x, mem);
+    if (!xd) return fail(TAD_ERR_HIP, "copy of x to the device failed");
+    double fv = 0.0;
+    TAD_TRY(eval_energy_batch(fn, xd, nullptr, nullptr, 1, &fv));
+    *f = fv;
+    return TAD_OK;
+}
+
+static tad_status eval_second(tad_fn fn, const double* x, int mode, double eps, double* f, double* g, double* H,
+                              tad_mem mem)
+{
+    TAD_TRY(check_fn(fn));
+    if (!x || !f) return fail(TAD_ERR_INVALID, "null argument");
+    TAD_HIP(hipSetDevice(fn->ctx->device));
+    TAD_TRY(ensure_ready(fn));
+    hipStream_t s = fn->s();
+    if (fn->n_elem_total == 0)
+    {
+        *f = 0.0;
+        if (g)
+        {
+            if (mem == TAD_MEM_DEVICE) TAD_HIP(launch_fill(g, fn->n_vars, 0.0, s));
+            else std::fill(g, g + fn->n_vars, 0.0);
+        }
+        TAD_HIP(hipStreamSynchronize(s));
+        return TAD_OK;
+    }
+    const double* xd = stage_x(fn, x, mem);
+    if (!xd) return fail(TAD_ERR_HIP, "copy of x to the device failed");
+    double* g_dev = (mem == TAD_MEM_DEVICE && g) ? g : fn->g_buf.p;
+    double* H_dev = (mode >= MODE_HESS) ? ((mem == TAD_MEM_DEVICE && H) ? H : fn->H_buf.p) : nullptr;
+    TAD_TRY(eval_core(fn, xd, mode, eps, g_dev, H_dev, f));
+    fn->h_valid = (H_dev == fn->H_buf.p);
+    if (mem == TAD_MEM_HOST)
+    {
+        if (g) TAD_HIP(hipMemcpyAsync(g, g_dev, fn->n_vars * sizeof(double), hipMemcpyDeviceToHost, s));
+        if (H && H_dev) TAD_HIP(hipMemcpyAsync(H, H_dev, fn->nnz * sizeof(double), hipMemcpyDeviceToHost, s));
+        TAD_HIP(hipStreamSynchronize(s));
+    }
+    return TAD_OK;
+}
+
+tad_status tad_fn_eval_with_gradient(tad_fn fn, const double* x, double* f, double* g, tad_mem mem)
+{
+    return eval_second(fn, x, MODE_GRAD, 0.0, f, g, nullptr, mem);
+}
+
+tad_status tad_fn_eval_with_derivatives(tad_fn fn, const double* x, double* f, double* g, double* H_values,
+                                        tad_mem mem)
+{
+    return eval_second(fn, x, MODE_HESS, 0.0, f, g, H_values, mem);
+}
+
+tad_status tad_fn_eval_with_hessian_proj(tad_fn fn, const double* x, double eps, double* f, double* g,
+                                         double* H_values, tad_mem mem)
+{
+    if (eps != eps) return fail(TAD_ERR_INVALID, "projection eps is NaN");
+    return eval_second(fn, x, MODE_HESS_PROJ, eps, f, g, H_values, mem);
+}
+
+tad_status tad_fn_device_buffers(tad_fn fn, const double** g_dev, const double** H_values_dev)
+{
+    TAD_TRY(check_fn(fn));
+    TAD_TRY(ensure_ready(fn));
+    if (g_dev) *g_dev = fn->g_buf.p;
+    if (H_values_dev) *H_values_dev = fn->H_buf.p;
+    return TAD_OK;
+}
+
+tad_status tad_solve_default_opts(tad_solve_opts* o)
+{
+    if (!o) return fail(TAD_ERR_INVALID, "null argument");
+    o->w_identity = 0.0;
+    o->tol = 1e-10;
+    o->max_iters = 20000;
+    o->check_every = 16;
+    return TAD_OK;
+}
+
+tad_status tad_fn_solve(tad_fn fn, const double* H_values, const double* rhs, double* sol, tad_mem mem,
+                        const tad_solve_opts* opts, tad_solve_stats* stats)
+{
+    TAD_TRY(check_fn(fn));
+    if (!rhs || !sol) return fail(TAD_ERR_INVALID, "null argument");
+    TAD_HIP(hipSetDevice(fn->ctx->device));
+    TAD_TRY(ensure_ready(fn));
+    tad_solve_opts o;
+    tad_solve_default_opts(&o);
+    if (opts) o = *opts;
+    hipStream_t s = fn->s();
+    const int64_t n = fn->n_vars;
+    const double* Hd;
+    if (!H_values)
+    {
+        if (!fn->h_valid) return fail(TAD_ERR_INVALID, "no Hessian evaluated yet");
+        Hd = fn->H_buf.p;
+    }
+    else if (mem == TAD_MEM_DEVICE)
+        Hd = H_values;
+    else
+    {
+        TAD_HIP(hipMemcpyAsync(fn->H_buf.p, H_values, fn->nnz * sizeof(double), hipMemcpyHostToDevice, s));
+        fn->h_valid = true;
+        Hd = fn->H_buf.p;
+    }
+    TAD_HIP(fn->sol.ensure(n));
+    const double* bd = rhs;
+    double* xd = sol;
+    if (mem == TAD_MEM_HOST)
+    {
+        TAD_HIP(fn->x_tmp.ensure(n));
+        TAD_HIP(hipMemcpyAsync(fn->x_tmp.p, rhs, n * sizeof(double), hipMemcpyHostToDevice, s));
+        bd = fn->x_tmp.p;
+        xd = fn->sol.p;
+    }
+    tad_status st = solve_core(fn, Hd, bd, xd, o, stats);
+    if (mem == TAD_MEM_HOST)
+    {
+        TAD_HIP(hipMemcpyAsync(sol, xd, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        TAD_HIP(hipStreamSynchronize(s));
+    }
+    return st;
+}
+
+tad_status tad_newton_default_opts(tad_newton_opts* o)
+{
+    if (!o) return fail(TAD_ERR_INVALID, "null argument");
+    o->projection_eps = 1e-9;
+    tad_solve_default_opts(&o->solve);
+    o->ls_s_max = 1.0;
+    o->ls_shrink = 0.8;
+    o->ls_max_iters = 64;
+    o->ls_armijo = 1e-4;
+    o->ls_batch = 4;
+    return TAD_OK;
+}
+
+tad_status tad_line_search(tad_fn fn, const double* x0, const double* dir, double f, const double* g, double* x_out,
+                           tad_mem mem, const tad_newton_opts* opts, double* f_out, double* step_out, int* trials_out)
+{
+    TAD_TRY(check_fn(fn));
+    if (!x0 || !dir || !g || !x_out) return fail(TAD_ERR_INVALID, "null argument");
+    if (mem != TAD_MEM_DEVICE) return fail(TAD_ERR_UNSUPPORTED, "tad_line_search takes device pointers");
+    TAD_HIP(hipSetDevice(fn->ctx->device));
+    TAD_TRY(ensure_ready(fn));
+    tad_newton_opts o;
+    tad_newton_default_opts(&o);
+    if (opts) o = *opts;
+    double dg = 0.0;
+    TAD_TRY(dot_dev(fn, dir, g, &dg));
+    double fo = 0, so = 0;
+    int tr = 0;
+    bool failed = false;
+    TAD_TRY(line_search_core(fn, x0, dir, 1.0, f, dg, x_out, o, &fo, &so, &tr, &failed));
+    if (f_out) *f_out = fo;
+    if (step_out) *step_out = so;
+    if (trials_out) *trials_out = failed ? -tr : tr;
+    return TAD_OK;
+}
+
+tad_status tad_newton_step(tad_fn fn, double* x, tad_mem mem, const tad_newton_opts* opts, tad_newton_stats* stats)
+{
+    TAD_TRY(check_fn(fn));
+    if (!x) return fail(TAD_ERR_INVALID, "null argument");
+    TAD_HIP(hipSetDevice(fn->ctx->device));
+    TAD_TRY(ensure_ready(fn));
+    tad_newton_opts o;
+    tad_newton_default_opts(&o);
+    if (opts) o = *opts;
+    hipStream_t s = fn->s();
+    const int64_t n = fn->n_vars;
+    double* xd = x;
+    if (mem == TAD_MEM_HOST)
+    {
+        TAD_HIP(hipMemcpyAsync(fn->x_buf.p, x, n * sizeof(double), hipMemcpyHostToDevice, s));
+        xd = fn->x_buf.p;
+    }
+    tad_newton_stats st;
+    std::memset(&st, 0, sizeof(st));
+    double f = 0.0;
+    TAD_TRY(eval_core(fn, xd, MODE_HESS_PROJ, o.projection_eps, fn->g_buf.p, fn->H_buf.p, &f));
+    fn->h_valid = true;
+    st.f = f;
+    // solve (wI + H) d' = g; the Newton direction is d = -d'
+    TAD_HIP(fn->sol.ensure(n));
+    tad_solve_stats ss;
+    std::memset(&ss, 0, sizeof(ss));
+    TAD_TRY(solve_core(fn, fn->H_buf.p, fn->g_buf.p, fn->sol.p, o.solve, &ss));
+    st.cg_iters = ss.iters;
+    st.solve_converged = ss.converged;
+    double dpg = 0.0;
+    TAD_TRY(dot_dev(fn, fn->sol.p, fn->g_buf.p, &dpg));
+    const double dg = -dpg;  // d^T g
+    st.newton_decrement = -0.5 * dg;
+    double fo = 0, so = 0;
+    int tr = 0;
+    bool failed = false;
+    TAD_TRY(line_search_core(fn, xd, fn->sol.p, -1.0, f, dg, xd, o, &fo, &so, &tr, &failed));
+    st.f_new = fo;
+    st.step = so;
+    st.ls_trials = tr;
+    st.ls_failed = failed ? 1 : 0;
+    st.grad_max = NAN;
+    if (mem == TAD_MEM_HOST)
+    {
+        TAD_HIP(hipMemcpyAsync(x, xd, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        TAD_HIP(hipStreamSynchronize(s));
+    }
+    if (stats) *stats = st;
+    return TAD_OK;
+}
+
+tad_status tad_selftest_dual(tad_ctx ctx, int which, double x, double y, double* out)
+{
+    if (!ctx || !out) return fail(TAD_ERR_INVALID, "null argument");
+    TAD_HIP(hipSetDevice(ctx->device));
+    double* dptr = nullptr;
+    TAD_HIP(hipMalloc((void**)&dptr, 8 * sizeof(double)));
+    hipError_t e = launch_selftest_dual(which, x, y, dptr, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ctx->pinned, dptr, 7 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(dptr);
+    if (e != hipSuccess) return fail(TAD_ERR_HIP, hipGetErrorString(e));
+    std::memcpy(out, ctx->pinned, 7 * sizeof(double));
+    return TAD_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,90 @@
+// Internal interface between the host library (tad_api.cpp) and the kernel
+// launchers (tad_kernels.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace tad {
+
+enum EvalMode
+{
+    MODE_GRAD = 1,      // value + gradient
+    MODE_HESS = 2,      // value + gradient + Hessian
+    MODE_HESS_PROJ = 3, // value + gradient + projected Hessian
+};
+
+struct TermDev
+{
+    int energy = 0;
+    int N = 0;          // valence
+    int n_params = 0;
+    int64_t n_elem = 0;
+    int32_t* conn = nullptr;   // device SoA: conn[a * n_elem + e]
+    double* params = nullptr;  // device SoA: params[k * n_elem + e]
+    int64_t rec_base = 0;      // first arena entry of this term's records
+    int64_t f_base = 0;        // first entry of this term in the element-energy array
+};
+
+struct PcgScalars
+{
+    double rho, rho_new, pq, rr, bb, alpha, beta, tol2;
+    int iters, max_iters, done, breakdown;
+    unsigned int counter;
+    unsigned int pad[3];
+};
+
+// Kernel timing hook (set by the host library when timing is enabled).
+struct Timer
+{
+    virtual void begin(const char* name, hipStream_t s) = 0;
+    virtual void end(const char* name, hipStream_t s) = 0;
+    virtual ~Timer() {}
+};
+
+constexpr int kReduceBlocks = 256;
+
+// MODE: 1 grad, 2 hessian, 3 projected hessian.  precision: 0 fp64, 1 fp32 records.
+hipError_t launch_element(const TermDev& t, int d, int precision, int mode, const double* x, double eps,
+                          double* elem_f, void* arena, int* err, hipStream_t s);
+
+// Energies at x + steps[c] * dir (dir may be null: plain x) for c < ncand,
+// written to elem_f[c * stride + f_base + e].
+hipError_t launch_energy(const TermDev& t, int d, const double* x, const double* dir, const double* steps,
+                         int ncand, double* elem_f, int64_t stride, hipStream_t s);
+
+// Deterministic sum of ncand rows of n doubles (row stride `stride`) into out[c].
+hipError_t launch_sum(const double* in, int64_t n, int ncand, int64_t stride, double* partials, double* out,
+                      hipStream_t s);
+
+hipError_t launch_assemble_hessian(int d, int precision, int64_t n_blocks, const int32_t* block_row,
+                                   const int32_t* brow_ptr, const int64_t* hmap_ptr, const uint32_t* hmap,
+                                   const void* arena, const int32_t* row_ptr, double* H, hipStream_t s);
+
+hipError_t launch_assemble_gradient(int d, int precision, int64_t n_handles, const int64_t* gmap_ptr,
+                                    const uint32_t* gmap, const void* arena, double* g, hipStream_t s);
+
+hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
+
+// x_out = x0 + step * dir  (same rounding as the line-search energy kernel)
+hipError_t launch_axpy_point(const double* x0, const double* dir, const double* step, int64_t n, double* x_out,
+                             hipStream_t s);
+
+// dot(a, b) -> out (deterministic)
+hipError_t launch_dot(const double* a, const double* b, int64_t n, double* partials, double* out, hipStream_t s);
+
+// --- PCG -------------------------------------------------------------------
+hipError_t launch_pcg_setup(int d, int64_t n_handles, const int32_t* brow_ptr, const int32_t* diag_block,
+                            const int32_t* row_ptr, const double* H, double w, double* Dinv, hipStream_t s);
+
+hipError_t launch_pcg_init(int d, int64_t n_handles, const double* b, const double* Dinv, double* x, double* r,
+                           double* z, double* p, double* partials, PcgScalars* sc, hipStream_t s);
+
+hipError_t launch_pcg_iteration(int d, int64_t n_handles, const int32_t* brow_ptr, const int32_t* bcol,
+                                const int32_t* row_ptr, const double* H, double w, const double* Dinv, double* x,
+                                double* r, double* z, double* p, double* q, double* partials, PcgScalars* sc,
+                                hipStream_t s, Timer* timer);
+
+hipError_t launch_selftest_dual(int which, double x, double y, double* out_dev, hipStream_t s);
+
+} // namespace tad

@@ -1,0 +1,583 @@
+// HIP kernels of the sparse second-order path for gfx950 (linear algebra part)
+// and their launchers.
+//
+//   k_sum_*, k_dot_*   deterministic FP64 sums (fixed tree, no atomics)
+//   k_assemble_*       element-order gather of gradient and CSR values
+//                      through the precomputed maps (replaces the serial
+//                      triplet loop ScalarObjectiveTerm.hh:257-277 and
+//                      setFromTriplets, ScalarFunctionImpl.hh:398)
+//   k_pcg_*, k_spmv    block-Jacobi PCG on the block-structured CSR
+//                      (replaces SimplicialLDLT, NewtonDirection.hh:25-48)
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "tad_internal.hpp"
+#include "tad_energies.hpp"
+#include "tad_elements.hpp"
+
+namespace tad {
+
+// ---------------------------------------------------------------------------
+// Deterministic reductions
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double block_sum(double v, double* sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+#pragma unroll
+    for (int off = kBlock / 2; off > 0; off >>= 1)
+    {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// block b of row c sums the contiguous chunk [b*chunk, (b+1)*chunk)
+__global__ __launch_bounds__(kBlock) void k_sum_stage1(const double* __restrict__ in, int64_t n, int64_t stride,
+                                                       double* __restrict__ partials)
+{
+    __shared__ double sh[kBlock];
+    const int c = blockIdx.y;
+    const int64_t chunk = (n + kReduceBlocks - 1) / kReduceBlocks;
+    const int64_t lo = (int64_t)blockIdx.x * chunk;
+    const int64_t hi = lo + chunk < n ? lo + chunk : n;
+    const double* row = in + c * stride;
+    double acc = 0.0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) acc += row[i];
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) partials[c * kReduceBlocks + blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_stage2(const double* __restrict__ partials, double* __restrict__ out)
+{
+    __shared__ double sh[kBlock];
+    const int c = blockIdx.x;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < kReduceBlocks; i += kBlock) acc += partials[c * kReduceBlocks + i];
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) out[c] = s;
+}
+
+hipError_t launch_sum(const double* in, int64_t n, int ncand, int64_t stride, double* partials, double* out,
+                      hipStream_t s)
+{
+    k_sum_stage1<<<dim3(kReduceBlocks, ncand), kBlock, 0, s>>>(in, n, stride, partials);
+    k_sum_stage2<<<ncand, kBlock, 0, s>>>(partials, out);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot_stage1(const double* __restrict__ a, const double* __restrict__ b,
+                                                       int64_t n, double* __restrict__ partials)
+{
+    __shared__ double sh[kBlock];
+    const int64_t chunk = (n + kReduceBlocks - 1) / kReduceBlocks;
+    const int64_t lo = (int64_t)blockIdx.x * chunk;
+    const int64_t hi = lo + chunk < n ? lo + chunk : n;
+    double acc = 0.0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) acc += a[i] * b[i];
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+hipError_t launch_dot(const double* a, const double* b, int64_t n, double* partials, double* out, hipStream_t s)
+{
+    k_dot_stage1<<<kReduceBlocks, kBlock, 0, s>>>(a, b, n, partials);
+    k_sum_stage2<<<1, kBlock, 0, s>>>(partials, out);
+    return hipGetLastError();
+}
+
+__global__ void k_fill(double* p, int64_t n, double v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    k_fill<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
+__global__ void k_axpy_point(const double* __restrict__ x0, const double* __restrict__ dir,
+                             const double* __restrict__ step, int64_t n, double* __restrict__ xo)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) xo[i] = fma(step[0], dir[i], x0[i]);
+}
+
+hipError_t launch_axpy_point(const double* x0, const double* dir, const double* step, int64_t n, double* x_out,
+                             hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    k_axpy_point<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(x0, dir, step, n, x_out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Assembly: element-order gather through precomputed maps
+// ---------------------------------------------------------------------------
+// hmap entry: arena offset of a d x d block record; top bit set = transposed
+template <class T, int D>
+__global__ __launch_bounds__(kBlock) void k_assemble_hessian(int64_t nb, const int32_t* __restrict__ block_row,
+                                                             const int32_t* __restrict__ brow_ptr,
+                                                             const int64_t* __restrict__ hmap_ptr,
+                                                             const uint32_t* __restrict__ hmap,
+                                                             const T* __restrict__ arena,
+                                                             const int32_t* __restrict__ row_ptr,
+                                                             double* __restrict__ H)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const int u = block_row[b];
+    const int b0 = brow_ptr[u];
+    const int k = (int)(b - b0);
+    const int nbu = brow_ptr[u + 1] - b0;
+    double acc[D * D];
+#pragma unroll
+    for (int m = 0; m < D * D; ++m) acc[m] = 0.0;
+    const int64_t c1 = hmap_ptr[b + 1];
+    for (int64_t c = hmap_ptr[b]; c < c1; ++c)
+    {
+        const uint32_t o = hmap[c];
+        const bool tr = (o >> 31) != 0;
+        const T* src = arena + (o & 0x7fffffffu);
+        T vals[D * D];
+#pragma unroll
+        for (int m = 0; m < D * D; ++m) vals[m] = src[m];
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j) acc[i * D + j] += (double)(tr ? vals[j * D + i] : vals[i * D + j]);
+    }
+    const int64_t base = row_ptr[D * u];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) H[base + (int64_t)i * D * nbu + D * k + j] = acc[i * D + j];
+}
+
+template <class T, int D>
+__global__ __launch_bounds__(kBlock) void k_assemble_gradient(int64_t nh, const int64_t* __restrict__ gmap_ptr,
+                                                              const uint32_t* __restrict__ gmap,
+                                                              const T* __restrict__ arena, double* __restrict__ g)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nh) return;
+    double acc[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) acc[i] = 0.0;
+    const int64_t c1 = gmap_ptr[v + 1];
+    for (int64_t c = gmap_ptr[v]; c < c1; ++c)
+    {
+        const T* src = arena + gmap[c];
+#pragma unroll
+        for (int i = 0; i < D; ++i) acc[i] += (double)src[i];
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) g[v * D + i] = acc[i];
+}
+
+template <class T>
+static hipError_t launch_assemble_hessian_t(int d, int64_t nb, const int32_t* block_row, const int32_t* brow_ptr,
+                                            const int64_t* hmap_ptr, const uint32_t* hmap, const T* arena,
+                                            const int32_t* row_ptr, double* H, hipStream_t s)
+{
+    if (nb == 0) return hipSuccess;
+    const unsigned grid = (unsigned)((nb + kBlock - 1) / kBlock);
+    if (d == 1) k_assemble_hessian<T, 1><<<grid, kBlock, 0, s>>>(nb, block_row, brow_ptr, hmap_ptr, hmap, arena, row_ptr, H);
+    else if (d == 2) k_assemble_hessian<T, 2><<<grid, kBlock, 0, s>>>(nb, block_row, brow_ptr, hmap_ptr, hmap, arena, row_ptr, H);
+    else k_assemble_hessian<T, 3><<<grid, kBlock, 0, s>>>(nb, block_row, brow_ptr, hmap_ptr, hmap, arena, row_ptr, H);
+    return hipGetLastError();
+}
+
+hipError_t launch_assemble_hessian(int d, int precision, int64_t nb, const int32_t* block_row, const int32_t* brow_ptr,
+                                   const int64_t* hmap_ptr, const uint32_t* hmap, const void* arena,
+                                   const int32_t* row_ptr, double* H, hipStream_t s)
+{
+    if (precision == 0)
+        return launch_assemble_hessian_t<double>(d, nb, block_row, brow_ptr, hmap_ptr, hmap, (const double*)arena, row_ptr, H, s);
+    return launch_assemble_hessian_t<float>(d, nb, block_row, brow_ptr, hmap_ptr, hmap, (const float*)arena, row_ptr, H, s);
+}
+
+template <class T>
+static hipError_t launch_assemble_gradient_t(int d, int64_t nh, const int64_t* gmap_ptr, const uint32_t* gmap,
+                                             const T* arena, double* g, hipStream_t s)
+{
+    if (nh == 0) return hipSuccess;
+    const unsigned grid = (unsigned)((nh + kBlock - 1) / kBlock);
+    if (d == 1) k_assemble_gradient<T, 1><<<grid, kBlock, 0, s>>>(nh, gmap_ptr, gmap, arena, g);
+    else if (d == 2) k_assemble_gradient<T, 2><<<grid, kBlock, 0, s>>>(nh, gmap_ptr, gmap, arena, g);
+    else k_assemble_gradient<T, 3><<<grid, kBlock, 0, s>>>(nh, gmap_ptr, gmap, arena, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_assemble_gradient(int d, int precision, int64_t nh, const int64_t* gmap_ptr, const uint32_t* gmap,
+                                    const void* arena, double* g, hipStream_t s)
+{
+    if (precision == 0)
+        return launch_assemble_gradient_t<double>(d, nh, gmap_ptr, gmap, (const double*)arena, g, s);
+    return launch_assemble_gradient_t<float>(d, nh, gmap_ptr, gmap, (const float*)arena, g, s);
+}
+
+// ---------------------------------------------------------------------------
+// PCG
+// ---------------------------------------------------------------------------
+// Last-block-done reduction: every block publishes its partial, the block
+// that finishes last (integer atomic ticket) sums all partials in index order.
+__device__ __forceinline__ bool publish_partials(double* partials, int n_vals, const double* vals, PcgScalars* sc)
+{
+    __shared__ bool last;
+    if (threadIdx.x == 0)
+    {
+        for (int m = 0; m < n_vals; ++m) partials[m * gridDim.x + blockIdx.x] = vals[m];
+        __threadfence();
+        const unsigned t = atomicAdd(&sc->counter, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last) __threadfence();
+    return last;
+}
+
+__device__ __forceinline__ double sum_partials(const double* partials, int m, double* sh)
+{
+    double acc = 0.0;
+    for (unsigned i = threadIdx.x; i < gridDim.x; i += kBlock) acc += partials[m * gridDim.x + i];
+    return block_sum(acc, sh);
+}
+
+template <int D>
+__device__ __forceinline__ void invert_block(const double* B, double* o)
+{
+    if constexpr (D == 1)
+        o[0] = 1.0 / B[0];
+    else if constexpr (D == 2)
+    {
+        const double det = B[0] * B[3] - B[1] * B[2];
+        o[0] = B[3] / det; o[1] = -B[1] / det; o[2] = -B[2] / det; o[3] = B[0] / det;
+    }
+    else
+    {
+        const double c00 = B[4] * B[8] - B[5] * B[7];
+        const double c01 = B[5] * B[6] - B[3] * B[8];
+        const double c02 = B[3] * B[7] - B[4] * B[6];
+        const double det = B[0] * c00 + B[1] * c01 + B[2] * c02;
+        const double id = 1.0 / det;
+        o[0] = c00 * id; o[3] = c01 * id; o[6] = c02 * id;
+        o[1] = (B[2] * B[7] - B[1] * B[8]) * id;
+        o[4] = (B[0] * B[8] - B[2] * B[6]) * id;
+        o[7] = (B[1] * B[6] - B[0] * B[7]) * id;
+        o[2] = (B[1] * B[5] - B[2] * B[4]) * id;
+        o[5] = (B[2] * B[3] - B[0] * B[5]) * id;
+        o[8] = (B[0] * B[4] - B[1] * B[3]) * id;
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_pcg_setup(int64_t nh, const int32_t* __restrict__ brow_ptr,
+                                                      const int32_t* __restrict__ diag_block,
+                                                      const int32_t* __restrict__ row_ptr,
+                                                      const double* __restrict__ H, double w,
+                                                      double* __restrict__ Dinv)
+{
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nh) return;
+    const int nbu = brow_ptr[u + 1] - brow_ptr[u];
+    const int k = diag_block[u];
+    const int64_t base = row_ptr[D * u];
+    double B[D * D], o[D * D];
+    if (k < 0)
+    {
+        // handle used by no element: its rows are empty, (w I + H) is w I there
+#pragma unroll
+        for (int m = 0; m < D * D; ++m) Dinv[u * D * D + m] = (m % (D + 1) == 0) ? (w > 0.0 ? 1.0 / w : 1.0) : 0.0;
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j)
+            B[i * D + j] = H[base + (int64_t)i * D * nbu + D * k + j] + (i == j ? w : 0.0);
+    invert_block<D>(B, o);
+#pragma unroll
+    for (int m = 0; m < D * D; ++m) Dinv[u * D * D + m] = o[m];
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t nh, const double* __restrict__ b,
+                                                     const double* __restrict__ Dinv, double* __restrict__ x,
+                                                     double* __restrict__ r, double* __restrict__ z,
+                                                     double* __restrict__ p, double* __restrict__ partials,
+                                                     PcgScalars* sc)
+{
+    __shared__ double sh[kBlock];
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double rz = 0.0, rr = 0.0;
+    if (u < nh)
+    {
+        double rv[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+        {
+            rv[i] = b[u * D + i];
+            x[u * D + i] = 0.0;
+            r[u * D + i] = rv[i];
+        }
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+        {
+            double zi = 0.0;
+#pragma unroll
+            for (int j = 0; j < D; ++j) zi += Dinv[u * D * D + i * D + j] * rv[j];
+            z[u * D + i] = zi;
+            p[u * D + i] = zi;
+            rz += rv[i] * zi;
+            rr += rv[i] * rv[i];
+        }
+    }
+    double vals[2];
+    vals[0] = block_sum(rz, sh);
+    vals[1] = block_sum(rr, sh);
+    if (publish_partials(partials, 2, vals, sc))
+    {
+        const double trz = sum_partials(partials, 0, sh);
+        const double trr = sum_partials(partials, 1, sh);
+        if (threadIdx.x == 0)
+        {
+            sc->rho = trz;
+            sc->rr = trr;
+            sc->bb = trr;
+            sc->iters = 0;
+            sc->breakdown = 0;
+            sc->done = (trr == 0.0 || sc->max_iters <= 0) ? 1 : 0;
+            sc->counter = 0;
+        }
+    }
+}
+
+// q = (w I + H) p, partial p.q; last block: alpha = rho / p.q
+template <int D, int LPB>
+__global__ __launch_bounds__(kBlock) void k_spmv(int64_t nh, const int32_t* __restrict__ brow_ptr,
+                                                 const int32_t* __restrict__ bcol, const int32_t* __restrict__ row_ptr,
+                                                 const double* __restrict__ H, double w, const double* __restrict__ p,
+                                                 double* __restrict__ q, double* __restrict__ partials, PcgScalars* sc)
+{
+    if (sc->done) return;
+    __shared__ double sh[kBlock];
+    const int lane = threadIdx.x % LPB;
+    const int64_t u = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / LPB;
+    double acc[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) acc[i] = 0.0;
+    if (u < nh)
+    {
+        const int b0 = brow_ptr[u];
+        const int nbu = brow_ptr[u + 1] - b0;
+        const int64_t base = row_ptr[D * u];
+        for (int k = lane; k < nbu; k += LPB)
+        {
+            const int64_t v = bcol[b0 + k];
+            double pv[D];
+#pragma unroll
+            for (int j = 0; j < D; ++j) pv[j] = p[v * D + j];
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+#pragma unroll
+                for (int j = 0; j < D; ++j) acc[i] += H[base + (int64_t)i * D * nbu + D * k + j] * pv[j];
+        }
+    }
+#pragma unroll
+    for (int off = LPB / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int i = 0; i < D; ++i) acc[i] += __shfl_xor(acc[i], off, LPB);
+    double pq = 0.0;
+    if (u < nh && lane == 0)
+    {
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+        {
+            const double pi = p[u * D + i];
+            const double qi = acc[i] + w * pi;
+            q[u * D + i] = qi;
+            pq += pi * qi;
+        }
+    }
+    double vals[1];
+    vals[0] = block_sum(pq, sh);
+    if (publish_partials(partials, 1, vals, sc))
+    {
+        const double tpq = sum_partials(partials, 0, sh);
+        if (threadIdx.x == 0)
+        {
+            sc->pq = tpq;
+            if (!(tpq > 0.0))
+            {
+                sc->breakdown = 1;
+                sc->done = 1;
+                sc->alpha = 0.0;
+            }
+            else
+                sc->alpha = sc->rho / tpq;
+            sc->counter = 0;
+        }
+    }
+}
+
+// x += alpha p, r -= alpha q, z = M^-1 r, partials r.z, r.r; last block: beta, convergence
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t nh, const double* __restrict__ Dinv,
+                                                       double* __restrict__ x, double* __restrict__ r,
+                                                       double* __restrict__ z, const double* __restrict__ p,
+                                                       const double* __restrict__ q, double* __restrict__ partials,
+                                                       PcgScalars* sc)
+{
+    if (sc->done) return;
+    __shared__ double sh[kBlock];
+    const double alpha = sc->alpha;
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double rz = 0.0, rr = 0.0;
+    if (u < nh)
+    {
+        double rv[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+        {
+            x[u * D + i] += alpha * p[u * D + i];
+            rv[i] = r[u * D + i] - alpha * q[u * D + i];
+            r[u * D + i] = rv[i];
+        }
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+        {
+            double zi = 0.0;
+#pragma unroll
+            for (int j = 0; j < D; ++j) zi += Dinv[u * D * D + i * D + j] * rv[j];
+            z[u * D + i] = zi;
+            rz += rv[i] * zi;
+            rr += rv[i] * rv[i];
+        }
+    }
+    double vals[2];
+    vals[0] = block_sum(rz, sh);
+    vals[1] = block_sum(rr, sh);
+    if (publish_partials(partials, 2, vals, sc))
+    {
+        const double trz = sum_partials(partials, 0, sh);
+        const double trr = sum_partials(partials, 1, sh);
+        if (threadIdx.x == 0)
+        {
+            sc->beta = trz / sc->rho;
+            sc->rho = trz;
+            sc->rr = trr;
+            sc->iters += 1;
+            if (trr <= sc->tol2 * sc->bb || sc->iters >= sc->max_iters) sc->done = 1;
+            sc->counter = 0;
+        }
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_pcg_dir(int64_t nh, const double* __restrict__ z, double* __restrict__ p,
+                                                    const PcgScalars* sc)
+{
+    if (sc->done) return;
+    const double beta = sc->beta;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nh * D) p[i] = z[i] + beta * p[i];
+}
+
+hipError_t launch_pcg_setup(int d, int64_t nh, const int32_t* brow_ptr, const int32_t* diag_block,
+                            const int32_t* row_ptr, const double* H, double w, double* Dinv, hipStream_t s)
+{
+    const unsigned grid = (unsigned)((nh + kBlock - 1) / kBlock);
+    if (d == 1) k_pcg_setup<1><<<grid, kBlock, 0, s>>>(nh, brow_ptr, diag_block, row_ptr, H, w, Dinv);
+    else if (d == 2) k_pcg_setup<2><<<grid, kBlock, 0, s>>>(nh, brow_ptr, diag_block, row_ptr, H, w, Dinv);
+    else k_pcg_setup<3><<<grid, kBlock, 0, s>>>(nh, brow_ptr, diag_block, row_ptr, H, w, Dinv);
+    return hipGetLastError();
+}
+
+hipError_t launch_pcg_init(int d, int64_t nh, const double* b, const double* Dinv, double* x, double* r, double* z,
+                           double* p, double* partials, PcgScalars* sc, hipStream_t s)
+{
+    const unsigned grid = (unsigned)((nh + kBlock - 1) / kBlock);
+    if (d == 1) k_pcg_init<1><<<grid, kBlock, 0, s>>>(nh, b, Dinv, x, r, z, p, partials, sc);
+    else if (d == 2) k_pcg_init<2><<<grid, kBlock, 0, s>>>(nh, b, Dinv, x, r, z, p, partials, sc);
+    else k_pcg_init<3><<<grid, kBlock, 0, s>>>(nh, b, Dinv, x, r, z, p, partials, sc);
+    return hipGetLastError();
+}
+
+template <int D, int LPB>
+static void launch_spmv_t(int64_t nh, const int32_t* brow_ptr, const int32_t* bcol, const int32_t* row_ptr,
+                          const double* H, double w, const double* p, double* q, double* partials, PcgScalars* sc,
+                          hipStream_t s)
+{
+    const int64_t rows_per_block = kBlock / LPB;
+    const unsigned grid = (unsigned)((nh + rows_per_block - 1) / rows_per_block);
+    k_spmv<D, LPB><<<grid, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, p, q, partials, sc);
+}
+
+hipError_t launch_pcg_iteration(int d, int64_t nh, const int32_t* brow_ptr, const int32_t* bcol,
+                                const int32_t* row_ptr, const double* H, double w, const double* Dinv, double* x,
+                                double* r, double* z, double* p, double* q, double* partials, PcgScalars* sc,
+                                hipStream_t s, Timer* timer)
+{
+    const unsigned grid = (unsigned)((nh + kBlock - 1) / kBlock);
+    const unsigned grid_v = (unsigned)((nh * d + kBlock - 1) / kBlock);
+    if (timer) timer->begin("spmv", s);
+    if (d == 1) launch_spmv_t<1, 8>(nh, brow_ptr, bcol, row_ptr, H, w, p, q, partials, sc, s);
+    else if (d == 2) launch_spmv_t<2, 8>(nh, brow_ptr, bcol, row_ptr, H, w, p, q, partials, sc, s);
+    else launch_spmv_t<3, 16>(nh, brow_ptr, bcol, row_ptr, H, w, p, q, partials, sc, s);
+    if (timer) timer->end("spmv", s);
+    if (timer) timer->begin("pcg_update", s);
+    if (d == 1) k_pcg_update<1><<<grid, kBlock, 0, s>>>(nh, Dinv, x, r, z, p, q, partials, sc);
+    else if (d == 2) k_pcg_update<2><<<grid, kBlock, 0, s>>>(nh, Dinv, x, r, z, p, q, partials, sc);
+    else k_pcg_update<3><<<grid, kBlock, 0, s>>>(nh, Dinv, x, r, z, p, q, partials, sc);
+    if (timer) timer->end("pcg_update", s);
+    if (timer) timer->begin("pcg_dir", s);
+    if (d == 1) k_pcg_dir<1><<<grid_v, kBlock, 0, s>>>(nh, z, p, sc);
+    else if (d == 2) k_pcg_dir<2><<<grid_v, kBlock, 0, s>>>(nh, z, p, sc);
+    else k_pcg_dir<3><<<grid_v, kBlock, 0, s>>>(nh, z, p, sc);
+    if (timer) timer->end("pcg_dir", s);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Dual self test (reference operator known answers)
+// ---------------------------------------------------------------------------
+__global__ void k_selftest_dual(int which, double xv, double yv, double* out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const Dual<2, double> x = dual_var<2, double>(xv, 0), y = dual_var<2, double>(yv, 1);
+    Dual<2, double> f;
+    switch (which)
+    {
+        case 0: f = sqr(x) / y; break;
+        case 1:
+        {
+            const Dual<2, double> a = 0.5 * sqr(x) - sqr(y) + 2.0 * x - y;
+            const Dual<2, double> b = -sqr(x - 2.0) - sqr(y - 3.0) + 1.0;
+            f = a / b;
+            break;
+        }
+        case 2: f = (sqr(x) + x) * (sqr(y) - y) / (y - 1.0); break;
+        default: f = -(x * y) + x - y + 2.0 * x * 3.0 - 1.0; break;
+    }
+    out[0] = f.v;
+    out[1] = f.g[0];
+    out[2] = f.g[1];
+    out[3] = f.h[Sym<2>::idx(0, 0)];
+    out[4] = f.h[Sym<2>::idx(0, 1)];
+    out[5] = f.h[Sym<2>::idx(0, 1)];
+    out[6] = f.h[Sym<2>::idx(1, 1)];
+}
+
+hipError_t launch_selftest_dual(int which, double x, double y, double* out_dev, hipStream_t s)
+{
+    k_selftest_dual<<<1, 64, 0, s>>>(which, x, y, out_dev);
+    return hipGetLastError();
+}
+
+} // namespace tad
