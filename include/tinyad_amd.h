@@ -97,6 +97,8 @@ const char* tad_version(void);
  * given an existing one.  Replaces the reference's OpenMP team
  * (Detail/Parallel.hh:20-69, EvalSettings.hh:14-21). */
 tad_status tad_ctx_create(int device, tad_ctx* out);
+/* Releases the caller's reference; the context lives on until the last
+ * function created on it is destroyed. */
 tad_status tad_ctx_destroy(tad_ctx ctx);
 /* hip_stream: a hipStream_t (NULL = the context's own stream). */
 tad_status tad_ctx_set_stream(tad_ctx ctx, void* hip_stream);
@@ -143,7 +145,7 @@ tad_status tad_fn_eval_with_gradient(tad_fn fn, const double* x, double* f, doub
 
 /* eval_with_derivatives(x, f, g, H) (ScalarFunctionImpl.hh:316-352): unprojected
  * Hessian.  H_values: nnz values in the pattern order (may be NULL: the values
- * stay in the function's device buffer, see tad_fn_hessian_values). */
+ * stay in the function's device buffer, see tad_fn_device_buffers). */
 tad_status tad_fn_eval_with_derivatives(tad_fn fn, const double* x, double* f, double* g, double* H_values,
                                         tad_mem mem);
 

@@ -1,0 +1,361 @@
+// C++17 front-end of libtinyad_amd.so with the shape of TinyAD's
+// ScalarFunction API (include/TinyAD/ScalarFunction.hh, Utils/NewtonDirection.hh,
+// Utils/LineSearch.hh, Utils/Helpers.hh).  Header-only; every call goes
+// through the C ABI of include/tinyad_amd.h and non-zero statuses become
+// std::runtime_error, as the reference's TINYAD_ERROR_throw / TINYAD_ASSERT do
+// (Utils/Out.hh:64-157).
+//
+// Differences from the reference, by design:
+//   * element functions are entries of the library's energy catalog plus
+//     per-element parameter rows (the reference instantiates a user lambda
+//     for three scalar types, ScalarObjectiveTerm.hh:54-115);
+//   * Eigen is optional: vectors are std::vector<double>, the Hessian is a
+//     CSR SparseMatrix with the same pattern Eigen's setFromTriplets yields.
+//
+//   auto func = tinyad_amd::scalar_function<2>(tinyad_amd::range(n_vertices));
+//   func.add_elements<3>(tinyad_amd::range(n_faces), TAD_E_SYMDIR_TRI,
+//       [&](int f) { return std::array<int, 3>{F[f][0], F[f][1], F[f][2]}; },
+//       [&](int f) { return rest_params[f]; });                  // 5 doubles
+//   std::vector<double> x = func.x_from_data([&](int v) { return std::array<double, 2>{...}; });
+//   auto [f, g, H_proj] = func.eval_with_hessian_proj(x);
+#pragma once
+
+#include <tinyad_amd.h>
+
+#include <array>
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+namespace tinyad_amd {
+
+inline void check(tad_status s)
+{
+    if (s != TAD_OK)
+        throw std::runtime_error(std::string("[tinyad_amd] ") + tad_last_error());
+}
+
+constexpr double default_hessian_projection_eps = 1e-9;  // HessianProjection.hh:16
+
+using VectorXd = std::vector<double>;
+
+// range(n) = {0, ..., n-1} (Helpers.hh:17-27)
+inline std::vector<int> range(int n)
+{
+    if (n < 0) throw std::runtime_error("[tinyad_amd] range: negative size");
+    std::vector<int> r(n);
+    for (int i = 0; i < n; ++i) r[i] = i;
+    return r;
+}
+
+// CSR matrix (row-major).  For the symmetric Hessians returned here it equals
+// the column-major Eigen::SparseMatrix the reference returns.
+struct SparseMatrix
+{
+    int64_t n_rows = 0, n_cols = 0;
+    std::vector<int32_t> outer;  // row offsets, n_rows + 1
+    std::vector<int32_t> inner;  // column indices, sorted within each row
+    std::vector<double> values;
+
+    int64_t rows() const { return n_rows; }
+    int64_t cols() const { return n_cols; }
+    int64_t nonZeros() const { return (int64_t)values.size(); }
+    double coeff(int64_t r, int64_t c) const
+    {
+        for (int32_t k = outer[r]; k < outer[r + 1]; ++k)
+            if (inner[k] == c) return values[k];
+        return 0.0;
+    }
+    VectorXd operator*(const VectorXd& v) const
+    {
+        VectorXd y(n_rows, 0.0);
+        for (int64_t r = 0; r < n_rows; ++r)
+            for (int32_t k = outer[r]; k < outer[r + 1]; ++k) y[r] += values[k] * v[inner[k]];
+        return y;
+    }
+};
+
+struct EvalSettings
+{
+    int device = 0;
+    tad_precision precision = TAD_PREC_FP64;
+};
+
+// One per device, shared by the functions created on it.
+class Context
+{
+public:
+    explicit Context(int device = 0) { check(tad_ctx_create(device, &h_)); }
+    ~Context() { tad_ctx_destroy(h_); }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    tad_ctx handle() const { return h_; }
+
+    static std::shared_ptr<Context> on_device(int device)
+    {
+        static std::vector<std::weak_ptr<Context>> cache;
+        if ((int)cache.size() <= device) cache.resize(device + 1);
+        std::shared_ptr<Context> c = cache[device].lock();
+        if (!c)
+        {
+            c = std::make_shared<Context>(device);
+            cache[device] = c;
+        }
+        return c;
+    }
+
+private:
+    tad_ctx h_ = nullptr;
+};
+
+// ScalarFunction<d> (ScalarFunction.hh:36-236).  Movable, not copyable.
+template <int variable_dimension>
+class ScalarFunction
+{
+    static_assert(variable_dimension >= 1 && variable_dimension <= 3, "variable dimension must be 1, 2 or 3");
+
+public:
+    static constexpr int d = variable_dimension;
+    using PassiveVariableVectorType = std::array<double, d>;
+
+    ScalarFunction() = default;
+    ScalarFunction(std::vector<int> variable_handles, const EvalSettings& settings_ = {})
+        : settings(settings_), variable_handles(std::move(variable_handles))
+    {
+        // handle indices must be a permutation of 0..n-1 (Element.hh:176-194)
+        std::vector<char> used(this->variable_handles.size(), 0);
+        for (int v : this->variable_handles)
+        {
+            if (v < 0 || v >= (int)used.size() || used[v])
+                throw std::runtime_error("[tinyad_amd] variable handles are not a permutation of 0..n-1");
+            used[v] = 1;
+        }
+        if (this->variable_handles.empty()) throw std::runtime_error("[tinyad_amd] no variable handles");
+        ctx_ = Context::on_device(settings.device);
+        n_vars = d * (int64_t)this->variable_handles.size();
+        tad_fn h = nullptr;
+        check(tad_fn_create(ctx_->handle(), d, (int64_t)this->variable_handles.size(), settings.precision, &h));
+        fn_.reset(h, [](tad_fn p) { tad_fn_destroy(p); });
+    }
+    ScalarFunction(const ScalarFunction&) = delete;
+    ScalarFunction& operator=(const ScalarFunction&) = delete;
+    ScalarFunction(ScalarFunction&&) = default;
+    ScalarFunction& operator=(ScalarFunction&&) = default;
+
+    // add_elements<N>(range, ...) (ScalarFunctionImpl.hh:63-100): conn_of(e)
+    // returns the N handles element e accesses, in access order; params_of(e)
+    // returns its parameter row (any container with size() and operator[]).
+    template <int element_valence, class Range, class ConnFn, class ParamFn>
+    void add_elements(const Range& element_range, tad_energy energy, ConnFn conn_of, ParamFn params_of)
+    {
+        require_fn();
+        std::vector<int32_t> conn;
+        std::vector<double> params;
+        int64_t n = 0;
+        int np = -1;
+        for (auto e : element_range)
+        {
+            const std::array<int, element_valence> c = conn_of(e);
+            for (int a = 0; a < element_valence; ++a) conn.push_back(c[a]);
+            const auto p = params_of(e);
+            if (np < 0) np = (int)p.size();
+            if ((int)p.size() != np) throw std::runtime_error("[tinyad_amd] parameter rows differ in length");
+            for (int k = 0; k < np; ++k) params.push_back(p[k]);
+            ++n;
+        }
+        check(tad_fn_add_elements(fn_.get(), energy, element_valence, n, conn.data(), params.data(),
+                                  np < 0 ? 0 : np));
+        n_elements += n;
+    }
+
+    // x_from_data (ScalarFunctionImpl.hh:216-235)
+    VectorXd x_from_data(std::function<PassiveVariableVectorType(int)> read_user_data) const
+    {
+        VectorXd x(n_vars);
+        for (int v : variable_handles)
+        {
+            const PassiveVariableVectorType p = read_user_data(v);
+            for (int i = 0; i < d; ++i) x[d * v + i] = p[i];
+        }
+        for (double xi : x)
+            if (!(xi == xi) || xi == 1.0 / 0.0 || xi == -1.0 / 0.0)
+                throw std::runtime_error("[tinyad_amd] x_from_data: non-finite value");
+        return x;
+    }
+
+    // x_to_data (ScalarFunctionImpl.hh:237-254)
+    void x_to_data(const VectorXd& x, std::function<void(int, const PassiveVariableVectorType&)> write_user_data) const
+    {
+        if ((int64_t)x.size() != n_vars) throw std::runtime_error("[tinyad_amd] x_to_data: size mismatch");
+        for (int v : variable_handles)
+        {
+            PassiveVariableVectorType p;
+            for (int i = 0; i < d; ++i) p[i] = x[d * v + i];
+            write_user_data(v, p);
+        }
+    }
+
+    double eval(const VectorXd& x) const
+    {
+        require_fn();
+        check_size(x);
+        double f = 0.0;
+        check(tad_fn_eval(fn_.get(), x.data(), TAD_MEM_HOST, &f));
+        return f;
+    }
+    double operator()(const VectorXd& x) const { return eval(x); }
+
+    std::tuple<double, VectorXd> eval_with_gradient(const VectorXd& x) const
+    {
+        require_fn();
+        check_size(x);
+        double f = 0.0;
+        VectorXd g(n_vars);
+        check(tad_fn_eval_with_gradient(fn_.get(), x.data(), &f, g.data(), TAD_MEM_HOST));
+        return { f, std::move(g) };
+    }
+
+    std::tuple<double, VectorXd, SparseMatrix> eval_with_derivatives(const VectorXd& x) const
+    {
+        require_fn();
+        check_size(x);
+        double f = 0.0;
+        VectorXd g(n_vars);
+        SparseMatrix H = empty_hessian();
+        check(tad_fn_eval_with_derivatives(fn_.get(), x.data(), &f, g.data(), H.values.data(), TAD_MEM_HOST));
+        return { f, std::move(g), std::move(H) };
+    }
+
+    SparseMatrix eval_hessian(const VectorXd& x) const { return std::get<2>(eval_with_derivatives(x)); }
+    SparseMatrix eval_hessian_of_quadratic() const { return eval_hessian(VectorXd(n_vars, 0.0)); }
+
+    std::tuple<double, VectorXd, SparseMatrix> eval_with_hessian_proj(
+        const VectorXd& x, double projection_eps = default_hessian_projection_eps) const
+    {
+        require_fn();
+        check_size(x);
+        double f = 0.0;
+        VectorXd g(n_vars);
+        SparseMatrix H = empty_hessian();
+        check(tad_fn_eval_with_hessian_proj(fn_.get(), x.data(), projection_eps, &f, g.data(), H.values.data(),
+                                            TAD_MEM_HOST));
+        return { f, std::move(g), std::move(H) };
+    }
+
+    tad_fn handle() const { return fn_.get(); }
+
+    EvalSettings settings;
+    int64_t n_vars = 0;
+    int64_t n_elements = 0;
+    std::vector<int> variable_handles;
+
+private:
+    void require_fn() const
+    {
+        if (!fn_) throw std::runtime_error("[tinyad_amd] empty (default-constructed or moved-from) ScalarFunction");
+    }
+    void check_size(const VectorXd& x) const
+    {
+        if ((int64_t)x.size() != n_vars)
+            throw std::runtime_error("[tinyad_amd] x has " + std::to_string(x.size()) + " entries, expected " +
+                                     std::to_string(n_vars));
+    }
+    SparseMatrix empty_hessian() const
+    {
+        SparseMatrix H;
+        int64_t nnz = 0;
+        check(tad_fn_pattern(fn_.get(), &nnz, nullptr, nullptr, nullptr, nullptr));
+        H.n_rows = H.n_cols = n_vars;
+        H.outer.resize(n_vars + 1);
+        H.inner.resize(nnz);
+        H.values.resize(nnz > 0 ? nnz : 1);
+        check(tad_fn_pattern(fn_.get(), nullptr, nullptr, nullptr, H.outer.data(), H.inner.data()));
+        H.values.resize(nnz);
+        return H;
+    }
+
+    std::shared_ptr<Context> ctx_;
+    std::shared_ptr<tad_fn_s> fn_;
+};
+
+// scalar_function<d>(range) (ScalarFunctionImpl.hh:418-442)
+template <int variable_dimension>
+ScalarFunction<variable_dimension> scalar_function(const std::vector<int>& variable_range,
+                                                   const EvalSettings& settings = {})
+{
+    return ScalarFunction<variable_dimension>(variable_range, settings);
+}
+
+// LinearSolver (LinearSolver.hh:13-19): solver options kept across Newton
+// iterations (the pattern is fixed per function, so nothing is re-analyzed).
+struct LinearSolver
+{
+    tad_solve_opts opts = []() { tad_solve_opts o; tad_solve_default_opts(&o); return o; }();
+    tad_solve_stats last_stats {};
+};
+
+// newton_direction(g, H_proj, solver, w) (NewtonDirection.hh:25-48):
+// solves (w I + H) d = -g; throws if the solve fails.
+template <int dd>
+VectorXd newton_direction(const ScalarFunction<dd>& func, const VectorXd& g, const SparseMatrix& H_proj,
+                          LinearSolver& solver, double w_identity = 0.0)
+{
+    VectorXd minus_g(g.size());
+    for (size_t i = 0; i < g.size(); ++i) minus_g[i] = -g[i];
+    VectorXd dvec(g.size());
+    tad_solve_opts o = solver.opts;
+    o.w_identity = w_identity;
+    check(tad_fn_solve(func.handle(), H_proj.values.data(), minus_g.data(), dvec.data(), TAD_MEM_HOST, &o,
+                       &solver.last_stats));
+    if (!solver.last_stats.converged) throw std::runtime_error("[tinyad_amd] Linear solve failed.");
+    return dvec;
+}
+
+// line_search(x0, d, f, g, eval, s_max, shrink, max_iters, armijo) (LineSearch.hh:26-65),
+// evaluated on the host through func.eval (one energy launch per trial).
+template <int dd>
+VectorXd line_search(const VectorXd& x0, const VectorXd& dvec, double f, const VectorXd& g,
+                     const ScalarFunction<dd>& func, double s_max = 1.0, double shrink = 0.8, int max_iters = 64,
+                     double armijo_const = 1e-4)
+{
+    if (x0.size() != g.size()) throw std::runtime_error("[tinyad_amd] line_search: size mismatch");
+    if (s_max <= 0.0) throw std::runtime_error("[tinyad_amd] Max step size not positive.");
+    const bool try_one = s_max > 1.0;
+    double dg = 0.0;
+    for (size_t i = 0; i < g.size(); ++i) dg += dvec[i] * g[i];
+    VectorXd x_new = x0;
+    double s = s_max;
+    for (int i = 0; i < max_iters; ++i)
+    {
+        for (size_t j = 0; j < x0.size(); ++j) x_new[j] = x0[j] + s * dvec[j];
+        const double f_new = func.eval(x_new);
+        if (f_new != f_new) throw std::runtime_error("[tinyad_amd] line_search: NaN energy");
+        if (f_new <= f + armijo_const * s * dg) return x_new;
+        if (try_one && s > 1.0 && s * shrink < 1.0) s = 1.0;
+        else s *= shrink;
+    }
+    return x0;
+}
+
+// newton_decrement (NewtonDecrement.hh:20-26)
+inline double newton_decrement(const VectorXd& dvec, const VectorXd& g)
+{
+    double s = 0.0;
+    for (size_t i = 0; i < g.size(); ++i) s += dvec[i] * g[i];
+    return -0.5 * s;
+}
+
+// Whole projected-Newton iteration on the device (eval -> PCG -> batched line search).
+template <int dd>
+tad_newton_stats newton_step(const ScalarFunction<dd>& func, VectorXd& x, const tad_newton_opts* opts = nullptr)
+{
+    tad_newton_stats st {};
+    check(tad_newton_step(func.handle(), x.data(), TAD_MEM_HOST, opts, &st));
+    return st;
+}
+
+} // namespace tinyad_amd

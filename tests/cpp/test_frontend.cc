@@ -1,0 +1,176 @@
+// C++ front-end checks, written like the reference's gtest cases
+// (tests/NewtonTest.cc, tests/ScalarFunctionTest.cc, tests/ExceptionTest.cc)
+// but against include/tinyad_amd/ScalarFunction.hh.  Prints one line per
+// case; exit status 0 when every case passes.  Needs a GPU.
+#include <tinyad_amd/ScalarFunction.hh>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+static int g_failed = 0;
+#define EXPECT(cond)                                                                   \
+    do                                                                                 \
+    {                                                                                  \
+        if (!(cond))                                                                   \
+        {                                                                              \
+            std::printf("  check failed at line %d: %s\n", __LINE__, #cond);        \
+            ok = false;                                                                \
+        }                                                                              \
+    } while (0)
+
+using tinyad_amd::VectorXd;
+
+static std::vector<double> quad_params(const std::vector<double>& A, const std::vector<double>& b, double c)
+{
+    std::vector<double> p = A;
+    p.insert(p.end(), b.begin(), b.end());
+    p.push_back(c);
+    return p;
+}
+
+static bool test_basic()
+{
+    bool ok = true;
+    auto func = tinyad_amd::scalar_function<2>(tinyad_amd::range(1));
+    func.add_elements<1>(tinyad_amd::range(1), TAD_E_QUADRATIC, [](int) { return std::array<int, 1>{ 0 }; },
+                         [](int) { return quad_params({ 2, 0, 0, 2 }, { 0, 0 }, 0); });
+    VectorXd x = { 1.0, 2.0 };
+    EXPECT(func.eval(x) == 5.0);
+    auto [f, g] = func.eval_with_gradient(x);
+    EXPECT(f == 5.0 && g[0] == 2.0 && g[1] == 4.0);
+    auto [fh, gh, H] = func.eval_with_derivatives(x);
+    EXPECT(fh == 5.0 && H.coeff(0, 0) == 2.0 && H.coeff(1, 1) == 2.0);
+    // move construction / assignment keep the function usable (ScalarFunctionTest.cc:181-252)
+    auto func2 = std::move(func);
+    EXPECT(func2.eval(x) == 5.0);
+    tinyad_amd::ScalarFunction<2> func3;
+    func3 = std::move(func2);
+    EXPECT(func3(x) == 5.0);
+    bool threw = false;
+    try { func2.eval(x); } catch (const std::runtime_error&) { threw = true; }
+    EXPECT(threw);
+    return ok;
+}
+
+static bool test_2d_nonconvex()
+{
+    bool ok = true;
+    auto func = tinyad_amd::scalar_function<2>(tinyad_amd::range(1));
+    func.add_elements<1>(tinyad_amd::range(1), TAD_E_QUADRATIC, [](int) { return std::array<int, 1>{ 0 }; },
+                         [](int) { return quad_params({ -4, -2, -2, -2 }, { -1, 0 }, -1); });
+    VectorXd x = { 1.0, 2.0 };
+    auto [f, g, H] = func.eval_with_derivatives(x);
+    auto [f2, g2, Hp] = func.eval_with_hessian_proj(x);
+    EXPECT(f == -12 && f2 == -12 && g[0] == -9 && g[1] == -6);
+    EXPECT(H.coeff(0, 0) == -4 && H.coeff(0, 1) == -2 && H.coeff(1, 1) == -2);
+    const double a = Hp.coeff(0, 0), b = Hp.coeff(0, 1), c = Hp.coeff(1, 1);
+    EXPECT(a > 0 && a * c - b * b > 0);  // positive definite
+    return ok;
+}
+
+// tests/NewtonTest.cc:12-95 on the 6-vertex planar fixture (tests/Meshes.hh:12-42)
+static bool test_newton(VectorXd* x_out)
+{
+    bool ok = true;
+    const double V_rest[6][2] = { { 0, 0 }, { 1, 0 }, { 0, 1 }, { 1, 1 }, { 0, 2 }, { 1, 2 } };
+    const int F[4][3] = { { 0, 1, 2 }, { 1, 3, 2 }, { 2, 3, 5 }, { 2, 5, 4 } };
+    const int b[2] = { 0, 4 };
+    const double bc[2][2] = { { 0.0, 0.0 }, { -2.0, 0.0 } };
+    auto func = tinyad_amd::scalar_function<2>(tinyad_amd::range(6));
+    func.add_elements<3>(tinyad_amd::range(4), TAD_E_SYMDIR_TRI,
+        [&](int f) { return std::array<int, 3>{ F[f][0], F[f][1], F[f][2] }; },
+        [&](int f) {
+            const double m00 = V_rest[F[f][1]][0] - V_rest[F[f][0]][0], m10 = V_rest[F[f][1]][1] - V_rest[F[f][0]][1];
+            const double m01 = V_rest[F[f][2]][0] - V_rest[F[f][0]][0], m11 = V_rest[F[f][2]][1] - V_rest[F[f][0]][1];
+            const double det = m00 * m11 - m01 * m10;
+            return std::vector<double>{ m11 / det, -m01 / det, -m10 / det, m00 / det, 1.0 / 4.0 };
+        });
+    func.add_elements<1>(tinyad_amd::range(2), TAD_E_PENALTY, [&](int i) { return std::array<int, 1>{ b[i] }; },
+                         [&](int i) { return std::vector<double>{ bc[i][0], bc[i][1], 1.0 }; });
+    VectorXd x = func.x_from_data([&](int v) { return std::array<double, 2>{ 0.5 * V_rest[v][0], 0.25 * V_rest[v][1] }; });
+    {
+        auto [f, g, H_proj] = func.eval_with_hessian_proj(x);
+        EXPECT(H_proj.nonZeros() == 4 * 6 + 8 * (6 + 4 - 1));
+    }
+    tinyad_amd::LinearSolver solver;
+    solver.opts.tol = 1e-14;
+    for (int i = 0; i < 10; ++i)
+    {
+        auto [f, g, H_proj] = func.eval_with_hessian_proj(x);
+        VectorXd d = tinyad_amd::newton_direction(func, g, H_proj, solver);
+        x = tinyad_amd::line_search(x, d, f, g, func);
+    }
+    auto [f, g, H_proj] = func.eval_with_hessian_proj(x);
+    double gmax = 0;
+    for (double gi : g) gmax = std::fmax(gmax, std::fabs(gi));
+    EXPECT(func.eval(x) == f);
+    EXPECT(std::fabs(f - 4.0) <= 1e-12);
+    EXPECT(gmax <= 1e-10);
+    std::vector<std::array<double, 2>> V(6);
+    func.x_to_data(x, [&](int v, const std::array<double, 2>& p) { V[v] = p; });
+    EXPECT(V[4][0] == x[8] && V[4][1] == x[9]);
+    if (x_out) *x_out = x;
+    return ok;
+}
+
+static bool test_newton_deterministic()
+{
+    bool ok = true;
+    VectorXd x0, x1;
+    ok = test_newton(&x0) && ok;
+    ok = test_newton(&x1) && ok;
+    EXPECT(std::memcmp(x0.data(), x1.data(), x0.size() * sizeof(double)) == 0);
+    return ok;
+}
+
+// ExceptionTest.cc:8-54 semantics: a failing call throws, the object stays usable
+static bool test_errors_and_recovery()
+{
+    bool ok = true;
+    auto func = tinyad_amd::scalar_function<1>(tinyad_amd::range(1));
+    func.add_elements<1>(tinyad_amd::range(10), TAD_E_QUADRATIC, [](int) { return std::array<int, 1>{ 0 }; },
+                         [](int) { return quad_params({ 0 }, { 0 }, 1.0); });
+    int caught = 0;
+    for (int i = 0; i < 2; ++i)
+    {
+        try { func.eval_with_hessian_proj(VectorXd(3, 0.0)); }
+        catch (const std::runtime_error&) { ++caught; }
+    }
+    EXPECT(caught == 2);
+    auto [f, g, H] = func.eval_with_hessian_proj(VectorXd(1, 0.0));
+    EXPECT(std::fabs(f - 10.0) <= 1e-12);
+    bool threw = false;
+    try
+    {
+        func.add_elements<1>(tinyad_amd::range(1), TAD_E_QUADRATIC, [](int) { return std::array<int, 1>{ 5 }; },
+                             [](int) { return quad_params({ 0 }, { 0 }, 1.0); });
+    }
+    catch (const std::runtime_error&) { threw = true; }
+    EXPECT(threw);
+    return ok;
+}
+
+int main()
+{
+    struct Case { const char* name; bool (*fn)(); };
+    const Case cases[] = {
+        { "ScalarFunctionTest.Basic+Move", test_basic },
+        { "ScalarFunctionTest.2DNonConvex", test_2d_nonconvex },
+        { "NewtonTest.2DDeformation", []() { return test_newton(nullptr); } },
+        { "NewtonTest.Deterministic", test_newton_deterministic },
+        { "ExceptionTest", test_errors_and_recovery },
+    };
+    for (const Case& c : cases)
+    {
+        bool ok = false;
+        try { ok = c.fn(); }
+        catch (const std::exception& ex) { std::printf("  exception: %s\n", ex.what()); }
+        std::printf("%s %s\n", ok ? "PASS" : "FAIL", c.name);
+        if (!ok) ++g_failed;
+    }
+    std::printf("%d failed\n", g_failed);
+    return g_failed == 0 ? 0 : 1;
+}

@@ -68,7 +68,9 @@ class Context:
     @classmethod
     def default(cls):
         if cls._default is None:
+            import atexit
             cls._default = Context(0)
+            atexit.register(cls._default.close)
         return cls._default
 
     def close(self):

@@ -67,6 +67,14 @@ def lib():
     global _lib
     if _lib is not None:
         return _lib
+    # PyTorch ships its own HIP runtime with the same SONAME as ROCm's
+    # (libamdhip64.so.7).  Loading torch first makes this library bind to
+    # that already-loaded runtime, so one process never holds two HIP
+    # runtimes (which breaks device discovery for whichever loads second).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -m tinyad_amd.build` "
                           "(or __graft_entry__.build())")

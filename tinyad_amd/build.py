@@ -52,5 +52,24 @@ def build(force: bool = False, jobs: int = 4) -> str:
     return LIB
 
 
+def build_cpp_tests() -> list:
+    """Compiles the C++ front-end test programs (tests/cpp/*.cc) with g++
+    against include/ and the library; binaries go to tests/cpp/bin/."""
+    src_dir = os.path.join(ROOT, "tests", "cpp")
+    out_dir = os.path.join(src_dir, "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    outs = []
+    for src in sorted(f for f in os.listdir(src_dir) if f.endswith(".cc")):
+        exe = os.path.join(out_dir, os.path.splitext(src)[0])
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), os.path.join(src_dir, src),
+               "-L" + HERE, "-ltinyad_amd", "-Wl,-rpath,$ORIGIN/../../../tinyad_amd", "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {src}:\n{r.stderr[-4000:]}")
+        outs.append(exe)
+    return outs
+
+
 if __name__ == "__main__":
     print(build(force="--force" in sys.argv))
+    print(build_cpp_tests())

@@ -163,8 +163,23 @@ struct tad_ctx_s
     hipStream_t stream = nullptr;
     hipStream_t own_stream = nullptr;
     KernelTimer timer;
-    double* pinned = nullptr;  // small pinned staging buffer (64 doubles)
+    double* pinned = nullptr;  // small pinned staging buffer (256 doubles)
+    // one reference held by the creator plus one per live function: the
+    // context is released when the last of them is gone, whatever the order
+    // of tad_ctx_destroy and tad_fn_destroy calls
+    int refs = 1;
 };
+
+static void ctx_release(tad_ctx ctx)
+{
+    if (--ctx->refs > 0) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->timer.collect();
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+}
 
 struct HostTerm
 {
@@ -607,12 +622,9 @@ tad_status tad_ctx_create(int device, tad_ctx* out)
 tad_status tad_ctx_destroy(tad_ctx ctx)
 {
     if (!ctx) return TAD_OK;
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    ctx->timer.collect();
-    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
-    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
-    delete ctx;
+    // a stream borrowed from the caller may be gone by now: fall back to ours
+    ctx->stream = ctx->own_stream;
+    ctx_release(ctx);
     return TAD_OK;
 }
 
@@ -671,6 +683,7 @@ tad_status tad_fn_create(tad_ctx ctx, int d, int64_t n_handles, tad_precision pr
     if (precision != TAD_PREC_FP64 && precision != TAD_PREC_MIXED) return fail(TAD_ERR_INVALID, "bad precision");
     tad_fn f = new tad_fn_s();
     f->ctx = ctx;
+    ctx->refs += 1;
     f->d = d;
     f->n_handles = n_handles;
     f->n_vars = d * n_handles;
@@ -695,7 +708,9 @@ tad_status tad_fn_destroy(tad_fn fn)
     fn->ls_f.release(); fn->x_tmp.release();
     fn->Dinv.release(); fn->r.release(); fn->z.release(); fn->p.release(); fn->q.release(); fn->sol.release();
     fn->scal.release();
+    tad_ctx ctx = fn->ctx;
     delete fn;
+    ctx_release(ctx);
     return TAD_OK;
 }
 

@@ -56,6 +56,24 @@ __device__ __forceinline__ double tad_inf() { return __longlong_as_double(0x7ff0
 template <class T>
 __device__ __forceinline__ bool finite_val(T x) { return isfinite(x); }
 
+// positive_diagonally_dominant (HessianProjection.hh:23-42) on a packed
+// symmetric K x K matrix: H_ii >= sum_{j != i} |H_ij| + eps for every row.
+template <int K, class T>
+__device__ __forceinline__ bool positive_diagonally_dominant(const T* Hp, T eps)
+{
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+    {
+        T off = T(0);
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (j != i) off += fabs(Hp[Sym<K>::at(i, j)]);
+        ok = ok && !(Hp[Sym<K>::idx(i, i)] < off + eps);
+    }
+    return ok;
+}
+
 // ---------------------------------------------------------------------------
 // Passive energies (FP64), shared by the line-search energy kernel and by the
 // value output of the derivative kernels so both give the same bits.
@@ -258,10 +276,64 @@ template <> struct Helmert<4>
     }
 };
 
-struct ElemStatus
+// Unprojected element Hessian in x coordinates, packed KX x KX:
+// H_X[(v,i),(u,i')] = w sum_{j,j'} D_vj D_uj' H_J[(i,j),(i',j')]
+template <int Dm, class T>
+__device__ __forceinline__ void symdir_hessian_x(const Dual<Dm * Dm, T>& E, const T (*Dmat)[Dm], T w, T* Hx)
 {
-    bool nonfinite;
-};
+    constexpr int N = Dm + 1;
+    constexpr int KJ = Dm * Dm;
+    constexpr int KX = N * Dm;
+    // T1[(i,j)][u][i'] = sum_j' H_J[(i,j),(i',j')] D_uj'
+    T T1[KJ][N][Dm];
+#pragma unroll
+    for (int r = 0; r < KJ; ++r)
+#pragma unroll
+        for (int u = 0; u < N; ++u)
+#pragma unroll
+            for (int ip = 0; ip < Dm; ++ip)
+            {
+                T s = T(0);
+#pragma unroll
+                for (int jp = 0; jp < Dm; ++jp) s += E.h[Sym<KJ>::at(r, ip * Dm + jp)] * Dmat[u][jp];
+                T1[r][u][ip] = s;
+            }
+#pragma unroll
+    for (int v = 0; v < N; ++v)
+#pragma unroll
+        for (int u = v; u < N; ++u)
+#pragma unroll
+            for (int i = 0; i < Dm; ++i)
+#pragma unroll
+                for (int ip = 0; ip < Dm; ++ip)
+                {
+                    if (u == v && ip < i) continue;
+                    T s = T(0);
+#pragma unroll
+                    for (int j = 0; j < Dm; ++j) s += Dmat[v][j] * T1[i * Dm + j][u][ip];
+                    Hx[Sym<KX>::idx(v * Dm + i, u * Dm + ip)] = w * s;
+                }
+}
+
+// Writes the N(N+1)/2 upper d x d blocks of a packed (N d) x (N d) matrix.
+template <int N, int Dm, class T>
+__device__ __forceinline__ void write_blocks(const T* Hp, T* hrec, bool& nonfinite)
+{
+    constexpr int K = N * Dm;
+#pragma unroll
+    for (int v = 0; v < N; ++v)
+#pragma unroll
+        for (int u = v; u < N; ++u)
+#pragma unroll
+            for (int i = 0; i < Dm; ++i)
+#pragma unroll
+                for (int ip = 0; ip < Dm; ++ip)
+                {
+                    const T val = Hp[Sym<K>::at(v * Dm + i, u * Dm + ip)];
+                    hrec[block_index(N, v, u) * Dm * Dm + i * Dm + ip] = val;
+                    nonfinite = nonfinite || !finite_val(val);
+                }
+}
 
 // Second-order evaluation of one symmetric-Dirichlet simplex.
 // X: element coordinates (FP64, AoS N x Dm); p: [Mr^-1, w] (FP64).
@@ -351,41 +423,27 @@ __device__ __forceinline__ double symdir_element(const double* X, const double* 
 
     if constexpr (MODE == MODE_HESS)
     {
-        // H_X[(v,i),(u,i')] = w sum_{j,j'} D_vj D_uj' H_J[(i,j),(i',j')]
-        // first T1[(i,j),(u,i')] = sum_j' H_J[(i,j),(i',j')] D_uj'
-        T T1[KJ][N][Dm];
-#pragma unroll
-        for (int r = 0; r < KJ; ++r)
-#pragma unroll
-            for (int u = 0; u < N; ++u)
-#pragma unroll
-                for (int ip = 0; ip < Dm; ++ip)
-                {
-                    T s = T(0);
-#pragma unroll
-                    for (int jp = 0; jp < Dm; ++jp) s += E.h[Sym<KJ>::at(r, ip * Dm + jp)] * Dmat[u][jp];
-                    T1[r][u][ip] = s;
-                }
-#pragma unroll
-        for (int v = 0; v < N; ++v)
-#pragma unroll
-            for (int u = v; u < N; ++u)
-#pragma unroll
-                for (int i = 0; i < Dm; ++i)
-#pragma unroll
-                    for (int ip = 0; ip < Dm; ++ip)
-                    {
-                        T s = T(0);
-#pragma unroll
-                        for (int j = 0; j < Dm; ++j) s += Dmat[v][j] * T1[i * Dm + j][u][ip];
-                        const T val = w * s;
-                        hrec[block_index(N, v, u) * Dm * Dm + i * Dm + ip] = val;
-                        nonfinite = nonfinite || !finite_val(val);
-                    }
+        T Hx[Sym<KX>::n];
+        symdir_hessian_x<Dm, T>(E, Dmat, w, Hx);
+        write_blocks<N, Dm, T>(Hx, hrec, nonfinite);
         return f;
     }
     else
     {
+        // Diagonal-dominance early out of the reference.  With eps > 0 it
+        // cannot fire here: the translations are in the kernel of H, so
+        // H_ii = -sum_{u != v} H_(v,i),(u,i) up to rounding and H_ii >=
+        // sum |H_ij| + eps is impossible unless eps is below rounding level.
+        if (eps <= T(0))
+        {
+            T Hx[Sym<KX>::n];
+            symdir_hessian_x<Dm, T>(E, Dmat, w, Hx);
+            if (positive_diagonally_dominant<KX, T>(Hx, eps))
+            {
+                write_blocks<N, Dm, T>(Hx, hrec, nonfinite);
+                return f;
+            }
+        }
         // E_aj = sum_v U_va D_vj ((N-1) x Dm, square)
         T Em[Dm][Dm];
 #pragma unroll
@@ -484,21 +542,9 @@ __device__ __forceinline__ void write_dense_record(const T* g, T* Hp, T eps, T* 
     if constexpr (MODE >= MODE_HESS)
     {
         if constexpr (MODE == MODE_HESS_PROJ)
-            project_psd<K, T>(Hp, eps);
-        T* hrec = rec + K;
-#pragma unroll
-        for (int v = 0; v < N; ++v)
-#pragma unroll
-            for (int u = v; u < N; ++u)
-#pragma unroll
-                for (int i = 0; i < Dm; ++i)
-#pragma unroll
-                    for (int ip = 0; ip < Dm; ++ip)
-                    {
-                        const T val = Hp[Sym<K>::at(v * Dm + i, u * Dm + ip)];
-                        hrec[block_index(N, v, u) * Dm * Dm + i * Dm + ip] = val;
-                        nonfinite = nonfinite || !finite_val(val);
-                    }
+            if (!positive_diagonally_dominant<K, T>(Hp, eps))
+                project_psd<K, T>(Hp, eps);
+        write_blocks<N, Dm, T>(Hp, rec + K, nonfinite);
     }
 }
 
