@@ -214,6 +214,7 @@ struct tad_fn_s
     // solver
     DevBuf<double> Dinv, r, z, p, q, sol;
     DevBuf<PcgScalars> scal;
+    DevBuf<double> pcg_parts;
     bool h_valid = false;
 
     hipStream_t s() const { return ctx->stream; }
@@ -489,12 +490,8 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
     TAD_HIP(fn->p.ensure(n));
     TAD_HIP(fn->q.ensure(n));
     TAD_HIP(fn->scal.ensure(1));
-    // per-block partials of the PCG kernels: the SpMV grid has nh * 16 / 256
-    // blocks (16 lanes per block row), the vector kernels nh / 256; two
-    // partial arrays each
-    const int64_t spmv_blocks = (nh * 16 + 255) / 256;
-    const int64_t vec_blocks = (nh + 255) / 256;
-    TAD_HIP(fn->partials.ensure((size_t)(2 * std::max(spmv_blocks, vec_blocks) + 8 * kReduceBlocks + 64)));
+    TAD_HIP(fn->pcg_parts.ensure((size_t)pcg_parts_size(d, nh)));
+    if (o.max_iters <= 0) return fail(TAD_ERR_INVALID, "max_iters must be positive");
     tm.begin("pcg_setup", s);
     TAD_HIP(launch_pcg_setup(d, nh, fn->brow_ptr.p, fn->diag_block.p, fn->row_ptr.p, H_dev, o.w_identity, fn->Dinv.p, s));
     tm.end("pcg_setup", s);
@@ -506,25 +503,24 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
     std::memcpy(pin, &hs, sizeof(hs));
     TAD_HIP(hipMemcpyAsync(fn->scal.p, pin, sizeof(hs), hipMemcpyHostToDevice, s));
     tm.begin("pcg_init", s);
-    TAD_HIP(launch_pcg_init(d, nh, rhs_dev, fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->p.p, fn->partials.p, fn->scal.p, s));
+    TAD_HIP(launch_pcg_init(d, nh, rhs_dev, fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->p.p, fn->pcg_parts.p, fn->scal.p, s));
     tm.end("pcg_init", s);
     const int every = std::max(1, o.check_every);
     Timer* timer = fn->ctx->timer.enabled ? &fn->ctx->timer : nullptr;
-    int64_t launched = 0;
+    // iterations are launched in chunks; every kernel returns at once after
+    // the device has recorded termination, so overshooting a chunk is cheap
+    int it = 0;
     for (;;)
     {
+        if (it >= o.max_iters) break;
+        const int n_launch = std::min(every, o.max_iters - it);
+        for (int k = 0; k < n_launch; ++k, ++it)
+            TAD_HIP(launch_pcg_iteration(d, nh, fn->brow_ptr.p, fn->bcol.p, fn->row_ptr.p, H_dev, o.w_identity,
+                                         fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->p.p, fn->q.p, fn->pcg_parts.p,
+                                         fn->scal.p, it, s, timer));
         TAD_HIP(hipMemcpyAsync(pin, fn->scal.p, sizeof(hs), hipMemcpyDeviceToHost, s));
         TAD_HIP(hipStreamSynchronize(s));
         if (pin->done) break;
-        // the device sets done at max_iters; this host bound only guards
-        // against corrupted solver state
-        if (launched > (int64_t)o.max_iters + every)
-            return fail(TAD_ERR_SOLVE, "PCG did not report termination");
-        launched += every;
-        for (int k = 0; k < every; ++k)
-            TAD_HIP(launch_pcg_iteration(d, nh, fn->brow_ptr.p, fn->bcol.p, fn->row_ptr.p, H_dev, o.w_identity,
-                                         fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->p.p, fn->q.p, fn->partials.p,
-                                         fn->scal.p, s, timer));
     }
     std::memcpy(&hs, pin, sizeof(hs));
     const double rel = hs.bb > 0 ? std::sqrt(hs.rr / hs.bb) : 0.0;
@@ -707,7 +703,7 @@ tad_status tad_fn_destroy(tad_fn fn)
     fn->partials.release(); fn->results.release(); fn->steps.release(); fn->err.release();
     fn->ls_f.release(); fn->x_tmp.release();
     fn->Dinv.release(); fn->r.release(); fn->z.release(); fn->p.release(); fn->q.release(); fn->sol.release();
-    fn->scal.release();
+    fn->scal.release(); fn->pcg_parts.release();
     tad_ctx ctx = fn->ctx;
     delete fn;
     ctx_release(ctx);

@@ -26,12 +26,11 @@ struct TermDev
     int64_t f_base = 0;        // first entry of this term in the element-energy array
 };
 
+// Solver bookkeeping written by block 0 of the PCG kernels (see tad_linalg.hip).
 struct PcgScalars
 {
-    double rho, rho_new, pq, rr, bb, alpha, beta, tol2;
+    double bb, rr, tol2;
     int iters, max_iters, done, breakdown;
-    unsigned int counter;
-    unsigned int pad[3];
 };
 
 // Kernel timing hook (set by the host library when timing is enabled).
@@ -82,8 +81,11 @@ hipError_t launch_pcg_init(int d, int64_t n_handles, const double* b, const doub
 
 hipError_t launch_pcg_iteration(int d, int64_t n_handles, const int32_t* brow_ptr, const int32_t* bcol,
                                 const int32_t* row_ptr, const double* H, double w, const double* Dinv, double* x,
-                                double* r, double* z, double* p, double* q, double* partials, PcgScalars* sc,
+                                double* r, double* z, double* p, double* q, double* partials, PcgScalars* sc, int it,
                                 hipStream_t s, Timer* timer);
+
+// Size (doubles) of the partial-sum buffer the PCG kernels need.
+int64_t pcg_parts_size(int d, int64_t n_handles);
 
 hipError_t launch_selftest_dual(int which, double x, double y, double* out_dev, hipStream_t s);
 

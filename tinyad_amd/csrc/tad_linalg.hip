@@ -10,6 +10,7 @@
 //                      (replaces SimplicialLDLT, NewtonDirection.hh:25-48)
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <algorithm>
 #include "tad_internal.hpp"
 #include "tad_energies.hpp"
 #include "tad_elements.hpp"
@@ -225,27 +226,31 @@ hipError_t launch_assemble_gradient(int d, int precision, int64_t nh, const int6
 // ---------------------------------------------------------------------------
 // PCG
 // ---------------------------------------------------------------------------
-// Last-block-done reduction: every block publishes its partial, the block
-// that finishes last (integer atomic ticket) sums all partials in index order.
-__device__ __forceinline__ bool publish_partials(double* partials, int n_vals, const double* vals, PcgScalars* sc)
+// Reductions without atomics or fences: every kernel writes one partial per
+// block; the kernel that consumes a scalar re-sums the producer's partials in
+// index order in every block (identical bits everywhere, a few KB from L2).
+// Scalars that change per iteration live in double-buffered partial slots
+// indexed by the iteration number, so no block of a kernel ever reads a value
+// another block of the same kernel writes.  Block 0 records bookkeeping
+// (iteration count, residual, termination) for later kernels and the host.
+//
+// Partial buffer layout (doubles): [pq: Gs][rz: 2 x Gv][rr: 2 x Gv][rr0: Gv]
+
+// Block-uniform read of the termination flag: block 0 of the same kernel may
+// set it while other blocks start, so one thread reads it for the whole block
+// and every thread then takes the same branch around the barriers below.
+__device__ __forceinline__ bool block_done(const PcgScalars* sc)
 {
-    __shared__ bool last;
-    if (threadIdx.x == 0)
-    {
-        for (int m = 0; m < n_vals; ++m) partials[m * gridDim.x + blockIdx.x] = vals[m];
-        __threadfence();
-        const unsigned t = atomicAdd(&sc->counter, 1u);
-        last = (t == gridDim.x - 1);
-    }
+    __shared__ int flag;
+    if (threadIdx.x == 0) flag = sc->done;
     __syncthreads();
-    if (last) __threadfence();
-    return last;
+    return flag != 0;
 }
 
-__device__ __forceinline__ double sum_partials(const double* partials, int m, double* sh)
+__device__ __forceinline__ double sum_parts(const double* __restrict__ parts, int n, double* sh)
 {
     double acc = 0.0;
-    for (unsigned i = threadIdx.x; i < gridDim.x; i += kBlock) acc += partials[m * gridDim.x + i];
+    for (int i = threadIdx.x; i < n; i += kBlock) acc += parts[i];
     return block_sum(acc, sh);
 }
 
@@ -306,17 +311,17 @@ __global__ __launch_bounds__(kBlock) void k_pcg_setup(int64_t nh, const int32_t*
     for (int m = 0; m < D * D; ++m) Dinv[u * D * D + m] = o[m];
 }
 
+// x = 0, r = b, z = p = M^-1 r; partials r.z (slot 1) and r.r (rr0)
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t nh, const double* __restrict__ b,
                                                      const double* __restrict__ Dinv, double* __restrict__ x,
                                                      double* __restrict__ r, double* __restrict__ z,
-                                                     double* __restrict__ p, double* __restrict__ partials,
-                                                     PcgScalars* sc)
+                                                     double* __restrict__ p, double* __restrict__ parts, int Gs,
+                                                     int Gv, PcgScalars* sc)
 {
     __shared__ double sh[kBlock];
-    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double rz = 0.0, rr = 0.0;
-    if (u < nh)
+    for (int64_t u = (int64_t)blockIdx.x * kBlock + threadIdx.x; u < nh; u += (int64_t)gridDim.x * kBlock)
     {
         double rv[D];
 #pragma unroll
@@ -338,108 +343,113 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t nh, const double* _
             rr += rv[i] * rv[i];
         }
     }
-    double vals[2];
-    vals[0] = block_sum(rz, sh);
-    vals[1] = block_sum(rr, sh);
-    if (publish_partials(partials, 2, vals, sc))
+    const double srz = block_sum(rz, sh);
+    const double srr = block_sum(rr, sh);
+    if (threadIdx.x == 0)
     {
-        const double trz = sum_partials(partials, 0, sh);
-        const double trr = sum_partials(partials, 1, sh);
-        if (threadIdx.x == 0)
+        parts[Gs + Gv + blockIdx.x] = srz;      // rz slot 1 (iteration -1)
+        parts[Gs + 4 * Gv + blockIdx.x] = srr;  // rr0
+        if (blockIdx.x == 0)
         {
-            sc->rho = trz;
-            sc->rr = trr;
-            sc->bb = trr;
             sc->iters = 0;
+            sc->done = 0;
             sc->breakdown = 0;
-            sc->done = (trr == 0.0 || sc->max_iters <= 0) ? 1 : 0;
-            sc->counter = 0;
         }
     }
 }
 
-// q = (w I + H) p, partial p.q; last block: alpha = rho / p.q
+// q = (w I + H) p; partial p.q.  Iteration 0 also records ||b||^2.
 template <int D, int LPB>
 __global__ __launch_bounds__(kBlock) void k_spmv(int64_t nh, const int32_t* __restrict__ brow_ptr,
                                                  const int32_t* __restrict__ bcol, const int32_t* __restrict__ row_ptr,
                                                  const double* __restrict__ H, double w, const double* __restrict__ p,
-                                                 double* __restrict__ q, double* __restrict__ partials, PcgScalars* sc)
+                                                 double* __restrict__ q, double* __restrict__ parts, int Gs, int Gv,
+                                                 int it, PcgScalars* sc)
 {
-    if (sc->done) return;
+    if (block_done(sc)) return;
     __shared__ double sh[kBlock];
-    const int lane = threadIdx.x % LPB;
-    const int64_t u = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / LPB;
-    double acc[D];
-#pragma unroll
-    for (int i = 0; i < D; ++i) acc[i] = 0.0;
-    if (u < nh)
+    if (it == 0)
     {
-        const int b0 = brow_ptr[u];
-        const int nbu = brow_ptr[u + 1] - b0;
-        const int64_t base = row_ptr[D * u];
-        for (int k = lane; k < nbu; k += LPB)
+        const double bb = sum_parts(parts + Gs + 4 * Gv, Gv, sh);
+        if (blockIdx.x == 0 && threadIdx.x == 0)
         {
-            const int64_t v = bcol[b0 + k];
-            double pv[D];
+            sc->bb = bb;
+            sc->rr = bb;
+            if (bb == 0.0 || sc->max_iters <= 0) sc->done = 1;
+        }
+    }
+    const int lane = threadIdx.x % LPB;
+    constexpr int rows_per_block = kBlock / LPB;
+    double pq = 0.0;
+    for (int64_t u0 = (int64_t)blockIdx.x * rows_per_block; u0 < nh; u0 += (int64_t)gridDim.x * rows_per_block)
+    {
+        const int64_t u = u0 + threadIdx.x / LPB;
+        double acc[D];
 #pragma unroll
-            for (int j = 0; j < D; ++j) pv[j] = p[v * D + j];
+        for (int i = 0; i < D; ++i) acc[i] = 0.0;
+        if (u < nh)
+        {
+            const int b0 = brow_ptr[u];
+            const int nbu = brow_ptr[u + 1] - b0;
+            const int64_t base = row_ptr[D * u];
+            for (int k = lane; k < nbu; k += LPB)
+            {
+                const int64_t v = bcol[b0 + k];
+                double pv[D];
+#pragma unroll
+                for (int j = 0; j < D; ++j) pv[j] = p[v * D + j];
+#pragma unroll
+                for (int i = 0; i < D; ++i)
+#pragma unroll
+                    for (int j = 0; j < D; ++j) acc[i] += H[base + (int64_t)i * D * nbu + D * k + j] * pv[j];
+            }
+        }
+#pragma unroll
+        for (int off = LPB / 2; off > 0; off >>= 1)
+#pragma unroll
+            for (int i = 0; i < D; ++i) acc[i] += __shfl_xor(acc[i], off, LPB);
+        if (u < nh && lane == 0)
+        {
 #pragma unroll
             for (int i = 0; i < D; ++i)
-#pragma unroll
-                for (int j = 0; j < D; ++j) acc[i] += H[base + (int64_t)i * D * nbu + D * k + j] * pv[j];
-        }
-    }
-#pragma unroll
-    for (int off = LPB / 2; off > 0; off >>= 1)
-#pragma unroll
-        for (int i = 0; i < D; ++i) acc[i] += __shfl_xor(acc[i], off, LPB);
-    double pq = 0.0;
-    if (u < nh && lane == 0)
-    {
-#pragma unroll
-        for (int i = 0; i < D; ++i)
-        {
-            const double pi = p[u * D + i];
-            const double qi = acc[i] + w * pi;
-            q[u * D + i] = qi;
-            pq += pi * qi;
-        }
-    }
-    double vals[1];
-    vals[0] = block_sum(pq, sh);
-    if (publish_partials(partials, 1, vals, sc))
-    {
-        const double tpq = sum_partials(partials, 0, sh);
-        if (threadIdx.x == 0)
-        {
-            sc->pq = tpq;
-            if (!(tpq > 0.0))
             {
-                sc->breakdown = 1;
-                sc->done = 1;
-                sc->alpha = 0.0;
+                const double pi = p[u * D + i];
+                const double qi = acc[i] + w * pi;
+                q[u * D + i] = qi;
+                pq += pi * qi;
             }
-            else
-                sc->alpha = sc->rho / tpq;
-            sc->counter = 0;
         }
     }
+    const double s = block_sum(pq, sh);
+    if (threadIdx.x == 0) parts[blockIdx.x] = s;
 }
 
-// x += alpha p, r -= alpha q, z = M^-1 r, partials r.z, r.r; last block: beta, convergence
+// alpha = rho / p.q; x += alpha p, r -= alpha q, z = M^-1 r; partials r.z, r.r (slot it & 1)
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t nh, const double* __restrict__ Dinv,
                                                        double* __restrict__ x, double* __restrict__ r,
                                                        double* __restrict__ z, const double* __restrict__ p,
-                                                       const double* __restrict__ q, double* __restrict__ partials,
-                                                       PcgScalars* sc)
+                                                       const double* __restrict__ q, double* __restrict__ parts,
+                                                       int Gs, int Gv, int it, PcgScalars* sc)
 {
-    if (sc->done) return;
+    if (block_done(sc)) return;
     __shared__ double sh[kBlock];
-    const double alpha = sc->alpha;
-    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double pq = sum_parts(parts, Gs, sh);
+    const double rho = sum_parts(parts + Gs + ((it + 1) & 1) * Gv, Gv, sh);
+    if (!(pq > 0.0))
+    {
+        // not positive definite along p: the same decision in every block
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+        {
+            sc->breakdown = 1;
+            sc->done = 1;
+            sc->iters = it;
+        }
+        return;
+    }
+    const double alpha = rho / pq;
     double rz = 0.0, rr = 0.0;
-    if (u < nh)
+    for (int64_t u = (int64_t)blockIdx.x * kBlock + threadIdx.x; u < nh; u += (int64_t)gridDim.x * kBlock)
     {
         double rv[D];
 #pragma unroll
@@ -460,33 +470,54 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t nh, const double*
             rr += rv[i] * rv[i];
         }
     }
-    double vals[2];
-    vals[0] = block_sum(rz, sh);
-    vals[1] = block_sum(rr, sh);
-    if (publish_partials(partials, 2, vals, sc))
+    const double srz = block_sum(rz, sh);
+    const double srr = block_sum(rr, sh);
+    if (threadIdx.x == 0)
     {
-        const double trz = sum_partials(partials, 0, sh);
-        const double trr = sum_partials(partials, 1, sh);
-        if (threadIdx.x == 0)
-        {
-            sc->beta = trz / sc->rho;
-            sc->rho = trz;
-            sc->rr = trr;
-            sc->iters += 1;
-            if (trr <= sc->tol2 * sc->bb || sc->iters >= sc->max_iters) sc->done = 1;
-            sc->counter = 0;
-        }
+        parts[Gs + (it & 1) * Gv + blockIdx.x] = srz;
+        parts[Gs + 2 * Gv + (it & 1) * Gv + blockIdx.x] = srr;
     }
 }
 
+// beta = rho_new / rho; p = z + beta p, or record termination
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_pcg_dir(int64_t nh, const double* __restrict__ z, double* __restrict__ p,
-                                                    const PcgScalars* sc)
+                                                    const double* __restrict__ parts, int Gs, int Gv, int it,
+                                                    PcgScalars* sc)
 {
-    if (sc->done) return;
-    const double beta = sc->beta;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nh * D) p[i] = z[i] + beta * p[i];
+    if (block_done(sc)) return;
+    __shared__ double sh[kBlock];
+    const double rho_new = sum_parts(parts + Gs + (it & 1) * Gv, Gv, sh);
+    const double rho = sum_parts(parts + Gs + ((it + 1) & 1) * Gv, Gv, sh);
+    const double rr = sum_parts(parts + Gs + 2 * Gv + (it & 1) * Gv, Gv, sh);
+    const bool stop = rr <= sc->tol2 * sc->bb || it + 1 >= sc->max_iters;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        sc->iters = it + 1;
+        sc->rr = rr;
+        if (stop) sc->done = 1;
+    }
+    if (stop) return;
+    const double beta = rho_new / rho;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nh * D; i += (int64_t)gridDim.x * kBlock)
+        p[i] = z[i] + beta * p[i];
+}
+
+void pcg_grids(int d, int64_t nh, int* Gs, int* Gv)
+{
+    const int lpb = d == 3 ? 16 : 8;
+    const int64_t rows_per_block = kBlock / lpb;
+    const int64_t gs = (nh + rows_per_block - 1) / rows_per_block;
+    const int64_t gv = (nh + kBlock - 1) / kBlock;
+    *Gs = (int)std::max<int64_t>(1, std::min<int64_t>(gs, 2048));
+    *Gv = (int)std::max<int64_t>(1, std::min<int64_t>(gv, 1024));
+}
+
+int64_t pcg_parts_size(int d, int64_t nh)
+{
+    int Gs, Gv;
+    pcg_grids(d, nh, &Gs, &Gv);
+    return Gs + 5 * (int64_t)Gv;
 }
 
 hipError_t launch_pcg_setup(int d, int64_t nh, const int32_t* brow_ptr, const int32_t* diag_block,
@@ -500,46 +531,37 @@ hipError_t launch_pcg_setup(int d, int64_t nh, const int32_t* brow_ptr, const in
 }
 
 hipError_t launch_pcg_init(int d, int64_t nh, const double* b, const double* Dinv, double* x, double* r, double* z,
-                           double* p, double* partials, PcgScalars* sc, hipStream_t s)
+                           double* p, double* parts, PcgScalars* sc, hipStream_t s)
 {
-    const unsigned grid = (unsigned)((nh + kBlock - 1) / kBlock);
-    if (d == 1) k_pcg_init<1><<<grid, kBlock, 0, s>>>(nh, b, Dinv, x, r, z, p, partials, sc);
-    else if (d == 2) k_pcg_init<2><<<grid, kBlock, 0, s>>>(nh, b, Dinv, x, r, z, p, partials, sc);
-    else k_pcg_init<3><<<grid, kBlock, 0, s>>>(nh, b, Dinv, x, r, z, p, partials, sc);
+    int Gs, Gv;
+    pcg_grids(d, nh, &Gs, &Gv);
+    if (d == 1) k_pcg_init<1><<<Gv, kBlock, 0, s>>>(nh, b, Dinv, x, r, z, p, parts, Gs, Gv, sc);
+    else if (d == 2) k_pcg_init<2><<<Gv, kBlock, 0, s>>>(nh, b, Dinv, x, r, z, p, parts, Gs, Gv, sc);
+    else k_pcg_init<3><<<Gv, kBlock, 0, s>>>(nh, b, Dinv, x, r, z, p, parts, Gs, Gv, sc);
     return hipGetLastError();
-}
-
-template <int D, int LPB>
-static void launch_spmv_t(int64_t nh, const int32_t* brow_ptr, const int32_t* bcol, const int32_t* row_ptr,
-                          const double* H, double w, const double* p, double* q, double* partials, PcgScalars* sc,
-                          hipStream_t s)
-{
-    const int64_t rows_per_block = kBlock / LPB;
-    const unsigned grid = (unsigned)((nh + rows_per_block - 1) / rows_per_block);
-    k_spmv<D, LPB><<<grid, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, p, q, partials, sc);
 }
 
 hipError_t launch_pcg_iteration(int d, int64_t nh, const int32_t* brow_ptr, const int32_t* bcol,
                                 const int32_t* row_ptr, const double* H, double w, const double* Dinv, double* x,
-                                double* r, double* z, double* p, double* q, double* partials, PcgScalars* sc,
+                                double* r, double* z, double* p, double* q, double* parts, PcgScalars* sc, int it,
                                 hipStream_t s, Timer* timer)
 {
-    const unsigned grid = (unsigned)((nh + kBlock - 1) / kBlock);
-    const unsigned grid_v = (unsigned)((nh * d + kBlock - 1) / kBlock);
+    int Gs, Gv;
+    pcg_grids(d, nh, &Gs, &Gv);
     if (timer) timer->begin("spmv", s);
-    if (d == 1) launch_spmv_t<1, 8>(nh, brow_ptr, bcol, row_ptr, H, w, p, q, partials, sc, s);
-    else if (d == 2) launch_spmv_t<2, 8>(nh, brow_ptr, bcol, row_ptr, H, w, p, q, partials, sc, s);
-    else launch_spmv_t<3, 16>(nh, brow_ptr, bcol, row_ptr, H, w, p, q, partials, sc, s);
+    if (d == 1) k_spmv<1, 8><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, p, q, parts, Gs, Gv, it, sc);
+    else if (d == 2) k_spmv<2, 8><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, p, q, parts, Gs, Gv, it, sc);
+    else k_spmv<3, 16><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, p, q, parts, Gs, Gv, it, sc);
     if (timer) timer->end("spmv", s);
     if (timer) timer->begin("pcg_update", s);
-    if (d == 1) k_pcg_update<1><<<grid, kBlock, 0, s>>>(nh, Dinv, x, r, z, p, q, partials, sc);
-    else if (d == 2) k_pcg_update<2><<<grid, kBlock, 0, s>>>(nh, Dinv, x, r, z, p, q, partials, sc);
-    else k_pcg_update<3><<<grid, kBlock, 0, s>>>(nh, Dinv, x, r, z, p, q, partials, sc);
+    if (d == 1) k_pcg_update<1><<<Gv, kBlock, 0, s>>>(nh, Dinv, x, r, z, p, q, parts, Gs, Gv, it, sc);
+    else if (d == 2) k_pcg_update<2><<<Gv, kBlock, 0, s>>>(nh, Dinv, x, r, z, p, q, parts, Gs, Gv, it, sc);
+    else k_pcg_update<3><<<Gv, kBlock, 0, s>>>(nh, Dinv, x, r, z, p, q, parts, Gs, Gv, it, sc);
     if (timer) timer->end("pcg_update", s);
     if (timer) timer->begin("pcg_dir", s);
-    if (d == 1) k_pcg_dir<1><<<grid_v, kBlock, 0, s>>>(nh, z, p, sc);
-    else if (d == 2) k_pcg_dir<2><<<grid_v, kBlock, 0, s>>>(nh, z, p, sc);
-    else k_pcg_dir<3><<<grid_v, kBlock, 0, s>>>(nh, z, p, sc);
+    if (d == 1) k_pcg_dir<1><<<Gv, kBlock, 0, s>>>(nh, z, p, parts, Gs, Gv, it, sc);
+    else if (d == 2) k_pcg_dir<2><<<Gv, kBlock, 0, s>>>(nh, z, p, parts, Gs, Gv, it, sc);
+    else k_pcg_dir<3><<<Gv, kBlock, 0, s>>>(nh, z, p, parts, Gs, Gv, it, sc);
     if (timer) timer->end("pcg_dir", s);
     return hipGetLastError();
 }
