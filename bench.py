@@ -129,7 +129,7 @@ def main():
             log(f"[bench] newton: f {st['f']:.10g} -> {st['f_new']:.10g} cg {st['cg_iters']} ls {st['ls_trials']} "
                 f"step {st['step']:.3g}")
         statsB = {k: ctx.kernel_stats(k) for k in ["element", "assemble_gradient", "assemble_hessian", "energy_sum",
-                                                   "spmv", "pcg_update", "pcg_dir", "pcg_setup", "pcg_init",
+                                                   "spmv", "pcg_update", "pcg_setup", "pcg_init",
                                                    "energy"]}
         n_cg = sum(s["cg_iters"] for s in sts)
         n_ls = sum(s["ls_trials"] for s in sts)
@@ -159,7 +159,9 @@ def main():
     avg_s = tot_ms / max(cnt, 1) / 1e3
     nb = (nnz // 9)
     if dom == "spmv":
-        traffic_alg = 8 * nnz + 4 * nb + 4 * (P.n_handles + 1) + 4 * P.n_handles + 8 * n + 8 * n
+        # values + block columns + block-row / row offsets + z and p_{k-1} read,
+        # p_k and q written (the direction update is fused into the SpMV)
+        traffic_alg = 8 * nnz + 4 * nb + 4 * (P.n_handles + 1) + 4 * P.n_handles + 4 * 8 * n
         roof = {"kernel": "k_spmv<3,16>", "bound": "hbm", "achieved": traffic_alg / avg_s / 1e9, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "bytes_per_launch": traffic_alg}
     elif dom == "element":

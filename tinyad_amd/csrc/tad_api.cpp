@@ -487,7 +487,7 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
     TAD_HIP(fn->Dinv.ensure((size_t)nh * d * d));
     TAD_HIP(fn->r.ensure(n));
     TAD_HIP(fn->z.ensure(n));
-    TAD_HIP(fn->p.ensure(n));
+    TAD_HIP(fn->p.ensure(2 * n));
     TAD_HIP(fn->q.ensure(n));
     TAD_HIP(fn->scal.ensure(1));
     TAD_HIP(fn->pcg_parts.ensure((size_t)pcg_parts_size(d, nh)));
@@ -503,25 +503,26 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
     std::memcpy(pin, &hs, sizeof(hs));
     TAD_HIP(hipMemcpyAsync(fn->scal.p, pin, sizeof(hs), hipMemcpyHostToDevice, s));
     tm.begin("pcg_init", s);
-    TAD_HIP(launch_pcg_init(d, nh, rhs_dev, fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->p.p, fn->pcg_parts.p, fn->scal.p, s));
+    TAD_HIP(launch_pcg_init(d, nh, rhs_dev, fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->pcg_parts.p, fn->scal.p, s));
     tm.end("pcg_init", s);
     const int every = std::max(1, o.check_every);
     Timer* timer = fn->ctx->timer.enabled ? &fn->ctx->timer : nullptr;
     // iterations are launched in chunks; every kernel returns at once after
-    // the device has recorded termination, so overshooting a chunk is cheap
-    int it = 0;
-    for (;;)
+    // the device has recorded termination, so overshooting a chunk is cheap.
+    // Iteration k = max_iters launches only the SpMV, whose termination test
+    // then always fires.
+    int k = 0;
+    while (k <= o.max_iters)
     {
-        if (it >= o.max_iters) break;
-        const int n_launch = std::min(every, o.max_iters - it);
-        for (int k = 0; k < n_launch; ++k, ++it)
+        for (int c = 0; c < every && k <= o.max_iters; ++c, ++k)
             TAD_HIP(launch_pcg_iteration(d, nh, fn->brow_ptr.p, fn->bcol.p, fn->row_ptr.p, H_dev, o.w_identity,
                                          fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->p.p, fn->q.p, fn->pcg_parts.p,
-                                         fn->scal.p, it, s, timer));
+                                         fn->scal.p, k, k < o.max_iters, s, timer));
         TAD_HIP(hipMemcpyAsync(pin, fn->scal.p, sizeof(hs), hipMemcpyDeviceToHost, s));
         TAD_HIP(hipStreamSynchronize(s));
         if (pin->done) break;
     }
+    if (!pin->done) return fail(TAD_ERR_SOLVE, "PCG did not report termination");
     std::memcpy(&hs, pin, sizeof(hs));
     const double rel = hs.bb > 0 ? std::sqrt(hs.rr / hs.bb) : 0.0;
     if (st)

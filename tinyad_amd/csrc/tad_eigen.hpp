@@ -11,8 +11,9 @@
 // A - max(eps, 0) I: it succeeds exactly when every eigenvalue exceeds the
 // clamp level, which covers every matrix the diagonal-dominance test accepts
 // (Gershgorin) and more, and in that case the reference also returns the
-// matrix unchanged.  Otherwise a cyclic Jacobi eigensolve (deterministic,
-// robust on clustered spectra) runs and the clamped spectrum is re-assembled.
+// matrix unchanged.  Otherwise a Jacobi eigensolve (deterministic, robust on
+// clustered spectra) in round-robin pair order runs and the clamped spectrum
+// is re-assembled.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -32,8 +33,57 @@ template <> struct EigTraits<float>
     static constexpr int max_sweeps = 10;
 };
 
-__device__ __forceinline__ double tad_rsqrt(double x) { return 1.0 / sqrt(x); }
-__device__ __forceinline__ float tad_rsqrt(float x) { return rsqrtf(x); }
+// Reciprocal square root and reciprocal from the hardware approximations
+// refined by Newton steps (two steps take the ~2^-23 relative error of the
+// FP64 instructions below 1 ulp).  Arguments are positive and finite where used.
+__device__ __forceinline__ double tad_rsqrt(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    const double hx = 0.5 * x;
+    y = y * (1.5 - hx * y * y);
+    y = y * (1.5 - hx * y * y);
+    return y;
+}
+__device__ __forceinline__ float tad_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+
+__device__ __forceinline__ double tad_rcp(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+    y = y * (2.0 - x * y);
+    y = y * (2.0 - x * y);
+    return y;
+}
+__device__ __forceinline__ float tad_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Round-robin ("tournament") order of the n(n-1)/2 index pairs: every round
+// holds floor(n/2) disjoint pairs, so the rotations of one round commute and
+// their angle computations are independent of each other.
+template <int n>
+struct Tournament
+{
+    static constexpr int m = n + (n & 1);        // players, with a dummy when n is odd
+    static constexpr int rounds = m - 1;
+    static constexpr int per_round = n / 2;      // real pairs per round
+    int p[rounds][per_round > 0 ? per_round : 1];
+    int q[rounds][per_round > 0 ? per_round : 1];
+    constexpr Tournament() : p(), q()
+    {
+        for (int r = 0; r < rounds; ++r)
+        {
+            int k = 0;
+            for (int i = 0; i < m / 2; ++i)
+            {
+                int a = (i == 0) ? m - 1 : (r + i) % (m - 1);
+                int b = (r - i + (m - 1)) % (m - 1);
+                if (i == 0) b = r;
+                if (a >= n || b >= n) continue;  // pair with the dummy
+                p[r][k] = a < b ? a : b;
+                q[r][k] = a < b ? b : a;
+                ++k;
+            }
+        }
+    }
+};
 
 // True when A - shift I is positive definite (Cholesky succeeds).
 template <int n, class T>
@@ -63,8 +113,67 @@ __device__ __forceinline__ bool cholesky_pd(const T* A, T shift)
     return ok;
 }
 
-// Cyclic Jacobi: on exit A holds (approximately) the eigenvalues on its
-// diagonal and V (row-major n x n) the eigenvectors as columns.
+// One Jacobi rotation of rows/columns p, q (angle from the current entries).
+template <int n, class T>
+__device__ __forceinline__ void jacobi_angle(const T* A, int p, int q, T& c, T& s, T& t)
+{
+    const T apq = A[Sym<n>::idx(p, q)];
+    const T dlt = A[Sym<n>::idx(q, q)] - A[Sym<n>::idx(p, p)];
+    const T y = dlt * dlt + T(4) * apq * apq;
+    // tangent of the rotation angle in the stable form sign(d) 2 a_pq / (|d| + sqrt(d^2 + 4 a_pq^2))
+    if constexpr (sizeof(T) == 4)
+    {
+        const T rt = y > T(0) ? y * tad_rsqrt(y > T(0) ? y : T(1)) : T(0);
+        const T den = fabs(dlt) + rt;
+        t = (dlt >= T(0) ? T(2) : T(-2)) * apq * tad_rcp(den > T(0) ? den : T(1));
+        t = (den > T(0)) ? t : T(0);
+        c = tad_rsqrt(T(1) + t * t);
+    }
+    else
+    {
+        // FP64 keeps the library sqrt / divide: with the refined hardware
+        // approximations the register allocator spills A and V to scratch
+        const T den = fabs(dlt) + sqrt(y);
+        t = (dlt >= T(0) ? T(2) : T(-2)) * apq / (den > T(0) ? den : T(1));
+        t = (den > T(0)) ? t : T(0);
+        c = T(1) / sqrt(T(1) + t * t);
+    }
+    s = t * c;
+}
+
+template <int n, class T>
+__device__ __forceinline__ void jacobi_apply(T* A, T* V, int p, int q, T c, T s, T t)
+{
+    const T apq = A[Sym<n>::idx(p, q)];
+    A[Sym<n>::idx(p, p)] -= t * apq;
+    A[Sym<n>::idx(q, q)] += t * apq;
+    A[Sym<n>::idx(p, q)] = T(0);
+#pragma unroll
+    for (int rr = 0; rr < n; ++rr)
+    {
+        if (rr == p || rr == q) continue;
+        const int irp = Sym<n>::at(rr, p), irq = Sym<n>::at(rr, q);
+        const T arp = A[irp], arq = A[irq];
+        A[irp] = c * arp - s * arq;
+        A[irq] = s * arp + c * arq;
+    }
+#pragma unroll
+    for (int rr = 0; rr < n; ++rr)
+    {
+        const T vrp = V[rr * n + p], vrq = V[rr * n + q];
+        V[rr * n + p] = c * vrp - s * vrq;
+        V[rr * n + q] = s * vrp + c * vrq;
+    }
+}
+
+// Jacobi eigensolve: on exit A holds (approximately) the eigenvalues on its
+// diagonal and V (row-major n x n) the eigenvectors as columns.  Sweeps stop
+// when the off-diagonal mass is below tol^2 of the total.
+// FP32 sweeps visit the pairs in round-robin order: each round's rotations
+// are disjoint, commute, and their angle computations (the long dependent
+// rsq/rcp chains) are independent, which gives the scheduler parallel work.
+// FP64 sweeps use the cyclic row order: A and V already take 252 of the 256
+// VGPRs, and in-flight rotations of a round would spill.
 template <int n, class T>
 __device__ __forceinline__ void jacobi_eigen(T* A, T* V)
 {
@@ -86,56 +195,58 @@ __device__ __forceinline__ void jacobi_eigen(T* A, T* V)
         const T tol = EigTraits<T>::tol;
         if (!(off > tol * tol * (dia + T(2) * off)))
             break;
+        if constexpr (sizeof(T) == 4)
+        {
+            constexpr Tournament<n> tour{};
+            constexpr int NR = Tournament<n>::rounds;
+            constexpr int NP = Tournament<n>::per_round > 0 ? Tournament<n>::per_round : 1;
 #pragma unroll
-        for (int p = 0; p < n - 1; ++p)
-#pragma unroll
-            for (int q = p + 1; q < n; ++q)
+            for (int r = 0; r < NR; ++r)
             {
-                const T apq = A[Sym<n>::idx(p, q)];
-                const T app = A[Sym<n>::idx(p, p)];
-                const T aqq = A[Sym<n>::idx(q, q)];
-                // tan of the rotation angle, stable form without dividing by apq
-                const T dlt = aqq - app;
-                const T den = fabs(dlt) + sqrt(dlt * dlt + T(4) * apq * apq);
-                T t = (dlt >= T(0) ? T(2) : T(-2)) * apq / (den > T(0) ? den : T(1));
-                t = (den > T(0)) ? t : T(0);
-                const T c = tad_rsqrt(T(1) + t * t);
-                const T s = t * c;
-                A[Sym<n>::idx(p, p)] = app - t * apq;
-                A[Sym<n>::idx(q, q)] = aqq + t * apq;
-                A[Sym<n>::idx(p, q)] = T(0);
+                T cs[NP], sn[NP], tn[NP];
 #pragma unroll
-                for (int r = 0; r < n; ++r)
-                {
-                    if (r == p || r == q) continue;
-                    const int irp = Sym<n>::at(r, p), irq = Sym<n>::at(r, q);
-                    const T arp = A[irp], arq = A[irq];
-                    A[irp] = c * arp - s * arq;
-                    A[irq] = s * arp + c * arq;
-                }
+                for (int k = 0; k < Tournament<n>::per_round; ++k)
+                    jacobi_angle<n, T>(A, tour.p[r][k], tour.q[r][k], cs[k], sn[k], tn[k]);
 #pragma unroll
-                for (int r = 0; r < n; ++r)
-                {
-                    const T vrp = V[r * n + p], vrq = V[r * n + q];
-                    V[r * n + p] = c * vrp - s * vrq;
-                    V[r * n + q] = s * vrp + c * vrq;
-                }
+                for (int k = 0; k < Tournament<n>::per_round; ++k)
+                    jacobi_apply<n, T>(A, V, tour.p[r][k], tour.q[r][k], cs[k], sn[k], tn[k]);
             }
+        }
+        else
+        {
+#pragma unroll
+            for (int p = 0; p < n - 1; ++p)
+#pragma unroll
+                for (int q = p + 1; q < n; ++q)
+                {
+                    T c, s, t;
+                    jacobi_angle<n, T>(A, p, q, c, s, t);
+                    jacobi_apply<n, T>(A, V, p, q, c, s, t);
+                }
+        }
     }
 }
 
 // Projects packed symmetric A in place.  Returns true when A was modified.
-// keep_unchanged: when no eigenvalue needed clamping, A is left bit-identical
-// (the reference returns the input untouched in that case).
-template <int n, class T>
+// KeepUnchanged: when no eigenvalue needs clamping, A is left bit-identical
+// (the reference returns its input untouched then); this keeps a copy of A
+// alive through the eigensolve.  Without it the spectrum is re-assembled
+// whenever the Cholesky test failed (equal to A up to rounding when nothing
+// was clamped), which saves n(n+1)/2 registers during the eigensolve.
+template <int n, class T, bool KeepUnchanged = true>
 __device__ __forceinline__ bool project_psd(T* A, T eps)
 {
     const T shift = eps > T(0) ? eps : T(0);
     if (cholesky_pd<n, T>(A, shift))
         return false;
-    T W[Sym<n>::n];
+    T Wbuf[KeepUnchanged ? Sym<n>::n : 1];
+    T* W = A;
+    if constexpr (KeepUnchanged)
+    {
 #pragma unroll
-    for (int i = 0; i < Sym<n>::n; ++i) W[i] = A[i];
+        for (int i = 0; i < Sym<n>::n; ++i) Wbuf[i] = A[i];
+        W = Wbuf;
+    }
     T V[n * n];
     jacobi_eigen<n, T>(W, V);
     T lam[n];
@@ -156,8 +267,9 @@ __device__ __forceinline__ bool project_psd(T* A, T eps)
         }
         lam[i] = l;
     }
-    if (!changed)
-        return false;
+    if constexpr (KeepUnchanged)
+        if (!changed)
+            return false;
 #pragma unroll
     for (int i = 0; i < n; ++i)
 #pragma unroll

@@ -276,28 +276,57 @@ template <> struct Helmert<4>
     }
 };
 
-// Unprojected element Hessian in x coordinates, packed KX x KX:
+// Entry of the unprojected element Hessian in x coordinates:
 // H_X[(v,i),(u,i')] = w sum_{j,j'} D_vj D_uj' H_J[(i,j),(i',j')]
 template <int Dm, class T>
-__device__ __forceinline__ void symdir_hessian_x(const Dual<Dm * Dm, T>& E, const T (*Dmat)[Dm], T w, T* Hx)
+__device__ __forceinline__ T symdir_hx_entry(const Dual<Dm * Dm, T>& E, const T (*Dmat)[Dm], T w, int v, int i,
+                                             int u, int ip)
+{
+    T s = T(0);
+#pragma unroll
+    for (int j = 0; j < Dm; ++j)
+    {
+        T t = T(0);
+#pragma unroll
+        for (int jp = 0; jp < Dm; ++jp) t += E.h[Sym<Dm * Dm>::at(i * Dm + j, ip * Dm + jp)] * Dmat[u][jp];
+        s += Dmat[v][j] * t;
+    }
+    return w * s;
+}
+
+// positive_diagonally_dominant of the unprojected element Hessian, entries
+// computed on the fly (this branch is cold: it only runs for eps <= 0).
+template <int Dm, class T>
+__device__ __forceinline__ bool symdir_hx_pdd(const Dual<Dm * Dm, T>& E, const T (*Dmat)[Dm], T w, T eps)
 {
     constexpr int N = Dm + 1;
-    constexpr int KJ = Dm * Dm;
-    constexpr int KX = N * Dm;
-    // T1[(i,j)][u][i'] = sum_j' H_J[(i,j),(i',j')] D_uj'
-    T T1[KJ][N][Dm];
+    bool ok = true;
 #pragma unroll
-    for (int r = 0; r < KJ; ++r)
+    for (int v = 0; v < N; ++v)
 #pragma unroll
-        for (int u = 0; u < N; ++u)
+        for (int i = 0; i < Dm; ++i)
+        {
+            T off = T(0), dia = T(0);
 #pragma unroll
-            for (int ip = 0; ip < Dm; ++ip)
-            {
-                T s = T(0);
+            for (int u = 0; u < N; ++u)
 #pragma unroll
-                for (int jp = 0; jp < Dm; ++jp) s += E.h[Sym<KJ>::at(r, ip * Dm + jp)] * Dmat[u][jp];
-                T1[r][u][ip] = s;
-            }
+                for (int ip = 0; ip < Dm; ++ip)
+                {
+                    const T h = symdir_hx_entry<Dm, T>(E, Dmat, w, v, i, u, ip);
+                    if (u == v && ip == i) dia = h;
+                    else off += fabs(h);
+                }
+            ok = ok && !(dia < off + eps);
+        }
+    return ok;
+}
+
+// Writes the unprojected element Hessian as N(N+1)/2 upper d x d blocks.
+template <int Dm, class T>
+__device__ __forceinline__ void symdir_write_hx(const Dual<Dm * Dm, T>& E, const T (*Dmat)[Dm], T w, T* hrec,
+                                                bool& nonfinite)
+{
+    constexpr int N = Dm + 1;
 #pragma unroll
     for (int v = 0; v < N; ++v)
 #pragma unroll
@@ -307,11 +336,9 @@ __device__ __forceinline__ void symdir_hessian_x(const Dual<Dm * Dm, T>& E, cons
 #pragma unroll
                 for (int ip = 0; ip < Dm; ++ip)
                 {
-                    if (u == v && ip < i) continue;
-                    T s = T(0);
-#pragma unroll
-                    for (int j = 0; j < Dm; ++j) s += Dmat[v][j] * T1[i * Dm + j][u][ip];
-                    Hx[Sym<KX>::idx(v * Dm + i, u * Dm + ip)] = w * s;
+                    const T val = symdir_hx_entry<Dm, T>(E, Dmat, w, v, i, u, ip);
+                    hrec[block_index(N, v, u) * Dm * Dm + i * Dm + ip] = val;
+                    nonfinite = nonfinite || !finite_val(val);
                 }
 }
 
@@ -423,9 +450,7 @@ __device__ __forceinline__ double symdir_element(const double* X, const double* 
 
     if constexpr (MODE == MODE_HESS)
     {
-        T Hx[Sym<KX>::n];
-        symdir_hessian_x<Dm, T>(E, Dmat, w, Hx);
-        write_blocks<N, Dm, T>(Hx, hrec, nonfinite);
+        symdir_write_hx<Dm, T>(E, Dmat, w, hrec, nonfinite);
         return f;
     }
     else
@@ -434,15 +459,10 @@ __device__ __forceinline__ double symdir_element(const double* X, const double* 
         // cannot fire here: the translations are in the kernel of H, so
         // H_ii = -sum_{u != v} H_(v,i),(u,i) up to rounding and H_ii >=
         // sum |H_ij| + eps is impossible unless eps is below rounding level.
-        if (eps <= T(0))
+        if (eps <= T(0) && symdir_hx_pdd<Dm, T>(E, Dmat, w, eps))
         {
-            T Hx[Sym<KX>::n];
-            symdir_hessian_x<Dm, T>(E, Dmat, w, Hx);
-            if (positive_diagonally_dominant<KX, T>(Hx, eps))
-            {
-                write_blocks<N, Dm, T>(Hx, hrec, nonfinite);
-                return f;
-            }
+            symdir_write_hx<Dm, T>(E, Dmat, w, hrec, nonfinite);
+            return f;
         }
         // E_aj = sum_v U_va D_vj ((N-1) x Dm, square)
         T Em[Dm][Dm];
@@ -488,7 +508,7 @@ __device__ __forceinline__ double symdir_element(const double* X, const double* 
                         for (int j = 0; j < Dm; ++j) s += Em[a][j] * T1[i * Dm + j][b][ip];
                         Km[Sym<KJ>::idx(r1, r2)] = w * s;
                     }
-        project_psd<KJ, T>(Km, eps);
+        project_psd<KJ, T, false>(Km, eps);
         // H[(v,i),(u,i')] = sum_{a,b} U_va U_ub P[(a,i),(b,i')] + max(eps,0)/N delta_ii'
         const T ct = eps > T(0) ? eps / T(N) : T(0);
         T S1[N][Dm][KJ]; // S1[v][i][(b,i')] = sum_a U_va P[(a,i),(b,i')]

@@ -77,12 +77,14 @@ hipError_t launch_pcg_setup(int d, int64_t n_handles, const int32_t* brow_ptr, c
                             const int32_t* row_ptr, const double* H, double w, double* Dinv, hipStream_t s);
 
 hipError_t launch_pcg_init(int d, int64_t n_handles, const double* b, const double* Dinv, double* x, double* r,
-                           double* z, double* p, double* partials, PcgScalars* sc, hipStream_t s);
+                           double* z, double* partials, PcgScalars* sc, hipStream_t s);
 
+// One CG iteration k: SpMV with termination test and on-the-fly direction
+// update, then (with_update) the x / r / z update.  pbuf holds 2 * n values.
 hipError_t launch_pcg_iteration(int d, int64_t n_handles, const int32_t* brow_ptr, const int32_t* bcol,
                                 const int32_t* row_ptr, const double* H, double w, const double* Dinv, double* x,
-                                double* r, double* z, double* p, double* q, double* partials, PcgScalars* sc, int it,
-                                hipStream_t s, Timer* timer);
+                                double* r, double* z, double* pbuf, double* q, double* partials, PcgScalars* sc, int k,
+                                bool with_update, hipStream_t s, Timer* timer);
 
 // Size (doubles) of the partial-sum buffer the PCG kernels need.
 int64_t pcg_parts_size(int d, int64_t n_handles);

@@ -20,18 +20,19 @@ namespace tad {
 // ---------------------------------------------------------------------------
 // Deterministic reductions
 // ---------------------------------------------------------------------------
+// Fixed-order block reduction: butterfly inside each 64-lane wave, then the
+// four wave totals in order.  Same bits for the same inputs on every run.
 __device__ __forceinline__ double block_sum(double v, double* sh)
 {
-    sh[threadIdx.x] = v;
-    __syncthreads();
 #pragma unroll
-    for (int off = kBlock / 2; off > 0; off >>= 1)
-    {
-        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-        __syncthreads();
-    }
-    const double r = sh[0];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int wave = threadIdx.x >> 6;
     __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[wave] = v;
+    __syncthreads();
+    double r = sh[0];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) r += sh[w];
     return r;
 }
 
@@ -311,13 +312,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_setup(int64_t nh, const int32_t*
     for (int m = 0; m < D * D; ++m) Dinv[u * D * D + m] = o[m];
 }
 
-// x = 0, r = b, z = p = M^-1 r; partials r.z (slot 1) and r.r (rr0)
+// x = 0, r = b, z = M^-1 r; partials rho_0 = r.z (slot 1) and ||b||^2 (rr0)
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t nh, const double* __restrict__ b,
                                                      const double* __restrict__ Dinv, double* __restrict__ x,
                                                      double* __restrict__ r, double* __restrict__ z,
-                                                     double* __restrict__ p, double* __restrict__ parts, int Gs,
-                                                     int Gv, PcgScalars* sc)
+                                                     double* __restrict__ parts, int Gs, int Gv, PcgScalars* sc)
 {
     __shared__ double sh[kBlock];
     double rz = 0.0, rr = 0.0;
@@ -338,7 +338,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t nh, const double* _
 #pragma unroll
             for (int j = 0; j < D; ++j) zi += Dinv[u * D * D + i * D + j] * rv[j];
             z[u * D + i] = zi;
-            p[u * D + i] = zi;
             rz += rv[i] * zi;
             rr += rv[i] * rv[i];
         }
@@ -347,7 +346,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t nh, const double* _
     const double srr = block_sum(rr, sh);
     if (threadIdx.x == 0)
     {
-        parts[Gs + Gv + blockIdx.x] = srz;      // rz slot 1 (iteration -1)
+        parts[Gs + Gv + blockIdx.x] = srz;      // rho_0 in slot 1
         parts[Gs + 4 * Gv + blockIdx.x] = srr;  // rr0
         if (blockIdx.x == 0)
         {
@@ -358,26 +357,48 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t nh, const double* _
     }
 }
 
-// q = (w I + H) p; partial p.q.  Iteration 0 also records ||b||^2.
+// CG iteration k, first half.  p_k = z_k + beta_k p_{k-1} (p_0 = z_0) is formed
+// on the fly for every gathered vertex and written for the block's own rows
+// into P[k & 1]; q = (w I + H) p_k; partial p_k.q.  Before that, the
+// termination test on rr_k = ||r_k||^2 (k = 0: ||b||^2, which is recorded).
 template <int D, int LPB>
 __global__ __launch_bounds__(kBlock) void k_spmv(int64_t nh, const int32_t* __restrict__ brow_ptr,
                                                  const int32_t* __restrict__ bcol, const int32_t* __restrict__ row_ptr,
-                                                 const double* __restrict__ H, double w, const double* __restrict__ p,
-                                                 double* __restrict__ q, double* __restrict__ parts, int Gs, int Gv,
-                                                 int it, PcgScalars* sc)
+                                                 const double* __restrict__ H, double w, const double* __restrict__ z,
+                                                 double* __restrict__ pbuf, double* __restrict__ q,
+                                                 double* __restrict__ parts, int Gs, int Gv, int k, PcgScalars* sc)
 {
     if (block_done(sc)) return;
     __shared__ double sh[kBlock];
-    if (it == 0)
+    double rr, bb;
+    if (k == 0)
     {
-        const double bb = sum_parts(parts + Gs + 4 * Gv, Gv, sh);
-        if (blockIdx.x == 0 && threadIdx.x == 0)
-        {
-            sc->bb = bb;
-            sc->rr = bb;
-            if (bb == 0.0 || sc->max_iters <= 0) sc->done = 1;
-        }
+        bb = sum_parts(parts + Gs + 4 * Gv, Gv, sh);
+        rr = bb;
     }
+    else
+    {
+        rr = sum_parts(parts + Gs + 2 * Gv + ((k + 1) & 1) * Gv, Gv, sh);
+        bb = sc->bb;
+    }
+    const bool stop = rr <= sc->tol2 * bb || k >= sc->max_iters;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        if (k == 0) sc->bb = bb;
+        sc->rr = rr;
+        sc->iters = k;
+        if (stop) sc->done = 1;
+    }
+    if (stop) return;
+    double beta = 0.0;
+    if (k > 0)
+    {
+        const double rho = sum_parts(parts + Gs + ((k + 1) & 1) * Gv, Gv, sh);
+        const double rho_old = sum_parts(parts + Gs + (k & 1) * Gv, Gv, sh);
+        beta = rho / rho_old;
+    }
+    double* __restrict__ pk = pbuf + (int64_t)(k & 1) * nh * D;
+    const double* __restrict__ pold = pbuf + (int64_t)((k + 1) & 1) * nh * D;
     const int lane = threadIdx.x % LPB;
     constexpr int rows_per_block = kBlock / LPB;
     double pq = 0.0;
@@ -392,16 +413,16 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t nh, const int32_t* __re
             const int b0 = brow_ptr[u];
             const int nbu = brow_ptr[u + 1] - b0;
             const int64_t base = row_ptr[D * u];
-            for (int k = lane; k < nbu; k += LPB)
+            for (int kk = lane; kk < nbu; kk += LPB)
             {
-                const int64_t v = bcol[b0 + k];
+                const int64_t v = bcol[b0 + kk];
                 double pv[D];
 #pragma unroll
-                for (int j = 0; j < D; ++j) pv[j] = p[v * D + j];
+                for (int j = 0; j < D; ++j) pv[j] = k > 0 ? fma(beta, pold[v * D + j], z[v * D + j]) : z[v * D + j];
 #pragma unroll
                 for (int i = 0; i < D; ++i)
 #pragma unroll
-                    for (int j = 0; j < D; ++j) acc[i] += H[base + (int64_t)i * D * nbu + D * k + j] * pv[j];
+                    for (int j = 0; j < D; ++j) acc[i] += H[base + (int64_t)i * D * nbu + D * kk + j] * pv[j];
             }
         }
 #pragma unroll
@@ -413,7 +434,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t nh, const int32_t* __re
 #pragma unroll
             for (int i = 0; i < D; ++i)
             {
-                const double pi = p[u * D + i];
+                const double pi = k > 0 ? fma(beta, pold[u * D + i], z[u * D + i]) : z[u * D + i];
+                pk[u * D + i] = pi;
                 const double qi = acc[i] + w * pi;
                 q[u * D + i] = qi;
                 pq += pi * qi;
@@ -424,18 +446,19 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t nh, const int32_t* __re
     if (threadIdx.x == 0) parts[blockIdx.x] = s;
 }
 
-// alpha = rho / p.q; x += alpha p, r -= alpha q, z = M^-1 r; partials r.z, r.r (slot it & 1)
+// CG iteration k, second half: alpha = rho_k / p_k.q; x += alpha p_k,
+// r -= alpha q, z = M^-1 r; partials of rho_{k+1} = r.z and r.r into slot k & 1.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t nh, const double* __restrict__ Dinv,
                                                        double* __restrict__ x, double* __restrict__ r,
-                                                       double* __restrict__ z, const double* __restrict__ p,
+                                                       double* __restrict__ z, const double* __restrict__ pbuf,
                                                        const double* __restrict__ q, double* __restrict__ parts,
-                                                       int Gs, int Gv, int it, PcgScalars* sc)
+                                                       int Gs, int Gv, int k, PcgScalars* sc)
 {
     if (block_done(sc)) return;
     __shared__ double sh[kBlock];
     const double pq = sum_parts(parts, Gs, sh);
-    const double rho = sum_parts(parts + Gs + ((it + 1) & 1) * Gv, Gv, sh);
+    const double rho = sum_parts(parts + Gs + ((k + 1) & 1) * Gv, Gv, sh);
     if (!(pq > 0.0))
     {
         // not positive definite along p: the same decision in every block
@@ -443,11 +466,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t nh, const double*
         {
             sc->breakdown = 1;
             sc->done = 1;
-            sc->iters = it;
+            sc->iters = k;
         }
         return;
     }
     const double alpha = rho / pq;
+    const double* __restrict__ p = pbuf + (int64_t)(k & 1) * nh * D;
     double rz = 0.0, rr = 0.0;
     for (int64_t u = (int64_t)blockIdx.x * kBlock + threadIdx.x; u < nh; u += (int64_t)gridDim.x * kBlock)
     {
@@ -474,33 +498,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t nh, const double*
     const double srr = block_sum(rr, sh);
     if (threadIdx.x == 0)
     {
-        parts[Gs + (it & 1) * Gv + blockIdx.x] = srz;
-        parts[Gs + 2 * Gv + (it & 1) * Gv + blockIdx.x] = srr;
+        parts[Gs + (k & 1) * Gv + blockIdx.x] = srz;
+        parts[Gs + 2 * Gv + (k & 1) * Gv + blockIdx.x] = srr;
     }
-}
-
-// beta = rho_new / rho; p = z + beta p, or record termination
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_pcg_dir(int64_t nh, const double* __restrict__ z, double* __restrict__ p,
-                                                    const double* __restrict__ parts, int Gs, int Gv, int it,
-                                                    PcgScalars* sc)
-{
-    if (block_done(sc)) return;
-    __shared__ double sh[kBlock];
-    const double rho_new = sum_parts(parts + Gs + (it & 1) * Gv, Gv, sh);
-    const double rho = sum_parts(parts + Gs + ((it + 1) & 1) * Gv, Gv, sh);
-    const double rr = sum_parts(parts + Gs + 2 * Gv + (it & 1) * Gv, Gv, sh);
-    const bool stop = rr <= sc->tol2 * sc->bb || it + 1 >= sc->max_iters;
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-    {
-        sc->iters = it + 1;
-        sc->rr = rr;
-        if (stop) sc->done = 1;
-    }
-    if (stop) return;
-    const double beta = rho_new / rho;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nh * D; i += (int64_t)gridDim.x * kBlock)
-        p[i] = z[i] + beta * p[i];
 }
 
 void pcg_grids(int d, int64_t nh, int* Gs, int* Gv)
@@ -531,38 +531,34 @@ hipError_t launch_pcg_setup(int d, int64_t nh, const int32_t* brow_ptr, const in
 }
 
 hipError_t launch_pcg_init(int d, int64_t nh, const double* b, const double* Dinv, double* x, double* r, double* z,
-                           double* p, double* parts, PcgScalars* sc, hipStream_t s)
+                           double* parts, PcgScalars* sc, hipStream_t s)
 {
     int Gs, Gv;
     pcg_grids(d, nh, &Gs, &Gv);
-    if (d == 1) k_pcg_init<1><<<Gv, kBlock, 0, s>>>(nh, b, Dinv, x, r, z, p, parts, Gs, Gv, sc);
-    else if (d == 2) k_pcg_init<2><<<Gv, kBlock, 0, s>>>(nh, b, Dinv, x, r, z, p, parts, Gs, Gv, sc);
-    else k_pcg_init<3><<<Gv, kBlock, 0, s>>>(nh, b, Dinv, x, r, z, p, parts, Gs, Gv, sc);
+    if (d == 1) k_pcg_init<1><<<Gv, kBlock, 0, s>>>(nh, b, Dinv, x, r, z, parts, Gs, Gv, sc);
+    else if (d == 2) k_pcg_init<2><<<Gv, kBlock, 0, s>>>(nh, b, Dinv, x, r, z, parts, Gs, Gv, sc);
+    else k_pcg_init<3><<<Gv, kBlock, 0, s>>>(nh, b, Dinv, x, r, z, parts, Gs, Gv, sc);
     return hipGetLastError();
 }
 
 hipError_t launch_pcg_iteration(int d, int64_t nh, const int32_t* brow_ptr, const int32_t* bcol,
                                 const int32_t* row_ptr, const double* H, double w, const double* Dinv, double* x,
-                                double* r, double* z, double* p, double* q, double* parts, PcgScalars* sc, int it,
-                                hipStream_t s, Timer* timer)
+                                double* r, double* z, double* pbuf, double* q, double* parts, PcgScalars* sc, int k,
+                                bool with_update, hipStream_t s, Timer* timer)
 {
     int Gs, Gv;
     pcg_grids(d, nh, &Gs, &Gv);
     if (timer) timer->begin("spmv", s);
-    if (d == 1) k_spmv<1, 8><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, p, q, parts, Gs, Gv, it, sc);
-    else if (d == 2) k_spmv<2, 8><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, p, q, parts, Gs, Gv, it, sc);
-    else k_spmv<3, 16><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, p, q, parts, Gs, Gv, it, sc);
+    if (d == 1) k_spmv<1, 8><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, pbuf, q, parts, Gs, Gv, k, sc);
+    else if (d == 2) k_spmv<2, 8><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, pbuf, q, parts, Gs, Gv, k, sc);
+    else k_spmv<3, 16><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, pbuf, q, parts, Gs, Gv, k, sc);
     if (timer) timer->end("spmv", s);
+    if (!with_update) return hipGetLastError();
     if (timer) timer->begin("pcg_update", s);
-    if (d == 1) k_pcg_update<1><<<Gv, kBlock, 0, s>>>(nh, Dinv, x, r, z, p, q, parts, Gs, Gv, it, sc);
-    else if (d == 2) k_pcg_update<2><<<Gv, kBlock, 0, s>>>(nh, Dinv, x, r, z, p, q, parts, Gs, Gv, it, sc);
-    else k_pcg_update<3><<<Gv, kBlock, 0, s>>>(nh, Dinv, x, r, z, p, q, parts, Gs, Gv, it, sc);
+    if (d == 1) k_pcg_update<1><<<Gv, kBlock, 0, s>>>(nh, Dinv, x, r, z, pbuf, q, parts, Gs, Gv, k, sc);
+    else if (d == 2) k_pcg_update<2><<<Gv, kBlock, 0, s>>>(nh, Dinv, x, r, z, pbuf, q, parts, Gs, Gv, k, sc);
+    else k_pcg_update<3><<<Gv, kBlock, 0, s>>>(nh, Dinv, x, r, z, pbuf, q, parts, Gs, Gv, k, sc);
     if (timer) timer->end("pcg_update", s);
-    if (timer) timer->begin("pcg_dir", s);
-    if (d == 1) k_pcg_dir<1><<<Gv, kBlock, 0, s>>>(nh, z, p, parts, Gs, Gv, it, sc);
-    else if (d == 2) k_pcg_dir<2><<<Gv, kBlock, 0, s>>>(nh, z, p, parts, Gs, Gv, it, sc);
-    else k_pcg_dir<3><<<Gv, kBlock, 0, s>>>(nh, z, p, parts, Gs, Gv, it, sc);
-    if (timer) timer->end("pcg_dir", s);
     return hipGetLastError();
 }
 
