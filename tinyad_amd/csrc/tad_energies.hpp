@@ -294,33 +294,6 @@ __device__ __forceinline__ T symdir_hx_entry(const Dual<Dm * Dm, T>& E, const T 
     return w * s;
 }
 
-// positive_diagonally_dominant of the unprojected element Hessian, entries
-// computed on the fly (this branch is cold: it only runs for eps <= 0).
-template <int Dm, class T>
-__device__ __forceinline__ bool symdir_hx_pdd(const Dual<Dm * Dm, T>& E, const T (*Dmat)[Dm], T w, T eps)
-{
-    constexpr int N = Dm + 1;
-    bool ok = true;
-#pragma unroll
-    for (int v = 0; v < N; ++v)
-#pragma unroll
-        for (int i = 0; i < Dm; ++i)
-        {
-            T off = T(0), dia = T(0);
-#pragma unroll
-            for (int u = 0; u < N; ++u)
-#pragma unroll
-                for (int ip = 0; ip < Dm; ++ip)
-                {
-                    const T h = symdir_hx_entry<Dm, T>(E, Dmat, w, v, i, u, ip);
-                    if (u == v && ip == i) dia = h;
-                    else off += fabs(h);
-                }
-            ok = ok && !(dia < off + eps);
-        }
-    return ok;
-}
-
 // Writes the unprojected element Hessian as N(N+1)/2 upper d x d blocks.
 template <int Dm, class T>
 __device__ __forceinline__ void symdir_write_hx(const Dual<Dm * Dm, T>& E, const T (*Dmat)[Dm], T w, T* hrec,
@@ -360,6 +333,40 @@ __device__ __forceinline__ void write_blocks(const T* Hp, T* hrec, bool& nonfini
                     hrec[block_index(N, v, u) * Dm * Dm + i * Dm + ip] = val;
                     nonfinite = nonfinite || !finite_val(val);
                 }
+}
+
+// positive_diagonally_dominant (HessianProjection.hh:23-42) of H = Q K Q^T,
+// entries H[(v,i),(u,i')] = sum_{a,b} U_va U_ub K[(a,i),(b,i')] formed on the fly.
+template <int N, int Dm, class T>
+__device__ __forceinline__ bool deflated_pdd(const T* Km, T eps)
+{
+    constexpr int KJ = Dm * Dm;
+    bool ok = true;
+#pragma unroll
+    for (int v = 0; v < N; ++v)
+#pragma unroll
+        for (int i = 0; i < Dm; ++i)
+        {
+            T off = T(0), dia = T(0);
+#pragma unroll
+            for (int u = 0; u < N; ++u)
+#pragma unroll
+                for (int ip = 0; ip < Dm; ++ip)
+                {
+                    T h = T(0);
+#pragma unroll
+                    for (int a = 0; a < Dm; ++a)
+#pragma unroll
+                        for (int b = 0; b < Dm; ++b)
+                            if (Helmert<N>::u(v, a) != 0.0 && Helmert<N>::u(u, b) != 0.0)
+                                h += T(Helmert<N>::u(v, a) * Helmert<N>::u(u, b)) *
+                                     Km[Sym<KJ>::at(a * Dm + i, b * Dm + ip)];
+                    if (u == v && ip == i) dia = h;
+                    else off += fabs(h);
+                }
+            ok = ok && !(dia < off + eps);
+        }
+    return ok;
 }
 
 // Second-order evaluation of one symmetric-Dirichlet simplex.
@@ -459,11 +466,6 @@ __device__ __forceinline__ double symdir_element(const double* X, const double* 
         // cannot fire here: the translations are in the kernel of H, so
         // H_ii = -sum_{u != v} H_(v,i),(u,i) up to rounding and H_ii >=
         // sum |H_ij| + eps is impossible unless eps is below rounding level.
-        if (eps <= T(0) && symdir_hx_pdd<Dm, T>(E, Dmat, w, eps))
-        {
-            symdir_write_hx<Dm, T>(E, Dmat, w, hrec, nonfinite);
-            return f;
-        }
         // E_aj = sum_v U_va D_vj ((N-1) x Dm, square)
         T Em[Dm][Dm];
 #pragma unroll
@@ -508,7 +510,11 @@ __device__ __forceinline__ double symdir_element(const double* X, const double* 
                         for (int j = 0; j < Dm; ++j) s += Em[a][j] * T1[i * Dm + j][b][ip];
                         Km[Sym<KJ>::idx(r1, r2)] = w * s;
                     }
-        project_psd<KJ, T, false>(Km, eps);
+        // Diagonal-dominance early out of the reference on H = Q K Q^T (exact
+        // when the translations are in the kernel of H); it leaves H
+        // unprojected.  Only evaluated for eps <= 0, see above.
+        const bool pdd = (eps <= T(0)) && deflated_pdd<N, Dm, T>(Km, eps);
+        if (!pdd) project_psd<KJ, T, false>(Km, eps);
         // H[(v,i),(u,i')] = sum_{a,b} U_va U_ub P[(a,i),(b,i')] + max(eps,0)/N delta_ii'
         const T ct = eps > T(0) ? eps / T(N) : T(0);
         T S1[N][Dm][KJ]; // S1[v][i][(b,i')] = sum_a U_va P[(a,i),(b,i')]
