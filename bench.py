@@ -43,10 +43,16 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--workload", choices=["twist_cube", "disk_batch"], default="twist_cube")
     ap.add_argument("--m", type=int, default=60, help="cube cells per side")
+    ap.add_argument("--disks", type=int, default=256, help="disks in the disk_batch workload")
+    ap.add_argument("--autotune", action="store_true", help="pick the element-kernel launch shape first")
     ap.add_argument("--pcg-tol", type=float, default=1e-10)
     ap.add_argument("--cpu-sample", type=int, default=60000, help="tets in the CPU baseline sample (0: skip)")
     ap.add_argument("--no-newton", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="first run 3 device-to-device copies of this many MB (a plain copy kernel of known "
+                         "size, used to scale the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -60,12 +66,30 @@ def main():
     import tinyad_amd as tad
     from tinyad_amd import configs, _lib
 
+    if args.copy_calibration_mb:
+        nel = args.copy_calibration_mb * (1 << 20) // 8
+        src = torch.ones(nel, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration: 3 copies of {args.copy_calibration_mb} MiB")
+        del src, dst
+
     t_setup = time.time()
-    P = configs.twist_cube(m=args.m, seed=7)
+    seg_off = None
+    if args.workload == "disk_batch":
+        disks = configs.disk_batch(count=args.disks, seed=3)
+        P, seg_off = configs.concat(disks, name=f"disk_batch_{args.disks}")
+        args.no_newton = True
+    else:
+        P = configs.twist_cube(m=args.m, seed=7)
     n_tets = P.terms[0].conn.shape[0]
-    log(f"[bench] {P.name}: {n_tets} tets, {P.n_handles} vertices, precision {args.precision}")
+    log(f"[bench] {P.name}: {n_tets} elements, {P.n_handles} vertices, precision {args.precision}")
     ctx = tad.Context(0)
     fn = tad.ScalarFunction.from_problem(P, ctx, precision=args.precision)
+    if seg_off is not None:
+        fn.set_segments(seg_off)
     nnz = fn.nnz
     n = fn.n_vars
     dev = torch.device("cuda", 0)
@@ -81,6 +105,10 @@ def main():
         return f.value
 
     log(f"[bench] setup {time.time() - t_setup:.1f}s, nnz {nnz}")
+    tune_report = None
+    if args.autotune:
+        tune_report = fn.autotune(x0)
+        log("[bench] autotune:\n" + tune_report)
     for _ in range(args.warmup):
         eval_hp(x0)
     ctx.synchronize()
@@ -165,8 +193,11 @@ def main():
         roof = {"kernel": "k_spmv<3,16>", "bound": "hbm", "achieved": traffic_alg / avg_s / 1e9, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "bytes_per_launch": traffic_alg}
     elif dom == "element":
-        per_launch = n_tets * (4 * 4 + 10 * 8) + 8 * n
-        roof = {"kernel": "k_element<SymdirElem<3>>", "bound": "hbm", "achieved": per_launch / avg_s / 1e9,
+        Nv = P.terms[0].valence
+        r = P.terms[0].params.shape[1]
+        per_launch = n_tets * (4 * Nv + r * 8) + 8 * n
+        kname = "k_element<SymdirElem<3>>" if Nv == 4 else "k_element<SymdirElem<2>>"
+        roof = {"kernel": kname, "bound": "hbm", "achieved": per_launch / avg_s / 1e9,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "bytes_per_launch": per_launch,
                 "note": "VALU-bound kernel; HBM figure from algorithmic input bytes"}
     else:
@@ -180,7 +211,7 @@ def main():
 
     # ---------------- CPU baseline (oracle on a bounded sample) ----------------
     cpu = None
-    if args.cpu_sample > 0:
+    if args.cpu_sample > 0 and args.workload == "twist_cube":
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
         S = min(args.cpu_sample, n_tets)
@@ -208,9 +239,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32-element/f64-reduce",
         "data": "synthetic",
-        "config": {"workload": f"twist_cube m={args.m}: {n_tets} tets, {P.n_handles} vertices, n={n}, nnz={nnz}, "
-                               "symmetric Dirichlet + pins, eval_with_hessian_proj (eps 1e-9)",
-                   "precision": args.precision, "pcg_tol": args.pcg_tol},
+        "config": {"workload": (f"twist_cube m={args.m}: {n_tets} tets, {P.n_handles} vertices, n={n}, nnz={nnz}, "
+                                "symmetric Dirichlet + pins, eval_with_hessian_proj (eps 1e-9)")
+                   if args.workload == "twist_cube" else
+                   (f"disk_batch: {args.disks} radius-clipped height-field disks, {n_tets} triangles, "
+                    f"{P.n_handles} vertices, nnz={nnz}, one launch per term, per-disk energies"),
+                   "precision": args.precision, "pcg_tol": args.pcg_tol,
+                   "autotuned": bool(args.autotune)},
         "roofline": roof,
         "cpu_baseline": cpu,
         "newton": newton,

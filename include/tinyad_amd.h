@@ -155,6 +155,45 @@ tad_status tad_fn_eval_with_derivatives(tad_fn fn, const double* x, double* f, d
 tad_status tad_fn_eval_with_hessian_proj(tad_fn fn, const double* x, double eps, double* f, double* g,
                                          double* H_values, tad_mem mem);
 
+/* --- batches of independent problems ---------------------------------------- */
+
+/* Declares consecutive handle ranges [handle_offsets[s], handle_offsets[s+1])
+ * as independent problems (e.g. many meshes evaluated in one launch).  Every
+ * element must lie inside one segment and, within each term, elements must be
+ * ordered by segment.  The reference has no counterpart: it evaluates one
+ * ScalarFunction per mesh. */
+tad_status tad_fn_set_segments(tad_fn fn, int64_t n_segments, const int64_t* handle_offsets);
+
+/* Per-segment energies of the last evaluation (n_segments values). */
+tad_status tad_fn_segment_energies(tad_fn fn, double* f, tad_mem mem);
+
+/* Replaces the parameter rows of term `term` (n_elem x params_per_elem,
+ * row-major, same layout as add_elements); the sparsity pattern is kept.
+ * Counterpart of changing the data an add_elements lambda captures by
+ * reference (ScalarFunction.hh:60-77: the element lambda reads it on every
+ * evaluation). */
+tad_status tad_fn_set_params(tad_fn fn, int term, const double* params, tad_mem mem);
+
+/* Rest-shape mixed products for implicit differentiation.  For a
+ * TAD_E_SYMDIR_TET term whose parameters are derived from per-vertex rest
+ * positions theta (same layout as x) as Mr(theta)^-1 and
+ * w_e = w_scale det(Mr_e(theta)) / 6, writes
+ *   out = sum_e (d2 E_e / dtheta dx)^T lambda        (n_vars values)
+ * at x.  Inverted elements (det M <= 0, energy +inf) contribute nothing.
+ * No kernel counterpart in the reference; its semantics are those of the
+ * truncated mixed Hessian block Scalar<24, T, true, 0, 12, 12, 12>
+ * (Scalar.hh:24-47, :188-196) contracted with lambda. */
+tad_status tad_fn_rest_vjp(tad_fn fn, int term, const double* x, const double* rest, const double* lambda,
+                           double w_scale, double* out, tad_mem mem);
+
+/* Times launch shapes of the projected-Hessian element kernel on x for every
+ * symmetric-Dirichlet term and keeps the fastest: the plain grid (one tile of
+ * 256 elements per workgroup) or a persistent grid of 64 / 128 / 256 lanes
+ * with 1, 2, 4 or 8 workgroups per CU and a strided or per-XCD tile order.
+ * Results are bit-identical for every shape.  report (optional) receives a
+ * text table of the timings.  No counterpart in the reference. */
+tad_status tad_fn_autotune(tad_fn fn, const double* x, tad_mem mem, double eps, char* report, size_t report_len);
+
 /* Device pointers of the function's own buffers holding the gradient and CSR
  * values of the last second-order evaluation (valid until the next call). */
 tad_status tad_fn_device_buffers(tad_fn fn, const double** g_dev, const double** H_values_dev);
@@ -187,7 +226,8 @@ tad_status tad_fn_solve(tad_fn fn, const double* H_values, const double* rhs, do
 
 typedef struct tad_newton_opts
 {
-    double projection_eps;  /* eval_with_hessian_proj eps (default 1e-9)                */
+    double projection_eps;  /* eval_with_hessian_proj eps (default 1e-9); NaN: unprojected */
+                            /* Hessian (eval_with_derivatives, ScalarFunctionImpl.hh:333)  */
     tad_solve_opts solve;   /* Newton direction solve                                   */
     double ls_s_max;        /* line_search defaults (LineSearch.hh:33-36):               */
     double ls_shrink;       /*   s_max 1, shrink 0.8, 64 trials, Armijo c 1e-4          */
@@ -202,7 +242,7 @@ typedef struct tad_newton_stats
     double f_new;           /* energy at the returned x                                */
     double step;            /* accepted step size (0 if the line search failed)         */
     double newton_decrement;/* -0.5 d^T g (NewtonDecrement.hh:20-26)                     */
-    double grad_max;        /* max |g| at the input x                                  */
+    double grad_norm;       /* ||g||_2 at the input x                                  */
     int cg_iters;
     int ls_trials;          /* energies evaluated in the line search                    */
     int ls_failed;          /* 1: no step accepted, x unchanged (LineSearch.hh:62-64)   */

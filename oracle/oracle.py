@@ -54,6 +54,7 @@ def lib():
         L.or_line_search.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, _dp, ctypes.c_double,
                                      ctypes.c_double, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
         L.or_dual_known_answer.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp]
+        L.or_tet_rest_mixed_vjp.argtypes = [ctypes.c_long, _ip, _dp, _dp, _dp, ctypes.c_double, ctypes.c_long, _dp]
         _lib = L
     return _lib
 
@@ -188,3 +189,15 @@ def dual_known_answer(which: int, x: float, y: float):
     out = np.zeros(7)
     lib().or_dual_known_answer(which, x, y, _d(out))
     return out[0], out[1:3].copy(), out[3:].reshape(2, 2).copy()
+
+
+def tet_rest_mixed_vjp(conn, x, rest, lam, w_scale):
+    """sum_e (d2E_e / dx dtheta)^T lambda_e for symmetric Dirichlet tets with
+    w_e = w_scale det(Mr_e(theta)) / 6 (TinyAD-style truncated Hessian block)."""
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    rest = np.ascontiguousarray(rest, dtype=np.float64)
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    out = np.zeros_like(x)
+    lib().or_tet_rest_mixed_vjp(conn.shape[0], _i(conn), _d(x), _d(rest), _d(lam), float(w_scale), x.size, _d(out))
+    return out

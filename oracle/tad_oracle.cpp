@@ -857,4 +857,87 @@ void or_dual_known_answer(int which, double xv, double yv, double* out)
     out[3] = f.hess[0][0]; out[4] = f.hess[0][1]; out[5] = f.hess[1][0]; out[6] = f.hess[1][1];
 }
 
+// Mixed-derivative VJP of the symmetric Dirichlet tet energy with respect to
+// the rest positions theta (per vertex, same AoS layout as x), as TinyAD would
+// compute it: a second-order scalar over the 24 variables (x_e, theta_e) whose
+// Hessian block rows 0..11 / columns 12..23 is d2E / dx dtheta (the truncated
+// Hessian block of Scalar<24, T, true, 0, 12, 12, 12>, Scalar.hh:24-47,
+// :188-196, pinned to the full block by ScalarTestHessianBlock.cc:50-100).
+// E_e = w_scale det(Mr(theta)) / 6 (||J||^2 + ||J^-1||^2), J = M(x) Mr(theta)^-1.
+// out[3 v + i] = sum_e sum_k (d2E_e / dx_k dtheta_{v,i}) lambda_k, elements in order.
+int or_tet_rest_mixed_vjp(long n_tets, const int* conn, const double* x, const double* rest, const double* lambda,
+                          double w_scale, long n_vars, double* out)
+{
+    for (long i = 0; i < n_vars; ++i) out[i] = 0.0;
+    std::vector<double> contrib((size_t)n_tets * 12);
+    #pragma omp parallel for schedule(static)
+    for (long e = 0; e < n_tets; ++e)
+    {
+        using D = Dual<24>;
+        D X[12], R[12];
+        for (int a = 0; a < 4; ++a)
+            for (int i = 0; i < 3; ++i)
+            {
+                X[3 * a + i] = D(x[3 * conn[4 * e + a] + i], 3 * a + i);
+                R[3 * a + i] = D(rest[3 * conn[4 * e + a] + i], 12 + 3 * a + i);
+            }
+        D M[3][3], Mr[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < 3; ++k)
+            {
+                M[i][k] = X[3 * (k + 1) + i] - X[i];
+                Mr[i][k] = R[3 * (k + 1) + i] - R[i];
+            }
+        const D detM = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1])
+                     - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0])
+                     + M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+        double* c = &contrib[(size_t)e * 12];
+        if (detM.val <= 0.0)
+        {
+            for (int k = 0; k < 12; ++k) c[k] = 0.0;
+            continue;
+        }
+        // Mr^-1 via cofactors
+        const D r00 = Mr[1][1] * Mr[2][2] - Mr[1][2] * Mr[2][1];
+        const D r01 = Mr[1][2] * Mr[2][0] - Mr[1][0] * Mr[2][2];
+        const D r02 = Mr[1][0] * Mr[2][1] - Mr[1][1] * Mr[2][0];
+        const D detR = Mr[0][0] * r00 + Mr[0][1] * r01 + Mr[0][2] * r02;
+        const D idR = 1.0 / detR;
+        D Ri[3][3];
+        Ri[0][0] = r00 * idR; Ri[1][0] = r01 * idR; Ri[2][0] = r02 * idR;
+        Ri[0][1] = (Mr[0][2] * Mr[2][1] - Mr[0][1] * Mr[2][2]) * idR;
+        Ri[1][1] = (Mr[0][0] * Mr[2][2] - Mr[0][2] * Mr[2][0]) * idR;
+        Ri[2][1] = (Mr[0][1] * Mr[2][0] - Mr[0][0] * Mr[2][1]) * idR;
+        Ri[0][2] = (Mr[0][1] * Mr[1][2] - Mr[0][2] * Mr[1][1]) * idR;
+        Ri[1][2] = (Mr[0][2] * Mr[1][0] - Mr[0][0] * Mr[1][2]) * idR;
+        Ri[2][2] = (Mr[0][0] * Mr[1][1] - Mr[0][1] * Mr[1][0]) * idR;
+        D J[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) J[i][j] = M[i][0] * Ri[0][j] + M[i][1] * Ri[1][j] + M[i][2] * Ri[2][j];
+        const D C00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        const D C01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        const D C02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        const D detJ = J[0][0] * C00 + J[0][1] * C01 + J[0][2] * C02;
+        const D inv = 1.0 / detJ;
+        const D Ci[9] = { C00, C01, C02,
+                          J[0][2] * J[2][1] - J[0][1] * J[2][2], J[0][0] * J[2][2] - J[0][2] * J[2][0],
+                          J[0][1] * J[2][0] - J[0][0] * J[2][1], J[0][1] * J[1][2] - J[0][2] * J[1][1],
+                          J[0][2] * J[1][0] - J[0][0] * J[1][2], J[0][0] * J[1][1] - J[0][1] * J[1][0] };
+        D nI = sqr(Ci[0] * inv);
+        for (int t = 1; t < 9; ++t) nI = nI + sqr(Ci[t] * inv);
+        D nJ = sqr(J[0][0]);
+        for (int t = 1; t < 9; ++t) nJ = nJ + sqr(J[t / 3][t % 3]);
+        const D E = (w_scale / 6.0) * detR * (nJ + nI);
+        for (int k = 0; k < 12; ++k)
+        {
+            double acc = 0.0;
+            for (int r = 0; r < 12; ++r) acc += E.hess[r][12 + k] * lambda[3 * conn[4 * e + r / 3] + r % 3];
+            c[k] = acc;
+        }
+    }
+    for (long e = 0; e < n_tets; ++e)
+        for (int k = 0; k < 12; ++k) out[3 * conn[4 * e + k / 3] + k % 3] += contrib[(size_t)e * 12 + k];
+    return 0;
+}
+
 } // extern "C"

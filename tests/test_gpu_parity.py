@@ -346,3 +346,37 @@ def test_newton_matches_oracle_trajectory_grid(ctx):
     fo_end = fo.eval(xo)
     fg_end = fg.eval(xg)
     assert abs(fg_end - fo_end) <= 1e-8 * abs(fo_end)
+
+
+# ---------------------------------------------------------------------------
+# Batches of independent meshes, autotuned launch shapes
+# ---------------------------------------------------------------------------
+
+def test_segments_match_per_mesh_oracle(ctx):
+    disks = configs.disk_batch(count=5, seed=9, n_min=8, n_max=14)
+    P, off = configs.concat(disks)
+    fg = tad.ScalarFunction.from_problem(P, ctx, precision="fp64")
+    fg.set_segments(off)
+    f, g, H = fg.eval_with_hessian_proj(P.x0)
+    fs = fg.segment_energies()
+    for k, D in enumerate(disks):
+        fo = oracle.Function.from_problem(D)
+        f1, g1, H1 = fo.eval_with_hessian_proj(D.x0)
+        assert abs(fs[k] - f1) <= 1e-12 * abs(f1)
+        sl = slice(2 * off[k], 2 * off[k + 1])
+        assert nrel(g[sl], g1) <= FP64_TOL
+        assert nrel(H[sl, sl].toarray(), H1.toarray()) <= FP64_TOL
+    assert abs(fs.sum() - f) <= 1e-12 * abs(f)
+    fe = fg.eval(P.x0)
+    assert np.array_equal(fg.segment_energies(), fs) and fe == f
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_autotune_keeps_bits(ctx, precision):
+    P = perturbed_cube(6, 0.15, 31)
+    fg = tad.ScalarFunction.from_problem(P, ctx, precision=precision)
+    f1, g1, H1 = fg.eval_with_hessian_proj(P.x0)
+    rep = fg.autotune(P.x0)
+    assert "best" in rep
+    f2, g2, H2 = fg.eval_with_hessian_proj(P.x0)
+    assert f1 == f2 and np.array_equal(g1, g2) and np.array_equal(H1.data, H2.data)

@@ -385,3 +385,79 @@ def test_line_search_armijo_and_failure():
     # an ascent direction never satisfies Armijo: x0 comes back after 64 trials
     xn, fn_, s, n = fo.line_search(P.x0, g, f, g)
     assert n == -64 and np.array_equal(xn, P.x0) and s == 0.0
+
+
+# ---------------------------------------------------------------------------
+# Rest-shape mixed products (adjoint reference)
+# ---------------------------------------------------------------------------
+
+def _perturbed_rest_problem(m=3, seed=21):
+    P = configs.twist_cube(m=m)
+    h = P.info["h"]
+    theta = P.info["rest"] + 0.05 * h * configs.normal(seed, P.n_vars, stream=6)
+    lam = configs.normal(seed + 1, P.n_vars, stream=7)
+    return P, theta, lam
+
+
+def _torch_tet_energy(x, theta, T, w_scale):
+    import torch
+    X = x.reshape(-1, 3)
+    R = theta.reshape(-1, 3)
+    M = torch.stack([X[T[:, k]] - X[T[:, 0]] for k in (1, 2, 3)], dim=2)
+    Mr = torch.stack([R[T[:, k]] - R[T[:, 0]] for k in (1, 2, 3)], dim=2)
+    J = M @ torch.linalg.inv(Mr)
+    w = w_scale * torch.linalg.det(Mr) / 6.0
+    return (w * ((J * J).sum((1, 2)) + (torch.linalg.inv(J) ** 2).sum((1, 2)))).sum()
+
+
+def test_rest_mixed_vjp_matches_torch_double_backward():
+    import torch
+    P, theta, lam = _perturbed_rest_problem()
+    T = P.terms[0].conn
+    ref = oracle.tet_rest_mixed_vjp(T, P.x0, theta, lam, 0.7)
+    xt = torch.tensor(P.x0, dtype=torch.float64, requires_grad=True)
+    tt = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
+    E = _torch_tet_energy(xt, tt, torch.tensor(T, dtype=torch.long), 0.7)
+    (gx,) = torch.autograd.grad(E, xt, create_graph=True)
+    (v,) = torch.autograd.grad(gx @ torch.tensor(lam), tt)
+    assert np.abs(ref - v.numpy()).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_rest_mixed_vjp_matches_fd_of_gradient():
+    """d/dtheta (grad_x E . lambda) by central differences of the oracle gradient."""
+    from tinyad_amd.adjoint import tet_rest_params
+    P, theta, lam = _perturbed_rest_problem()
+    T = P.terms[0].conn
+    ws = 0.7
+    ref = oracle.tet_rest_mixed_vjp(T, P.x0, theta, lam, ws)
+
+    def phi(th):
+        fo = oracle.Function(3, P.n_handles)
+        fo.add_elements(configs.E_SYMDIR_TET, T, tet_rest_params(th, T, ws))
+        return fo.eval_with_gradient(P.x0)[1] @ lam
+
+    eps = 1e-6
+    for k in range(4):
+        v = configs.normal(100 + k, P.n_vars, stream=8)
+        v /= np.linalg.norm(v)
+        fd = (phi(theta + eps * v) - phi(theta - eps * v)) / (2 * eps)
+        assert abs(fd - ref @ v) <= 1e-7 * np.abs(ref).sum() * np.abs(v).max()
+
+
+def test_rest_mixed_vjp_inverted_and_translation():
+    P, theta, lam = _perturbed_rest_problem()
+    T = P.terms[0].conn
+    x = P.x0.copy().reshape(-1, 3)
+    # flip one vertex through the opposite face plane: the elements it inverts add nothing
+    v0 = T[0, 0]
+    x[v0] = 2 * x[T[0, 1:]].mean(axis=0) - x[v0]
+    x = x.ravel()
+    X = x.reshape(-1, 3)
+    det = np.linalg.det(np.stack([X[T[:, k]] - X[T[:, 0]] for k in (1, 2, 3)], axis=2))
+    ok = det > 0
+    assert not ok.all()
+    full = oracle.tet_rest_mixed_vjp(T, x, theta, lam, 1.0)
+    part = oracle.tet_rest_mixed_vjp(T[ok], x, theta, lam, 1.0)
+    assert np.array_equal(full, part)
+    # the energy depends on theta through edge vectors only: a rest translation is free
+    assert np.abs(full.reshape(-1, 3).sum(axis=0)).max() <= 1e-12 * np.abs(full).sum()

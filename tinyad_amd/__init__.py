@@ -281,6 +281,61 @@ class ScalarFunction:
         check(lib().tad_fn_eval_with_hessian_proj(self._h, ptr, float(eps), ctypes.byref(f), gp, hp, mem))
         return f.value, g, self._wrap_h(H, mem)
 
+    # -- parameters and rest-shape products ---------------------------------
+    def set_params(self, term: int, params):
+        """Replaces the parameter rows of term `term` (the pattern is kept)."""
+        if _is_torch(params):
+            import torch
+            params = params.detach().to(torch.float64).contiguous()
+            _order_after_torch(self.ctx, params.device)
+            check(lib().tad_fn_set_params(self._h, int(term), params.data_ptr(), _lib.MEM_DEVICE))
+            return
+        params = np.ascontiguousarray(params, dtype=np.float64)
+        check(lib().tad_fn_set_params(self._h, int(term), params.ctypes.data, _lib.MEM_HOST))
+
+    def rest_vjp(self, term: int, x, rest, lam, w_scale: float):
+        """sum_e (d2E_e / dtheta dx)^T lam for a symmetric-Dirichlet tet term whose
+        parameters come from rest positions theta (w_e = w_scale det(Mr_e) / 6)."""
+        if _is_torch(x):
+            import torch
+            xs = [self._x(x)[0]]
+            for a in (rest, lam):
+                if not (_is_torch(a) and a.dtype == torch.float64 and a.is_contiguous() and a.numel() == self.n_vars):
+                    raise TadError(1, "rest and lam must be contiguous float64 CUDA tensors like x")
+            out = torch.empty_like(x)
+            check(lib().tad_fn_rest_vjp(self._h, int(term), xs[0], rest.data_ptr(), lam.data_ptr(), float(w_scale),
+                                        out.data_ptr(), _lib.MEM_DEVICE))
+            return out
+        ptr, _ = self._x(x)
+        rest = np.ascontiguousarray(rest, dtype=np.float64)
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        if rest.size != self.n_vars or lam.size != self.n_vars:
+            raise TadError(1, "rest and lam must have n_vars entries")
+        out = np.zeros(self.n_vars)
+        check(lib().tad_fn_rest_vjp(self._h, int(term), ptr, rest.ctypes.data, lam.ctypes.data, float(w_scale),
+                                    out.ctypes.data, _lib.MEM_HOST))
+        return out
+
+    # -- batches of independent problems ----------------------------------
+    def set_segments(self, handle_offsets):
+        """Declares handle ranges [off[s], off[s+1]) as independent problems."""
+        off = np.ascontiguousarray(handle_offsets, dtype=np.int64)
+        check(lib().tad_fn_set_segments(self._h, off.size - 1, off.ctypes.data))
+        self.n_segments = off.size - 1
+
+    def segment_energies(self) -> np.ndarray:
+        """Per-segment energies of the last evaluation."""
+        f = np.zeros(self.n_segments)
+        check(lib().tad_fn_segment_energies(self._h, f.ctypes.data, _lib.MEM_HOST))
+        return f
+
+    def autotune(self, x, eps: float = DEFAULT_HESSIAN_PROJECTION_EPS) -> str:
+        """Picks the fastest element-kernel launch shape on x; returns the timing table."""
+        ptr, mem = self._x(x)
+        buf = ctypes.create_string_buffer(16384)
+        check(lib().tad_fn_autotune(self._h, ptr, mem, float(eps), buf, len(buf)))
+        return buf.value.decode()
+
     # -- solver ------------------------------------------------------------
     def solve(self, rhs, H=None, **opts):
         """Solves (w I + H) sol = rhs with the library's PCG; H=None uses the last evaluated Hessian."""

@@ -26,6 +26,8 @@ EXPORTED = [
     "tad_fn_pattern", "tad_fn_eval", "tad_fn_eval_with_gradient", "tad_fn_eval_with_derivatives",
     "tad_fn_eval_with_hessian_proj", "tad_fn_device_buffers",
     "tad_solve_default_opts", "tad_fn_solve", "tad_newton_default_opts", "tad_newton_step", "tad_line_search",
+    "tad_fn_set_segments", "tad_fn_segment_energies", "tad_fn_autotune",
+    "tad_fn_set_params", "tad_fn_rest_vjp",
     "tad_selftest_dual",
 ]
 
@@ -53,7 +55,7 @@ class NewtonOpts(ctypes.Structure):
 
 class NewtonStats(ctypes.Structure):
     _fields_ = [("f", ctypes.c_double), ("f_new", ctypes.c_double), ("step", ctypes.c_double),
-                ("newton_decrement", ctypes.c_double), ("grad_max", ctypes.c_double), ("cg_iters", ctypes.c_int),
+                ("newton_decrement", ctypes.c_double), ("grad_norm", ctypes.c_double), ("cg_iters", ctypes.c_int),
                 ("ls_trials", ctypes.c_int), ("ls_failed", ctypes.c_int), ("solve_converged", ctypes.c_int)]
 
     def as_dict(self):
@@ -111,6 +113,11 @@ def lib():
         "tad_newton_step": [vp, vp, i32, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonStats)],
         "tad_line_search": [vp, vp, vp, dbl, vp, vp, i32, ctypes.POINTER(NewtonOpts), pdbl, pdbl,
                             ctypes.POINTER(ctypes.c_int)],
+        "tad_fn_set_segments": [vp, i64, vp],
+        "tad_fn_segment_energies": [vp, vp, i32],
+        "tad_fn_autotune": [vp, vp, i32, dbl, ctypes.c_char_p, ctypes.c_size_t],
+        "tad_fn_set_params": [vp, i32, vp, i32],
+        "tad_fn_rest_vjp": [vp, i32, vp, vp, vp, dbl, vp, i32],
         "tad_selftest_dual": [vp, i32, dbl, dbl, pdbl],
     }
     for name, args in sig.items():

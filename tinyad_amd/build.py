@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(HERE, "libtinyad_amd.so")
-SOURCES = ["tad_element.hip", "tad_energy.hip", "tad_linalg.hip", "tad_api.cpp"]
+SOURCES = ["tad_element.hip", "tad_energy.hip", "tad_linalg.hip", "tad_adjoint.hip", "tad_api.cpp"]
 HEADERS = ["tad_dual.hpp", "tad_eigen.hpp", "tad_energies.hpp", "tad_elements.hpp", "tad_internal.hpp"]
 ARCH = os.environ.get("TAD_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",

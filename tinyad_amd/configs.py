@@ -293,3 +293,21 @@ def min_signed_volume(problem: Problem, x: np.ndarray | None = None) -> float:
             M = np.stack([X[t.conn[:, k]] - a for k in range(1, t.valence)], axis=2)
             out = min(out, float(np.linalg.det(M).min()))
     return out
+
+
+def concat(problems, name: str = "batch"):
+    """Concatenates independent problems of equal d into one; returns it and
+    the handle offsets of the pieces (the segment boundaries)."""
+    d = problems[0].d
+    offsets = [0]
+    terms_by_energy = {}
+    xs = []
+    for P in problems:
+        assert P.d == d
+        for t in P.terms:
+            terms_by_energy.setdefault((t.energy, t.valence), []).append((t.conn + offsets[-1], t.params))
+        xs.append(P.x0)
+        offsets.append(offsets[-1] + P.n_handles)
+    terms = [Term(e, np.concatenate([c for c, _ in v]).astype(np.int32), np.concatenate([p for _, p in v]))
+             for (e, _), v in terms_by_energy.items()]
+    return Problem(name, d, offsets[-1], terms, np.concatenate(xs)), np.array(offsets, dtype=np.int64)

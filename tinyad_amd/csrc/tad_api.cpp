@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -216,6 +217,13 @@ struct tad_fn_s
     DevBuf<PcgScalars> scal;
     DevBuf<double> pcg_parts;
     bool h_valid = false;
+    // segments (independent problems)
+    int64_t n_seg = 0;
+    std::vector<int64_t> seg_offsets;
+    DevBuf<int64_t> seg_ranges;
+    DevBuf<double> seg_f;
+    // adjoint products (FP64 records when the arena holds FP32 ones)
+    DevBuf<double> adj_arena, adj_rest, adj_lam;
 
     hipStream_t s() const { return ctx->stream; }
 };
@@ -425,6 +433,8 @@ tad_status eval_core(tad_fn fn, const double* x_dev, int mode, double eps, doubl
     }
     tm.begin("energy_sum", s);
     TAD_HIP(launch_sum(fn->elem_f.p, fn->n_elem_total, 1, fn->n_elem_total, fn->partials.p, fn->results.p, s));
+    if (fn->n_seg > 0)
+        TAD_HIP(launch_segment_sum(fn->elem_f.p, fn->seg_ranges.p, (int)fn->terms.size(), fn->n_seg, fn->seg_f.p, s));
     tm.end("energy_sum", s);
     double* pin = fn->ctx->pinned;
     TAD_HIP(hipMemcpyAsync(pin, fn->results.p, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -458,6 +468,8 @@ tad_status eval_energy_batch(tad_fn fn, const double* x_dev, const double* dir_d
     tm.end("energy", s);
     tm.begin("energy_sum", s);
     TAD_HIP(launch_sum(dst, ne, nc, ne, fn->partials.p, fn->results.p, s));
+    if (!dir_dev && fn->n_seg > 0)
+        TAD_HIP(launch_segment_sum(fn->elem_f.p, fn->seg_ranges.p, (int)fn->terms.size(), fn->n_seg, fn->seg_f.p, s));
     tm.end("energy_sum", s);
     double* pin = fn->ctx->pinned;
     TAD_HIP(hipMemcpyAsync(pin + 32, fn->results.p, nc * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -704,7 +716,7 @@ tad_status tad_fn_destroy(tad_fn fn)
     fn->partials.release(); fn->results.release(); fn->steps.release(); fn->err.release();
     fn->ls_f.release(); fn->x_tmp.release();
     fn->Dinv.release(); fn->r.release(); fn->z.release(); fn->p.release(); fn->q.release(); fn->sol.release();
-    fn->scal.release(); fn->pcg_parts.release();
+    fn->scal.release(); fn->pcg_parts.release(); fn->seg_ranges.release(); fn->seg_f.release();
     tad_ctx ctx = fn->ctx;
     delete fn;
     ctx_release(ctx);
@@ -777,6 +789,7 @@ tad_status tad_fn_add_elements(tad_fn fn, tad_energy energy, int valence, int64_
     fn->n_elem_total += n_elem;
     free_pattern(fn);
     fn->h_valid = false;
+    fn->n_seg = 0;  // segment ranges must be declared again
     return TAD_OK;
 }
 
@@ -996,7 +1009,10 @@ tad_status tad_newton_step(tad_fn fn, double* x, tad_mem mem, const tad_newton_o
     tad_newton_stats st;
     std::memset(&st, 0, sizeof(st));
     double f = 0.0;
-    TAD_TRY(eval_core(fn, xd, MODE_HESS_PROJ, o.projection_eps, fn->g_buf.p, fn->H_buf.p, &f));
+    // a NaN eps selects the unprojected Hessian, as eval_with_derivatives
+    // passes eps = NAN to the shared evaluation (ScalarFunctionImpl.hh:333)
+    const int mode = std::isnan(o.projection_eps) ? MODE_HESS : MODE_HESS_PROJ;
+    TAD_TRY(eval_core(fn, xd, mode, o.projection_eps, fn->g_buf.p, fn->H_buf.p, &f));
     fn->h_valid = true;
     st.f = f;
     // solve (wI + H) d' = g; the Newton direction is d = -d'
@@ -1009,6 +1025,9 @@ tad_status tad_newton_step(tad_fn fn, double* x, tad_mem mem, const tad_newton_o
     double dpg = 0.0;
     TAD_TRY(dot_dev(fn, fn->sol.p, fn->g_buf.p, &dpg));
     const double dg = -dpg;  // d^T g
+    double gg = 0.0;
+    TAD_TRY(dot_dev(fn, fn->g_buf.p, fn->g_buf.p, &gg));
+    st.grad_norm = std::sqrt(gg);
     st.newton_decrement = -0.5 * dg;
     double fo = 0, so = 0;
     int tr = 0;
@@ -1018,13 +1037,219 @@ tad_status tad_newton_step(tad_fn fn, double* x, tad_mem mem, const tad_newton_o
     st.step = so;
     st.ls_trials = tr;
     st.ls_failed = failed ? 1 : 0;
-    st.grad_max = NAN;
     if (mem == TAD_MEM_HOST)
     {
         TAD_HIP(hipMemcpyAsync(x, xd, n * sizeof(double), hipMemcpyDeviceToHost, s));
         TAD_HIP(hipStreamSynchronize(s));
     }
     if (stats) *stats = st;
+    return TAD_OK;
+}
+
+tad_status tad_fn_set_segments(tad_fn fn, int64_t n_segments, const int64_t* off)
+{
+    TAD_TRY(check_fn(fn));
+    if (n_segments < 1 || !off) return fail(TAD_ERR_INVALID, "need at least one segment");
+    if (off[0] != 0 || off[n_segments] != fn->n_handles)
+        return fail(TAD_ERR_INVALID, "segment offsets must start at 0 and end at n_handles");
+    for (int64_t k = 0; k < n_segments; ++k)
+        if (off[k + 1] < off[k]) return fail(TAD_ERR_INVALID, "segment offsets must be non-decreasing");
+    auto seg_of = [&](int32_t v) -> int64_t {
+        return (int64_t)(std::upper_bound(off, off + n_segments + 1, (int64_t)v) - off) - 1;
+    };
+    const size_t nt = fn->hterms.size();
+    std::vector<int64_t> ranges(nt * n_segments * 2, 0);
+    for (size_t ti = 0; ti < nt; ++ti)
+    {
+        const HostTerm& t = fn->hterms[ti];
+        const int64_t base = fn->terms[ti].f_base;
+        for (int64_t k = 0; k < n_segments; ++k)
+            ranges[(ti * n_segments + k) * 2] = ranges[(ti * n_segments + k) * 2 + 1] = base;
+        int64_t prev = -1;
+        for (int64_t e = 0; e < t.n_elem; ++e)
+        {
+            const int64_t sg = seg_of(t.conn[e * t.N]);
+            for (int a = 1; a < t.N; ++a)
+                if (seg_of(t.conn[e * t.N + a]) != sg)
+                    return fail(TAD_ERR_INVALID, "element " + std::to_string(e) + " spans two segments");
+            if (sg < prev) return fail(TAD_ERR_INVALID, "elements of a term must be ordered by segment");
+            if (sg != prev)
+            {
+                ranges[(ti * n_segments + sg) * 2] = base + e;
+                prev = sg;
+            }
+            ranges[(ti * n_segments + sg) * 2 + 1] = base + e + 1;
+        }
+    }
+    TAD_HIP(fn->seg_ranges.ensure(ranges.size()));
+    TAD_HIP(fn->seg_f.ensure((size_t)n_segments));
+    TAD_HIP(hipMemcpy(fn->seg_ranges.p, ranges.data(), ranges.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    TAD_HIP(hipMemset(fn->seg_f.p, 0, n_segments * sizeof(double)));
+    fn->n_seg = n_segments;
+    fn->seg_offsets.assign(off, off + n_segments + 1);
+    return TAD_OK;
+}
+
+tad_status tad_fn_segment_energies(tad_fn fn, double* f, tad_mem mem)
+{
+    TAD_TRY(check_fn(fn));
+    if (!f) return fail(TAD_ERR_INVALID, "null argument");
+    if (fn->n_seg == 0) return fail(TAD_ERR_INVALID, "no segments declared");
+    TAD_HIP(hipMemcpyAsync(f, fn->seg_f.p, fn->n_seg * sizeof(double),
+                           mem == TAD_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, fn->s()));
+    TAD_HIP(hipStreamSynchronize(fn->s()));
+    return TAD_OK;
+}
+
+tad_status tad_fn_autotune(tad_fn fn, const double* x, tad_mem mem, double eps, char* report, size_t report_len)
+{
+    TAD_TRY(check_fn(fn));
+    if (!x) return fail(TAD_ERR_INVALID, "null argument");
+    TAD_HIP(hipSetDevice(fn->ctx->device));
+    TAD_TRY(ensure_ready(fn));
+    hipStream_t s = fn->s();
+    const double* xd = stage_x(fn, x, mem);
+    if (!xd) return fail(TAD_ERR_HIP, "copy of x to the device failed");
+    int n_cu = 0;
+    TAD_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, fn->ctx->device));
+    hipEvent_t e0, e1;
+    TAD_HIP(hipEventCreate(&e0));
+    TAD_HIP(hipEventCreate(&e1));
+    std::string rep;
+    for (TermDev& t : fn->terms)
+    {
+        if (!(t.energy == TAD_E_SYMDIR_TRI || t.energy == TAD_E_SYMDIR_TET) || t.n_elem == 0) continue;
+        struct Cand { int block, grid, map; double ms; };
+        std::vector<Cand> cands;
+        cands.push_back({ 0, 0, 0, 0.0 });
+        for (int b : { 64, 128, 256 })
+            for (int wpc : { 1, 2, 4, 8 })
+                for (int map : { 0, 1 })
+                {
+                    int grid = n_cu * wpc;
+                    grid = (grid + 7) / 8 * 8;
+                    cands.push_back({ b, grid, map, 0.0 });
+                }
+        for (Cand& c : cands)
+        {
+            t.tune_block = c.block;
+            t.tune_grid = c.grid;
+            t.tune_map = c.map;
+            TAD_HIP(launch_element(t, fn->d, fn->precision, MODE_HESS_PROJ, xd, eps, fn->elem_f.p, fn->arena.p,
+                                   fn->err.p, s));
+            float best = 1e30f;
+            for (int rep_i = 0; rep_i < 3; ++rep_i)
+            {
+                TAD_HIP(hipEventRecord(e0, s));
+                TAD_HIP(launch_element(t, fn->d, fn->precision, MODE_HESS_PROJ, xd, eps, fn->elem_f.p, fn->arena.p,
+                                       fn->err.p, s));
+                TAD_HIP(hipEventRecord(e1, s));
+                TAD_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                TAD_HIP(hipEventElapsedTime(&ms, e0, e1));
+                best = std::min(best, ms);
+            }
+            c.ms = best;
+        }
+        const Cand* bestc = &cands[0];
+        for (const Cand& c : cands)
+            if (c.ms < bestc->ms) bestc = &c;
+        t.tune_block = bestc->block;
+        t.tune_grid = bestc->grid;
+        t.tune_map = bestc->map;
+        char line[256];
+        std::snprintf(line, sizeof(line), "term energy %d (%lld elements): plain grid %.4f ms; best %s block %d grid %d map %s %.4f ms\n",
+                      t.energy, (long long)t.n_elem, cands[0].ms, bestc->block ? "persistent" : "plain", bestc->block,
+                      bestc->grid, bestc->map ? "xcd" : "strided", bestc->ms);
+        rep += line;
+        for (const Cand& c : cands)
+        {
+            std::snprintf(line, sizeof(line), "  block %3d grid %5d map %-7s %.4f ms\n", c.block, c.grid,
+                          c.block == 0 ? "-" : (c.map ? "xcd" : "strided"), c.ms);
+            rep += line;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (report && report_len > 0)
+    {
+        std::strncpy(report, rep.c_str(), report_len - 1);
+        report[report_len - 1] = 0;
+    }
+    return TAD_OK;
+}
+
+tad_status tad_fn_set_params(tad_fn fn, int term, const double* params, tad_mem mem)
+{
+    TAD_TRY(check_fn(fn));
+    if (term < 0 || term >= (int)fn->terms.size()) return fail(TAD_ERR_INVALID, "term index out of range");
+    if (!params) return fail(TAD_ERR_INVALID, "null argument");
+    TAD_HIP(hipSetDevice(fn->ctx->device));
+    TermDev& t = fn->terms[term];
+    const int64_t ne = t.n_elem;
+    const int np = t.n_params;
+    if (ne == 0 || np == 0) return TAD_OK;
+    std::vector<double> host;
+    const double* src = params;
+    if (mem == TAD_MEM_DEVICE)
+    {
+        host.resize((size_t)ne * np);
+        TAD_HIP(hipMemcpyAsync(host.data(), params, host.size() * sizeof(double), hipMemcpyDeviceToHost, fn->s()));
+        TAD_HIP(hipStreamSynchronize(fn->s()));
+        src = host.data();
+    }
+    std::vector<double> ps((size_t)ne * np);
+    for (int64_t e = 0; e < ne; ++e)
+        for (int k = 0; k < np; ++k) ps[k * ne + e] = src[e * np + k];
+    TAD_HIP(hipMemcpyAsync(t.params, ps.data(), ps.size() * sizeof(double), hipMemcpyHostToDevice, fn->s()));
+    TAD_HIP(hipStreamSynchronize(fn->s()));
+    fn->h_valid = false;
+    return TAD_OK;
+}
+
+tad_status tad_fn_rest_vjp(tad_fn fn, int term, const double* x, const double* rest, const double* lam,
+                           double w_scale, double* out, tad_mem mem)
+{
+    TAD_TRY(check_fn(fn));
+    if (term < 0 || term >= (int)fn->terms.size()) return fail(TAD_ERR_INVALID, "term index out of range");
+    if (!x || !rest || !lam || !out) return fail(TAD_ERR_INVALID, "null argument");
+    if (fn->terms[term].energy != TAD_E_SYMDIR_TET || fn->d != 3)
+        return fail(TAD_ERR_UNSUPPORTED, "rest-shape products exist for symmetric-Dirichlet tets in 3D only");
+    TAD_HIP(hipSetDevice(fn->ctx->device));
+    TAD_TRY(ensure_ready(fn));
+    hipStream_t s = fn->s();
+    const size_t nv = (size_t)fn->n_vars;
+    const double *xd = x, *rd = rest, *ld = lam;
+    if (mem == TAD_MEM_HOST)
+    {
+        TAD_HIP(fn->adj_rest.ensure(nv));
+        TAD_HIP(fn->adj_lam.ensure(nv));
+        TAD_HIP(hipMemcpyAsync(fn->x_buf.p, x, nv * sizeof(double), hipMemcpyHostToDevice, s));
+        TAD_HIP(hipMemcpyAsync(fn->adj_rest.p, rest, nv * sizeof(double), hipMemcpyHostToDevice, s));
+        TAD_HIP(hipMemcpyAsync(fn->adj_lam.p, lam, nv * sizeof(double), hipMemcpyHostToDevice, s));
+        xd = fn->x_buf.p;
+        rd = fn->adj_rest.p;
+        ld = fn->adj_lam.p;
+    }
+    // FP64 records: the arena itself in FP64 mode (its contents are consumed
+    // by the gathers of the evaluation that wrote them), a side buffer of the
+    // same layout in mixed mode.
+    double* rec = nullptr;
+    if (fn->precision == TAD_PREC_FP64) rec = (double*)fn->arena.p;
+    else
+    {
+        TAD_HIP(fn->adj_arena.ensure((size_t)fn->arena_size));
+        rec = fn->adj_arena.p;
+    }
+    double* out_dev = (mem == TAD_MEM_DEVICE) ? out : fn->g_buf.p;
+    for (size_t ti = 0; ti < fn->terms.size(); ++ti)
+    {
+        if ((int)ti == term) TAD_HIP(launch_tet_rest_vjp(fn->terms[ti], fn->d, xd, rd, ld, w_scale, rec, s));
+        else TAD_HIP(launch_zero_grad_slots(fn->terms[ti], fn->d, rec, s));
+    }
+    TAD_HIP(launch_assemble_gradient(fn->d, 0, fn->n_handles, fn->gmap_ptr.p, fn->gmap.p, rec, out_dev, s));
+    if (mem == TAD_MEM_HOST) TAD_HIP(hipMemcpyAsync(out, out_dev, nv * sizeof(double), hipMemcpyDeviceToHost, s));
+    TAD_HIP(hipStreamSynchronize(s));
     return TAD_OK;
 }
 

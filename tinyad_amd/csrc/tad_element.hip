@@ -12,13 +12,11 @@ namespace tad {
 // Element kernel
 // ---------------------------------------------------------------------------
 template <class E, class T, int MODE>
-__global__ __launch_bounds__(kBlock) void k_element(int64_t n, const int32_t* __restrict__ conn,
-                                                    const double* __restrict__ params, const double* __restrict__ x,
-                                                    double eps, double* __restrict__ elem_f, T* __restrict__ rec,
-                                                    int* __restrict__ err)
+__device__ __forceinline__ void element_body(int64_t e, int64_t n, const int32_t* __restrict__ conn,
+                                             const double* __restrict__ params, const double* __restrict__ x,
+                                             double eps, double* __restrict__ elem_f, T* __restrict__ rec,
+                                             int* __restrict__ err)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
     constexpr int RS = record_size(E::N, E::D);
     double X[E::N * E::D];
 #pragma unroll
@@ -36,15 +34,83 @@ __global__ __launch_bounds__(kBlock) void k_element(int64_t n, const int32_t* __
     if (nf) *err = 1;
 }
 
+// One lane per element, one tile of blockDim elements per workgroup.
+template <class E, class T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_element(int64_t n, const int32_t* __restrict__ conn,
+                                                    const double* __restrict__ params, const double* __restrict__ x,
+                                                    double eps, double* __restrict__ elem_f, T* __restrict__ rec,
+                                                    int* __restrict__ err)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    element_body<E, T, MODE>(e, n, conn, params, x, eps, elem_f, rec, err);
+}
+
+// Persistent variant: a fixed grid walks the tiles.  map_xcd = 1 gives each of
+// the 8 XCDs (workgroups are dispatched to XCDs round-robin by id) one
+// contiguous eighth of the tiles, so the vertex gathers of neighbouring
+// elements hit the same XCD's L2; map_xcd = 0 strides tiles across the grid.
+// Requires gridDim.x to be a multiple of 8 when map_xcd = 1.
+template <class E, class T, int MODE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_element_persistent(int64_t n, const int32_t* __restrict__ conn,
+                                                              const double* __restrict__ params,
+                                                              const double* __restrict__ x, double eps,
+                                                              double* __restrict__ elem_f, T* __restrict__ rec,
+                                                              int* __restrict__ err, int map_xcd)
+{
+    const int64_t n_tiles = (n + BLOCK - 1) / BLOCK;
+    int64_t t0, t1, stride, first;
+    if (map_xcd)
+    {
+        const int xcd = blockIdx.x % 8;
+        const int64_t local = blockIdx.x / 8;
+        stride = gridDim.x / 8;
+        t0 = n_tiles * xcd / 8;
+        t1 = n_tiles * (xcd + 1) / 8;
+        first = t0 + local;
+    }
+    else
+    {
+        t1 = n_tiles;
+        stride = gridDim.x;
+        first = blockIdx.x;
+    }
+    for (int64_t t = first; t < t1; t += stride)
+    {
+        const int64_t e = t * BLOCK + threadIdx.x;
+        if (e < n) element_body<E, T, MODE>(e, n, conn, params, x, eps, elem_f, rec, err);
+    }
+}
+
+template <class E, class T, int MODE, int BLOCK>
+static void launch_persistent(const TermDev& t, const double* x, double eps, double* ef, T* rec, int* err,
+                              hipStream_t s)
+{
+    k_element_persistent<E, T, MODE, BLOCK><<<t.tune_grid, BLOCK, 0, s>>>(t.n_elem, t.conn, t.params, x, eps, ef, rec,
+                                                                         err, t.tune_map);
+}
+
 template <class E, class T>
 static hipError_t launch_element_t(const TermDev& t, int mode, const double* x, double eps, double* elem_f,
                                    T* arena, int* err, hipStream_t s)
 {
     const int64_t n = t.n_elem;
     if (n == 0) return hipSuccess;
-    const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
     T* rec = arena + t.rec_base;
     double* ef = elem_f + t.f_base;
+    if constexpr (E::tunable)
+    {
+        // the launch shape chosen by tad_fn_autotune (projected Hessian path)
+        if (mode == MODE_HESS_PROJ && t.tune_block > 0)
+        {
+            if (t.tune_map == 1 && t.tune_grid % 8 != 0) return hipErrorInvalidValue;
+            if (t.tune_block == 64) launch_persistent<E, T, MODE_HESS_PROJ, 64>(t, x, eps, ef, rec, err, s);
+            else if (t.tune_block == 128) launch_persistent<E, T, MODE_HESS_PROJ, 128>(t, x, eps, ef, rec, err, s);
+            else launch_persistent<E, T, MODE_HESS_PROJ, 256>(t, x, eps, ef, rec, err, s);
+            return hipGetLastError();
+        }
+    }
+    const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
     switch (mode)
     {
         case MODE_GRAD: k_element<E, T, MODE_GRAD><<<grid, kBlock, 0, s>>>(n, t.conn, t.params, x, eps, ef, rec, err); break;

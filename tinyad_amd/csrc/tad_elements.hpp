@@ -17,6 +17,7 @@ template <int Dm>
 struct SymdirElem
 {
     static constexpr int N = Dm + 1, D = Dm, NP = Dm * Dm + 1;
+    static constexpr bool tunable = true;  // has persistent launch variants
     template <class T, int MODE>
     __device__ static double eval(const double* X, const double* p, T eps, T* rec, bool& nf)
     {
@@ -29,6 +30,7 @@ template <int Dm>
 struct PenaltyElem
 {
     static constexpr int N = 1, D = Dm, NP = Dm + 1;
+    static constexpr bool tunable = false;
     template <class T, int MODE>
     __device__ static double eval(const double* X, const double* p, T eps, T* rec, bool& nf)
     {
@@ -41,6 +43,7 @@ template <int NN, int Dm>
 struct QuadElem
 {
     static constexpr int N = NN, D = Dm, K = NN * Dm, NP = K * K + K + 1;
+    static constexpr bool tunable = false;
     template <class T, int MODE>
     __device__ static double eval(const double* X, const double* p, T eps, T* rec, bool& nf)
     {

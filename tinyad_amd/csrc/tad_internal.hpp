@@ -24,12 +24,20 @@ struct TermDev
     double* params = nullptr;  // device SoA: params[k * n_elem + e]
     int64_t rec_base = 0;      // first arena entry of this term's records
     int64_t f_base = 0;        // first entry of this term in the element-energy array
+    // launch shape of the projected-Hessian element kernel chosen by the
+    // autotuner: tune_block = 0 is the plain one-tile-per-workgroup grid,
+    // otherwise a persistent grid of tune_grid workgroups of tune_block lanes
+    // with tile map tune_map (0 strided, 1 per-XCD contiguous)
+    int tune_block = 0;
+    int tune_grid = 0;
+    int tune_map = 0;
 };
 
 // Solver bookkeeping written by block 0 of the PCG kernels (see tad_linalg.hip).
 struct PcgScalars
 {
     double bb, rr, tol2;
+    double rho_hist[2], alpha_hist[2];  // rho_k and alpha_k, slot k & 1
     int iters, max_iters, done, breakdown;
 };
 
@@ -79,8 +87,9 @@ hipError_t launch_pcg_setup(int d, int64_t n_handles, const int32_t* brow_ptr, c
 hipError_t launch_pcg_init(int d, int64_t n_handles, const double* b, const double* Dinv, double* x, double* r,
                            double* z, double* partials, PcgScalars* sc, hipStream_t s);
 
-// One CG iteration k: SpMV with termination test and on-the-fly direction
-// update, then (with_update) the x / r / z update.  pbuf holds 2 * n values.
+// One CG iteration k: SpMV on z with the termination test, then (with_update)
+// the scalar recurrences and the p / q / x / r / z update.  pbuf holds 2 * n
+// values (p, then s = (w I + H) z).
 hipError_t launch_pcg_iteration(int d, int64_t n_handles, const int32_t* brow_ptr, const int32_t* bcol,
                                 const int32_t* row_ptr, const double* H, double w, const double* Dinv, double* x,
                                 double* r, double* z, double* pbuf, double* q, double* partials, PcgScalars* sc, int k,
@@ -89,6 +98,14 @@ hipError_t launch_pcg_iteration(int d, int64_t n_handles, const int32_t* brow_pt
 // Size (doubles) of the partial-sum buffer the PCG kernels need.
 int64_t pcg_parts_size(int d, int64_t n_handles);
 
+// Per-segment energies: ranges[(t * n_seg + s) * 2 + {0,1}] is the element
+// range of segment s in term t (absolute indices into elem_f).
+hipError_t launch_segment_sum(const double* elem_f, const int64_t* ranges, int n_terms, int64_t n_seg, double* out,
+                              hipStream_t s);
+
+hipError_t launch_tet_rest_vjp(const TermDev& t, int d, const double* x, const double* rest, const double* lam,
+                               double w_scale, double* arena, hipStream_t s);
+hipError_t launch_zero_grad_slots(const TermDev& t, int d, double* arena, hipStream_t s);
 hipError_t launch_selftest_dual(int which, double x, double y, double* out_dev, hipStream_t s);
 
 } // namespace tad

@@ -357,16 +357,23 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t nh, const double* _
     }
 }
 
-// CG iteration k, first half.  p_k = z_k + beta_k p_{k-1} (p_0 = z_0) is formed
-// on the fly for every gathered vertex and written for the block's own rows
-// into P[k & 1]; q = (w I + H) p_k; partial p_k.q.  Before that, the
-// termination test on rr_k = ||r_k||^2 (k = 0: ||b||^2, which is recorded).
+// Chronopoulos-Gear form of preconditioned CG (the iterates of standard PCG
+// in exact arithmetic, one reduction point per iteration):
+//   s_k = A z_k,  delta_k = z_k.s_k,  rho_k = r_k.z_k
+//   k = 0: beta = 0, alpha = rho_0 / delta_0,            p = z, q = s
+//   k > 0: beta = rho_k / rho_{k-1}, alpha = rho_k / (delta_k - beta rho_k / alpha_{k-1}),
+//          p = z + beta p,  q = s + beta q
+//   x += alpha p,  r -= alpha q,  z = M^-1 r
+// so the SpMV gathers only z, which is complete when it runs.
+
+// CG iteration k, first half: termination test on rr_k = ||r_k||^2 (k = 0:
+// ||b||^2, recorded), then s = (w I + H) z and partial z.s.
 template <int D, int LPB>
 __global__ __launch_bounds__(kBlock) void k_spmv(int64_t nh, const int32_t* __restrict__ brow_ptr,
                                                  const int32_t* __restrict__ bcol, const int32_t* __restrict__ row_ptr,
                                                  const double* __restrict__ H, double w, const double* __restrict__ z,
-                                                 double* __restrict__ pbuf, double* __restrict__ q,
-                                                 double* __restrict__ parts, int Gs, int Gv, int k, PcgScalars* sc)
+                                                 double* __restrict__ sv, double* __restrict__ parts, int Gs, int Gv,
+                                                 int k, PcgScalars* sc)
 {
     if (block_done(sc)) return;
     __shared__ double sh[kBlock];
@@ -390,18 +397,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t nh, const int32_t* __re
         if (stop) sc->done = 1;
     }
     if (stop) return;
-    double beta = 0.0;
-    if (k > 0)
-    {
-        const double rho = sum_parts(parts + Gs + ((k + 1) & 1) * Gv, Gv, sh);
-        const double rho_old = sum_parts(parts + Gs + (k & 1) * Gv, Gv, sh);
-        beta = rho / rho_old;
-    }
-    double* __restrict__ pk = pbuf + (int64_t)(k & 1) * nh * D;
-    const double* __restrict__ pold = pbuf + (int64_t)((k + 1) & 1) * nh * D;
     const int lane = threadIdx.x % LPB;
     constexpr int rows_per_block = kBlock / LPB;
-    double pq = 0.0;
+    double zs = 0.0;
     for (int64_t u0 = (int64_t)blockIdx.x * rows_per_block; u0 < nh; u0 += (int64_t)gridDim.x * rows_per_block)
     {
         const int64_t u = u0 + threadIdx.x / LPB;
@@ -416,13 +414,13 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t nh, const int32_t* __re
             for (int kk = lane; kk < nbu; kk += LPB)
             {
                 const int64_t v = bcol[b0 + kk];
-                double pv[D];
+                double zv[D];
 #pragma unroll
-                for (int j = 0; j < D; ++j) pv[j] = k > 0 ? fma(beta, pold[v * D + j], z[v * D + j]) : z[v * D + j];
+                for (int j = 0; j < D; ++j) zv[j] = z[v * D + j];
 #pragma unroll
                 for (int i = 0; i < D; ++i)
 #pragma unroll
-                    for (int j = 0; j < D; ++j) acc[i] += H[base + (int64_t)i * D * nbu + D * kk + j] * pv[j];
+                    for (int j = 0; j < D; ++j) acc[i] += H[base + (int64_t)i * D * nbu + D * kk + j] * zv[j];
             }
         }
 #pragma unroll
@@ -434,32 +432,41 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t nh, const int32_t* __re
 #pragma unroll
             for (int i = 0; i < D; ++i)
             {
-                const double pi = k > 0 ? fma(beta, pold[u * D + i], z[u * D + i]) : z[u * D + i];
-                pk[u * D + i] = pi;
-                const double qi = acc[i] + w * pi;
-                q[u * D + i] = qi;
-                pq += pi * qi;
+                const double zi = z[u * D + i];
+                const double si = acc[i] + w * zi;
+                sv[u * D + i] = si;
+                zs += zi * si;
             }
         }
     }
-    const double s = block_sum(pq, sh);
-    if (threadIdx.x == 0) parts[blockIdx.x] = s;
+    const double tot = block_sum(zs, sh);
+    if (threadIdx.x == 0) parts[blockIdx.x] = tot;
 }
 
-// CG iteration k, second half: alpha = rho_k / p_k.q; x += alpha p_k,
-// r -= alpha q, z = M^-1 r; partials of rho_{k+1} = r.z and r.r into slot k & 1.
+// CG iteration k, second half: scalars, direction recurrences, x / r / z
+// update; partials of rho_{k+1} = r.z and ||r||^2 into slot k & 1.  alpha and
+// rho of iteration k are kept (slot k & 1 of sc) for iteration k + 1.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t nh, const double* __restrict__ Dinv,
                                                        double* __restrict__ x, double* __restrict__ r,
-                                                       double* __restrict__ z, const double* __restrict__ pbuf,
-                                                       const double* __restrict__ q, double* __restrict__ parts,
-                                                       int Gs, int Gv, int k, PcgScalars* sc)
+                                                       double* __restrict__ z, double* __restrict__ p,
+                                                       double* __restrict__ q, const double* __restrict__ sv,
+                                                       double* __restrict__ parts, int Gs, int Gv, int k,
+                                                       PcgScalars* sc)
 {
     if (block_done(sc)) return;
     __shared__ double sh[kBlock];
-    const double pq = sum_parts(parts, Gs, sh);
+    const double delta = sum_parts(parts, Gs, sh);
     const double rho = sum_parts(parts + Gs + ((k + 1) & 1) * Gv, Gv, sh);
-    if (!(pq > 0.0))
+    double beta = 0.0, den = delta;
+    if (k > 0)
+    {
+        const double rho_prev = sc->rho_hist[(k + 1) & 1];
+        const double alpha_prev = sc->alpha_hist[(k + 1) & 1];
+        beta = rho / rho_prev;
+        den = delta - beta * rho / alpha_prev;
+    }
+    if (!(den > 0.0))
     {
         // not positive definite along p: the same decision in every block
         if (blockIdx.x == 0 && threadIdx.x == 0)
@@ -470,8 +477,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t nh, const double*
         }
         return;
     }
-    const double alpha = rho / pq;
-    const double* __restrict__ p = pbuf + (int64_t)(k & 1) * nh * D;
+    const double alpha = rho / den;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        sc->rho_hist[k & 1] = rho;
+        sc->alpha_hist[k & 1] = alpha;
+    }
     double rz = 0.0, rr = 0.0;
     for (int64_t u = (int64_t)blockIdx.x * kBlock + threadIdx.x; u < nh; u += (int64_t)gridDim.x * kBlock)
     {
@@ -479,8 +490,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t nh, const double*
 #pragma unroll
         for (int i = 0; i < D; ++i)
         {
-            x[u * D + i] += alpha * p[u * D + i];
-            rv[i] = r[u * D + i] - alpha * q[u * D + i];
+            const double pi = k > 0 ? fma(beta, p[u * D + i], z[u * D + i]) : z[u * D + i];
+            const double qi = k > 0 ? fma(beta, q[u * D + i], sv[u * D + i]) : sv[u * D + i];
+            p[u * D + i] = pi;
+            q[u * D + i] = qi;
+            x[u * D + i] += alpha * pi;
+            rv[i] = r[u * D + i] - alpha * qi;
             r[u * D + i] = rv[i];
         }
 #pragma unroll
@@ -548,17 +563,47 @@ hipError_t launch_pcg_iteration(int d, int64_t nh, const int32_t* brow_ptr, cons
 {
     int Gs, Gv;
     pcg_grids(d, nh, &Gs, &Gv);
+    double* p = pbuf;
+    double* sv = pbuf + nh * d;
     if (timer) timer->begin("spmv", s);
-    if (d == 1) k_spmv<1, 8><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, pbuf, q, parts, Gs, Gv, k, sc);
-    else if (d == 2) k_spmv<2, 8><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, pbuf, q, parts, Gs, Gv, k, sc);
-    else k_spmv<3, 16><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, pbuf, q, parts, Gs, Gv, k, sc);
+    if (d == 1) k_spmv<1, 8><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, sv, parts, Gs, Gv, k, sc);
+    else if (d == 2) k_spmv<2, 8><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, sv, parts, Gs, Gv, k, sc);
+    else k_spmv<3, 16><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, sv, parts, Gs, Gv, k, sc);
     if (timer) timer->end("spmv", s);
     if (!with_update) return hipGetLastError();
     if (timer) timer->begin("pcg_update", s);
-    if (d == 1) k_pcg_update<1><<<Gv, kBlock, 0, s>>>(nh, Dinv, x, r, z, pbuf, q, parts, Gs, Gv, k, sc);
-    else if (d == 2) k_pcg_update<2><<<Gv, kBlock, 0, s>>>(nh, Dinv, x, r, z, pbuf, q, parts, Gs, Gv, k, sc);
-    else k_pcg_update<3><<<Gv, kBlock, 0, s>>>(nh, Dinv, x, r, z, pbuf, q, parts, Gs, Gv, k, sc);
+    if (d == 1) k_pcg_update<1><<<Gv, kBlock, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, sc);
+    else if (d == 2) k_pcg_update<2><<<Gv, kBlock, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, sc);
+    else k_pcg_update<3><<<Gv, kBlock, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, sc);
     if (timer) timer->end("pcg_update", s);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Per-segment energy sums (fixed order: terms in order, fixed thread stride
+// within each range, fixed block tree)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_segment_sum(const double* __restrict__ elem_f,
+                                                        const int64_t* __restrict__ ranges, int n_terms,
+                                                        int64_t n_seg, double* __restrict__ out)
+{
+    __shared__ double sh[kBlock];
+    const int64_t sgm = blockIdx.x;
+    double acc = 0.0;
+    for (int t = 0; t < n_terms; ++t)
+    {
+        const int64_t a = ranges[(t * n_seg + sgm) * 2], b = ranges[(t * n_seg + sgm) * 2 + 1];
+        for (int64_t i = a + threadIdx.x; i < b; i += kBlock) acc += elem_f[i];
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) out[sgm] = s;
+}
+
+hipError_t launch_segment_sum(const double* elem_f, const int64_t* ranges, int n_terms, int64_t n_seg, double* out,
+                              hipStream_t s)
+{
+    if (n_seg == 0) return hipSuccess;
+    k_segment_sum<<<(unsigned)n_seg, kBlock, 0, s>>>(elem_f, ranges, n_terms, n_seg, out);
     return hipGetLastError();
 }
 
