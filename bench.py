@@ -187,9 +187,9 @@ def main():
     avg_s = tot_ms / max(cnt, 1) / 1e3
     nb = (nnz // 9)
     if dom == "spmv":
-        # values + block columns + block-row / row offsets + z and p_{k-1} read,
-        # p_k and q written (the direction update is fused into the SpMV)
-        traffic_alg = 8 * nnz + 4 * nb + 4 * (P.n_handles + 1) + 4 * P.n_handles + 4 * 8 * n
+        # s = (w I + H) z: CSR values, block columns, block-row offsets, the
+        # first scalar-row offset of each block row, z read once, s written
+        traffic_alg = 8 * nnz + 4 * nb + 4 * (P.n_handles + 1) + 4 * P.n_handles + 2 * 8 * n
         roof = {"kernel": "k_spmv<3,16>", "bound": "hbm", "achieved": traffic_alg / avg_s / 1e9, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "bytes_per_launch": traffic_alg}
     elif dom == "element":

@@ -70,7 +70,7 @@ typedef enum tad_precision
  *     M = col_mat(v1 - v0, v2 - v0[, v3 - v0])   (tests/NewtonTest.cc:28-44)
  *   TAD_E_PENALTY     N=1, any d: [target (d), w]     E = w ||target - x_v||^2
  *                                                       (tests/NewtonTest.cc:48-55)
- *   TAD_E_QUADRATIC   any N, d with N*d <= 12: [A (KxK), b (K), c], K = N*d
+ *   TAD_E_QUADRATIC   any N, d with N*d <= 8: [A (KxK), b (K), c], K = N*d
  *     E = 0.5 x_e^T A x_e + b^T x_e + c over the element's local variables
  *     (tests/ScalarFunctionTest.cc, tests/DynamicElementsTest.cc:100-111)      */
 typedef enum tad_energy

@@ -282,6 +282,31 @@ def test_size_mismatch_raises(ctx):
     assert ei.value.status == 1
 
 
+def test_dense_quadratic_k8_and_limit(ctx):
+    """K = N d = 8 dense elements project in registers (indefinite A, all eps
+    modes, against the oracle); K > 8 is rejected as unsupported."""
+    rng = np.random.default_rng(4)
+    Q, _ = np.linalg.qr(rng.standard_normal((8, 8)))
+    A = (Q * np.array([-3.0, -1e-3, 0.0, 1e-12, 0.5, 1.0, 2.0, 10.0])) @ Q.T
+    A = 0.5 * (A + A.T)
+    b = rng.standard_normal(8)
+    x = rng.standard_normal(8)
+    prm = quad_params(A, b, 0.25)
+    fg = tad.ScalarFunction(2, 4, ctx)
+    fg.add_elements(4, tad.QUADRATIC, [[0, 1, 2, 3]], prm)
+    fo = oracle.Function(2, 4)
+    fo.add_elements(4, [[0, 1, 2, 3]], prm)
+    for eps in (1e-9, -1.0, 0.05):
+        f, g, H = fg.eval_with_hessian_proj(x, eps)
+        fr, gr, Hr = fo.eval_with_hessian_proj(x, eps)
+        assert abs(f - fr) <= 1e-12 * abs(fr) and nrel(g, gr) <= 1e-12
+        assert nrel(H.toarray(), Hr.toarray()) <= 1e-12
+    f3 = tad.ScalarFunction(3, 3, ctx)
+    with pytest.raises(tad.TadError) as ei:
+        f3.add_elements(3, tad.QUADRATIC, [[0, 1, 2]], quad_params(np.eye(9), np.zeros(9), 0.0))
+    assert ei.value.status == 6
+
+
 # ---------------------------------------------------------------------------
 # Solver and Newton
 # ---------------------------------------------------------------------------
@@ -345,6 +370,29 @@ def test_newton_matches_oracle_trajectory_grid(ctx):
         fg.newton_step(xg, opts)
     fo_end = fo.eval(xo)
     fg_end = fg.eval(xg)
+    assert abs(fg_end - fo_end) <= 1e-8 * abs(fo_end)
+
+
+def test_grid100_twenty_newton_iterations(ctx):
+    """100 x 100 grid parametrization (19,602 triangles): 20 projected-Newton
+    iterations on the GPU (PCG 1e-13) and on the oracle (direct solves as the
+    SimplicialLDLT stand-in) end at the same energy within 1e-8 relative."""
+    import scipy.sparse.linalg as spla
+    P = configs.grid_param(100, seed=1)
+    assert P.terms[0].conn.shape[0] == 19602
+    fo = oracle.Function.from_problem(P)
+    fg = tad.ScalarFunction.from_problem(P, ctx)
+    xo = P.x0.copy()
+    xg = P.x0.copy()
+    opts = tad.newton_default_opts(solve_tol=1e-13, solve_max_iters=100000)
+    for _ in range(20):
+        f, g, H = fo.eval_with_hessian_proj(xo)
+        d = spla.spsolve(H.tocsc(), -g)
+        xo, _, _, _ = fo.line_search(xo, d, f, g)
+        fg.newton_step(xg, opts)
+    fo_end = fo.eval(xo)
+    fg_end = fg.eval(xg)
+    print(f"grid 100: oracle f = {fo_end:.15e}, GPU f = {fg_end:.15e}")
     assert abs(fg_end - fo_end) <= 1e-8 * abs(fo_end)
 
 

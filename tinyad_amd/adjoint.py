@@ -87,14 +87,17 @@ class RestShapeProblem:
 
         Projected Newton (newton_step with eps 1e-9) until the Newton decrement
         falls below switch_rel |f|, then Newton with the unprojected Hessian
-        (quadratic convergence near a strict minimum) until the line search
-        finds no decrease (f is at rounding level) or the decrement vanishes.
+        (quadratic convergence near a strict minimum) until the gradient stops
+        shrinking: a step that leaves ||g|| above a quarter of the previous
+        step's value has reached the rounding floor (the Armijo test accepts
+        f_new == f there, so it cannot signal convergence by itself).
         Returns a list of per-iteration stats dicts.
         """
         proj = newton_default_opts(solve_tol=proj_tol)
         exact = newton_default_opts(projection_eps=math.nan, solve_tol=exact_tol)
         hist = []
         phase = "proj"
+        prev_g = math.inf
         for _ in range(max_iters):
             try:
                 st = self.fn.newton_step(x, exact if phase == "exact" else proj)
@@ -102,14 +105,18 @@ class RestShapeProblem:
                 if phase != "exact" or e.status != 4:
                     raise
                 phase = "proj"  # unprojected system not positive definite here
+                prev_g = math.inf
                 continue
             st["phase"] = phase
             hist.append(st)
-            small = st["newton_decrement"] <= switch_rel * max(abs(st["f"]), 1e-300)
-            if phase == "proj" and (small or st["ls_failed"]):
-                phase = "exact"
-            elif phase == "exact" and (st["ls_failed"] or st["newton_decrement"] <= 1e-30 * abs(st["f"])):
-                break
+            if phase == "proj":
+                small = st["newton_decrement"] <= switch_rel * max(abs(st["f"]), 1e-300)
+                if small or st["ls_failed"]:
+                    phase = "exact"
+            else:
+                if st["ls_failed"] or st["grad_norm"] > 0.25 * prev_g or st["newton_decrement"] == 0.0:
+                    break
+                prev_g = st["grad_norm"]
         return hist
 
     # -- adjoint ---------------------------------------------------------------

@@ -735,6 +735,10 @@ tad_status tad_fn_add_elements(tad_fn fn, tad_energy energy, int valence, int64_
     if (energy == TAD_E_SYMDIR_TET && !(d == 3 && N == 4))
         return fail(TAD_ERR_INVALID, "symmetric Dirichlet tets need d = 3 and valence 4");
     if (energy == TAD_E_PENALTY && N != 1) return fail(TAD_ERR_INVALID, "penalty elements have valence 1");
+    // dense K x K quadratics above K = 8 would keep A and its eigenvectors in
+    // scratch memory; the catalog stops where every kernel stays in registers
+    if (energy == TAD_E_QUADRATIC && d * N > 8)
+        return fail(TAD_ERR_UNSUPPORTED, "quadratic elements need d * valence <= 8");
     const int np = energy_param_count(energy, d, N);
     if (np < 0) return fail(TAD_ERR_INVALID, "unknown energy");
     if (np != params_per_elem)

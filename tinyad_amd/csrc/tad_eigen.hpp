@@ -11,79 +11,15 @@
 // A - max(eps, 0) I: it succeeds exactly when every eigenvalue exceeds the
 // clamp level, which covers every matrix the diagonal-dominance test accepts
 // (Gershgorin) and more, and in that case the reference also returns the
-// matrix unchanged.  Otherwise a Jacobi eigensolve (deterministic, robust on
-// clustered spectra) in round-robin pair order runs and the clamped spectrum
-// is re-assembled.
+// matrix unchanged.  Otherwise a Householder tridiagonalization and implicit
+// QL iteration (the two stages SelfAdjointEigenSolver also uses) give the
+// spectrum, and the clamped spectrum is re-assembled.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include "tad_dual.hpp"
 
 namespace tad {
-
-template <class T> struct EigTraits;
-template <> struct EigTraits<double>
-{
-    static constexpr double tol = 1e-15;
-    static constexpr int max_sweeps = 14;
-};
-template <> struct EigTraits<float>
-{
-    static constexpr float tol = 2e-7f;
-    static constexpr int max_sweeps = 10;
-};
-
-// Reciprocal square root and reciprocal from the hardware approximations
-// refined by Newton steps (two steps take the ~2^-23 relative error of the
-// FP64 instructions below 1 ulp).  Arguments are positive and finite where used.
-__device__ __forceinline__ double tad_rsqrt(double x)
-{
-    double y = __builtin_amdgcn_rsq(x);
-    const double hx = 0.5 * x;
-    y = y * (1.5 - hx * y * y);
-    y = y * (1.5 - hx * y * y);
-    return y;
-}
-__device__ __forceinline__ float tad_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
-
-__device__ __forceinline__ double tad_rcp(double x)
-{
-    double y = __builtin_amdgcn_rcp(x);
-    y = y * (2.0 - x * y);
-    y = y * (2.0 - x * y);
-    return y;
-}
-__device__ __forceinline__ float tad_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-
-// Round-robin ("tournament") order of the n(n-1)/2 index pairs: every round
-// holds floor(n/2) disjoint pairs, so the rotations of one round commute and
-// their angle computations are independent of each other.
-template <int n>
-struct Tournament
-{
-    static constexpr int m = n + (n & 1);        // players, with a dummy when n is odd
-    static constexpr int rounds = m - 1;
-    static constexpr int per_round = n / 2;      // real pairs per round
-    int p[rounds][per_round > 0 ? per_round : 1];
-    int q[rounds][per_round > 0 ? per_round : 1];
-    constexpr Tournament() : p(), q()
-    {
-        for (int r = 0; r < rounds; ++r)
-        {
-            int k = 0;
-            for (int i = 0; i < m / 2; ++i)
-            {
-                int a = (i == 0) ? m - 1 : (r + i) % (m - 1);
-                int b = (r - i + (m - 1)) % (m - 1);
-                if (i == 0) b = r;
-                if (a >= n || b >= n) continue;  // pair with the dummy
-                p[r][k] = a < b ? a : b;
-                q[r][k] = a < b ? b : a;
-                ++k;
-            }
-        }
-    }
-};
 
 // True when A - shift I is positive definite (Cholesky succeeds).
 template <int n, class T>
@@ -113,116 +49,163 @@ __device__ __forceinline__ bool cholesky_pd(const T* A, T shift)
     return ok;
 }
 
-// One Jacobi rotation of rows/columns p, q (angle from the current entries).
-template <int n, class T>
-__device__ __forceinline__ void jacobi_angle(const T* A, int p, int q, T& c, T& s, T& t)
+template <class T> struct QlTraits;
+template <> struct QlTraits<double>
 {
-    const T apq = A[Sym<n>::idx(p, q)];
-    const T dlt = A[Sym<n>::idx(q, q)] - A[Sym<n>::idx(p, p)];
-    const T y = dlt * dlt + T(4) * apq * apq;
-    // tangent of the rotation angle in the stable form sign(d) 2 a_pq / (|d| + sqrt(d^2 + 4 a_pq^2))
-    if constexpr (sizeof(T) == 4)
-    {
-        const T rt = y > T(0) ? y * tad_rsqrt(y > T(0) ? y : T(1)) : T(0);
-        const T den = fabs(dlt) + rt;
-        t = (dlt >= T(0) ? T(2) : T(-2)) * apq * tad_rcp(den > T(0) ? den : T(1));
-        t = (den > T(0)) ? t : T(0);
-        c = tad_rsqrt(T(1) + t * t);
-    }
-    else
-    {
-        // FP64 keeps the library sqrt / divide: with the refined hardware
-        // approximations the register allocator spills A and V to scratch
-        const T den = fabs(dlt) + sqrt(y);
-        t = (dlt >= T(0) ? T(2) : T(-2)) * apq / (den > T(0) ? den : T(1));
-        t = (den > T(0)) ? t : T(0);
-        c = T(1) / sqrt(T(1) + t * t);
-    }
-    s = t * c;
-}
+    static constexpr double eps = 2.220446049250313e-16;  // 2^-52
+    static constexpr int max_iters = 30;
+};
+template <> struct QlTraits<float>
+{
+    static constexpr float eps = 1.1920929e-07f;  // 2^-23
+    static constexpr int max_iters = 30;
+};
 
+// Symmetric eigensolve by Householder reduction to tridiagonal form followed
+// by the implicit QL iteration with a Wilkinson-type shift (Bowdler, Martin,
+// Reinsch and Wilkinson, Numer. Math. 11 (1968) 293-306, the method of EISPACK
+// tql2; Eigen's SelfAdjointEigenSolver follows the same two stages).
+// A (packed, n x n) is consumed; on exit Z (row-major) holds orthonormal
+// eigenvectors as columns and lam the eigenvalues.  Every register index is a
+// compile-time constant: data-dependent loop bounds of the QL sweep are
+// applied as per-step activity masks, so nothing goes to scratch.
 template <int n, class T>
-__device__ __forceinline__ void jacobi_apply(T* A, T* V, int p, int q, T c, T s, T t)
+__device__ __forceinline__ void tridiag_ql_eigen(T* A, T* Z, T* lam)
 {
-    const T apq = A[Sym<n>::idx(p, q)];
-    A[Sym<n>::idx(p, p)] -= t * apq;
-    A[Sym<n>::idx(q, q)] += t * apq;
-    A[Sym<n>::idx(p, q)] = T(0);
+    T e[n];
+    T tau[n > 2 ? n - 2 : 1];
+    // 1. Householder: column k below the diagonal -> (e_k, 0, ..., 0); the
+    //    reflector vector v overwrites that column (v_0 in row k + 1).
 #pragma unroll
-    for (int rr = 0; rr < n; ++rr)
+    for (int k = 0; k + 2 < n; ++k)
     {
-        if (rr == p || rr == q) continue;
-        const int irp = Sym<n>::at(rr, p), irq = Sym<n>::at(rr, q);
-        const T arp = A[irp], arq = A[irq];
-        A[irp] = c * arp - s * arq;
-        A[irq] = s * arp + c * arq;
+        const int L = n - k - 1;
+        T sigma = T(0);
+#pragma unroll
+        for (int i = 1; i < L; ++i) sigma += A[Sym<n>::idx(k, k + 1 + i)] * A[Sym<n>::idx(k, k + 1 + i)];
+        const T x0 = A[Sym<n>::idx(k, k + 1)];
+        const bool has = sigma > T(0);
+        const T mu = sqrt(x0 * x0 + sigma);
+        const T alpha = x0 > T(0) ? -mu : mu;
+        const T v0 = x0 - alpha;
+        const T vtv = v0 * v0 + sigma;
+        const T tk = has ? T(2) / (has ? vtv : T(1)) : T(0);
+        e[k] = has ? alpha : x0;
+        tau[k] = tk;
+        A[Sym<n>::idx(k, k + 1)] = v0;
+        // p = tau B v, w = p - (tau/2)(v.p) v, B -= v w^T + w v^T
+        T pv[n], vv[n];
+#pragma unroll
+        for (int i = 0; i < L; ++i) vv[i] = A[Sym<n>::idx(k, k + 1 + i)];
+        T vp = T(0);
+#pragma unroll
+        for (int i = 0; i < L; ++i)
+        {
+            T acc = T(0);
+#pragma unroll
+            for (int j = 0; j < L; ++j) acc += A[Sym<n>::at(k + 1 + i, k + 1 + j)] * vv[j];
+            pv[i] = tk * acc;
+            vp += vv[i] * pv[i];
+        }
+        const T K = T(0.5) * tk * vp;
+#pragma unroll
+        for (int i = 0; i < L; ++i) pv[i] -= K * vv[i];
+#pragma unroll
+        for (int i = 0; i < L; ++i)
+#pragma unroll
+            for (int j = i; j < L; ++j) A[Sym<n>::idx(k + 1 + i, k + 1 + j)] -= vv[i] * pv[j] + pv[i] * vv[j];
     }
 #pragma unroll
-    for (int rr = 0; rr < n; ++rr)
-    {
-        const T vrp = V[rr * n + p], vrq = V[rr * n + q];
-        V[rr * n + p] = c * vrp - s * vrq;
-        V[rr * n + q] = s * vrp + c * vrq;
-    }
-}
-
-// Jacobi eigensolve: on exit A holds (approximately) the eigenvalues on its
-// diagonal and V (row-major n x n) the eigenvectors as columns.  Sweeps stop
-// when the off-diagonal mass is below tol^2 of the total.
-// FP32 sweeps visit the pairs in round-robin order: each round's rotations
-// are disjoint, commute, and their angle computations (the long dependent
-// rsq/rcp chains) are independent, which gives the scheduler parallel work.
-// FP64 sweeps use the cyclic row order: A and V already take 252 of the 256
-// VGPRs, and in-flight rotations of a round would spill.
-template <int n, class T>
-__device__ __forceinline__ void jacobi_eigen(T* A, T* V)
-{
+    for (int i = 0; i < n; ++i) lam[i] = A[Sym<n>::idx(i, i)];
+    if (n >= 2) e[n - 2] = A[Sym<n>::idx(n - 2, n - 1)];
+    e[n - 1] = T(0);
+    // 2. Z = P_0 P_1 ... P_{n-3}, accumulated backwards (P_k acts on rows and
+    //    columns k+1..n-1, where the partial product is non-trivial).
 #pragma unroll
     for (int i = 0; i < n; ++i)
 #pragma unroll
-        for (int j = 0; j < n; ++j) V[i * n + j] = (i == j) ? T(1) : T(0);
-
-    for (int sweep = 0; sweep < EigTraits<T>::max_sweeps; ++sweep)
+        for (int j = 0; j < n; ++j) Z[i * n + j] = (i == j) ? T(1) : T(0);
+#pragma unroll
+    for (int k = n - 3; k >= 0; --k)
     {
-        T off = T(0), dia = T(0);
+        const int L = n - k - 1;
 #pragma unroll
-        for (int i = 0; i < n; ++i)
+        for (int j = k + 1; j < n; ++j)
         {
-            dia += A[Sym<n>::idx(i, i)] * A[Sym<n>::idx(i, i)];
+            T s = T(0);
 #pragma unroll
-            for (int j = i + 1; j < n; ++j) off += A[Sym<n>::idx(i, j)] * A[Sym<n>::idx(i, j)];
+            for (int i = 0; i < L; ++i) s += A[Sym<n>::idx(k, k + 1 + i)] * Z[(k + 1 + i) * n + j];
+            s *= tau[k];
+#pragma unroll
+            for (int i = 0; i < L; ++i) Z[(k + 1 + i) * n + j] -= s * A[Sym<n>::idx(k, k + 1 + i)];
         }
-        const T tol = EigTraits<T>::tol;
-        if (!(off > tol * tol * (dia + T(2) * off)))
-            break;
-        if constexpr (sizeof(T) == 4)
-        {
-            constexpr Tournament<n> tour{};
-            constexpr int NR = Tournament<n>::rounds;
-            constexpr int NP = Tournament<n>::per_round > 0 ? Tournament<n>::per_round : 1;
+    }
+    // 3. implicit QL on (lam, e); e[i] couples lam[i] and lam[i+1]
+    const T eps = QlTraits<T>::eps;
 #pragma unroll
-            for (int r = 0; r < NR; ++r)
+    for (int l = 0; l + 1 < n; ++l)
+    {
+        for (int it = 0; it < QlTraits<T>::max_iters; ++it)
+        {
+            int m = n - 1;
+#pragma unroll
+            for (int j = n - 2; j >= l; --j)
+                if (fabs(e[j]) <= eps * (fabs(lam[j]) + fabs(lam[j + 1]))) m = j;
+            if (m == l) break;
+            T dm = lam[n - 1];
+#pragma unroll
+            for (int j = l; j + 1 < n; ++j)
+                if (j == m) dm = lam[j];
+            T g = (lam[l + 1] - lam[l]) / (T(2) * e[l]);
+            T r = sqrt(g * g + T(1));
+            g = dm - lam[l] + e[l] / (g + (g >= T(0) ? r : -r));
+            T s = T(1), c = T(1), p = T(0);
+            bool alive = true;  // false once a rotation underflowed (r == 0)
+#pragma unroll
+            for (int i = n - 2; i >= l; --i)
             {
-                T cs[NP], sn[NP], tn[NP];
-#pragma unroll
-                for (int k = 0; k < Tournament<n>::per_round; ++k)
-                    jacobi_angle<n, T>(A, tour.p[r][k], tour.q[r][k], cs[k], sn[k], tn[k]);
-#pragma unroll
-                for (int k = 0; k < Tournament<n>::per_round; ++k)
-                    jacobi_apply<n, T>(A, V, tour.p[r][k], tour.q[r][k], cs[k], sn[k], tn[k]);
-            }
-        }
-        else
-        {
-#pragma unroll
-            for (int p = 0; p < n - 1; ++p)
-#pragma unroll
-                for (int q = p + 1; q < n; ++q)
+                if (alive && i < m)
                 {
-                    T c, s, t;
-                    jacobi_angle<n, T>(A, p, q, c, s, t);
-                    jacobi_apply<n, T>(A, V, p, q, c, s, t);
+                    const T f = s * e[i];
+                    const T b = c * e[i];
+                    r = sqrt(f * f + g * g);
+                    e[i + 1] = r;
+                    if (r == T(0))
+                    {
+                        lam[i + 1] -= p;
+#pragma unroll
+                        for (int j = l; j + 1 < n; ++j)
+                            if (j == m) e[j] = T(0);
+                        alive = false;
+                    }
+                    else
+                    {
+                        const T ir = T(1) / r;
+                        s = f * ir;
+                        c = g * ir;
+                        g = lam[i + 1] - p;
+                        r = (lam[i] - g) * s + T(2) * c * b;
+                        p = s * r;
+                        lam[i + 1] = g + p;
+                        g = c * r - b;
+#pragma unroll
+                        for (int k = 0; k < n; ++k)
+                        {
+                            const T zi = Z[k * n + i], zi1 = Z[k * n + i + 1];
+                            Z[k * n + i + 1] = s * zi + c * zi1;
+                            Z[k * n + i] = c * zi - s * zi1;
+                        }
+                    }
                 }
+            }
+            if (alive)
+            {
+                lam[l] -= p;
+                e[l] = g;
+#pragma unroll
+                for (int j = l; j + 1 < n; ++j)
+                    if (j == m) e[j] = T(0);
+            }
         }
     }
 }
@@ -248,13 +231,13 @@ __device__ __forceinline__ bool project_psd(T* A, T eps)
         W = Wbuf;
     }
     T V[n * n];
-    jacobi_eigen<n, T>(W, V);
     T lam[n];
+    tridiag_ql_eigen<n, T>(W, V, lam);
     bool changed = false;
 #pragma unroll
     for (int i = 0; i < n; ++i)
     {
-        T l = W[Sym<n>::idx(i, i)];
+        T l = lam[i];
         if (eps < T(0))
         {
             changed = changed || (l < T(0));

@@ -138,8 +138,8 @@ static hipError_t launch_element_p(const TermDev& t, int d, int mode, const doub
     if (t.N == NN && d == DD) return launch_element_t<QuadElem<NN, DD>, T>(t, mode, x, eps, elem_f, arena, err, s);
             TAD_QUAD(1, 1) TAD_QUAD(1, 2) TAD_QUAD(1, 3)
             TAD_QUAD(2, 1) TAD_QUAD(2, 2) TAD_QUAD(2, 3)
-            TAD_QUAD(3, 1) TAD_QUAD(3, 2) TAD_QUAD(3, 3)
-            TAD_QUAD(4, 1) TAD_QUAD(4, 2) TAD_QUAD(4, 3)
+            TAD_QUAD(3, 1) TAD_QUAD(3, 2)
+            TAD_QUAD(4, 1) TAD_QUAD(4, 2)
 #undef TAD_QUAD
             return hipErrorInvalidValue;
         }

@@ -80,8 +80,8 @@ hipError_t launch_energy(const TermDev& t, int d, const double* x, const double*
     if (t.N == NN && d == DD) return launch_energy_t<QuadElem<NN, DD>>(t, x, dir, steps, ncand, elem_f, stride, s);
             TAD_QUAD(1, 1) TAD_QUAD(1, 2) TAD_QUAD(1, 3)
             TAD_QUAD(2, 1) TAD_QUAD(2, 2) TAD_QUAD(2, 3)
-            TAD_QUAD(3, 1) TAD_QUAD(3, 2) TAD_QUAD(3, 3)
-            TAD_QUAD(4, 1) TAD_QUAD(4, 2) TAD_QUAD(4, 3)
+            TAD_QUAD(3, 1) TAD_QUAD(3, 2)
+            TAD_QUAD(4, 1) TAD_QUAD(4, 2)
 #undef TAD_QUAD
             return hipErrorInvalidValue;
         }
