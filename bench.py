@@ -30,6 +30,23 @@ sys.path.insert(0, ROOT)
 
 METRIC = "M elements/s for eval_with_hessian_proj; Newton iters/s at 1.3M tets"
 HBM_PEAK_GBS = 8000.0
+
+
+def pmc_traffic(args, pmc_name):
+    """HBM bytes per launch of `pmc_name` from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, scaled by the plain
+    512 MiB copy), or None when no summary exists for it."""
+    if not pmc_name:
+        return None
+    tag = f"{args.workload}_{args.m if args.workload == 'twist_cube' else args.disks}_{args.precision}"
+    path = os.path.join(ROOT, "profiles", f"traffic_{tag}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 6 and cells[0] == f"`{pmc_name}`":
+            return (float(cells[4]) + float(cells[5])) * 1e6
+    return None
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -46,10 +63,13 @@ def main():
     ap.add_argument("--workload", choices=["twist_cube", "disk_batch"], default="twist_cube")
     ap.add_argument("--m", type=int, default=60, help="cube cells per side")
     ap.add_argument("--disks", type=int, default=256, help="disks in the disk_batch workload")
-    ap.add_argument("--autotune", action="store_true", help="pick the element-kernel launch shape first")
+    ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
+                    help="pick the element-kernel launch shape first (untimed; default on)")
     ap.add_argument("--pcg-tol", type=float, default=1e-10)
     ap.add_argument("--cpu-sample", type=int, default=60000, help="tets in the CPU baseline sample (0: skip)")
     ap.add_argument("--no-newton", action="store_true")
+    ap.add_argument("--newton-iters", type=int, default=30,
+                    help="Newton iterations of the converged-energy comparison (mixed precision)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MB (a plain copy kernel of known "
                          "size, used to scale the FETCH_SIZE / WRITE_SIZE counters)")
@@ -114,8 +134,9 @@ def main():
     ctx.synchronize()
 
     # ---------------- phase A: eval_with_hessian_proj ----------------
-    ctx.reset_stats()
-    ctx.enable_timing(True)
+    # timed region without per-kernel events; an identical second pass records
+    # HIP events around every launch (on the library's stream) for the kernel
+    # table and the roofline (events add gaps between kernels, not to them)
     torch.cuda.synchronize()
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -124,6 +145,11 @@ def main():
     ctx.synchronize()
     torch.cuda.synchronize()
     tA = time.perf_counter() - t0
+    ctx.reset_stats()
+    ctx.enable_timing(True)
+    for _ in range(args.steps):
+        eval_hp(x0)
+    ctx.synchronize()
     ctx.enable_timing(False)
     statsA = {k: ctx.kernel_stats(k) for k in ["element", "assemble_gradient", "assemble_hessian", "energy_sum"]}
     ms_eval = tA / args.steps * 1e3
@@ -141,8 +167,7 @@ def main():
             log(f"[bench] warmup newton {i}: f {st['f']:.10g} -> {st['f_new']:.10g} cg {st['cg_iters']} "
                 f"ls {st['ls_trials']} step {st['step']:.3g}")
         ctx.set_stream(None)
-        ctx.reset_stats()
-        ctx.enable_timing(True)
+        x_start = x.clone()
         torch.cuda.synchronize()
         ctx.synchronize()
         t0 = time.perf_counter()
@@ -152,7 +177,16 @@ def main():
         ctx.synchronize()
         torch.cuda.synchronize()
         tB = time.perf_counter() - t0
+        # the same iterations again from the same point with per-kernel events
+        # (deterministic: identical CG counts and bits)
+        xi = x_start.clone()
+        ctx.reset_stats()
+        ctx.enable_timing(True)
+        sts_i = [fn.newton_step(xi, opts) for _ in range(args.steps)]
+        ctx.synchronize()
         ctx.enable_timing(False)
+        if [s["cg_iters"] for s in sts_i] != [s["cg_iters"] for s in sts] or not torch.equal(xi, x):
+            raise RuntimeError("instrumented Newton pass diverged from the timed pass")
         for st in sts:
             log(f"[bench] newton: f {st['f']:.10g} -> {st['f_new']:.10g} cg {st['cg_iters']} ls {st['ls_trials']} "
                 f"step {st['step']:.3g}")
@@ -191,7 +225,18 @@ def main():
         # first scalar-row offset of each block row, z read once, s written
         traffic_alg = 8 * nnz + 4 * nb + 4 * (P.n_handles + 1) + 4 * P.n_handles + 2 * 8 * n
         roof = {"kernel": "k_spmv<3,16>", "bound": "hbm", "achieved": traffic_alg / avg_s / 1e9, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "bytes_per_launch": traffic_alg}
+                "unit": "GB/s", "bytes_per_launch": traffic_alg, "pmc_name": "tad::k_spmv<3, 16>"}
+    elif dom == "assemble_hessian":
+        # contributions read (n_e N^2 blocks of d x d in the element precision)
+        # and the CSR values written
+        tsz = 8 if args.precision == "fp64" else 4
+        d = P.d
+        nblk = sum(t.conn.shape[0] * t.valence ** 2 for t in P.terms)
+        per_launch = nblk * d * d * tsz + 8 * nnz
+        kname = f"k_assemble_hessian<{'double' if tsz == 8 else 'float'},{d}>"
+        roof = {"kernel": kname, "bound": "hbm", "achieved": per_launch / avg_s / 1e9, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "bytes_per_launch": per_launch,
+                "pmc_name": f"tad::k_assemble_hessian<{'double' if tsz == 8 else 'float'}, {d}>"}
     elif dom == "element":
         Nv = P.terms[0].valence
         r = P.terms[0].params.shape[1]
@@ -206,7 +251,7 @@ def main():
         roof["frac"] = roof["achieved"] / roof["peak"]
     roof["avg_launch_ms"] = avg_s * 1e3
     roof["launches"] = cnt
-    roof["traffic"] = None
+    roof["traffic"] = pmc_traffic(args, roof.pop("pmc_name", None))
     kernel_ms = {k: {"launches": v[0], "total_ms": round(v[1], 4)} for k, v in stats.items() if v[0]}
 
     # ---------------- CPU baseline (oracle on a bounded sample) ----------------
@@ -225,6 +270,47 @@ def main():
                          f"per-element eigen-projection, serial triplets + sorted CSR), median of 2 after warm-up",
                "wall_s": round(time.time() - t0, 2)}
         log(f"[bench] cpu baseline: {cpu['value']:.4f} M elements/s on {cpu['cores']} threads")
+
+    # ---------------- mixed precision: errors against the FP64 path ----------------
+    prec = None
+    if args.precision == "mixed":
+        f64 = tad.ScalarFunction.from_problem(P, ctx, precision="fp64")
+        fm, gm, Hm = fn.eval_with_hessian_proj(x0)
+        fd, gd, Hd = f64.eval_with_hessian_proj(x0)
+
+        def errs(a, b):
+            dlt = (a - b).abs()
+            return {"max_rel": float(dlt.max() / b.abs().max()),
+                    "rms_rel": float(torch.sqrt((dlt * dlt).mean()) / torch.sqrt((b * b).mean()))}
+
+        prec = {"reference": "FP64 GPU path at x0 (itself equal to the oracle within 1e-14 normwise in the parity "
+                             "tests)",
+                "f_rel": abs(fm - fd) / abs(fd), "g": errs(gm, gd), "H_proj": errs(Hm, Hd)}
+        if newton is not None:
+            opts64 = tad.newton_default_opts(solve_tol=args.pcg_tol)
+            x64 = x0.clone()
+            for _ in range(args.warmup):
+                f64.newton_step(x64, opts64)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st64 = [f64.newton_step(x64, opts64) for _ in range(args.steps)]
+            ctx.synchronize()
+            t64 = time.perf_counter() - t0
+            prec["newton_ms_per_iter_fp64"] = t64 / args.steps * 1e3
+            prec["newton_speedup_vs_fp64"] = (t64 / args.steps * 1e3) / newton["ms_per_iter"]
+            # converged energies: the Newton run of the workload (30 iterations
+            # from x0) in both precisions, untimed
+            xa, xb = x0.clone(), x0.clone()
+            for _ in range(args.newton_iters):
+                f64.newton_step(xa, opts64)
+                fn.newton_step(xb, opts)
+            fa, fb = f64.eval(xa), f64.eval(xb)
+            prec["converged_newton_iters"] = args.newton_iters
+            prec["converged_f_fp64"] = fa
+            prec["converged_f_mixed"] = fb
+            prec["converged_f_rel"] = abs(fa - fb) / abs(fa)
+        log(f"[bench] mixed vs fp64: {json.dumps(prec)}")
+        f64.close()
 
     out = {
         "metric": METRIC,
@@ -249,6 +335,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "newton": newton,
+        "mixed_precision": prec,
         "kernels": kernel_ms,
     }
     print(json.dumps(out), flush=True)

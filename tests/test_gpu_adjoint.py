@@ -144,7 +144,11 @@ def _fd_check(ctx, m, n_dirs, eps=1e-6):
 
     def loss(th):
         xx = optimum(th, rp, xs)
+        h = rp.last_history
+        info.append((len(h), h[-1]["grad_norm"], h[-1]["step"]))
         return float(((xx - xt) ** 2).sum())
+
+    info = []
 
     t0 = time.time()
     worst = 0.0
@@ -156,7 +160,7 @@ def _fd_check(ctx, m, n_dirs, eps=1e-6):
         ad = float(g @ v)
         err = abs(fd - ad) / (abs(ad) + 1e-9 * np.linalg.norm(g))
         worst = max(worst, err)
-        assert err <= FD_TOL, (k, fd, ad, err)
+        assert err <= FD_TOL, (k, fd, ad, err, info[-2:])
     print(f"m={m}: {n_dirs} directions, worst relative FD error {worst:.3e}, {time.time() - t0:.1f} s")
 
 

@@ -82,41 +82,44 @@ class RestShapeProblem:
         self.fn.set_params(self.term, tet_rest_params(th.reshape(-1), self.conn, self.w_scale))
 
     def minimize(self, x, max_iters: int = 200, switch_rel: float = 1e-12, proj_tol: float = 1e-10,
-                 exact_tol: float = 1e-13):
+                 exact_tol: float = 1e-12):
         """Newton to the optimum in place on the device tensor x.
 
-        Projected Newton (newton_step with eps 1e-9) until the Newton decrement
-        falls below switch_rel |f|, then Newton with the unprojected Hessian
-        (quadratic convergence near a strict minimum) until the gradient stops
-        shrinking: a step that leaves ||g|| above a quarter of the previous
-        step's value has reached the rounding floor (the Armijo test accepts
-        f_new == f there, so it cannot signal convergence by itself).
-        Returns a list of per-iteration stats dicts.
+        Projected Newton steps with line search (newton_step, eps 1e-9) until
+        the Newton decrement falls below switch_rel |f|; then full Newton steps
+        with the unprojected Hessian (eval_with_derivatives + solve) until the
+        gradient norm stops shrinking by at least 4x, i.e. reaches its rounding
+        floor.  In that last phase the decrement is far below one ulp of f, so
+        an energy-based line search could not tell steps apart; near a strict
+        minimum full steps converge quadratically.  Returns per-iteration stats.
         """
+        import torch
         proj = newton_default_opts(solve_tol=proj_tol)
-        exact = newton_default_opts(projection_eps=math.nan, solve_tol=exact_tol)
         hist = []
-        phase = "proj"
-        prev_g = math.inf
         for _ in range(max_iters):
-            try:
-                st = self.fn.newton_step(x, exact if phase == "exact" else proj)
-            except TadError as e:
-                if phase != "exact" or e.status != 4:
-                    raise
-                phase = "proj"  # unprojected system not positive definite here
-                prev_g = math.inf
-                continue
-            st["phase"] = phase
+            st = self.fn.newton_step(x, proj)
+            st["phase"] = "proj"
             hist.append(st)
-            if phase == "proj":
-                small = st["newton_decrement"] <= switch_rel * max(abs(st["f"]), 1e-300)
-                if small or st["ls_failed"]:
-                    phase = "exact"
-            else:
-                if st["ls_failed"] or st["grad_norm"] > 0.25 * prev_g or st["newton_decrement"] == 0.0:
-                    break
-                prev_g = st["grad_norm"]
+            if st["ls_failed"] or st["newton_decrement"] <= switch_rel * max(abs(st["f"]), 1e-300):
+                break
+        prev_g = math.inf
+        x_prev = None
+        for _ in range(max_iters):
+            f, g, H = self.fn.eval_with_derivatives(x)
+            gn = float(torch.linalg.vector_norm(g))
+            if gn > 0.25 * prev_g:
+                if gn > prev_g:
+                    x.copy_(x_prev)  # the last step did not help: keep the better point
+                hist.append({"phase": "exact", "f": f, "f_new": f, "grad_norm": gn, "step": 0.0,
+                             "newton_decrement": 0.0, "cg_iters": 0, "ls_failed": 0})
+                break
+            d, sst = self.fn.solve(g, H=H, tol=exact_tol)
+            x_prev = x.clone()
+            x.sub_(d)
+            hist.append({"phase": "exact", "f": f, "f_new": f, "grad_norm": gn, "step": 1.0,
+                         "newton_decrement": 0.5 * float(torch.dot(d, g)), "cg_iters": sst.iters,
+                         "ls_failed": 0})
+            prev_g = gn
         return hist
 
     # -- adjoint ---------------------------------------------------------------

@@ -83,12 +83,13 @@ __device__ __forceinline__ M3 edges(const double* v)
     return r;
 }
 
-// out: 12 values per element in the gradient slots of the element's record.
+// out: the element's 4 x 3 values, written to its gradient slots
+// (gpos[a n + e]: slot of vertex a), summed per vertex by the gradient assembly.
 __global__ __launch_bounds__(256) void k_tet_rest_vjp(int64_t n, const int32_t* __restrict__ conn,
                                                       const double* __restrict__ x,
                                                       const double* __restrict__ rest,
                                                       const double* __restrict__ lam, double w_scale,
-                                                      double* __restrict__ rec, int rec_stride)
+                                                      const uint32_t* __restrict__ gpos, double* __restrict__ gslots)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
@@ -105,7 +106,9 @@ __global__ __launch_bounds__(256) void k_tet_rest_vjp(int64_t n, const int32_t* 
             Rv[3 * a + i] = rest[3 * v + i];
         }
     }
-    double* out = rec + e * rec_stride;
+    double* dst[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) dst[a] = gslots + (int64_t)gpos[a * n + e] * 3;
     const M3 M = edges(Xv);
     const M3 Mr = edges(Rv);
     const double detM = det(M);
@@ -114,7 +117,9 @@ __global__ __launch_bounds__(256) void k_tet_rest_vjp(int64_t n, const int32_t* 
     if (!(detM > 0.0) || detR == 0.0)
     {
 #pragma unroll
-        for (int k = 0; k < 12; ++k) out[k] = 0.0;
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) dst[a][i] = 0.0;
         return;
     }
     const M3 R = inverse(Mr, detR);
@@ -156,36 +161,19 @@ __global__ __launch_bounds__(256) void k_tet_rest_vjp(int64_t n, const int32_t* 
 #pragma unroll
         for (int k = 0; k < 3; ++k)
         {
-            out[3 * (k + 1) + i] = w * Gm.a[i][k];
+            dst[k + 1][i] = w * Gm.a[i][k];
             sum += Gm.a[i][k];
         }
-        out[i] = -w * sum;
+        dst[0][i] = -w * sum;
     }
 }
 
-__global__ void k_zero_grad_slots(int64_t n, int width, int rec_stride, double* __restrict__ rec)
-{
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    for (int k = 0; k < width; ++k) rec[e * rec_stride + k] = 0.0;
-}
-
-hipError_t launch_tet_rest_vjp(const TermDev& t, int d, const double* x, const double* rest, const double* lam,
-                               double w_scale, double* arena, hipStream_t s)
+hipError_t launch_tet_rest_vjp(const TermDev& t, const double* x, const double* rest, const double* lam,
+                               double w_scale, double* G, hipStream_t s)
 {
     if (t.n_elem == 0) return hipSuccess;
-    const int RS = record_size(t.N, d);
     const unsigned grid = (unsigned)((t.n_elem + 255) / 256);
-    k_tet_rest_vjp<<<grid, 256, 0, s>>>(t.n_elem, t.conn, x, rest, lam, w_scale, arena + t.rec_base, RS);
-    return hipGetLastError();
-}
-
-hipError_t launch_zero_grad_slots(const TermDev& t, int d, double* arena, hipStream_t s)
-{
-    if (t.n_elem == 0) return hipSuccess;
-    const int RS = record_size(t.N, d);
-    const unsigned grid = (unsigned)((t.n_elem + 255) / 256);
-    k_zero_grad_slots<<<grid, 256, 0, s>>>(t.n_elem, t.N * d, RS, arena + t.rec_base);
+    k_tet_rest_vjp<<<grid, 256, 0, s>>>(t.n_elem, t.conn, x, rest, lam, w_scale, t.gpos, G);
     return hipGetLastError();
 }
 

@@ -6,11 +6,13 @@
 //   tad_fn   -- ScalarFunction: variable dimension d, handle count, objective
 //               terms added by tad_fn_add_elements (ScalarObjectiveTerm), plus
 //               the device-resident state: connectivity and parameters (SoA),
-//               the CSR pattern, the element-order gather maps, the element
-//               record arena and the solver work vectors.
+//               the CSR pattern, the scatter maps (slot of every element
+//               block / gradient part), the destination-sorted contribution
+//               arrays and the solver work vectors.
 //
-// Evaluation = element kernel per term (records into the arena) -> gather of
-// g and CSR values in element order -> deterministic FP64 energy sum.
+// Evaluation = element kernel per term (scatters its blocks to their slots)
+// -> contiguous per-destination sums, in element order, into g and the CSR
+// values -> deterministic FP64 energy sum.
 #include "tinyad_amd.h"
 #include "tad_internal.hpp"
 
@@ -63,8 +65,6 @@ int energy_param_count(int energy, int d, int N)
         default: return -1;
     }
 }
-
-constexpr int rec_size(int N, int d) { return N * d + (N * (N + 1) / 2) * d * d; }
 
 template <class T>
 struct DevBuf
@@ -197,17 +197,18 @@ struct tad_fn_s
     int precision = 0;
     std::vector<TermDev> terms;
     std::vector<HostTerm> hterms;
-    int64_t arena_size = 0;  // entries of the record arena
 
     // pattern (block rows = handles)
     bool pattern_ready = false;
     int64_t n_blocks = 0, nnz = 0;
     DevBuf<int32_t> brow_ptr, bcol, block_row, diag_block, row_ptr, col_idx;
-    DevBuf<int64_t> hmap_ptr, gmap_ptr;
-    DevBuf<uint32_t> hmap, gmap;
+    // destination-sorted contribution slots: CSR block b owns Hessian slots
+    // [hslot_ptr[b], hslot_ptr[b+1]), handle u gradient slots [gslot_ptr[u], ..)
+    int64_t n_hslots = 0, n_gslots = 0;
+    DevBuf<int64_t> hslot_ptr, gslot_ptr;
 
     // evaluation buffers
-    DevBuf<unsigned char> arena;
+    DevBuf<unsigned char> gc_buf, hc_buf;  // contributions (FP64 or FP32 by precision)
     DevBuf<double> elem_f, x_buf, g_buf, H_buf, partials, results, steps;
     DevBuf<int> err;
     // line search
@@ -222,8 +223,8 @@ struct tad_fn_s
     std::vector<int64_t> seg_offsets;
     DevBuf<int64_t> seg_ranges;
     DevBuf<double> seg_f;
-    // adjoint products (FP64 records when the arena holds FP32 ones)
-    DevBuf<double> adj_arena, adj_rest, adj_lam;
+    // adjoint products (FP64 gradient slots when the contributions are FP32)
+    DevBuf<double> adj_g, adj_rest, adj_lam;
 
     hipStream_t s() const { return ctx->stream; }
 };
@@ -239,8 +240,13 @@ tad_status check_fn(tad_fn fn)
 void free_pattern(tad_fn fn)
 {
     fn->brow_ptr.release(); fn->bcol.release(); fn->block_row.release(); fn->diag_block.release();
-    fn->row_ptr.release(); fn->col_idx.release(); fn->hmap_ptr.release(); fn->gmap_ptr.release();
-    fn->hmap.release(); fn->gmap.release();
+    fn->row_ptr.release(); fn->col_idx.release(); fn->hslot_ptr.release(); fn->gslot_ptr.release();
+    for (TermDev& t : fn->terms)
+    {
+        if (t.gpos) (void)hipFree(t.gpos);
+        if (t.hpos) (void)hipFree(t.hpos);
+        t.gpos = t.hpos = nullptr;
+    }
     fn->pattern_ready = false;
 }
 
@@ -294,7 +300,9 @@ tad_status build_pattern(tad_fn fn)
         return (int64_t)(std::lower_bound(b, e, v) - bcol.data());
     };
 
-    // 2. Hessian gather map (element order within each block)
+    // 2. contribution slots: per destination (CSR block / handle) a contiguous
+    //    run, ordered by term and element; per element the slot of each of its
+    //    N^2 blocks and N gradient parts (SoA, coalesced in the element kernel)
     std::vector<int64_t> hptr(nb + 1, 0);
     std::vector<int64_t> gptr(nh + 1, 0);
     for (const HostTerm& t : fn->hterms)
@@ -307,31 +315,25 @@ tad_status build_pattern(tad_fn fn)
             }
     for (int64_t k = 0; k < nb; ++k) hptr[k + 1] += hptr[k];
     for (int64_t u = 0; u < nh; ++u) gptr[u + 1] += gptr[u];
-    std::vector<uint32_t> hmap(hptr[nb]), gmap(gptr[nh]);
+    if (hptr[nb] >= 4294967295LL || gptr[nh] >= 4294967295LL)
+        return fail(TAD_ERR_UNSUPPORTED, "more than 2^32 element contributions");
+    std::vector<std::vector<uint32_t>> gpos(fn->hterms.size()), hpos(fn->hterms.size());
     {
         std::vector<int64_t> hc(hptr.begin(), hptr.end() - 1), gc(gptr.begin(), gptr.end() - 1);
         for (size_t ti = 0; ti < fn->hterms.size(); ++ti)
         {
             const HostTerm& t = fn->hterms[ti];
-            const TermDev& td = fn->terms[ti];
-            const int RS = rec_size(t.N, d);
-            for (int64_t e = 0; e < t.n_elem; ++e)
-            {
-                const int64_t rec = td.rec_base + e * RS;
+            const int64_t ne = t.n_elem;
+            gpos[ti].resize((size_t)ne * t.N);
+            hpos[ti].resize((size_t)ne * t.N * t.N);
+            for (int64_t e = 0; e < ne; ++e)
                 for (int a = 0; a < t.N; ++a)
                 {
                     const int32_t u = t.conn[e * t.N + a];
-                    gmap[gc[u]++] = (uint32_t)(rec + a * d);
+                    gpos[ti][a * ne + e] = (uint32_t)gc[u]++;
                     for (int b = 0; b < t.N; ++b)
-                    {
-                        const int lo = std::min(a, b), hi = std::max(a, b);
-                        const int bi = lo * t.N - (lo * (lo - 1)) / 2 + (hi - lo);
-                        const int64_t off = rec + t.N * d + (int64_t)bi * d * d;
-                        const uint32_t flag = (a > b) ? 0x80000000u : 0u;
-                        hmap[hc[find_block(u, t.conn[e * t.N + b])]++] = (uint32_t)off | flag;
-                    }
+                        hpos[ti][(a * t.N + b) * ne + e] = (uint32_t)hc[find_block(u, t.conn[e * t.N + b])]++;
                 }
-            }
         }
     }
     // 3. scalar CSR
@@ -367,11 +369,20 @@ tad_status build_pattern(tad_fn fn)
     TAD_HIP(up(fn->diag_block, diag));
     TAD_HIP(up(fn->row_ptr, row_ptr));
     TAD_HIP(up(fn->col_idx, col_idx));
-    TAD_HIP(up(fn->hmap_ptr, hptr));
-    TAD_HIP(up(fn->hmap, hmap));
-    TAD_HIP(up(fn->gmap_ptr, gptr));
-    TAD_HIP(up(fn->gmap, gmap));
+    TAD_HIP(up(fn->hslot_ptr, hptr));
+    TAD_HIP(up(fn->gslot_ptr, gptr));
+    for (size_t ti = 0; ti < fn->terms.size(); ++ti)
+    {
+        TermDev& td = fn->terms[ti];
+        if (td.n_elem == 0) continue;
+        TAD_HIP(hipMalloc((void**)&td.gpos, gpos[ti].size() * sizeof(uint32_t)));
+        TAD_HIP(hipMalloc((void**)&td.hpos, hpos[ti].size() * sizeof(uint32_t)));
+        TAD_HIP(hipMemcpyAsync(td.gpos, gpos[ti].data(), gpos[ti].size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        TAD_HIP(hipMemcpyAsync(td.hpos, hpos[ti].data(), hpos[ti].size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    }
     TAD_HIP(hipStreamSynchronize(s));
+    fn->n_hslots = hptr[nb];
+    fn->n_gslots = gptr[nh];
     fn->n_blocks = nb;
     fn->nnz = nnz;
     fn->pattern_ready = true;
@@ -386,7 +397,8 @@ tad_status ensure_ready(tad_fn fn)
     }
     if (!fn->pattern_ready && !fn->terms.empty()) TAD_TRY(build_pattern(fn));
     const size_t tsz = fn->precision == TAD_PREC_FP64 ? sizeof(double) : sizeof(float);
-    TAD_HIP(fn->arena.ensure((size_t)fn->arena_size * tsz));
+    TAD_HIP(fn->gc_buf.ensure((size_t)fn->n_gslots * fn->d * tsz));
+    TAD_HIP(fn->hc_buf.ensure((size_t)fn->n_hslots * fn->d * fn->d * tsz));
     TAD_HIP(fn->elem_f.ensure((size_t)fn->n_elem_total));
     TAD_HIP(fn->x_buf.ensure((size_t)fn->n_vars));
     TAD_HIP(fn->g_buf.ensure((size_t)fn->n_vars));
@@ -415,20 +427,20 @@ tad_status eval_core(tad_fn fn, const double* x_dev, int mode, double eps, doubl
     TAD_HIP(hipMemsetAsync(fn->err.p, 0, sizeof(int), s));
     tm.begin("element", s);
     for (const TermDev& t : fn->terms)
-        TAD_HIP(launch_element(t, fn->d, fn->precision, mode, x_dev, eps, fn->elem_f.p, fn->arena.p, fn->err.p, s));
+        TAD_HIP(launch_element(t, fn->d, fn->precision, mode, x_dev, eps, fn->elem_f.p, fn->gc_buf.p, fn->hc_buf.p,
+                               fn->err.p, s));
     tm.end("element", s);
     if (g_dev)
     {
         tm.begin("assemble_gradient", s);
-        TAD_HIP(launch_assemble_gradient(fn->d, fn->precision, fn->n_handles, fn->gmap_ptr.p, fn->gmap.p, fn->arena.p,
-                                         g_dev, s));
+        TAD_HIP(launch_assemble_gradient(fn->d, fn->precision, fn->n_handles, fn->gslot_ptr.p, fn->gc_buf.p, g_dev, s));
         tm.end("assemble_gradient", s);
     }
     if (H_dev && mode >= MODE_HESS)
     {
         tm.begin("assemble_hessian", s);
         TAD_HIP(launch_assemble_hessian(fn->d, fn->precision, fn->n_blocks, fn->block_row.p, fn->brow_ptr.p,
-                                        fn->hmap_ptr.p, fn->hmap.p, fn->arena.p, fn->row_ptr.p, H_dev, s));
+                                        fn->hslot_ptr.p, fn->hc_buf.p, fn->row_ptr.p, H_dev, s));
         tm.end("assemble_hessian", s);
     }
     tm.begin("energy_sum", s);
@@ -712,11 +724,12 @@ tad_status tad_fn_destroy(tad_fn fn)
         if (t.params) (void)hipFree(t.params);
     }
     free_pattern(fn);
-    fn->arena.release(); fn->elem_f.release(); fn->x_buf.release(); fn->g_buf.release(); fn->H_buf.release();
+    fn->gc_buf.release(); fn->hc_buf.release(); fn->elem_f.release(); fn->x_buf.release(); fn->g_buf.release(); fn->H_buf.release();
     fn->partials.release(); fn->results.release(); fn->steps.release(); fn->err.release();
     fn->ls_f.release(); fn->x_tmp.release();
     fn->Dinv.release(); fn->r.release(); fn->z.release(); fn->p.release(); fn->q.release(); fn->sol.release();
     fn->scal.release(); fn->pcg_parts.release(); fn->seg_ranges.release(); fn->seg_f.release();
+    fn->adj_g.release(); fn->adj_rest.release(); fn->adj_lam.release();
     tad_ctx ctx = fn->ctx;
     delete fn;
     ctx_release(ctx);
@@ -757,16 +770,12 @@ tad_status tad_fn_add_elements(tad_fn fn, tad_energy energy, int valence, int64_
         }
     for (int64_t i = 0; i < n_elem * (int64_t)np; ++i)
         if (!std::isfinite(params[i])) return fail(TAD_ERR_INVALID, "non-finite element parameter");
-    const int RS = rec_size(N, d);
-    if ((double)(fn->arena_size + n_elem * (int64_t)RS) >= 2147483647.0)
-        return fail(TAD_ERR_UNSUPPORTED, "element record arena exceeds 2^31 entries");
 
     TermDev t;
     t.energy = energy;
     t.N = N;
     t.n_params = np;
     t.n_elem = n_elem;
-    t.rec_base = fn->arena_size;
     t.f_base = fn->n_elem_total;
     if (n_elem > 0)
     {
@@ -789,7 +798,6 @@ tad_status tad_fn_add_elements(tad_fn fn, tad_energy energy, int valence, int64_
     h.conn.assign(conn, conn + n_elem * N);
     fn->terms.push_back(t);
     fn->hterms.push_back(std::move(h));
-    fn->arena_size += n_elem * RS;
     fn->n_elem_total += n_elem;
     free_pattern(fn);
     fn->h_valid = false;
@@ -1139,13 +1147,13 @@ tad_status tad_fn_autotune(tad_fn fn, const double* x, tad_mem mem, double eps, 
             t.tune_block = c.block;
             t.tune_grid = c.grid;
             t.tune_map = c.map;
-            TAD_HIP(launch_element(t, fn->d, fn->precision, MODE_HESS_PROJ, xd, eps, fn->elem_f.p, fn->arena.p,
+            TAD_HIP(launch_element(t, fn->d, fn->precision, MODE_HESS_PROJ, xd, eps, fn->elem_f.p, fn->gc_buf.p, fn->hc_buf.p,
                                    fn->err.p, s));
             float best = 1e30f;
             for (int rep_i = 0; rep_i < 3; ++rep_i)
             {
                 TAD_HIP(hipEventRecord(e0, s));
-                TAD_HIP(launch_element(t, fn->d, fn->precision, MODE_HESS_PROJ, xd, eps, fn->elem_f.p, fn->arena.p,
+                TAD_HIP(launch_element(t, fn->d, fn->precision, MODE_HESS_PROJ, xd, eps, fn->elem_f.p, fn->gc_buf.p, fn->hc_buf.p,
                                        fn->err.p, s));
                 TAD_HIP(hipEventRecord(e1, s));
                 TAD_HIP(hipEventSynchronize(e1));
@@ -1235,23 +1243,21 @@ tad_status tad_fn_rest_vjp(tad_fn fn, int term, const double* x, const double* r
         rd = fn->adj_rest.p;
         ld = fn->adj_lam.p;
     }
-    // FP64 records: the arena itself in FP64 mode (its contents are consumed
-    // by the gathers of the evaluation that wrote them), a side buffer of the
-    // same layout in mixed mode.
-    double* rec = nullptr;
-    if (fn->precision == TAD_PREC_FP64) rec = (double*)fn->arena.p;
+    // FP64 gradient slots: the evaluation's own slots in FP64 mode (their
+    // contents were consumed by the assembly of the evaluation that wrote
+    // them), a side buffer of the same layout in mixed mode.  Slots of other
+    // terms are zero, so they add nothing.
+    double* gs = nullptr;
+    if (fn->precision == TAD_PREC_FP64) gs = (double*)fn->gc_buf.p;
     else
     {
-        TAD_HIP(fn->adj_arena.ensure((size_t)fn->arena_size));
-        rec = fn->adj_arena.p;
+        TAD_HIP(fn->adj_g.ensure((size_t)fn->n_gslots * fn->d));
+        gs = fn->adj_g.p;
     }
     double* out_dev = (mem == TAD_MEM_DEVICE) ? out : fn->g_buf.p;
-    for (size_t ti = 0; ti < fn->terms.size(); ++ti)
-    {
-        if ((int)ti == term) TAD_HIP(launch_tet_rest_vjp(fn->terms[ti], fn->d, xd, rd, ld, w_scale, rec, s));
-        else TAD_HIP(launch_zero_grad_slots(fn->terms[ti], fn->d, rec, s));
-    }
-    TAD_HIP(launch_assemble_gradient(fn->d, 0, fn->n_handles, fn->gmap_ptr.p, fn->gmap.p, rec, out_dev, s));
+    TAD_HIP(launch_fill(gs, fn->n_gslots * fn->d, 0.0, s));
+    TAD_HIP(launch_tet_rest_vjp(fn->terms[term], xd, rd, ld, w_scale, gs, s));
+    TAD_HIP(launch_assemble_gradient(fn->d, 0, fn->n_handles, fn->gslot_ptr.p, gs, out_dev, s));
     if (mem == TAD_MEM_HOST) TAD_HIP(hipMemcpyAsync(out, out_dev, nv * sizeof(double), hipMemcpyDeviceToHost, s));
     TAD_HIP(hipStreamSynchronize(s));
     return TAD_OK;

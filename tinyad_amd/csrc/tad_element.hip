@@ -11,13 +11,22 @@ namespace tad {
 // ---------------------------------------------------------------------------
 // Element kernel
 // ---------------------------------------------------------------------------
+// Scatter targets of one term: contribution arrays and the term's slot maps.
+template <class T>
+struct ElemOut
+{
+    T* G;
+    T* Hc;
+    const uint32_t* gpos;
+    const uint32_t* hpos;
+};
+
 template <class E, class T, int MODE>
 __device__ __forceinline__ void element_body(int64_t e, int64_t n, const int32_t* __restrict__ conn,
                                              const double* __restrict__ params, const double* __restrict__ x,
-                                             double eps, double* __restrict__ elem_f, T* __restrict__ rec,
+                                             double eps, double* __restrict__ elem_f, const ElemOut<T>& o,
                                              int* __restrict__ err)
 {
-    constexpr int RS = record_size(E::N, E::D);
     double X[E::N * E::D];
 #pragma unroll
     for (int a = 0; a < E::N; ++a)
@@ -30,7 +39,8 @@ __device__ __forceinline__ void element_body(int64_t e, int64_t n, const int32_t
 #pragma unroll
     for (int k = 0; k < E::NP; ++k) p[k] = params[k * n + e];
     bool nf = false;
-    elem_f[e] = E::template eval<T, MODE>(X, p, T(eps), rec + e * RS, nf);
+    const ScatterSink<E::N, E::D, T> out{o.G, o.Hc, o.gpos, o.hpos, e, n};
+    elem_f[e] = E::template eval<T, MODE>(X, p, T(eps), out, nf);
     if (nf) *err = 1;
 }
 
@@ -38,12 +48,12 @@ __device__ __forceinline__ void element_body(int64_t e, int64_t n, const int32_t
 template <class E, class T, int MODE>
 __global__ __launch_bounds__(kBlock) void k_element(int64_t n, const int32_t* __restrict__ conn,
                                                     const double* __restrict__ params, const double* __restrict__ x,
-                                                    double eps, double* __restrict__ elem_f, T* __restrict__ rec,
+                                                    double eps, double* __restrict__ elem_f, ElemOut<T> o,
                                                     int* __restrict__ err)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
-    element_body<E, T, MODE>(e, n, conn, params, x, eps, elem_f, rec, err);
+    element_body<E, T, MODE>(e, n, conn, params, x, eps, elem_f, o, err);
 }
 
 // Persistent variant: a fixed grid walks the tiles.  map_xcd = 1 gives each of
@@ -55,7 +65,7 @@ template <class E, class T, int MODE, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_element_persistent(int64_t n, const int32_t* __restrict__ conn,
                                                               const double* __restrict__ params,
                                                               const double* __restrict__ x, double eps,
-                                                              double* __restrict__ elem_f, T* __restrict__ rec,
+                                                              double* __restrict__ elem_f, ElemOut<T> o,
                                                               int* __restrict__ err, int map_xcd)
 {
     const int64_t n_tiles = (n + BLOCK - 1) / BLOCK;
@@ -78,25 +88,25 @@ __global__ __launch_bounds__(BLOCK) void k_element_persistent(int64_t n, const i
     for (int64_t t = first; t < t1; t += stride)
     {
         const int64_t e = t * BLOCK + threadIdx.x;
-        if (e < n) element_body<E, T, MODE>(e, n, conn, params, x, eps, elem_f, rec, err);
+        if (e < n) element_body<E, T, MODE>(e, n, conn, params, x, eps, elem_f, o, err);
     }
 }
 
 template <class E, class T, int MODE, int BLOCK>
-static void launch_persistent(const TermDev& t, const double* x, double eps, double* ef, T* rec, int* err,
-                              hipStream_t s)
+static void launch_persistent(const TermDev& t, const double* x, double eps, double* ef, const ElemOut<T>& o,
+                              int* err, hipStream_t s)
 {
-    k_element_persistent<E, T, MODE, BLOCK><<<t.tune_grid, BLOCK, 0, s>>>(t.n_elem, t.conn, t.params, x, eps, ef, rec,
+    k_element_persistent<E, T, MODE, BLOCK><<<t.tune_grid, BLOCK, 0, s>>>(t.n_elem, t.conn, t.params, x, eps, ef, o,
                                                                          err, t.tune_map);
 }
 
 template <class E, class T>
 static hipError_t launch_element_t(const TermDev& t, int mode, const double* x, double eps, double* elem_f,
-                                   T* arena, int* err, hipStream_t s)
+                                   T* G, T* Hc, int* err, hipStream_t s)
 {
     const int64_t n = t.n_elem;
     if (n == 0) return hipSuccess;
-    T* rec = arena + t.rec_base;
+    const ElemOut<T> out{G, Hc, t.gpos, t.hpos};
     double* ef = elem_f + t.f_base;
     if constexpr (E::tunable)
     {
@@ -104,38 +114,38 @@ static hipError_t launch_element_t(const TermDev& t, int mode, const double* x, 
         if (mode == MODE_HESS_PROJ && t.tune_block > 0)
         {
             if (t.tune_map == 1 && t.tune_grid % 8 != 0) return hipErrorInvalidValue;
-            if (t.tune_block == 64) launch_persistent<E, T, MODE_HESS_PROJ, 64>(t, x, eps, ef, rec, err, s);
-            else if (t.tune_block == 128) launch_persistent<E, T, MODE_HESS_PROJ, 128>(t, x, eps, ef, rec, err, s);
-            else launch_persistent<E, T, MODE_HESS_PROJ, 256>(t, x, eps, ef, rec, err, s);
+            if (t.tune_block == 64) launch_persistent<E, T, MODE_HESS_PROJ, 64>(t, x, eps, ef, out, err, s);
+            else if (t.tune_block == 128) launch_persistent<E, T, MODE_HESS_PROJ, 128>(t, x, eps, ef, out, err, s);
+            else launch_persistent<E, T, MODE_HESS_PROJ, 256>(t, x, eps, ef, out, err, s);
             return hipGetLastError();
         }
     }
     const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
     switch (mode)
     {
-        case MODE_GRAD: k_element<E, T, MODE_GRAD><<<grid, kBlock, 0, s>>>(n, t.conn, t.params, x, eps, ef, rec, err); break;
-        case MODE_HESS: k_element<E, T, MODE_HESS><<<grid, kBlock, 0, s>>>(n, t.conn, t.params, x, eps, ef, rec, err); break;
-        default: k_element<E, T, MODE_HESS_PROJ><<<grid, kBlock, 0, s>>>(n, t.conn, t.params, x, eps, ef, rec, err); break;
+        case MODE_GRAD: k_element<E, T, MODE_GRAD><<<grid, kBlock, 0, s>>>(n, t.conn, t.params, x, eps, ef, out, err); break;
+        case MODE_HESS: k_element<E, T, MODE_HESS><<<grid, kBlock, 0, s>>>(n, t.conn, t.params, x, eps, ef, out, err); break;
+        default: k_element<E, T, MODE_HESS_PROJ><<<grid, kBlock, 0, s>>>(n, t.conn, t.params, x, eps, ef, out, err); break;
     }
     return hipGetLastError();
 }
 
 template <class T>
 static hipError_t launch_element_p(const TermDev& t, int d, int mode, const double* x, double eps, double* elem_f,
-                                   T* arena, int* err, hipStream_t s)
+                                   T* G, T* Hc, int* err, hipStream_t s)
 {
     switch (t.energy)
     {
-        case E_SYMDIR_TRI: return launch_element_t<SymdirElem<2>, T>(t, mode, x, eps, elem_f, arena, err, s);
-        case E_SYMDIR_TET: return launch_element_t<SymdirElem<3>, T>(t, mode, x, eps, elem_f, arena, err, s);
+        case E_SYMDIR_TRI: return launch_element_t<SymdirElem<2>, T>(t, mode, x, eps, elem_f, G, Hc, err, s);
+        case E_SYMDIR_TET: return launch_element_t<SymdirElem<3>, T>(t, mode, x, eps, elem_f, G, Hc, err, s);
         case E_PENALTY:
-            if (d == 1) return launch_element_t<PenaltyElem<1>, T>(t, mode, x, eps, elem_f, arena, err, s);
-            if (d == 2) return launch_element_t<PenaltyElem<2>, T>(t, mode, x, eps, elem_f, arena, err, s);
-            return launch_element_t<PenaltyElem<3>, T>(t, mode, x, eps, elem_f, arena, err, s);
+            if (d == 1) return launch_element_t<PenaltyElem<1>, T>(t, mode, x, eps, elem_f, G, Hc, err, s);
+            if (d == 2) return launch_element_t<PenaltyElem<2>, T>(t, mode, x, eps, elem_f, G, Hc, err, s);
+            return launch_element_t<PenaltyElem<3>, T>(t, mode, x, eps, elem_f, G, Hc, err, s);
         case E_QUADRATIC:
         {
 #define TAD_QUAD(NN, DD) \
-    if (t.N == NN && d == DD) return launch_element_t<QuadElem<NN, DD>, T>(t, mode, x, eps, elem_f, arena, err, s);
+    if (t.N == NN && d == DD) return launch_element_t<QuadElem<NN, DD>, T>(t, mode, x, eps, elem_f, G, Hc, err, s);
             TAD_QUAD(1, 1) TAD_QUAD(1, 2) TAD_QUAD(1, 3)
             TAD_QUAD(2, 1) TAD_QUAD(2, 2) TAD_QUAD(2, 3)
             TAD_QUAD(3, 1) TAD_QUAD(3, 2)
@@ -148,11 +158,11 @@ static hipError_t launch_element_p(const TermDev& t, int d, int mode, const doub
 }
 
 hipError_t launch_element(const TermDev& t, int d, int precision, int mode, const double* x, double eps,
-                          double* elem_f, void* arena, int* err, hipStream_t s)
+                          double* elem_f, void* G, void* Hc, int* err, hipStream_t s)
 {
     if (precision == 0)
-        return launch_element_p<double>(t, d, mode, x, eps, elem_f, (double*)arena, err, s);
-    return launch_element_p<float>(t, d, mode, x, eps, elem_f, (float*)arena, err, s);
+        return launch_element_p<double>(t, d, mode, x, eps, elem_f, (double*)G, (double*)Hc, err, s);
+    return launch_element_p<float>(t, d, mode, x, eps, elem_f, (float*)G, (float*)Hc, err, s);
 }
 
 } // namespace tad

@@ -18,10 +18,10 @@ struct SymdirElem
 {
     static constexpr int N = Dm + 1, D = Dm, NP = Dm * Dm + 1;
     static constexpr bool tunable = true;  // has persistent launch variants
-    template <class T, int MODE>
-    __device__ static double eval(const double* X, const double* p, T eps, T* rec, bool& nf)
+    template <class T, int MODE, class Sink>
+    __device__ static double eval(const double* X, const double* p, T eps, const Sink& out, bool& nf)
     {
-        return symdir_element<Dm, T, MODE>(X, p, eps, rec, nf);
+        return symdir_element<Dm, T, MODE>(X, p, eps, out, nf);
     }
     __device__ static double energy(const double* X, const double* p) { return energy_symdir<Dm>(X, p); }
 };
@@ -31,10 +31,10 @@ struct PenaltyElem
 {
     static constexpr int N = 1, D = Dm, NP = Dm + 1;
     static constexpr bool tunable = false;
-    template <class T, int MODE>
-    __device__ static double eval(const double* X, const double* p, T eps, T* rec, bool& nf)
+    template <class T, int MODE, class Sink>
+    __device__ static double eval(const double* X, const double* p, T eps, const Sink& out, bool& nf)
     {
-        return penalty_element<Dm, T, MODE>(X, p, eps, rec, nf);
+        return penalty_element<Dm, T, MODE>(X, p, eps, out, nf);
     }
     __device__ static double energy(const double* X, const double* p) { return energy_penalty<Dm>(X, p); }
 };
@@ -44,10 +44,10 @@ struct QuadElem
 {
     static constexpr int N = NN, D = Dm, K = NN * Dm, NP = K * K + K + 1;
     static constexpr bool tunable = false;
-    template <class T, int MODE>
-    __device__ static double eval(const double* X, const double* p, T eps, T* rec, bool& nf)
+    template <class T, int MODE, class Sink>
+    __device__ static double eval(const double* X, const double* p, T eps, const Sink& out, bool& nf)
     {
-        return quadratic_element<NN, Dm, T, MODE>(X, p, eps, rec, nf);
+        return quadratic_element<NN, Dm, T, MODE>(X, p, eps, out, nf);
     }
     __device__ static double energy(const double* X, const double* p) { return energy_quadratic<K>(X, p); }
 };

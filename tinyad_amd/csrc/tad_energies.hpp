@@ -27,8 +27,10 @@
 // (include/TinyAD/Utils/HessianProjection.hh:48-101) computed on the 9 x 9
 // (tet) or 4 x 4 (triangle) block that can carry non-zero eigenvalues.
 //
-// Output record per element (type T): gradient (N*d), then the N(N+1)/2
-// upper-triangular d x d blocks (v <= u) of the element Hessian, row-major.
+// Output: every element hands its per-vertex gradient parts (d values) and
+// the N(N+1)/2 upper d x d Hessian blocks (v <= u, row-major) to a sink; the
+// library's sink scatters them (and the transposes of the off-diagonal
+// blocks) to their slots in the destination-sorted contribution arrays.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -48,8 +50,44 @@ enum EnergyId
 };
 
 __host__ __device__ constexpr int n_blocks(int N) { return N * (N + 1) / 2; }
-__host__ __device__ constexpr int block_index(int N, int v, int u) { return v * N - (v * (v - 1)) / 2 + (u - v); }
-__host__ __device__ constexpr int record_size(int N, int d) { return N * d + n_blocks(N) * d * d; }
+
+// Scatter of one element's derivative parts through the precomputed maps:
+// gpos[a n + e] is the slot of vertex a's gradient part in G (d values per
+// slot), hpos[(a N + b) n + e] the slot of block (a, b) in Hc (d*d values per
+// slot).  Slots are sorted by destination (vertex, CSR block) and, within a
+// destination, by term and element, so the assembly sums contiguous runs in
+// the reference's element order.
+template <int N, int Dm, class T>
+struct ScatterSink
+{
+    T* __restrict__ G;
+    T* __restrict__ Hc;
+    const uint32_t* __restrict__ gpos;
+    const uint32_t* __restrict__ hpos;
+    int64_t e, n;
+
+    __device__ __forceinline__ void grad(int v, const T* g) const
+    {
+        T* dst = G + (int64_t)gpos[v * n + e] * Dm;
+#pragma unroll
+        for (int i = 0; i < Dm; ++i) dst[i] = g[i];
+    }
+    // block (v, u), v <= u; for v < u also writes block (u, v) = block^T
+    __device__ __forceinline__ void block(int v, int u, const T* b) const
+    {
+        T* dst = Hc + (int64_t)hpos[(v * N + u) * n + e] * (Dm * Dm);
+#pragma unroll
+        for (int m = 0; m < Dm * Dm; ++m) dst[m] = b[m];
+        if (v != u)
+        {
+            T* dt = Hc + (int64_t)hpos[(u * N + v) * n + e] * (Dm * Dm);
+#pragma unroll
+            for (int i = 0; i < Dm; ++i)
+#pragma unroll
+                for (int j = 0; j < Dm; ++j) dt[j * Dm + i] = b[i * Dm + j];
+        }
+    }
+};
 
 __device__ __forceinline__ double tad_inf() { return __longlong_as_double(0x7ff0000000000000LL); }
 
@@ -247,8 +285,32 @@ __device__ __forceinline__ Dual<Dm * Dm, T> symdir_dual(const T* J)
         cofactor_accumulate<KJ, T, 1, 5, 2, 4>(J, A);
         cofactor_accumulate<KJ, T, 2, 3, 0, 5>(J, A);
         cofactor_accumulate<KJ, T, 0, 4, 1, 3>(J, A);
-        const Dual<KJ, T> q = A / sqr(det);
-        return nJ + q;
+        // E = nJ + A r with r = det^-2, entry by entry (the product rule of
+        // Scalar.hh:738-821 with r's derivatives from the chain rule):
+        //   r' = -2 det^-3 det',  r'' = 6 det^-4 det' det'^T - 2 det^-3 det''
+        // so only A, det and the result are live (det'' keeps its structural
+        // zeros).
+        const T id = T(1) / det.v;
+        const T rv = id * id;
+        const T c3 = T(-2) * rv * id;
+        const T c4 = T(6) * rv * rv;
+        T rg[KJ];
+#pragma unroll
+        for (int i = 0; i < KJ; ++i) rg[i] = c3 * det.g[i];
+        Dual<KJ, T> E;
+        E.v = nJ.v + A.v * rv;
+#pragma unroll
+        for (int i = 0; i < KJ; ++i) E.g[i] = nJ.g[i] + (A.g[i] * rv + A.v * rg[i]);
+#pragma unroll
+        for (int i = 0; i < KJ; ++i)
+#pragma unroll
+            for (int j = i; j < KJ; ++j)
+            {
+                const int k = Sym<KJ>::idx(i, j);
+                const T rh = c4 * det.g[i] * det.g[j] + c3 * det.h[k];
+                E.h[k] = nJ.h[k] + (A.v * rh + rv * A.h[k] + A.g[i] * rg[j] + rg[i] * A.g[j]);
+            }
+        return E;
     }
 }
 
@@ -295,8 +357,8 @@ __device__ __forceinline__ T symdir_hx_entry(const Dual<Dm * Dm, T>& E, const T 
 }
 
 // Writes the unprojected element Hessian as N(N+1)/2 upper d x d blocks.
-template <int Dm, class T>
-__device__ __forceinline__ void symdir_write_hx(const Dual<Dm * Dm, T>& E, const T (*Dmat)[Dm], T w, T* hrec,
+template <int Dm, class T, class Sink>
+__device__ __forceinline__ void symdir_write_hx(const Dual<Dm * Dm, T>& E, const T (*Dmat)[Dm], T w, const Sink& out,
                                                 bool& nonfinite)
 {
     constexpr int N = Dm + 1;
@@ -304,35 +366,43 @@ __device__ __forceinline__ void symdir_write_hx(const Dual<Dm * Dm, T>& E, const
     for (int v = 0; v < N; ++v)
 #pragma unroll
         for (int u = v; u < N; ++u)
+        {
+            T blk[Dm * Dm];
 #pragma unroll
             for (int i = 0; i < Dm; ++i)
 #pragma unroll
                 for (int ip = 0; ip < Dm; ++ip)
                 {
                     const T val = symdir_hx_entry<Dm, T>(E, Dmat, w, v, i, u, ip);
-                    hrec[block_index(N, v, u) * Dm * Dm + i * Dm + ip] = val;
+                    blk[i * Dm + ip] = val;
                     nonfinite = nonfinite || !finite_val(val);
                 }
+            out.block(v, u, blk);
+        }
 }
 
 // Writes the N(N+1)/2 upper d x d blocks of a packed (N d) x (N d) matrix.
-template <int N, int Dm, class T>
-__device__ __forceinline__ void write_blocks(const T* Hp, T* hrec, bool& nonfinite)
+template <int N, int Dm, class T, class Sink>
+__device__ __forceinline__ void write_blocks(const T* Hp, const Sink& out, bool& nonfinite)
 {
     constexpr int K = N * Dm;
 #pragma unroll
     for (int v = 0; v < N; ++v)
 #pragma unroll
         for (int u = v; u < N; ++u)
+        {
+            T blk[Dm * Dm];
 #pragma unroll
             for (int i = 0; i < Dm; ++i)
 #pragma unroll
                 for (int ip = 0; ip < Dm; ++ip)
                 {
                     const T val = Hp[Sym<K>::at(v * Dm + i, u * Dm + ip)];
-                    hrec[block_index(N, v, u) * Dm * Dm + i * Dm + ip] = val;
+                    blk[i * Dm + ip] = val;
                     nonfinite = nonfinite || !finite_val(val);
                 }
+            out.block(v, u, blk);
+        }
 }
 
 // positive_diagonally_dominant (HessianProjection.hh:23-42) of H = Q K Q^T,
@@ -372,8 +442,9 @@ __device__ __forceinline__ bool deflated_pdd(const T* Km, T eps)
 // Second-order evaluation of one symmetric-Dirichlet simplex.
 // X: element coordinates (FP64, AoS N x Dm); p: [Mr^-1, w] (FP64).
 // Writes the record (gradient + Hessian blocks) in type T; returns the FP64 energy.
-template <int Dm, class T, int MODE>
-__device__ __forceinline__ double symdir_element(const double* X, const double* p, T eps, T* rec, bool& nonfinite)
+template <int Dm, class T, int MODE, class Sink>
+__device__ __forceinline__ double symdir_element(const double* X, const double* p, T eps, const Sink& out,
+                                                 bool& nonfinite)
 {
     constexpr int N = Dm + 1;
     constexpr int KJ = Dm * Dm;
@@ -381,14 +452,13 @@ __device__ __forceinline__ double symdir_element(const double* X, const double* 
     constexpr int NB = n_blocks(N);
     const double f = energy_symdir<Dm>(X, p);
 
-    T* grec = rec;
-    T* hrec = rec + KX;
     if (!(f < tad_inf()))
     {
         // inverted element: +inf energy, zero derivatives; the projection of
         // the zero Hessian is eps I (clamp mode) or 0 (absolute-value mode)
+        const T zero[Dm] = {};
 #pragma unroll
-        for (int i = 0; i < KX; ++i) grec[i] = T(0);
+        for (int v = 0; v < N; ++v) out.grad(v, zero);
         if constexpr (MODE >= MODE_HESS)
         {
             const T dg = (MODE == MODE_HESS_PROJ && eps > T(0)) ? eps : T(0);
@@ -396,11 +466,14 @@ __device__ __forceinline__ double symdir_element(const double* X, const double* 
             for (int v = 0; v < N; ++v)
 #pragma unroll
                 for (int u = v; u < N; ++u)
+                {
+                    T blk[Dm * Dm];
 #pragma unroll
                     for (int i = 0; i < Dm; ++i)
 #pragma unroll
-                        for (int j = 0; j < Dm; ++j)
-                            hrec[block_index(N, v, u) * Dm * Dm + i * Dm + j] = (v == u && i == j) ? dg : T(0);
+                        for (int j = 0; j < Dm; ++j) blk[i * Dm + j] = (v == u && i == j) ? dg : T(0);
+                    out.block(v, u, blk);
+                }
         }
         return f;
     }
@@ -443,21 +516,25 @@ __device__ __forceinline__ double symdir_element(const double* X, const double* 
     // gradient: g_X[v, i] = w sum_j D_vj dE/dJ_ij
 #pragma unroll
     for (int v = 0; v < N; ++v)
+    {
+        T gv[Dm];
 #pragma unroll
         for (int i = 0; i < Dm; ++i)
         {
             T s = T(0);
 #pragma unroll
             for (int j = 0; j < Dm; ++j) s += Dmat[v][j] * E.g[i * Dm + j];
-            grec[v * Dm + i] = w * s;
-            nonfinite = nonfinite || !finite_val(grec[v * Dm + i]);
+            gv[i] = w * s;
+            nonfinite = nonfinite || !finite_val(gv[i]);
         }
+        out.grad(v, gv);
+    }
     if constexpr (MODE == MODE_GRAD)
         return f;
 
     if constexpr (MODE == MODE_HESS)
     {
-        symdir_write_hx<Dm, T>(E, Dmat, w, hrec, nonfinite);
+        symdir_write_hx<Dm, T>(E, Dmat, w, out, nonfinite);
         return f;
     }
     else
@@ -479,37 +556,37 @@ __device__ __forceinline__ double symdir_element(const double* X, const double* 
                     if (Helmert<N>::u(v, a) != 0.0) s += T(Helmert<N>::u(v, a)) * Dmat[v][j];
                 Em[a][j] = s;
             }
-        // K[(a,i),(b,i')] = w sum_{j,j'} E_aj E_bj' H_J[(i,j),(i',j')]
-        T T1[KJ][Dm][Dm]; // [(i,j)][b][i']
+        // K[(a,i),(b,i')] = w sum_{j,j'} E_aj E_bj' H_J[(i,j),(i',j')], one
+        // column (b, i') at a time: T1[(i,j)] = sum_j' H_J[(i,j),(i',j')] E_bj'
+        T Km[Sym<KJ>::n];
 #pragma unroll
-        for (int r = 0; r < KJ; ++r)
+        for (int b = 0; b < Dm; ++b)
 #pragma unroll
-            for (int b = 0; b < Dm; ++b)
+            for (int ip = 0; ip < Dm; ++ip)
+            {
+                const int r2 = b * Dm + ip;
+                T T1[KJ];
 #pragma unroll
-                for (int ip = 0; ip < Dm; ++ip)
+                for (int r = 0; r < KJ; ++r)
                 {
                     T s = T(0);
 #pragma unroll
                     for (int jp = 0; jp < Dm; ++jp) s += E.h[Sym<KJ>::at(r, ip * Dm + jp)] * Em[b][jp];
-                    T1[r][b][ip] = s;
+                    T1[r] = s;
                 }
-        T Km[Sym<KJ>::n];
 #pragma unroll
-        for (int a = 0; a < Dm; ++a)
+                for (int a = 0; a < Dm; ++a)
 #pragma unroll
-            for (int i = 0; i < Dm; ++i)
-#pragma unroll
-                for (int b = 0; b < Dm; ++b)
-#pragma unroll
-                    for (int ip = 0; ip < Dm; ++ip)
+                    for (int i = 0; i < Dm; ++i)
                     {
-                        const int r1 = a * Dm + i, r2 = b * Dm + ip;
+                        const int r1 = a * Dm + i;
                         if (r2 < r1) continue;
                         T s = T(0);
 #pragma unroll
-                        for (int j = 0; j < Dm; ++j) s += Em[a][j] * T1[i * Dm + j][b][ip];
+                        for (int j = 0; j < Dm; ++j) s += Em[a][j] * T1[i * Dm + j];
                         Km[Sym<KJ>::idx(r1, r2)] = w * s;
                     }
+            }
         // Diagonal-dominance early out of the reference on H = Q K Q^T (exact
         // when the translations are in the kernel of H); it leaves H
         // unprojected.  Only evaluated for eps <= 0, see above.
@@ -517,9 +594,12 @@ __device__ __forceinline__ double symdir_element(const double* X, const double* 
         if (!pdd) project_psd<KJ, T, false>(Km, eps);
         // H[(v,i),(u,i')] = sum_{a,b} U_va U_ub P[(a,i),(b,i')] + max(eps,0)/N delta_ii'
         const T ct = eps > T(0) ? eps / T(N) : T(0);
-        T S1[N][Dm][KJ]; // S1[v][i][(b,i')] = sum_a U_va P[(a,i),(b,i')]
+        // one vertex row at a time: S1[i][(b,i')] = sum_a U_va P[(a,i),(b,i')]
+        // is live for the blocks (v, u >= v) only
 #pragma unroll
         for (int v = 0; v < N; ++v)
+        {
+            T S1[Dm][KJ];
 #pragma unroll
             for (int i = 0; i < Dm; ++i)
 #pragma unroll
@@ -530,12 +610,12 @@ __device__ __forceinline__ double symdir_element(const double* X, const double* 
                     for (int a = 0; a < Dm; ++a)
                         if (Helmert<N>::u(v, a) != 0.0)
                             s += T(Helmert<N>::u(v, a)) * Km[Sym<KJ>::at(a * Dm + i, c2)];
-                    S1[v][i][c2] = s;
+                    S1[i][c2] = s;
                 }
 #pragma unroll
-        for (int v = 0; v < N; ++v)
-#pragma unroll
             for (int u = v; u < N; ++u)
+            {
+                T blk[Dm * Dm];
 #pragma unroll
                 for (int i = 0; i < Dm; ++i)
 #pragma unroll
@@ -545,37 +625,39 @@ __device__ __forceinline__ double symdir_element(const double* X, const double* 
 #pragma unroll
                         for (int b = 0; b < Dm; ++b)
                             if (Helmert<N>::u(u, b) != 0.0)
-                                s += T(Helmert<N>::u(u, b)) * S1[v][i][b * Dm + ip];
-                        hrec[block_index(N, v, u) * Dm * Dm + i * Dm + ip] = s;
+                                s += T(Helmert<N>::u(u, b)) * S1[i][b * Dm + ip];
+                        blk[i * Dm + ip] = s;
                         nonfinite = nonfinite || !finite_val(s);
                     }
+                out.block(v, u, blk);
+            }
+        }
         return f;
     }
 }
 
 // Generic small dense terms (penalty, quadratic): full K x K Hessian in
 // packed form, projected without deflation.
-template <int N, int Dm, class T, int MODE>
-__device__ __forceinline__ void write_dense_record(const T* g, T* Hp, T eps, T* rec, bool& nonfinite)
+template <int N, int Dm, class T, int MODE, class Sink>
+__device__ __forceinline__ void write_dense_record(const T* g, T* Hp, T eps, const Sink& out, bool& nonfinite)
 {
     constexpr int K = N * Dm;
 #pragma unroll
-    for (int i = 0; i < K; ++i)
-    {
-        rec[i] = g[i];
-        nonfinite = nonfinite || !finite_val(g[i]);
-    }
+    for (int i = 0; i < K; ++i) nonfinite = nonfinite || !finite_val(g[i]);
+#pragma unroll
+    for (int v = 0; v < N; ++v) out.grad(v, g + v * Dm);
     if constexpr (MODE >= MODE_HESS)
     {
         if constexpr (MODE == MODE_HESS_PROJ)
             if (!positive_diagonally_dominant<K, T>(Hp, eps))
                 project_psd<K, T>(Hp, eps);
-        write_blocks<N, Dm, T>(Hp, rec + K, nonfinite);
+        write_blocks<N, Dm, T>(Hp, out, nonfinite);
     }
 }
 
-template <int Dm, class T, int MODE>
-__device__ __forceinline__ double penalty_element(const double* X, const double* p, T eps, T* rec, bool& nonfinite)
+template <int Dm, class T, int MODE, class Sink>
+__device__ __forceinline__ double penalty_element(const double* X, const double* p, T eps, const Sink& out,
+                                                  bool& nonfinite)
 {
     const double f = energy_penalty<Dm>(X, p);
     T g[Dm], Hp[Sym<Dm>::n];
@@ -586,12 +668,13 @@ __device__ __forceinline__ double penalty_element(const double* X, const double*
     for (int i = 0; i < Dm; ++i)
 #pragma unroll
         for (int j = i; j < Dm; ++j) Hp[Sym<Dm>::idx(i, j)] = (i == j) ? T(2) * w : T(0);
-    write_dense_record<1, Dm, T, MODE>(g, Hp, eps, rec, nonfinite);
+    write_dense_record<1, Dm, T, MODE>(g, Hp, eps, out, nonfinite);
     return f;
 }
 
-template <int N, int Dm, class T, int MODE>
-__device__ __forceinline__ double quadratic_element(const double* X, const double* p, T eps, T* rec, bool& nonfinite)
+template <int N, int Dm, class T, int MODE, class Sink>
+__device__ __forceinline__ double quadratic_element(const double* X, const double* p, T eps, const Sink& out,
+                                                    bool& nonfinite)
 {
     constexpr int K = N * Dm;
     const double f = energy_quadratic<K>(X, p);
@@ -606,7 +689,7 @@ __device__ __forceinline__ double quadratic_element(const double* X, const doubl
 #pragma unroll
         for (int j = i; j < K; ++j) Hp[Sym<K>::idx(i, j)] = T(0.5 * (p[i * K + j] + p[j * K + i]));
     }
-    write_dense_record<N, Dm, T, MODE>(g, Hp, eps, rec, nonfinite);
+    write_dense_record<N, Dm, T, MODE>(g, Hp, eps, out, nonfinite);
     return f;
 }
 

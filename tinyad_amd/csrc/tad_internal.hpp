@@ -22,7 +22,8 @@ struct TermDev
     int64_t n_elem = 0;
     int32_t* conn = nullptr;   // device SoA: conn[a * n_elem + e]
     double* params = nullptr;  // device SoA: params[k * n_elem + e]
-    int64_t rec_base = 0;      // first arena entry of this term's records
+    uint32_t* gpos = nullptr;  // SoA [N][n_elem]: gradient-part slot of (vertex a, element e)
+    uint32_t* hpos = nullptr;  // SoA [N*N][n_elem]: Hessian-block slot of (block (a, b), element e)
     int64_t f_base = 0;        // first entry of this term in the element-energy array
     // launch shape of the projected-Hessian element kernel chosen by the
     // autotuner: tune_block = 0 is the plain one-tile-per-workgroup grid,
@@ -51,9 +52,11 @@ struct Timer
 
 constexpr int kReduceBlocks = 256;
 
-// MODE: 1 grad, 2 hessian, 3 projected hessian.  precision: 0 fp64, 1 fp32 records.
+// MODE: 1 grad, 2 hessian, 3 projected hessian.  precision: 0 fp64, 1 fp32
+// contributions.  G / Hc: destination-sorted gradient-part and Hessian-block
+// contribution arrays (d and d*d values per slot).
 hipError_t launch_element(const TermDev& t, int d, int precision, int mode, const double* x, double eps,
-                          double* elem_f, void* arena, int* err, hipStream_t s);
+                          double* elem_f, void* G, void* Hc, int* err, hipStream_t s);
 
 // Energies at x + steps[c] * dir (dir may be null: plain x) for c < ncand,
 // written to elem_f[c * stride + f_base + e].
@@ -64,12 +67,14 @@ hipError_t launch_energy(const TermDev& t, int d, const double* x, const double*
 hipError_t launch_sum(const double* in, int64_t n, int ncand, int64_t stride, double* partials, double* out,
                       hipStream_t s);
 
+// CSR block b = sum of Hessian slots [hslot_ptr[b], hslot_ptr[b+1]) of Hc;
+// gradient of handle u = sum of slots [gslot_ptr[u], gslot_ptr[u+1]) of G.
 hipError_t launch_assemble_hessian(int d, int precision, int64_t n_blocks, const int32_t* block_row,
-                                   const int32_t* brow_ptr, const int64_t* hmap_ptr, const uint32_t* hmap,
-                                   const void* arena, const int32_t* row_ptr, double* H, hipStream_t s);
+                                   const int32_t* brow_ptr, const int64_t* hslot_ptr, const void* Hc,
+                                   const int32_t* row_ptr, double* H, hipStream_t s);
 
-hipError_t launch_assemble_gradient(int d, int precision, int64_t n_handles, const int64_t* gmap_ptr,
-                                    const uint32_t* gmap, const void* arena, double* g, hipStream_t s);
+hipError_t launch_assemble_gradient(int d, int precision, int64_t n_handles, const int64_t* gslot_ptr,
+                                    const void* G, double* g, hipStream_t s);
 
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
 
@@ -103,9 +108,10 @@ int64_t pcg_parts_size(int d, int64_t n_handles);
 hipError_t launch_segment_sum(const double* elem_f, const int64_t* ranges, int n_terms, int64_t n_seg, double* out,
                               hipStream_t s);
 
-hipError_t launch_tet_rest_vjp(const TermDev& t, int d, const double* x, const double* rest, const double* lam,
-                               double w_scale, double* arena, hipStream_t s);
-hipError_t launch_zero_grad_slots(const TermDev& t, int d, double* arena, hipStream_t s);
+// Rest-shape mixed products of a symmetric-Dirichlet tet term, written to
+// the term's gradient slots of G (FP64).
+hipError_t launch_tet_rest_vjp(const TermDev& t, const double* x, const double* rest, const double* lam,
+                               double w_scale, double* G, hipStream_t s);
 hipError_t launch_selftest_dual(int which, double x, double y, double* out_dev, hipStream_t s);
 
 } // namespace tad

@@ -36,6 +36,22 @@ __device__ __forceinline__ double block_sum(double v, double* sh)
     return r;
 }
 
+// Same fixed-order reduction for a block of NT threads (NT a multiple of 64).
+template <int NT>
+__device__ __forceinline__ double block_sum_n(double v, double* sh)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int wave = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[wave] = v;
+    __syncthreads();
+    double r = sh[0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w) r += sh[w];
+    return r;
+}
+
 // block b of row c sums the contiguous chunk [b*chunk, (b+1)*chunk)
 __global__ __launch_bounds__(kBlock) void k_sum_stage1(const double* __restrict__ in, int64_t n, int64_t stride,
                                                        double* __restrict__ partials)
@@ -119,15 +135,16 @@ hipError_t launch_axpy_point(const double* x0, const double* dir, const double* 
 }
 
 // ---------------------------------------------------------------------------
-// Assembly: element-order gather through precomputed maps
+// Assembly: contiguous sums over destination-sorted contributions
 // ---------------------------------------------------------------------------
-// hmap entry: arena offset of a d x d block record; top bit set = transposed
+// The element kernels scatter every d x d block (and every per-vertex gradient
+// part) to its slot; the slots of one destination are contiguous and in
+// element order, so each output entry is a fixed-order sum over one run.
 template <class T, int D>
 __global__ __launch_bounds__(kBlock) void k_assemble_hessian(int64_t nb, const int32_t* __restrict__ block_row,
                                                              const int32_t* __restrict__ brow_ptr,
-                                                             const int64_t* __restrict__ hmap_ptr,
-                                                             const uint32_t* __restrict__ hmap,
-                                                             const T* __restrict__ arena,
+                                                             const int64_t* __restrict__ hslot_ptr,
+                                                             const T* __restrict__ Hc,
                                                              const int32_t* __restrict__ row_ptr,
                                                              double* __restrict__ H)
 {
@@ -140,19 +157,12 @@ __global__ __launch_bounds__(kBlock) void k_assemble_hessian(int64_t nb, const i
     double acc[D * D];
 #pragma unroll
     for (int m = 0; m < D * D; ++m) acc[m] = 0.0;
-    const int64_t c1 = hmap_ptr[b + 1];
-    for (int64_t c = hmap_ptr[b]; c < c1; ++c)
+    const int64_t c1 = hslot_ptr[b + 1];
+    for (int64_t c = hslot_ptr[b]; c < c1; ++c)
     {
-        const uint32_t o = hmap[c];
-        const bool tr = (o >> 31) != 0;
-        const T* src = arena + (o & 0x7fffffffu);
-        T vals[D * D];
+        const T* src = Hc + c * (D * D);
 #pragma unroll
-        for (int m = 0; m < D * D; ++m) vals[m] = src[m];
-#pragma unroll
-        for (int i = 0; i < D; ++i)
-#pragma unroll
-            for (int j = 0; j < D; ++j) acc[i * D + j] += (double)(tr ? vals[j * D + i] : vals[i * D + j]);
+        for (int m = 0; m < D * D; ++m) acc[m] += (double)src[m];
     }
     const int64_t base = row_ptr[D * u];
 #pragma unroll
@@ -162,19 +172,18 @@ __global__ __launch_bounds__(kBlock) void k_assemble_hessian(int64_t nb, const i
 }
 
 template <class T, int D>
-__global__ __launch_bounds__(kBlock) void k_assemble_gradient(int64_t nh, const int64_t* __restrict__ gmap_ptr,
-                                                              const uint32_t* __restrict__ gmap,
-                                                              const T* __restrict__ arena, double* __restrict__ g)
+__global__ __launch_bounds__(kBlock) void k_assemble_gradient(int64_t nh, const int64_t* __restrict__ gslot_ptr,
+                                                              const T* __restrict__ G, double* __restrict__ g)
 {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nh) return;
     double acc[D];
 #pragma unroll
     for (int i = 0; i < D; ++i) acc[i] = 0.0;
-    const int64_t c1 = gmap_ptr[v + 1];
-    for (int64_t c = gmap_ptr[v]; c < c1; ++c)
+    const int64_t c1 = gslot_ptr[v + 1];
+    for (int64_t c = gslot_ptr[v]; c < c1; ++c)
     {
-        const T* src = arena + gmap[c];
+        const T* src = G + c * D;
 #pragma unroll
         for (int i = 0; i < D; ++i) acc[i] += (double)src[i];
     }
@@ -184,44 +193,43 @@ __global__ __launch_bounds__(kBlock) void k_assemble_gradient(int64_t nh, const 
 
 template <class T>
 static hipError_t launch_assemble_hessian_t(int d, int64_t nb, const int32_t* block_row, const int32_t* brow_ptr,
-                                            const int64_t* hmap_ptr, const uint32_t* hmap, const T* arena,
-                                            const int32_t* row_ptr, double* H, hipStream_t s)
+                                            const int64_t* hslot_ptr, const T* Hc, const int32_t* row_ptr, double* H,
+                                            hipStream_t s)
 {
     if (nb == 0) return hipSuccess;
     const unsigned grid = (unsigned)((nb + kBlock - 1) / kBlock);
-    if (d == 1) k_assemble_hessian<T, 1><<<grid, kBlock, 0, s>>>(nb, block_row, brow_ptr, hmap_ptr, hmap, arena, row_ptr, H);
-    else if (d == 2) k_assemble_hessian<T, 2><<<grid, kBlock, 0, s>>>(nb, block_row, brow_ptr, hmap_ptr, hmap, arena, row_ptr, H);
-    else k_assemble_hessian<T, 3><<<grid, kBlock, 0, s>>>(nb, block_row, brow_ptr, hmap_ptr, hmap, arena, row_ptr, H);
+    if (d == 1) k_assemble_hessian<T, 1><<<grid, kBlock, 0, s>>>(nb, block_row, brow_ptr, hslot_ptr, Hc, row_ptr, H);
+    else if (d == 2) k_assemble_hessian<T, 2><<<grid, kBlock, 0, s>>>(nb, block_row, brow_ptr, hslot_ptr, Hc, row_ptr, H);
+    else k_assemble_hessian<T, 3><<<grid, kBlock, 0, s>>>(nb, block_row, brow_ptr, hslot_ptr, Hc, row_ptr, H);
     return hipGetLastError();
 }
 
 hipError_t launch_assemble_hessian(int d, int precision, int64_t nb, const int32_t* block_row, const int32_t* brow_ptr,
-                                   const int64_t* hmap_ptr, const uint32_t* hmap, const void* arena,
-                                   const int32_t* row_ptr, double* H, hipStream_t s)
+                                   const int64_t* hslot_ptr, const void* Hc, const int32_t* row_ptr, double* H,
+                                   hipStream_t s)
 {
     if (precision == 0)
-        return launch_assemble_hessian_t<double>(d, nb, block_row, brow_ptr, hmap_ptr, hmap, (const double*)arena, row_ptr, H, s);
-    return launch_assemble_hessian_t<float>(d, nb, block_row, brow_ptr, hmap_ptr, hmap, (const float*)arena, row_ptr, H, s);
+        return launch_assemble_hessian_t<double>(d, nb, block_row, brow_ptr, hslot_ptr, (const double*)Hc, row_ptr, H, s);
+    return launch_assemble_hessian_t<float>(d, nb, block_row, brow_ptr, hslot_ptr, (const float*)Hc, row_ptr, H, s);
 }
 
 template <class T>
-static hipError_t launch_assemble_gradient_t(int d, int64_t nh, const int64_t* gmap_ptr, const uint32_t* gmap,
-                                             const T* arena, double* g, hipStream_t s)
+static hipError_t launch_assemble_gradient_t(int d, int64_t nh, const int64_t* gslot_ptr, const T* G, double* g,
+                                             hipStream_t s)
 {
     if (nh == 0) return hipSuccess;
     const unsigned grid = (unsigned)((nh + kBlock - 1) / kBlock);
-    if (d == 1) k_assemble_gradient<T, 1><<<grid, kBlock, 0, s>>>(nh, gmap_ptr, gmap, arena, g);
-    else if (d == 2) k_assemble_gradient<T, 2><<<grid, kBlock, 0, s>>>(nh, gmap_ptr, gmap, arena, g);
-    else k_assemble_gradient<T, 3><<<grid, kBlock, 0, s>>>(nh, gmap_ptr, gmap, arena, g);
+    if (d == 1) k_assemble_gradient<T, 1><<<grid, kBlock, 0, s>>>(nh, gslot_ptr, G, g);
+    else if (d == 2) k_assemble_gradient<T, 2><<<grid, kBlock, 0, s>>>(nh, gslot_ptr, G, g);
+    else k_assemble_gradient<T, 3><<<grid, kBlock, 0, s>>>(nh, gslot_ptr, G, g);
     return hipGetLastError();
 }
 
-hipError_t launch_assemble_gradient(int d, int precision, int64_t nh, const int64_t* gmap_ptr, const uint32_t* gmap,
-                                    const void* arena, double* g, hipStream_t s)
+hipError_t launch_assemble_gradient(int d, int precision, int64_t nh, const int64_t* gslot_ptr, const void* G,
+                                    double* g, hipStream_t s)
 {
-    if (precision == 0)
-        return launch_assemble_gradient_t<double>(d, nh, gmap_ptr, gmap, (const double*)arena, g, s);
-    return launch_assemble_gradient_t<float>(d, nh, gmap_ptr, gmap, (const float*)arena, g, s);
+    if (precision == 0) return launch_assemble_gradient_t<double>(d, nh, gslot_ptr, (const double*)G, g, s);
+    return launch_assemble_gradient_t<float>(d, nh, gslot_ptr, (const float*)G, g, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -248,11 +256,12 @@ __device__ __forceinline__ bool block_done(const PcgScalars* sc)
     return flag != 0;
 }
 
+template <int NT = kBlock>
 __device__ __forceinline__ double sum_parts(const double* __restrict__ parts, int n, double* sh)
 {
     double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += kBlock) acc += parts[i];
-    return block_sum(acc, sh);
+    for (int i = threadIdx.x; i < n; i += NT) acc += parts[i];
+    return block_sum_n<NT>(acc, sh);
 }
 
 template <int D>
@@ -446,18 +455,28 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t nh, const int32_t* __re
 // CG iteration k, second half: scalars, direction recurrences, x / r / z
 // update; partials of rho_{k+1} = r.z and ||r||^2 into slot k & 1.  alpha and
 // rho of iteration k are kept (slot k & 1 of sc) for iteration k + 1.
+// A block of 64 D threads owns 64 consecutive vertices per step: thread t
+// updates variable t of the chunk (every vector access is a contiguous run),
+// and the D x D preconditioner blocks and the new residual pass through LDS
+// for z = M^-1 r.
+constexpr int kUpdVerts = 64;
+
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t nh, const double* __restrict__ Dinv,
-                                                       double* __restrict__ x, double* __restrict__ r,
-                                                       double* __restrict__ z, double* __restrict__ p,
-                                                       double* __restrict__ q, const double* __restrict__ sv,
-                                                       double* __restrict__ parts, int Gs, int Gv, int k,
-                                                       PcgScalars* sc)
+__global__ __launch_bounds__(kUpdVerts * D) void k_pcg_update(int64_t nh, const double* __restrict__ Dinv,
+                                                              double* __restrict__ x, double* __restrict__ r,
+                                                              double* __restrict__ z, double* __restrict__ p,
+                                                              double* __restrict__ q,
+                                                              const double* __restrict__ sv,
+                                                              double* __restrict__ parts, int Gs, int Gv, int k,
+                                                              PcgScalars* sc)
 {
+    constexpr int NT = kUpdVerts * D;
     if (block_done(sc)) return;
-    __shared__ double sh[kBlock];
-    const double delta = sum_parts(parts, Gs, sh);
-    const double rho = sum_parts(parts + Gs + ((k + 1) & 1) * Gv, Gv, sh);
+    __shared__ double sh[NT];
+    __shared__ double sD[kUpdVerts * D * D];
+    __shared__ double sR[NT];
+    const double delta = sum_parts<NT>(parts, Gs, sh);
+    const double rho = sum_parts<NT>(parts + Gs + ((k + 1) & 1) * Gv, Gv, sh);
     double beta = 0.0, den = delta;
     if (k > 0)
     {
@@ -465,10 +484,18 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t nh, const double*
         const double alpha_prev = sc->alpha_hist[(k + 1) & 1];
         beta = rho / rho_prev;
         den = delta - beta * rho / alpha_prev;
+        // p.Ap from the recurrence can cancel to <= 0 near convergence even
+        // for a positive definite matrix: restart along z (beta = 0, p = z,
+        // p.Ap = z.Az) instead of reporting a breakdown
+        if (!(den > 0.0))
+        {
+            beta = 0.0;
+            den = delta;
+        }
     }
     if (!(den > 0.0))
     {
-        // not positive definite along p: the same decision in every block
+        // not positive definite along z: the same decision in every block
         if (blockIdx.x == 0 && threadIdx.x == 0)
         {
             sc->breakdown = 1;
@@ -483,34 +510,47 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t nh, const double*
         sc->rho_hist[k & 1] = rho;
         sc->alpha_hist[k & 1] = alpha;
     }
+    const int64_t n = nh * D;
+    const int t = threadIdx.x;
     double rz = 0.0, rr = 0.0;
-    for (int64_t u = (int64_t)blockIdx.x * kBlock + threadIdx.x; u < nh; u += (int64_t)gridDim.x * kBlock)
+    for (int64_t c0 = (int64_t)blockIdx.x * kUpdVerts; c0 < nh; c0 += (int64_t)gridDim.x * kUpdVerts)
     {
-        double rv[D];
-#pragma unroll
-        for (int i = 0; i < D; ++i)
+        const int64_t iv = c0 * D + t;
+        double rv = 0.0;
+        if (iv < n)
         {
-            const double pi = k > 0 ? fma(beta, p[u * D + i], z[u * D + i]) : z[u * D + i];
-            const double qi = k > 0 ? fma(beta, q[u * D + i], sv[u * D + i]) : sv[u * D + i];
-            p[u * D + i] = pi;
-            q[u * D + i] = qi;
-            x[u * D + i] += alpha * pi;
-            rv[i] = r[u * D + i] - alpha * qi;
-            r[u * D + i] = rv[i];
+            const double pi = k > 0 ? fma(beta, p[iv], z[iv]) : z[iv];
+            const double qi = k > 0 ? fma(beta, q[iv], sv[iv]) : sv[iv];
+            p[iv] = pi;
+            q[iv] = qi;
+            x[iv] += alpha * pi;
+            rv = r[iv] - alpha * qi;
+            r[iv] = rv;
         }
+        __syncthreads();  // the previous chunk's LDS reads are complete
+        sR[t] = rv;
+        const int64_t d0 = c0 * D * D;
+        const int64_t dn = nh * D * D;
 #pragma unroll
-        for (int i = 0; i < D; ++i)
+        for (int m = 0; m < D; ++m)
         {
+            const int64_t id = d0 + t + (int64_t)m * NT;
+            sD[t + m * NT] = id < dn ? Dinv[id] : 0.0;
+        }
+        __syncthreads();
+        if (iv < n)
+        {
+            const int vl = t / D, i = t - vl * D;
             double zi = 0.0;
 #pragma unroll
-            for (int j = 0; j < D; ++j) zi += Dinv[u * D * D + i * D + j] * rv[j];
-            z[u * D + i] = zi;
-            rz += rv[i] * zi;
-            rr += rv[i] * rv[i];
+            for (int j = 0; j < D; ++j) zi += sD[vl * D * D + i * D + j] * sR[vl * D + j];
+            z[iv] = zi;
+            rz += rv * zi;
+            rr += rv * rv;
         }
     }
-    const double srz = block_sum(rz, sh);
-    const double srr = block_sum(rr, sh);
+    const double srz = block_sum_n<NT>(rz, sh);
+    const double srr = block_sum_n<NT>(rr, sh);
     if (threadIdx.x == 0)
     {
         parts[Gs + (k & 1) * Gv + blockIdx.x] = srz;
@@ -523,7 +563,7 @@ void pcg_grids(int d, int64_t nh, int* Gs, int* Gv)
     const int lpb = d == 3 ? 16 : 8;
     const int64_t rows_per_block = kBlock / lpb;
     const int64_t gs = (nh + rows_per_block - 1) / rows_per_block;
-    const int64_t gv = (nh + kBlock - 1) / kBlock;
+    const int64_t gv = (nh + kUpdVerts - 1) / kUpdVerts;
     *Gs = (int)std::max<int64_t>(1, std::min<int64_t>(gs, 2048));
     *Gv = (int)std::max<int64_t>(1, std::min<int64_t>(gv, 1024));
 }
@@ -572,9 +612,9 @@ hipError_t launch_pcg_iteration(int d, int64_t nh, const int32_t* brow_ptr, cons
     if (timer) timer->end("spmv", s);
     if (!with_update) return hipGetLastError();
     if (timer) timer->begin("pcg_update", s);
-    if (d == 1) k_pcg_update<1><<<Gv, kBlock, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, sc);
-    else if (d == 2) k_pcg_update<2><<<Gv, kBlock, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, sc);
-    else k_pcg_update<3><<<Gv, kBlock, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, sc);
+    if (d == 1) k_pcg_update<1><<<Gv, kUpdVerts, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, sc);
+    else if (d == 2) k_pcg_update<2><<<Gv, 2 * kUpdVerts, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, sc);
+    else k_pcg_update<3><<<Gv, 3 * kUpdVerts, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, sc);
     if (timer) timer->end("pcg_update", s);
     return hipGetLastError();
 }
