@@ -439,9 +439,15 @@ def test_segments_match_per_mesh_oracle(ctx):
     assert np.array_equal(fg.segment_energies(), fs) and fe == f
 
 
+@pytest.mark.parametrize("kind", ["cube", "disks"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_autotune_keeps_bits(ctx, precision):
-    P = perturbed_cube(6, 0.15, 31)
+def test_autotune_keeps_bits(ctx, precision, kind):
+    """Every element launch shape and both Hessian assembly shapes sum in the
+    same order: autotuning never changes a bit."""
+    if kind == "cube":
+        P = perturbed_cube(6, 0.15, 31)
+    else:
+        P, _ = configs.concat(configs.disk_batch(count=6, seed=5, n_min=10, n_max=30))
     fg = tad.ScalarFunction.from_problem(P, ctx, precision=precision)
     f1, g1, H1 = fg.eval_with_hessian_proj(P.x0)
     rep = fg.autotune(P.x0)

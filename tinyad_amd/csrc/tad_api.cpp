@@ -206,6 +206,8 @@ struct tad_fn_s
     // [hslot_ptr[b], hslot_ptr[b+1]), handle u gradient slots [gslot_ptr[u], ..)
     int64_t n_hslots = 0, n_gslots = 0;
     DevBuf<int64_t> hslot_ptr, gslot_ptr;
+    DevBuf<int32_t> block_mirror;  // block whose run assembles this block (itself, or the transposed upper one)
+    int asm_shape = 0;             // Hessian assembly launch shape (tad_fn_autotune)
 
     // evaluation buffers
     DevBuf<unsigned char> gc_buf, hc_buf;  // contributions (FP64 or FP32 by precision)
@@ -241,6 +243,7 @@ void free_pattern(tad_fn fn)
 {
     fn->brow_ptr.release(); fn->bcol.release(); fn->block_row.release(); fn->diag_block.release();
     fn->row_ptr.release(); fn->col_idx.release(); fn->hslot_ptr.release(); fn->gslot_ptr.release();
+    fn->block_mirror.release();
     for (TermDev& t : fn->terms)
     {
         if (t.gpos) (void)hipFree(t.gpos);
@@ -300,9 +303,13 @@ tad_status build_pattern(tad_fn fn)
         return (int64_t)(std::lower_bound(b, e, v) - bcol.data());
     };
 
-    // 2. contribution slots: per destination (CSR block / handle) a contiguous
-    //    run, ordered by term and element; per element the slot of each of its
-    //    N^2 blocks and N gradient parts (SoA, coalesced in the element kernel)
+    // 2. contribution slots: per destination a contiguous run, ordered by
+    //    term and element.  Hessian runs exist for the upper CSR blocks
+    //    (column >= row); an element writes each of its N(N+1)/2 blocks once,
+    //    in the orientation of that upper block (top bit of its slot: write
+    //    the transpose), and a lower block is assembled from its mirror's run.
+    //    Gradient runs per handle.  Per element the slot of each block and
+    //    gradient part (SoA, coalesced in the element kernel).
     std::vector<int64_t> hptr(nb + 1, 0);
     std::vector<int64_t> gptr(nh + 1, 0);
     for (const HostTerm& t : fn->hterms)
@@ -311,12 +318,20 @@ tad_status build_pattern(tad_fn fn)
             {
                 const int32_t u = t.conn[e * t.N + a];
                 gptr[u + 1] += 1;
-                for (int b = 0; b < t.N; ++b) hptr[find_block(u, t.conn[e * t.N + b]) + 1] += 1;
+                for (int b = a; b < t.N; ++b)
+                {
+                    const int32_t v = t.conn[e * t.N + b];
+                    hptr[find_block(std::min(u, v), std::max(u, v)) + 1] += 1;
+                }
             }
     for (int64_t k = 0; k < nb; ++k) hptr[k + 1] += hptr[k];
     for (int64_t u = 0; u < nh; ++u) gptr[u + 1] += gptr[u];
-    if (hptr[nb] >= 4294967295LL || gptr[nh] >= 4294967295LL)
-        return fail(TAD_ERR_UNSUPPORTED, "more than 2^32 element contributions");
+    if (hptr[nb] >= 2147483647LL || gptr[nh] >= 4294967295LL)
+        return fail(TAD_ERR_UNSUPPORTED, "more than 2^31 element contributions");
+    std::vector<int32_t> mirror(nb);
+    for (int64_t u = 0; u < nh; ++u)
+        for (int32_t k = brow_ptr[u]; k < brow_ptr[u + 1]; ++k)
+            mirror[k] = bcol[k] >= u ? k : (int32_t)find_block(bcol[k], (int32_t)u);
     std::vector<std::vector<uint32_t>> gpos(fn->hterms.size()), hpos(fn->hterms.size());
     {
         std::vector<int64_t> hc(hptr.begin(), hptr.end() - 1), gc(gptr.begin(), gptr.end() - 1);
@@ -324,15 +339,22 @@ tad_status build_pattern(tad_fn fn)
         {
             const HostTerm& t = fn->hterms[ti];
             const int64_t ne = t.n_elem;
+            const int NB = t.N * (t.N + 1) / 2;
             gpos[ti].resize((size_t)ne * t.N);
-            hpos[ti].resize((size_t)ne * t.N * t.N);
+            hpos[ti].resize((size_t)ne * NB);
             for (int64_t e = 0; e < ne; ++e)
                 for (int a = 0; a < t.N; ++a)
                 {
                     const int32_t u = t.conn[e * t.N + a];
                     gpos[ti][a * ne + e] = (uint32_t)gc[u]++;
-                    for (int b = 0; b < t.N; ++b)
-                        hpos[ti][(a * t.N + b) * ne + e] = (uint32_t)hc[find_block(u, t.conn[e * t.N + b])]++;
+                    for (int b = a; b < t.N; ++b)
+                    {
+                        const int32_t v = t.conn[e * t.N + b];
+                        const int bi = a * t.N - (a * (a - 1)) / 2 + (b - a);
+                        const uint32_t flag = (u > v) ? 0x80000000u : 0u;
+                        hpos[ti][(size_t)bi * ne + e] =
+                            (uint32_t)hc[find_block(std::min(u, v), std::max(u, v))]++ | flag;
+                    }
                 }
         }
     }
@@ -371,6 +393,7 @@ tad_status build_pattern(tad_fn fn)
     TAD_HIP(up(fn->col_idx, col_idx));
     TAD_HIP(up(fn->hslot_ptr, hptr));
     TAD_HIP(up(fn->gslot_ptr, gptr));
+    TAD_HIP(up(fn->block_mirror, mirror));
     for (size_t ti = 0; ti < fn->terms.size(); ++ti)
     {
         TermDev& td = fn->terms[ti];
@@ -439,8 +462,9 @@ tad_status eval_core(tad_fn fn, const double* x_dev, int mode, double eps, doubl
     if (H_dev && mode >= MODE_HESS)
     {
         tm.begin("assemble_hessian", s);
-        TAD_HIP(launch_assemble_hessian(fn->d, fn->precision, fn->n_blocks, fn->block_row.p, fn->brow_ptr.p,
-                                        fn->hslot_ptr.p, fn->hc_buf.p, fn->row_ptr.p, H_dev, s));
+        TAD_HIP(launch_assemble_hessian(fn->d, fn->precision, fn->asm_shape, fn->n_blocks, fn->block_row.p, fn->brow_ptr.p,
+                                        fn->hslot_ptr.p, fn->block_mirror.p, fn->hc_buf.p, fn->row_ptr.p, H_dev,
+                                        s));
         tm.end("assemble_hessian", s);
     }
     tm.begin("energy_sum", s);
@@ -1181,6 +1205,34 @@ tad_status tad_fn_autotune(tad_fn fn, const double* x, tad_mem mem, double eps, 
             rep += line;
         }
     }
+    // Hessian assembly shape (on the contributions the last launch wrote)
+    if (fn->n_blocks > 0)
+    {
+        float ms_shape[2];
+        for (int shape = 0; shape < 2; ++shape)
+        {
+            float best = 1e30f;
+            for (int rep_i = 0; rep_i < 4; ++rep_i)
+            {
+                TAD_HIP(hipEventRecord(e0, s));
+                TAD_HIP(launch_assemble_hessian(fn->d, fn->precision, shape, fn->n_blocks, fn->block_row.p,
+                                                fn->brow_ptr.p, fn->hslot_ptr.p, fn->block_mirror.p, fn->hc_buf.p,
+                                                fn->row_ptr.p, fn->H_buf.p, s));
+                TAD_HIP(hipEventRecord(e1, s));
+                TAD_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                TAD_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (rep_i > 0) best = std::min(best, ms);
+            }
+            ms_shape[shape] = best;
+        }
+        fn->asm_shape = ms_shape[1] < ms_shape[0] ? 1 : 0;
+        char line[256];
+        std::snprintf(line, sizeof(line), "Hessian assembly: %d lanes per block %.4f ms, one thread per block %.4f ms\n",
+                      fn->d * fn->d, ms_shape[0], ms_shape[1]);
+        rep += line;
+    }
+    fn->h_valid = false;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (report && report_len > 0)

@@ -50,13 +50,15 @@ enum EnergyId
 };
 
 __host__ __device__ constexpr int n_blocks(int N) { return N * (N + 1) / 2; }
+__host__ __device__ constexpr int block_index(int N, int v, int u) { return v * N - (v * (v - 1)) / 2 + (u - v); }
 
 // Scatter of one element's derivative parts through the precomputed maps:
 // gpos[a n + e] is the slot of vertex a's gradient part in G (d values per
-// slot), hpos[(a N + b) n + e] the slot of block (a, b) in Hc (d*d values per
-// slot).  Slots are sorted by destination (vertex, CSR block) and, within a
-// destination, by term and element, so the assembly sums contiguous runs in
-// the reference's element order.
+// slot); hpos[block_index(a, b) n + e] (a <= b) the slot of block (a, b) in Hc
+// (d*d values per slot), top bit set when the slot belongs to the transposed
+// (upper) CSR block.  Slots are sorted by destination (vertex, upper CSR
+// block) and, within a destination, by term and element, so the assembly sums
+// contiguous runs in the reference's element order.
 template <int N, int Dm, class T>
 struct ScatterSink
 {
@@ -72,19 +74,22 @@ struct ScatterSink
 #pragma unroll
         for (int i = 0; i < Dm; ++i) dst[i] = g[i];
     }
-    // block (v, u), v <= u; for v < u also writes block (u, v) = block^T
+    // block (v, u), v <= u, in the orientation of its slot
     __device__ __forceinline__ void block(int v, int u, const T* b) const
     {
-        T* dst = Hc + (int64_t)hpos[(v * N + u) * n + e] * (Dm * Dm);
-#pragma unroll
-        for (int m = 0; m < Dm * Dm; ++m) dst[m] = b[m];
-        if (v != u)
+        const uint32_t pos = hpos[block_index(N, v, u) * n + e];
+        T* dst = Hc + (int64_t)(pos & 0x7fffffffu) * (Dm * Dm);
+        if (pos >> 31)
         {
-            T* dt = Hc + (int64_t)hpos[(u * N + v) * n + e] * (Dm * Dm);
 #pragma unroll
             for (int i = 0; i < Dm; ++i)
 #pragma unroll
-                for (int j = 0; j < Dm; ++j) dt[j * Dm + i] = b[i * Dm + j];
+                for (int j = 0; j < Dm; ++j) dst[j * Dm + i] = b[i * Dm + j];
+        }
+        else
+        {
+#pragma unroll
+            for (int m = 0; m < Dm * Dm; ++m) dst[m] = b[m];
         }
     }
 };

@@ -23,7 +23,8 @@ struct TermDev
     int32_t* conn = nullptr;   // device SoA: conn[a * n_elem + e]
     double* params = nullptr;  // device SoA: params[k * n_elem + e]
     uint32_t* gpos = nullptr;  // SoA [N][n_elem]: gradient-part slot of (vertex a, element e)
-    uint32_t* hpos = nullptr;  // SoA [N*N][n_elem]: Hessian-block slot of (block (a, b), element e)
+    uint32_t* hpos = nullptr;  // SoA [N(N+1)/2][n_elem]: slot of upper block (a <= b) of element e;
+                               // top bit: the slot's block is (b, a), write the transpose
     int64_t f_base = 0;        // first entry of this term in the element-energy array
     // launch shape of the projected-Hessian element kernel chosen by the
     // autotuner: tune_block = 0 is the plain one-tile-per-workgroup grid,
@@ -67,11 +68,13 @@ hipError_t launch_energy(const TermDev& t, int d, const double* x, const double*
 hipError_t launch_sum(const double* in, int64_t n, int ncand, int64_t stride, double* partials, double* out,
                       hipStream_t s);
 
-// CSR block b = sum of Hessian slots [hslot_ptr[b], hslot_ptr[b+1]) of Hc;
-// gradient of handle u = sum of slots [gslot_ptr[u], gslot_ptr[u+1]) of G.
-hipError_t launch_assemble_hessian(int d, int precision, int64_t n_blocks, const int32_t* block_row,
-                                   const int32_t* brow_ptr, const int64_t* hslot_ptr, const void* Hc,
-                                   const int32_t* row_ptr, double* H, hipStream_t s);
+// CSR block b = sum of the Hessian slots [hslot_ptr[m], hslot_ptr[m+1]) of Hc,
+// m = mirror[b] (b itself for column >= row, else the upper block (col, row),
+// read transposed); gradient of handle u = sum of slots [gslot_ptr[u], ..) of G.
+// shape 0: d*d lanes per block, 1: one thread per block (identical results).
+hipError_t launch_assemble_hessian(int d, int precision, int shape, int64_t n_blocks, const int32_t* block_row,
+                                   const int32_t* brow_ptr, const int64_t* hslot_ptr, const int32_t* mirror,
+                                   const void* Hc, const int32_t* row_ptr, double* H, hipStream_t s);
 
 hipError_t launch_assemble_gradient(int d, int precision, int64_t n_handles, const int64_t* gslot_ptr,
                                     const void* G, double* g, hipStream_t s);

@@ -140,77 +140,161 @@ hipError_t launch_axpy_point(const double* x0, const double* dir, const double* 
 // The element kernels scatter every d x d block (and every per-vertex gradient
 // part) to its slot; the slots of one destination are contiguous and in
 // element order, so each output entry is a fixed-order sum over one run.
+// sum_{c in [c0, c1)} src[c * S] in index order, four loads in flight
+template <int S, class T>
+__device__ __forceinline__ double run_sum(const T* __restrict__ src, int64_t c0, int64_t c1)
+{
+    double acc = 0.0;
+    int64_t c = c0;
+    for (; c + 4 <= c1; c += 4)
+    {
+        const double a0 = (double)src[c * S], a1 = (double)src[(c + 1) * S];
+        const double a2 = (double)src[(c + 2) * S], a3 = (double)src[(c + 3) * S];
+        acc += a0;
+        acc += a1;
+        acc += a2;
+        acc += a3;
+    }
+    for (; c < c1; ++c) acc += (double)src[c * S];
+    return acc;
+}
+
+// Runs exist for the upper blocks; a lower block (u, v), u > v, sums the run
+// of (v, u) transposed.  D*D lanes share one block (lane m sums entry m over
+// the run), so every contribution is read as one contiguous D*D-value segment
+// by consecutive lanes; a 64-lane wave serves floor(64 / D^2) blocks.
 template <class T, int D>
 __global__ __launch_bounds__(kBlock) void k_assemble_hessian(int64_t nb, const int32_t* __restrict__ block_row,
                                                              const int32_t* __restrict__ brow_ptr,
                                                              const int64_t* __restrict__ hslot_ptr,
+                                                             const int32_t* __restrict__ mirror,
                                                              const T* __restrict__ Hc,
                                                              const int32_t* __restrict__ row_ptr,
                                                              double* __restrict__ H)
 {
+    constexpr int DD = D * D;
+    constexpr int BPW = 64 / DD;  // blocks per wave
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (lane >= BPW * DD) return;
+    const int64_t b = wave * BPW + lane / DD;
+    if (b >= nb) return;
+    const int m = lane % DD;
+    const int u = block_row[b];
+    const int b0 = brow_ptr[u];
+    const int k = (int)(b - b0);
+    const int nbu = brow_ptr[u + 1] - b0;
+    const int i = m / D, j = m % D;
+    const int64_t mb = mirror[b];
+    const int src = (mb == b) ? m : j * D + i;
+    const int64_t c1 = hslot_ptr[mb + 1];
+    const double acc = run_sum<DD>(Hc + src, hslot_ptr[mb], c1);
+    H[(int64_t)row_ptr[D * u] + (int64_t)i * D * nbu + D * k + j] = acc;
+}
+
+// The same sums with one thread per block (all D*D entries of a contribution
+// per thread, two contributions in flight); identical bits.  Which shape is
+// faster depends on d, the precision and the run lengths (the autotuner times
+// both).
+template <class T, int D>
+__global__ __launch_bounds__(kBlock) void k_assemble_hessian_tpb(int64_t nb, const int32_t* __restrict__ block_row,
+                                                                 const int32_t* __restrict__ brow_ptr,
+                                                                 const int64_t* __restrict__ hslot_ptr,
+                                                                 const int32_t* __restrict__ mirror,
+                                                                 const T* __restrict__ Hc,
+                                                                 const int32_t* __restrict__ row_ptr,
+                                                                 double* __restrict__ H)
+{
+    constexpr int DD = D * D;
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
     const int u = block_row[b];
     const int b0 = brow_ptr[u];
     const int k = (int)(b - b0);
     const int nbu = brow_ptr[u + 1] - b0;
-    double acc[D * D];
+    const int64_t mb = mirror[b];
+    double acc[DD];
 #pragma unroll
-    for (int m = 0; m < D * D; ++m) acc[m] = 0.0;
-    const int64_t c1 = hslot_ptr[b + 1];
-    for (int64_t c = hslot_ptr[b]; c < c1; ++c)
+    for (int m = 0; m < DD; ++m) acc[m] = 0.0;
+    int64_t c = hslot_ptr[mb];
+    const int64_t c1 = hslot_ptr[mb + 1];
+    for (; c + 2 <= c1; c += 2)
     {
-        const T* src = Hc + c * (D * D);
+        double v0[DD], v1[DD];
 #pragma unroll
-        for (int m = 0; m < D * D; ++m) acc[m] += (double)src[m];
+        for (int m = 0; m < DD; ++m)
+        {
+            v0[m] = (double)Hc[c * DD + m];
+            v1[m] = (double)Hc[(c + 1) * DD + m];
+        }
+#pragma unroll
+        for (int m = 0; m < DD; ++m)
+        {
+            acc[m] += v0[m];
+            acc[m] += v1[m];
+        }
     }
+    for (; c < c1; ++c)
+    {
+#pragma unroll
+        for (int m = 0; m < DD; ++m) acc[m] += (double)Hc[c * DD + m];
+    }
+    const bool tr = mb != b;
     const int64_t base = row_ptr[D * u];
 #pragma unroll
     for (int i = 0; i < D; ++i)
 #pragma unroll
-        for (int j = 0; j < D; ++j) H[base + (int64_t)i * D * nbu + D * k + j] = acc[i * D + j];
+        for (int j = 0; j < D; ++j)
+            H[base + (int64_t)i * D * nbu + D * k + j] = tr ? acc[j * D + i] : acc[i * D + j];
 }
 
+// D lanes per handle, floor(64 / D) handles per wave.
 template <class T, int D>
 __global__ __launch_bounds__(kBlock) void k_assemble_gradient(int64_t nh, const int64_t* __restrict__ gslot_ptr,
                                                               const T* __restrict__ G, double* __restrict__ g)
 {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int HPW = 64 / D;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (lane >= HPW * D) return;
+    const int64_t v = wave * HPW + lane / D;
     if (v >= nh) return;
-    double acc[D];
-#pragma unroll
-    for (int i = 0; i < D; ++i) acc[i] = 0.0;
-    const int64_t c1 = gslot_ptr[v + 1];
-    for (int64_t c = gslot_ptr[v]; c < c1; ++c)
-    {
-        const T* src = G + c * D;
-#pragma unroll
-        for (int i = 0; i < D; ++i) acc[i] += (double)src[i];
-    }
-#pragma unroll
-    for (int i = 0; i < D; ++i) g[v * D + i] = acc[i];
+    const int i = lane % D;
+    g[v * D + i] = run_sum<D>(G + i, gslot_ptr[v], gslot_ptr[v + 1]);
 }
 
 template <class T>
-static hipError_t launch_assemble_hessian_t(int d, int64_t nb, const int32_t* block_row, const int32_t* brow_ptr,
-                                            const int64_t* hslot_ptr, const T* Hc, const int32_t* row_ptr, double* H,
-                                            hipStream_t s)
+static hipError_t launch_assemble_hessian_t(int d, int shape, int64_t nb, const int32_t* block_row,
+                                            const int32_t* brow_ptr, const int64_t* hslot_ptr, const int32_t* mirror,
+                                            const T* Hc, const int32_t* row_ptr, double* H, hipStream_t s)
 {
     if (nb == 0) return hipSuccess;
-    const unsigned grid = (unsigned)((nb + kBlock - 1) / kBlock);
-    if (d == 1) k_assemble_hessian<T, 1><<<grid, kBlock, 0, s>>>(nb, block_row, brow_ptr, hslot_ptr, Hc, row_ptr, H);
-    else if (d == 2) k_assemble_hessian<T, 2><<<grid, kBlock, 0, s>>>(nb, block_row, brow_ptr, hslot_ptr, Hc, row_ptr, H);
-    else k_assemble_hessian<T, 3><<<grid, kBlock, 0, s>>>(nb, block_row, brow_ptr, hslot_ptr, Hc, row_ptr, H);
+    if (shape == 1)
+    {
+        const unsigned gt = (unsigned)((nb + kBlock - 1) / kBlock);
+        if (d == 1) k_assemble_hessian_tpb<T, 1><<<gt, kBlock, 0, s>>>(nb, block_row, brow_ptr, hslot_ptr, mirror, Hc, row_ptr, H);
+        else if (d == 2) k_assemble_hessian_tpb<T, 2><<<gt, kBlock, 0, s>>>(nb, block_row, brow_ptr, hslot_ptr, mirror, Hc, row_ptr, H);
+        else k_assemble_hessian_tpb<T, 3><<<gt, kBlock, 0, s>>>(nb, block_row, brow_ptr, hslot_ptr, mirror, Hc, row_ptr, H);
+        return hipGetLastError();
+    }
+    const int bpw = 64 / (d * d);
+    const int64_t waves = (nb + bpw - 1) / bpw;
+    const unsigned grid = (unsigned)((waves * 64 + kBlock - 1) / kBlock);
+    if (d == 1) k_assemble_hessian<T, 1><<<grid, kBlock, 0, s>>>(nb, block_row, brow_ptr, hslot_ptr, mirror, Hc, row_ptr, H);
+    else if (d == 2) k_assemble_hessian<T, 2><<<grid, kBlock, 0, s>>>(nb, block_row, brow_ptr, hslot_ptr, mirror, Hc, row_ptr, H);
+    else k_assemble_hessian<T, 3><<<grid, kBlock, 0, s>>>(nb, block_row, brow_ptr, hslot_ptr, mirror, Hc, row_ptr, H);
     return hipGetLastError();
 }
 
-hipError_t launch_assemble_hessian(int d, int precision, int64_t nb, const int32_t* block_row, const int32_t* brow_ptr,
-                                   const int64_t* hslot_ptr, const void* Hc, const int32_t* row_ptr, double* H,
-                                   hipStream_t s)
+hipError_t launch_assemble_hessian(int d, int precision, int shape, int64_t nb, const int32_t* block_row,
+                                   const int32_t* brow_ptr, const int64_t* hslot_ptr, const int32_t* mirror,
+                                   const void* Hc, const int32_t* row_ptr, double* H, hipStream_t s)
 {
     if (precision == 0)
-        return launch_assemble_hessian_t<double>(d, nb, block_row, brow_ptr, hslot_ptr, (const double*)Hc, row_ptr, H, s);
-    return launch_assemble_hessian_t<float>(d, nb, block_row, brow_ptr, hslot_ptr, (const float*)Hc, row_ptr, H, s);
+        return launch_assemble_hessian_t<double>(d, shape, nb, block_row, brow_ptr, hslot_ptr, mirror,
+                                                 (const double*)Hc, row_ptr, H, s);
+    return launch_assemble_hessian_t<float>(d, shape, nb, block_row, brow_ptr, hslot_ptr, mirror, (const float*)Hc,
+                                            row_ptr, H, s);
 }
 
 template <class T>
@@ -218,7 +302,9 @@ static hipError_t launch_assemble_gradient_t(int d, int64_t nh, const int64_t* g
                                              hipStream_t s)
 {
     if (nh == 0) return hipSuccess;
-    const unsigned grid = (unsigned)((nh + kBlock - 1) / kBlock);
+    const int hpw = 64 / d;
+    const int64_t waves = (nh + hpw - 1) / hpw;
+    const unsigned grid = (unsigned)((waves * 64 + kBlock - 1) / kBlock);
     if (d == 1) k_assemble_gradient<T, 1><<<grid, kBlock, 0, s>>>(nh, gslot_ptr, G, g);
     else if (d == 2) k_assemble_gradient<T, 2><<<grid, kBlock, 0, s>>>(nh, gslot_ptr, G, g);
     else k_assemble_gradient<T, 3><<<grid, kBlock, 0, s>>>(nh, gslot_ptr, G, g);
