@@ -265,11 +265,29 @@ def main():
         fo.add_elements(T.energy, T.conn[:S], T.params[:S])
         t0 = time.time()
         sec = fo.time_eval_with_hessian_proj(P.x0, 1e-9, reps=2)
-        cpu = {"value": S / sec / 1e6, "unit": "M elements/s", "cores": fo.threads, "kind": "port",
+        default_threads = fo.threads
+        # the same sample on every hardware thread the process may use
+        n_all = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        fa = oracle.Function(3, P.n_handles, n_threads=n_all)
+        fa.add_elements(T.energy, T.conn[:S], T.params[:S])
+        sec_all = fa.time_eval_with_hessian_proj(P.x0, 1e-9, reps=2)
+        cpu_model = ""
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+        cpu = {"value": S / sec / 1e6, "unit": "M elements/s", "cores": default_threads, "kind": "port",
                "sample": f"first {S} of the {n_tets} tets, eval_with_hessian_proj (dense Double<12>, "
-                         f"per-element eigen-projection, serial triplets + sorted CSR), median of 2 after warm-up",
+                         f"per-element eigen-projection, serial triplets + sorted CSR), median of 2 after warm-up, "
+                         f"{default_threads} threads = the reference's default (available threads - 1)",
+               "all_threads": {"threads": n_all, "value": S / sec_all / 1e6},
+               "host_cpu": cpu_model,
                "wall_s": round(time.time() - t0, 2)}
-        log(f"[bench] cpu baseline: {cpu['value']:.4f} M elements/s on {cpu['cores']} threads")
+        log(f"[bench] cpu baseline: {cpu['value']:.4f} M elements/s on {cpu['cores']} threads, "
+            f"{S / sec_all / 1e6:.4f} on {n_all}")
 
     # ---------------- mixed precision: errors against the FP64 path ----------------
     prec = None
