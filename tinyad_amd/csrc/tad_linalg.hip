@@ -342,11 +342,22 @@ __device__ __forceinline__ bool block_done(const PcgScalars* sc)
     return flag != 0;
 }
 
+// Every block re-sums all n partials: thread t adds parts[t], parts[t + NT],
+// ... in index order (four loads in flight), then the fixed block tree.
 template <int NT = kBlock>
 __device__ __forceinline__ double sum_parts(const double* __restrict__ parts, int n, double* sh)
 {
     double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += NT) acc += parts[i];
+    int i = threadIdx.x;
+    for (; i + 3 * NT < n; i += 4 * NT)
+    {
+        const double a0 = parts[i], a1 = parts[i + NT], a2 = parts[i + 2 * NT], a3 = parts[i + 3 * NT];
+        acc += a0;
+        acc += a1;
+        acc += a2;
+        acc += a3;
+    }
+    for (; i < n; i += NT) acc += parts[i];
     return block_sum_n<NT>(acc, sh);
 }
 
@@ -561,6 +572,33 @@ __global__ __launch_bounds__(kUpdVerts * D) void k_pcg_update(int64_t nh, const 
     __shared__ double sh[NT];
     __shared__ double sD[kUpdVerts * D * D];
     __shared__ double sR[NT];
+    const int64_t n = nh * D;
+    const int64_t dn = nh * D * D;
+    const int t = threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * kUpdVerts;
+    // operands of one chunk: this thread's variable and its third of the
+    // chunk's D x D preconditioner blocks
+    double zc = 0.0, pc = 0.0, qc = 0.0, sc_ = 0.0, xc = 0.0, rc = 0.0, dc[D];
+    auto load = [&](int64_t c0) {
+        const int64_t iv = c0 * D + t;
+        if (iv < n)
+        {
+            zc = z[iv];
+            pc = p[iv];
+            qc = q[iv];
+            sc_ = sv[iv];
+            xc = x[iv];
+            rc = r[iv];
+        }
+#pragma unroll
+        for (int m = 0; m < D; ++m)
+        {
+            const int64_t id = c0 * D * D + t + (int64_t)m * NT;
+            dc[m] = id < dn ? Dinv[id] : 0.0;
+        }
+    };
+    int64_t c0 = (int64_t)blockIdx.x * kUpdVerts;
+    if (c0 < nh) load(c0);  // in flight during the scalar re-sums
     const double delta = sum_parts<NT>(parts, Gs, sh);
     const double rho = sum_parts<NT>(parts + Gs + ((k + 1) & 1) * Gv, Gv, sh);
     double beta = 0.0, den = delta;
@@ -596,34 +634,27 @@ __global__ __launch_bounds__(kUpdVerts * D) void k_pcg_update(int64_t nh, const 
         sc->rho_hist[k & 1] = rho;
         sc->alpha_hist[k & 1] = alpha;
     }
-    const int64_t n = nh * D;
-    const int t = threadIdx.x;
     double rz = 0.0, rr = 0.0;
-    for (int64_t c0 = (int64_t)blockIdx.x * kUpdVerts; c0 < nh; c0 += (int64_t)gridDim.x * kUpdVerts)
+    for (; c0 < nh; c0 += stride)
     {
         const int64_t iv = c0 * D + t;
         double rv = 0.0;
         if (iv < n)
         {
-            const double pi = k > 0 ? fma(beta, p[iv], z[iv]) : z[iv];
-            const double qi = k > 0 ? fma(beta, q[iv], sv[iv]) : sv[iv];
+            const double pi = k > 0 ? fma(beta, pc, zc) : zc;
+            const double qi = k > 0 ? fma(beta, qc, sc_) : sc_;
             p[iv] = pi;
             q[iv] = qi;
-            x[iv] += alpha * pi;
-            rv = r[iv] - alpha * qi;
+            x[iv] = xc + alpha * pi;
+            rv = rc - alpha * qi;
             r[iv] = rv;
         }
         __syncthreads();  // the previous chunk's LDS reads are complete
         sR[t] = rv;
-        const int64_t d0 = c0 * D * D;
-        const int64_t dn = nh * D * D;
 #pragma unroll
-        for (int m = 0; m < D; ++m)
-        {
-            const int64_t id = d0 + t + (int64_t)m * NT;
-            sD[t + m * NT] = id < dn ? Dinv[id] : 0.0;
-        }
+        for (int m = 0; m < D; ++m) sD[t + m * NT] = dc[m];
         __syncthreads();
+        if (c0 + stride < nh) load(c0 + stride);  // next chunk in flight during this one's z
         if (iv < n)
         {
             const int vl = t / D, i = t - vl * D;
