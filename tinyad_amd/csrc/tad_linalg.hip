@@ -52,6 +52,23 @@ __device__ __forceinline__ double block_sum_n(double v, double* sh)
     return r;
 }
 
+// acc over v[i], v[i + S], v[i + 2S], ... < hi in that order, four loads in flight
+template <int S>
+__device__ __forceinline__ double strided_sum(const double* __restrict__ v, int64_t i, int64_t hi)
+{
+    double acc = 0.0;
+    for (; i + 3 * S < hi; i += 4 * S)
+    {
+        const double a0 = v[i], a1 = v[i + S], a2 = v[i + 2 * S], a3 = v[i + 3 * S];
+        acc += a0;
+        acc += a1;
+        acc += a2;
+        acc += a3;
+    }
+    for (; i < hi; i += S) acc += v[i];
+    return acc;
+}
+
 // block b of row c sums the contiguous chunk [b*chunk, (b+1)*chunk)
 __global__ __launch_bounds__(kBlock) void k_sum_stage1(const double* __restrict__ in, int64_t n, int64_t stride,
                                                        double* __restrict__ partials)
@@ -62,8 +79,7 @@ __global__ __launch_bounds__(kBlock) void k_sum_stage1(const double* __restrict_
     const int64_t lo = (int64_t)blockIdx.x * chunk;
     const int64_t hi = lo + chunk < n ? lo + chunk : n;
     const double* row = in + c * stride;
-    double acc = 0.0;
-    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) acc += row[i];
+    const double acc = strided_sum<kBlock>(row, lo + threadIdx.x, hi);
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) partials[c * kReduceBlocks + blockIdx.x] = s;
 }
@@ -750,7 +766,17 @@ __global__ __launch_bounds__(kBlock) void k_segment_sum(const double* __restrict
     for (int t = 0; t < n_terms; ++t)
     {
         const int64_t a = ranges[(t * n_seg + sgm) * 2], b = ranges[(t * n_seg + sgm) * 2 + 1];
-        for (int64_t i = a + threadIdx.x; i < b; i += kBlock) acc += elem_f[i];
+        int64_t i = a + threadIdx.x;
+        for (; i + 3 * kBlock < b; i += 4 * kBlock)  // four loads in flight, same order
+        {
+            const double v0 = elem_f[i], v1 = elem_f[i + kBlock], v2 = elem_f[i + 2 * kBlock],
+                         v3 = elem_f[i + 3 * kBlock];
+            acc += v0;
+            acc += v1;
+            acc += v2;
+            acc += v3;
+        }
+        for (; i < b; i += kBlock) acc += elem_f[i];
     }
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) out[sgm] = s;
