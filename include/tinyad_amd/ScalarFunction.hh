@@ -27,6 +27,7 @@
 #include <functional>
 #include <memory>
 #include <stdexcept>
+#include <cstring>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -246,6 +247,45 @@ public:
         return { f, std::move(g), std::move(H) };
     }
 
+    // --- beyond the reference's call shape (no counterpart in TinyAD) -------
+
+    // Handle ranges [offsets[s], offsets[s+1]) are independent meshes evaluated
+    // together; segment_energies() returns their energies from the last eval.
+    void set_segments(const std::vector<int64_t>& handle_offsets)
+    {
+        require_fn();
+        if (handle_offsets.size() < 2) throw std::runtime_error("[tinyad_amd] set_segments: need two offsets");
+        check(tad_fn_set_segments(fn_.get(), (int64_t)handle_offsets.size() - 1, handle_offsets.data()));
+        n_segments_ = (int64_t)handle_offsets.size() - 1;
+    }
+    std::vector<double> segment_energies() const
+    {
+        require_fn();
+        std::vector<double> f((size_t)n_segments_);
+        check(tad_fn_segment_energies(fn_.get(), f.data(), TAD_MEM_HOST));
+        return f;
+    }
+
+    // Times the element-kernel and assembly launch shapes on x and keeps the
+    // fastest; results are bit-identical for every shape.  Returns the report.
+    std::string autotune(const VectorXd& x, double projection_eps = default_hessian_projection_eps)
+    {
+        require_fn();
+        check_size(x);
+        std::string report(16384, '\0');
+        check(tad_fn_autotune(fn_.get(), x.data(), TAD_MEM_HOST, projection_eps, &report[0], report.size()));
+        report.resize(std::strlen(report.c_str()));
+        return report;
+    }
+
+    // Replaces the parameter rows of term `term` (the terms in add_elements
+    // order), n_elem x params_per_elem row-major; the pattern is kept.
+    void set_params(int term, const std::vector<double>& params)
+    {
+        require_fn();
+        check(tad_fn_set_params(fn_.get(), term, params.data(), TAD_MEM_HOST));
+    }
+
     tad_fn handle() const { return fn_.get(); }
 
     EvalSettings settings;
@@ -254,6 +294,7 @@ public:
     std::vector<int> variable_handles;
 
 private:
+    int64_t n_segments_ = 0;
     void require_fn() const
     {
         if (!fn_) throw std::runtime_error("[tinyad_amd] empty (default-constructed or moved-from) ScalarFunction");

@@ -4,6 +4,7 @@
 // case; exit status 0 when every case passes.  Needs a GPU.
 #include <tinyad_amd/ScalarFunction.hh>
 
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -153,6 +154,46 @@ static bool test_errors_and_recovery()
     return ok;
 }
 
+// Two 2-triangle meshes as segments of one function: per-mesh energies equal
+// separate functions, autotuning keeps every bit, set_params equals a function
+// built with the new rows.
+static bool test_segments_autotune_params()
+{
+    bool ok = true;
+    const std::array<int, 3> tris[2] = { { 0, 1, 2 }, { 1, 3, 2 } };
+    auto params = [](double w) { return std::vector<double>{ 1, 0, 0, 1, w }; };  // Mr = I
+    auto build = [&](int meshes, double w) {
+        auto f = tinyad_amd::scalar_function<2>(tinyad_amd::range(4 * meshes));
+        f.add_elements<3>(tinyad_amd::range(2 * meshes), TAD_E_SYMDIR_TRI,
+                          [&](int e) { auto t = tris[e % 2]; for (int& v : t) v += 4 * (e / 2); return t; },
+                          [&](int) { return params(w); });
+        return f;
+    };
+    VectorXd x = { 0, 0, 1.2, 0, 0, 0.9, 1.1, 1.0, 0, 0, 1.5, 0.1, 0.2, 1.1, 1.3, 1.2 };
+    auto both = build(2, 0.5);
+    both.set_segments({ 0, 4, 8 });
+    auto [f, g, H] = both.eval_with_hessian_proj(x);
+    std::vector<double> fs = both.segment_energies();
+    EXPECT(fs.size() == 2);
+    for (int m = 0; m < 2; ++m)
+    {
+        auto one = build(1, 0.5);
+        VectorXd xm(x.begin() + 8 * m, x.begin() + 8 * m + 8);
+        EXPECT(std::fabs(one.eval(xm) - fs[m]) <= 1e-12 * std::fabs(fs[m]));
+    }
+    EXPECT(std::fabs(fs[0] + fs[1] - f) <= 1e-12 * std::fabs(f));
+    const std::string rep = both.autotune(x);
+    EXPECT(rep.find("best") != std::string::npos);
+    auto [f2, g2, H2] = both.eval_with_hessian_proj(x);
+    EXPECT(f2 == f && g2 == g && H2.values == H.values);
+    std::vector<double> rows;
+    for (int e = 0; e < 4; ++e) { auto r = params(2.0); rows.insert(rows.end(), r.begin(), r.end()); }
+    both.set_params(0, rows);
+    auto fresh = build(2, 2.0);
+    EXPECT(both.eval(x) == fresh.eval(x));
+    return ok;
+}
+
 int main()
 {
     struct Case { const char* name; bool (*fn)(); };
@@ -162,6 +203,7 @@ int main()
         { "NewtonTest.2DDeformation", []() { return test_newton(nullptr); } },
         { "NewtonTest.Deterministic", test_newton_deterministic },
         { "ExceptionTest", test_errors_and_recovery },
+        { "Batching.SegmentsAutotuneParams", test_segments_autotune_params },
     };
     for (const Case& c : cases)
     {

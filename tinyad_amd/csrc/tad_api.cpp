@@ -1148,9 +1148,36 @@ tad_status tad_fn_autotune(tad_fn fn, const double* x, tad_mem mem, double eps, 
     if (!xd) return fail(TAD_ERR_HIP, "copy of x to the device failed");
     int n_cu = 0;
     TAD_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, fn->ctx->device));
-    hipEvent_t e0, e1;
-    TAD_HIP(hipEventCreate(&e0));
-    TAD_HIP(hipEventCreate(&e1));
+    // events released and launch shapes restored on every exit path; the new
+    // shapes are committed only when every measurement succeeded
+    struct Guard
+    {
+        tad_fn fn;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        std::vector<TermDev> saved;
+        int saved_shape = 0;
+        bool commit = false;
+        ~Guard()
+        {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (!commit)
+            {
+                for (size_t i = 0; i < fn->terms.size(); ++i)
+                {
+                    fn->terms[i].tune_block = saved[i].tune_block;
+                    fn->terms[i].tune_grid = saved[i].tune_grid;
+                    fn->terms[i].tune_map = saved[i].tune_map;
+                }
+                fn->asm_shape = saved_shape;
+            }
+        }
+    } guard{fn};
+    guard.saved = fn->terms;
+    guard.saved_shape = fn->asm_shape;
+    TAD_HIP(hipEventCreate(&guard.e0));
+    TAD_HIP(hipEventCreate(&guard.e1));
+    hipEvent_t e0 = guard.e0, e1 = guard.e1;
     std::string rep;
     for (TermDev& t : fn->terms)
     {
@@ -1233,8 +1260,7 @@ tad_status tad_fn_autotune(tad_fn fn, const double* x, tad_mem mem, double eps, 
         rep += line;
     }
     fn->h_valid = false;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    guard.commit = true;
     if (report && report_len > 0)
     {
         std::strncpy(report, rep.c_str(), report_len - 1);
