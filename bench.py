@@ -227,16 +227,18 @@ def main():
         roof = {"kernel": "k_spmv<3,16>", "bound": "hbm", "achieved": traffic_alg / avg_s / 1e9, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "bytes_per_launch": traffic_alg, "pmc_name": "tad::k_spmv<3, 16>"}
     elif dom == "assemble_hessian":
-        # contributions read (n_e N^2 blocks of d x d in the element precision)
-        # and the CSR values written
+        # contributions read once (n_e N(N+1)/2 upper blocks of d x d in the
+        # element precision) and the CSR values written
         tsz = 8 if args.precision == "fp64" else 4
         d = P.d
-        nblk = sum(t.conn.shape[0] * t.valence ** 2 for t in P.terms)
+        nblk = sum(t.conn.shape[0] * t.valence * (t.valence + 1) // 2 for t in P.terms)
         per_launch = nblk * d * d * tsz + 8 * nnz
-        kname = f"k_assemble_hessian<{'double' if tsz == 8 else 'float'},{d}>"
-        roof = {"kernel": kname, "bound": "hbm", "achieved": per_launch / avg_s / 1e9, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "bytes_per_launch": per_launch,
-                "pmc_name": f"tad::k_assemble_hessian<{'double' if tsz == 8 else 'float'}, {d}>"}
+        kbase = ["k_assemble_hessian", "k_assemble_hessian_tpb", "k_assemble_hessian_upper",
+                 "k_assemble_hessian_upper_tpb"][fn.assembly_shape]
+        tname = "double" if tsz == 8 else "float"
+        roof = {"kernel": f"{kbase}<{tname},{d}>", "bound": "hbm", "achieved": per_launch / avg_s / 1e9,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "bytes_per_launch": per_launch,
+                "pmc_name": f"tad::{kbase}<{tname}, {d}>"}
     elif dom == "element":
         Nv = P.terms[0].valence
         r = P.terms[0].params.shape[1]
@@ -349,7 +351,8 @@ def main():
                    (f"disk_batch: {args.disks} radius-clipped height-field disks, {n_tets} triangles, "
                     f"{P.n_handles} vertices, nnz={nnz}, one launch per term, per-disk energies"),
                    "precision": args.precision, "pcg_tol": args.pcg_tol,
-                   "autotuned": bool(args.autotune)},
+                   "autotuned": bool(args.autotune),
+                   "launch_shape": list(fn.launch_shape(0)), "assembly_shape": fn.assembly_shape},
         "roofline": roof,
         "cpu_baseline": cpu,
         "newton": newton,

@@ -194,6 +194,19 @@ tad_status tad_fn_rest_vjp(tad_fn fn, int term, const double* x, const double* r
  * text table of the timings.  No counterpart in the reference. */
 tad_status tad_fn_autotune(tad_fn fn, const double* x, tad_mem mem, double eps, char* report, size_t report_len);
 
+/* The launch shapes tad_fn_autotune chose, to store and restore without
+ * timing again.  Element launch of term `term` (projected Hessian path of a
+ * symmetric-Dirichlet term): block 0 is the plain grid; otherwise a
+ * persistent grid of `grid` workgroups (1 .. 2^20) of 64, 128 or 256 lanes,
+ * map 0 strided or 1 per-XCD contiguous (grid a multiple of 8).  Hessian
+ * assembly shape 0 .. 3 (d*d lanes or one thread per CSR block, d*d lanes or
+ * one thread per upper block writing it and its mirror).  Every shape gives
+ * bit-identical results.  No counterpart in the reference. */
+tad_status tad_fn_get_launch_shape(tad_fn fn, int term, int* block, int* grid, int* map);
+tad_status tad_fn_set_launch_shape(tad_fn fn, int term, int block, int grid, int map);
+tad_status tad_fn_get_assembly_shape(tad_fn fn, int* shape);
+tad_status tad_fn_set_assembly_shape(tad_fn fn, int shape);
+
 /* Device pointers of the function's own buffers holding the gradient and CSR
  * values of the last second-order evaluation (valid until the next call). */
 tad_status tad_fn_device_buffers(tad_fn fn, const double** g_dev, const double** H_values_dev);

@@ -278,6 +278,36 @@ public:
         return report;
     }
 
+    // Launch shapes (as chosen by autotune) to store and restore without timing.
+    struct LaunchShape
+    {
+        int block = 0, grid = 0, map = 0;
+    };
+    LaunchShape launch_shape(int term) const
+    {
+        require_fn();
+        LaunchShape ls;
+        check(tad_fn_get_launch_shape(fn_.get(), term, &ls.block, &ls.grid, &ls.map));
+        return ls;
+    }
+    void set_launch_shape(int term, const LaunchShape& ls)
+    {
+        require_fn();
+        check(tad_fn_set_launch_shape(fn_.get(), term, ls.block, ls.grid, ls.map));
+    }
+    int assembly_shape() const
+    {
+        require_fn();
+        int v = 0;
+        check(tad_fn_get_assembly_shape(fn_.get(), &v));
+        return v;
+    }
+    void set_assembly_shape(int shape)
+    {
+        require_fn();
+        check(tad_fn_set_assembly_shape(fn_.get(), shape));
+    }
+
     // Replaces the parameter rows of term `term` (the terms in add_elements
     // order), n_elem x params_per_elem row-major; the pattern is kept.
     void set_params(int term, const std::vector<double>& params)

@@ -442,7 +442,7 @@ def test_segments_match_per_mesh_oracle(ctx):
 @pytest.mark.parametrize("kind", ["cube", "disks"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_autotune_keeps_bits(ctx, precision, kind):
-    """Every element launch shape and both Hessian assembly shapes sum in the
+    """Every element launch shape and every Hessian assembly shape sums in the
     same order: autotuning never changes a bit."""
     if kind == "cube":
         P = perturbed_cube(6, 0.15, 31)
@@ -454,3 +454,24 @@ def test_autotune_keeps_bits(ctx, precision, kind):
     assert "best" in rep
     f2, g2, H2 = fg.eval_with_hessian_proj(P.x0)
     assert f1 == f2 and np.array_equal(g1, g2) and np.array_equal(H1.data, H2.data)
+    tuned = fg.launch_shape(0), fg.assembly_shape
+    for shape in range(4):
+        fg.assembly_shape = shape
+        for ls in [(0, 0, 0), (64, 8, 1), (128, 3, 0), (256, 1024, 1)]:
+            fg.set_launch_shape(0, *ls)
+            assert fg.launch_shape(0) == ls
+            f3, g3, H3 = fg.eval_with_hessian_proj(P.x0)
+            assert f1 == f3 and np.array_equal(g1, g3) and np.array_equal(H1.data, H3.data), (shape, ls)
+    fg.set_launch_shape(0, *tuned[0])
+    fg.assembly_shape = tuned[1]
+    for bad in [(32, 8, 0), (64, 0, 0), (64, 12, 1), (64, 8, 2)]:
+        with pytest.raises(tad.TadError):
+            fg.set_launch_shape(0, *bad)
+    with pytest.raises(tad.TadError):
+        fg.assembly_shape = 4
+    with pytest.raises(tad.TadError):
+        fg.set_launch_shape(len(P.terms), 0)
+    if P.terms[-1].energy != tad.SYMDIR_TET and P.terms[-1].energy != tad.SYMDIR_TRI:
+        with pytest.raises(tad.TadError):
+            fg.set_launch_shape(len(P.terms) - 1, 64, 8, 0)
+    assert (fg.launch_shape(0), fg.assembly_shape) == tuned

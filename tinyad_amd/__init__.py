@@ -336,6 +336,26 @@ class ScalarFunction:
         check(lib().tad_fn_autotune(self._h, ptr, mem, float(eps), buf, len(buf)))
         return buf.value.decode()
 
+    def launch_shape(self, term: int):
+        """(block, grid, map) of term's projected-Hessian element launch (block 0: plain grid)."""
+        b, g, m = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().tad_fn_get_launch_shape(self._h, int(term), ctypes.byref(b), ctypes.byref(g), ctypes.byref(m)))
+        return b.value, g.value, m.value
+
+    def set_launch_shape(self, term: int, block: int, grid: int = 0, map: int = 0):
+        check(lib().tad_fn_set_launch_shape(self._h, int(term), int(block), int(grid), int(map)))
+
+    @property
+    def assembly_shape(self) -> int:
+        """Hessian assembly shape 0..3 (bit-identical results; the autotuner picks one)."""
+        v = ctypes.c_int()
+        check(lib().tad_fn_get_assembly_shape(self._h, ctypes.byref(v)))
+        return v.value
+
+    @assembly_shape.setter
+    def assembly_shape(self, shape: int):
+        check(lib().tad_fn_set_assembly_shape(self._h, int(shape)))
+
     # -- solver ------------------------------------------------------------
     def solve(self, rhs, H=None, **opts):
         """Solves (w I + H) sol = rhs with the library's PCG; H=None uses the last evaluated Hessian."""

@@ -27,6 +27,7 @@ EXPORTED = [
     "tad_fn_eval_with_hessian_proj", "tad_fn_device_buffers",
     "tad_solve_default_opts", "tad_fn_solve", "tad_newton_default_opts", "tad_newton_step", "tad_line_search",
     "tad_fn_set_segments", "tad_fn_segment_energies", "tad_fn_autotune",
+    "tad_fn_get_launch_shape", "tad_fn_set_launch_shape", "tad_fn_get_assembly_shape", "tad_fn_set_assembly_shape",
     "tad_fn_set_params", "tad_fn_rest_vjp",
     "tad_selftest_dual",
 ]
@@ -116,6 +117,11 @@ def lib():
         "tad_fn_set_segments": [vp, i64, vp],
         "tad_fn_segment_energies": [vp, vp, i32],
         "tad_fn_autotune": [vp, vp, i32, dbl, ctypes.c_char_p, ctypes.c_size_t],
+        "tad_fn_get_launch_shape": [vp, i32, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                    ctypes.POINTER(ctypes.c_int)],
+        "tad_fn_set_launch_shape": [vp, i32, i32, i32, i32],
+        "tad_fn_get_assembly_shape": [vp, ctypes.POINTER(ctypes.c_int)],
+        "tad_fn_set_assembly_shape": [vp, i32],
         "tad_fn_set_params": [vp, i32, vp, i32],
         "tad_fn_rest_vjp": [vp, i32, vp, vp, vp, dbl, vp, i32],
         "tad_selftest_dual": [vp, i32, dbl, dbl, pdbl],

@@ -207,6 +207,10 @@ struct tad_fn_s
     int64_t n_hslots = 0, n_gslots = 0;
     DevBuf<int64_t> hslot_ptr, gslot_ptr;
     DevBuf<int32_t> block_mirror;  // block whose run assembles this block (itself, or the transposed upper one)
+    // per upper block: its run and (H offset, row stride) of it and its mirror
+    int64_t n_upper = 0;
+    DevBuf<int64_t> uslot_ptr;
+    DevBuf<int32_t> uplace;        // 4 per upper block
     int asm_shape = 0;             // Hessian assembly launch shape (tad_fn_autotune)
 
     // evaluation buffers
@@ -229,6 +233,20 @@ struct tad_fn_s
     DevBuf<double> adj_g, adj_rest, adj_lam;
 
     hipStream_t s() const { return ctx->stream; }
+    HessAsm hess_asm() const
+    {
+        HessAsm a;
+        a.n_blocks = n_blocks;
+        a.block_row = block_row.p;
+        a.brow_ptr = brow_ptr.p;
+        a.hslot_ptr = hslot_ptr.p;
+        a.mirror = block_mirror.p;
+        a.row_ptr = row_ptr.p;
+        a.n_upper = n_upper;
+        a.uslot_ptr = uslot_ptr.p;
+        a.uplace = reinterpret_cast<const int4*>(uplace.p);
+        return a;
+    }
 };
 
 namespace {
@@ -243,7 +261,7 @@ void free_pattern(tad_fn fn)
 {
     fn->brow_ptr.release(); fn->bcol.release(); fn->block_row.release(); fn->diag_block.release();
     fn->row_ptr.release(); fn->col_idx.release(); fn->hslot_ptr.release(); fn->gslot_ptr.release();
-    fn->block_mirror.release();
+    fn->block_mirror.release(); fn->uslot_ptr.release(); fn->uplace.release();
     for (TermDev& t : fn->terms)
     {
         if (t.gpos) (void)hipFree(t.gpos);
@@ -378,6 +396,54 @@ tad_status build_pattern(tad_fn fn)
                 for (int j = 0; j < d; ++j) col_idx[o++] = d * bcol[brow_ptr[u] + k] + j;
         }
     }
+    // 4. upper-run placement: run i of upper block (u, v) lands at (u, v) and,
+    //    transposed, at (v, u); every H entry must be written exactly once
+    std::vector<int64_t> uptr;
+    std::vector<int32_t> uplace;
+    uptr.reserve((nb + nh) / 2 + 1);
+    uplace.reserve(2 * (nb + nh) + 4);
+    {
+        std::vector<uint8_t> hits(nnz, 0);
+        auto place = [&](int64_t off, int64_t stride, bool tr) -> bool {
+            for (int i = 0; i < d; ++i)
+                for (int j = 0; j < d; ++j)
+                {
+                    const int64_t q = off + (tr ? (int64_t)j * stride + i : (int64_t)i * stride + j);
+                    if (q < 0 || q >= nnz || hits[q]++) return false;
+                }
+            return true;
+        };
+        for (int64_t u = 0; u < nh; ++u)
+        {
+            const int32_t nbu = brow_ptr[u + 1] - brow_ptr[u];
+            for (int32_t k = brow_ptr[u]; k < brow_ptr[u + 1]; ++k)
+            {
+                const int32_t v = bcol[k];
+                if (v < u) continue;
+                const int64_t off_up = row_ptr[d * u] + (int64_t)d * (k - brow_ptr[u]);
+                int64_t off_lo = -1, stride_lo = 0;
+                if (v != u)
+                {
+                    const int64_t lb = find_block(v, (int32_t)u);
+                    off_lo = row_ptr[d * v] + (int64_t)d * (lb - brow_ptr[v]);
+                    stride_lo = (int64_t)d * (brow_ptr[v + 1] - brow_ptr[v]);
+                }
+                if (!place(off_up, (int64_t)d * nbu, false) || (off_lo >= 0 && !place(off_lo, stride_lo, true)))
+                    return fail(TAD_ERR_UNSUPPORTED, "Hessian placement map overlaps");
+                uptr.push_back(hptr[k]);
+                uplace.push_back((int32_t)off_up);
+                uplace.push_back(d * nbu);
+                uplace.push_back((int32_t)off_lo);
+                uplace.push_back((int32_t)stride_lo);
+            }
+        }
+        for (int64_t q = 0; q < nnz; ++q)
+            if (hits[q] != 1) return fail(TAD_ERR_UNSUPPORTED, "Hessian placement map misses an entry");
+        uptr.push_back(hptr[nb]);
+        // runs of lower blocks are empty, so consecutive upper runs tile the slots
+        for (size_t i = 0; i + 1 < uptr.size(); ++i)
+            if (uptr[i] > uptr[i + 1]) return fail(TAD_ERR_UNSUPPORTED, "upper runs out of order");
+    }
     hipStream_t s = fn->s();
     auto up = [&](auto& buf, const auto& vec) -> hipError_t {
         hipError_t e = buf.ensure(vec.size());
@@ -394,6 +460,8 @@ tad_status build_pattern(tad_fn fn)
     TAD_HIP(up(fn->hslot_ptr, hptr));
     TAD_HIP(up(fn->gslot_ptr, gptr));
     TAD_HIP(up(fn->block_mirror, mirror));
+    TAD_HIP(up(fn->uslot_ptr, uptr));
+    TAD_HIP(up(fn->uplace, uplace));
     for (size_t ti = 0; ti < fn->terms.size(); ++ti)
     {
         TermDev& td = fn->terms[ti];
@@ -407,6 +475,7 @@ tad_status build_pattern(tad_fn fn)
     fn->n_hslots = hptr[nb];
     fn->n_gslots = gptr[nh];
     fn->n_blocks = nb;
+    fn->n_upper = (int64_t)uptr.size() - 1;
     fn->nnz = nnz;
     fn->pattern_ready = true;
     return TAD_OK;
@@ -462,9 +531,7 @@ tad_status eval_core(tad_fn fn, const double* x_dev, int mode, double eps, doubl
     if (H_dev && mode >= MODE_HESS)
     {
         tm.begin("assemble_hessian", s);
-        TAD_HIP(launch_assemble_hessian(fn->d, fn->precision, fn->asm_shape, fn->n_blocks, fn->block_row.p, fn->brow_ptr.p,
-                                        fn->hslot_ptr.p, fn->block_mirror.p, fn->hc_buf.p, fn->row_ptr.p, H_dev,
-                                        s));
+        TAD_HIP(launch_assemble_hessian(fn->d, fn->precision, fn->asm_shape, fn->hess_asm(), fn->hc_buf.p, H_dev, s));
         tm.end("assemble_hessian", s);
     }
     tm.begin("energy_sum", s);
@@ -1137,6 +1204,56 @@ tad_status tad_fn_segment_energies(tad_fn fn, double* f, tad_mem mem)
     return TAD_OK;
 }
 
+tad_status tad_fn_get_launch_shape(tad_fn fn, int term, int* block, int* grid, int* map)
+{
+    TAD_TRY(check_fn(fn));
+    if (!block || !grid || !map) return fail(TAD_ERR_INVALID, "null argument");
+    if (term < 0 || term >= (int)fn->terms.size()) return fail(TAD_ERR_INVALID, "term index out of range");
+    const TermDev& t = fn->terms[term];
+    *block = t.tune_block;
+    *grid = t.tune_grid;
+    *map = t.tune_map;
+    return TAD_OK;
+}
+
+tad_status tad_fn_set_launch_shape(tad_fn fn, int term, int block, int grid, int map)
+{
+    TAD_TRY(check_fn(fn));
+    if (term < 0 || term >= (int)fn->terms.size()) return fail(TAD_ERR_INVALID, "term index out of range");
+    TermDev& t = fn->terms[term];
+    if (block == 0)
+    {
+        t.tune_block = t.tune_grid = t.tune_map = 0;
+        return TAD_OK;
+    }
+    if (!(t.energy == TAD_E_SYMDIR_TRI || t.energy == TAD_E_SYMDIR_TET))
+        return fail(TAD_ERR_UNSUPPORTED, "only symmetric-Dirichlet terms have persistent launch shapes");
+    if (block != 64 && block != 128 && block != 256) return fail(TAD_ERR_INVALID, "block must be 0, 64, 128 or 256");
+    if (grid < 1 || grid > (1 << 20)) return fail(TAD_ERR_INVALID, "grid must be in 1 .. 2^20");
+    if (map != 0 && map != 1) return fail(TAD_ERR_INVALID, "map must be 0 or 1");
+    if (map == 1 && grid % 8 != 0) return fail(TAD_ERR_INVALID, "per-XCD map needs a grid that is a multiple of 8");
+    t.tune_block = block;
+    t.tune_grid = grid;
+    t.tune_map = map;
+    return TAD_OK;
+}
+
+tad_status tad_fn_get_assembly_shape(tad_fn fn, int* shape)
+{
+    TAD_TRY(check_fn(fn));
+    if (!shape) return fail(TAD_ERR_INVALID, "null argument");
+    *shape = fn->asm_shape;
+    return TAD_OK;
+}
+
+tad_status tad_fn_set_assembly_shape(tad_fn fn, int shape)
+{
+    TAD_TRY(check_fn(fn));
+    if (shape < 0 || shape >= kAsmShapes) return fail(TAD_ERR_INVALID, "assembly shape must be in 0 .. 3");
+    fn->asm_shape = shape;
+    return TAD_OK;
+}
+
 tad_status tad_fn_autotune(tad_fn fn, const double* x, tad_mem mem, double eps, char* report, size_t report_len)
 {
     TAD_TRY(check_fn(fn));
@@ -1235,16 +1352,15 @@ tad_status tad_fn_autotune(tad_fn fn, const double* x, tad_mem mem, double eps, 
     // Hessian assembly shape (on the contributions the last launch wrote)
     if (fn->n_blocks > 0)
     {
-        float ms_shape[2];
-        for (int shape = 0; shape < 2; ++shape)
+        float ms_shape[kAsmShapes];
+        const HessAsm pat = fn->hess_asm();
+        for (int shape = 0; shape < kAsmShapes; ++shape)
         {
             float best = 1e30f;
             for (int rep_i = 0; rep_i < 4; ++rep_i)
             {
                 TAD_HIP(hipEventRecord(e0, s));
-                TAD_HIP(launch_assemble_hessian(fn->d, fn->precision, shape, fn->n_blocks, fn->block_row.p,
-                                                fn->brow_ptr.p, fn->hslot_ptr.p, fn->block_mirror.p, fn->hc_buf.p,
-                                                fn->row_ptr.p, fn->H_buf.p, s));
+                TAD_HIP(launch_assemble_hessian(fn->d, fn->precision, shape, pat, fn->hc_buf.p, fn->H_buf.p, s));
                 TAD_HIP(hipEventRecord(e1, s));
                 TAD_HIP(hipEventSynchronize(e1));
                 float ms = 0.f;
@@ -1253,10 +1369,15 @@ tad_status tad_fn_autotune(tad_fn fn, const double* x, tad_mem mem, double eps, 
             }
             ms_shape[shape] = best;
         }
-        fn->asm_shape = ms_shape[1] < ms_shape[0] ? 1 : 0;
+        int bs = 0;
+        for (int shape = 1; shape < kAsmShapes; ++shape)
+            if (ms_shape[shape] < ms_shape[bs]) bs = shape;
+        fn->asm_shape = bs;
         char line[256];
-        std::snprintf(line, sizeof(line), "Hessian assembly: %d lanes per block %.4f ms, one thread per block %.4f ms\n",
-                      fn->d * fn->d, ms_shape[0], ms_shape[1]);
+        std::snprintf(line, sizeof(line),
+                      "Hessian assembly: %d lanes per block %.4f ms, one thread per block %.4f ms, "
+                      "%d lanes per upper block %.4f ms, one thread per upper block %.4f ms\n",
+                      fn->d * fn->d, ms_shape[0], ms_shape[1], fn->d * fn->d, ms_shape[2], ms_shape[3]);
         rep += line;
     }
     fn->h_valid = false;

@@ -68,13 +68,36 @@ hipError_t launch_energy(const TermDev& t, int d, const double* x, const double*
 hipError_t launch_sum(const double* in, int64_t n, int ncand, int64_t stride, double* partials, double* out,
                       hipStream_t s);
 
-// CSR block b = sum of the Hessian slots [hslot_ptr[m], hslot_ptr[m+1]) of Hc,
-// m = mirror[b] (b itself for column >= row, else the upper block (col, row),
-// read transposed); gradient of handle u = sum of slots [gslot_ptr[u], ..) of G.
-// shape 0: d*d lanes per block, 1: one thread per block (identical results).
-hipError_t launch_assemble_hessian(int d, int precision, int shape, int64_t n_blocks, const int32_t* block_row,
-                                   const int32_t* brow_ptr, const int64_t* hslot_ptr, const int32_t* mirror,
-                                   const void* Hc, const int32_t* row_ptr, double* H, hipStream_t s);
+// Device pattern of the Hessian assembly.
+//   Per CSR block b (shapes 0, 1): its row block_row[b], the run
+//   [hslot_ptr[m], hslot_ptr[m+1]) of m = mirror[b] (b itself for column >=
+//   row, else the upper block (col, row), read transposed).
+//   Per upper block i (shapes 2, 3; upper blocks in CSR order): its run
+//   [uslot_ptr[i], uslot_ptr[i+1]) and uplace[i] = (offset of entry (0,0) in
+//   H, row stride of its block row, the same two for the mirrored lower block
+//   or (-1, 0) on the diagonal).
+struct HessAsm
+{
+    int64_t n_blocks = 0;
+    const int32_t* block_row = nullptr;
+    const int32_t* brow_ptr = nullptr;
+    const int64_t* hslot_ptr = nullptr;
+    const int32_t* mirror = nullptr;
+    const int32_t* row_ptr = nullptr;
+    int64_t n_upper = 0;
+    const int64_t* uslot_ptr = nullptr;
+    const int4* uplace = nullptr;
+};
+
+constexpr int kAsmShapes = 4;
+
+// CSR values of H from the Hessian contributions Hc; gradient of handle u =
+// sum of slots [gslot_ptr[u], ..) of G.  shape 0: d*d lanes per block, 1: one
+// thread per block, 2: d*d lanes per upper block writing it and its mirror,
+// 3: one thread per upper block writing both.  Every shape sums each run in
+// slot order (identical results).
+hipError_t launch_assemble_hessian(int d, int precision, int shape, const HessAsm& a, const void* Hc, double* H,
+                                   hipStream_t s);
 
 hipError_t launch_assemble_gradient(int d, int precision, int64_t n_handles, const int64_t* gslot_ptr,
                                     const void* G, double* g, hipStream_t s);

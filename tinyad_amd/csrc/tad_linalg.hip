@@ -264,6 +264,81 @@ __global__ __launch_bounds__(kBlock) void k_assemble_hessian_tpb(int64_t nb, con
             H[base + (int64_t)i * D * nbu + D * k + j] = tr ? acc[j * D + i] : acc[i * D + j];
 }
 
+// Upper-run shapes: each run is summed once and written to its upper block
+// and, transposed, to the mirrored lower block, so every contribution is read
+// once (shapes 0 / 1 read the off-diagonal runs twice).  D*D lanes per upper
+// block, lane m = (i, j) writes H_upper(i, j) and H_lower(j, i).
+template <class T, int D>
+__global__ __launch_bounds__(kBlock) void k_assemble_hessian_upper(int64_t nu, const int64_t* __restrict__ uslot_ptr,
+                                                                   const int4* __restrict__ uplace,
+                                                                   const T* __restrict__ Hc, double* __restrict__ H)
+{
+    constexpr int DD = D * D;
+    constexpr int BPW = 64 / DD;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (lane >= BPW * DD) return;
+    const int64_t b = wave * BPW + lane / DD;
+    if (b >= nu) return;
+    const int m = lane % DD;
+    const int i = m / D, j = m % D;
+    const double acc = run_sum<DD>(Hc + m, uslot_ptr[b], uslot_ptr[b + 1]);
+    const int4 pl = uplace[b];
+    H[(int64_t)pl.x + (int64_t)i * pl.y + j] = acc;
+    if (pl.z >= 0) H[(int64_t)pl.z + (int64_t)j * pl.w + i] = acc;
+}
+
+// One thread per upper block, two contributions in flight.
+template <class T, int D>
+__global__ __launch_bounds__(kBlock) void k_assemble_hessian_upper_tpb(int64_t nu,
+                                                                       const int64_t* __restrict__ uslot_ptr,
+                                                                       const int4* __restrict__ uplace,
+                                                                       const T* __restrict__ Hc,
+                                                                       double* __restrict__ H)
+{
+    constexpr int DD = D * D;
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nu) return;
+    double acc[DD];
+#pragma unroll
+    for (int m = 0; m < DD; ++m) acc[m] = 0.0;
+    int64_t c = uslot_ptr[b];
+    const int64_t c1 = uslot_ptr[b + 1];
+    for (; c + 2 <= c1; c += 2)
+    {
+        double v0[DD], v1[DD];
+#pragma unroll
+        for (int m = 0; m < DD; ++m)
+        {
+            v0[m] = (double)Hc[c * DD + m];
+            v1[m] = (double)Hc[(c + 1) * DD + m];
+        }
+#pragma unroll
+        for (int m = 0; m < DD; ++m)
+        {
+            acc[m] += v0[m];
+            acc[m] += v1[m];
+        }
+    }
+    for (; c < c1; ++c)
+    {
+#pragma unroll
+        for (int m = 0; m < DD; ++m) acc[m] += (double)Hc[c * DD + m];
+    }
+    const int4 pl = uplace[b];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) H[(int64_t)pl.x + (int64_t)i * pl.y + j] = acc[i * D + j];
+    if (pl.z >= 0)
+    {
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j) H[(int64_t)pl.z + (int64_t)j * pl.w + i] = acc[i * D + j];
+    }
+}
+
 // D lanes per handle, floor(64 / D) handles per wave.
 template <class T, int D>
 __global__ __launch_bounds__(kBlock) void k_assemble_gradient(int64_t nh, const int64_t* __restrict__ gslot_ptr,
@@ -279,38 +354,41 @@ __global__ __launch_bounds__(kBlock) void k_assemble_gradient(int64_t nh, const 
     g[v * D + i] = run_sum<D>(G + i, gslot_ptr[v], gslot_ptr[v + 1]);
 }
 
-template <class T>
-static hipError_t launch_assemble_hessian_t(int d, int shape, int64_t nb, const int32_t* block_row,
-                                            const int32_t* brow_ptr, const int64_t* hslot_ptr, const int32_t* mirror,
-                                            const T* Hc, const int32_t* row_ptr, double* H, hipStream_t s)
+template <class T, int D>
+static void launch_assemble_hessian_td(int shape, const HessAsm& a, const T* Hc, double* H, hipStream_t s)
 {
-    if (nb == 0) return hipSuccess;
-    if (shape == 1)
-    {
-        const unsigned gt = (unsigned)((nb + kBlock - 1) / kBlock);
-        if (d == 1) k_assemble_hessian_tpb<T, 1><<<gt, kBlock, 0, s>>>(nb, block_row, brow_ptr, hslot_ptr, mirror, Hc, row_ptr, H);
-        else if (d == 2) k_assemble_hessian_tpb<T, 2><<<gt, kBlock, 0, s>>>(nb, block_row, brow_ptr, hslot_ptr, mirror, Hc, row_ptr, H);
-        else k_assemble_hessian_tpb<T, 3><<<gt, kBlock, 0, s>>>(nb, block_row, brow_ptr, hslot_ptr, mirror, Hc, row_ptr, H);
-        return hipGetLastError();
-    }
-    const int bpw = 64 / (d * d);
-    const int64_t waves = (nb + bpw - 1) / bpw;
-    const unsigned grid = (unsigned)((waves * 64 + kBlock - 1) / kBlock);
-    if (d == 1) k_assemble_hessian<T, 1><<<grid, kBlock, 0, s>>>(nb, block_row, brow_ptr, hslot_ptr, mirror, Hc, row_ptr, H);
-    else if (d == 2) k_assemble_hessian<T, 2><<<grid, kBlock, 0, s>>>(nb, block_row, brow_ptr, hslot_ptr, mirror, Hc, row_ptr, H);
-    else k_assemble_hessian<T, 3><<<grid, kBlock, 0, s>>>(nb, block_row, brow_ptr, hslot_ptr, mirror, Hc, row_ptr, H);
+    constexpr int bpw = 64 / (D * D);
+    const int64_t n = shape >= 2 ? a.n_upper : a.n_blocks;
+    const unsigned gt = (unsigned)((n + kBlock - 1) / kBlock);
+    const unsigned gl = (unsigned)((((n + bpw - 1) / bpw) * 64 + kBlock - 1) / kBlock);
+    if (shape == 0)
+        k_assemble_hessian<T, D><<<gl, kBlock, 0, s>>>(a.n_blocks, a.block_row, a.brow_ptr, a.hslot_ptr, a.mirror, Hc,
+                                                       a.row_ptr, H);
+    else if (shape == 1)
+        k_assemble_hessian_tpb<T, D><<<gt, kBlock, 0, s>>>(a.n_blocks, a.block_row, a.brow_ptr, a.hslot_ptr, a.mirror,
+                                                           Hc, a.row_ptr, H);
+    else if (shape == 2)
+        k_assemble_hessian_upper<T, D><<<gl, kBlock, 0, s>>>(a.n_upper, a.uslot_ptr, a.uplace, Hc, H);
+    else
+        k_assemble_hessian_upper_tpb<T, D><<<gt, kBlock, 0, s>>>(a.n_upper, a.uslot_ptr, a.uplace, Hc, H);
+}
+
+template <class T>
+static hipError_t launch_assemble_hessian_t(int d, int shape, const HessAsm& a, const T* Hc, double* H, hipStream_t s)
+{
+    if (a.n_blocks == 0) return hipSuccess;
+    if (shape < 0 || shape >= kAsmShapes) return hipErrorInvalidValue;
+    if (d == 1) launch_assemble_hessian_td<T, 1>(shape, a, Hc, H, s);
+    else if (d == 2) launch_assemble_hessian_td<T, 2>(shape, a, Hc, H, s);
+    else launch_assemble_hessian_td<T, 3>(shape, a, Hc, H, s);
     return hipGetLastError();
 }
 
-hipError_t launch_assemble_hessian(int d, int precision, int shape, int64_t nb, const int32_t* block_row,
-                                   const int32_t* brow_ptr, const int64_t* hslot_ptr, const int32_t* mirror,
-                                   const void* Hc, const int32_t* row_ptr, double* H, hipStream_t s)
+hipError_t launch_assemble_hessian(int d, int precision, int shape, const HessAsm& a, const void* Hc, double* H,
+                                   hipStream_t s)
 {
-    if (precision == 0)
-        return launch_assemble_hessian_t<double>(d, shape, nb, block_row, brow_ptr, hslot_ptr, mirror,
-                                                 (const double*)Hc, row_ptr, H, s);
-    return launch_assemble_hessian_t<float>(d, shape, nb, block_row, brow_ptr, hslot_ptr, mirror, (const float*)Hc,
-                                            row_ptr, H, s);
+    if (precision == 0) return launch_assemble_hessian_t<double>(d, shape, a, (const double*)Hc, H, s);
+    return launch_assemble_hessian_t<float>(d, shape, a, (const float*)Hc, H, s);
 }
 
 template <class T>
