@@ -311,28 +311,38 @@ inline bool positive_diagonally_dominant(const double* H, int k, double eps)
     return true;
 }
 
+// Cyclic Jacobi with Rutishauser's termination (Rutishauser 1971; the form in
+// Press et al., Numerical Recipes, sec. 11.1): the rotated pair is set to an
+// exact zero, the first three sweeps skip entries below a fifth of the mean
+// off-diagonal magnitude, later sweeps zero an entry that no longer changes
+// either diagonal entry it couples, and the iteration ends when every
+// off-diagonal entry is exactly zero.
 inline void jacobi_eigen(const double* H, int k, double* lambda, double* V)
 {
     std::vector<double> A(H, H + k * k);
     for (int i = 0; i < k * k; ++i) V[i] = 0.0;
     for (int i = 0; i < k; ++i) V[i * k + i] = 1.0;
-    for (int sweep = 0; sweep < 100; ++sweep)
+    for (int sweep = 1; sweep <= 64; ++sweep)
     {
-        double off = 0.0, nrm = 0.0;
-        for (int i = 0; i < k; ++i)
-            for (int j = 0; j < k; ++j)
-            {
-                nrm += A[i * k + j] * A[i * k + j];
-                if (i != j) off += A[i * k + j] * A[i * k + j];
-            }
-        if (off <= 1e-34 * nrm || off == 0.0)
+        double sm = 0.0;
+        for (int p = 0; p < k - 1; ++p)
+            for (int q = p + 1; q < k; ++q) sm += std::fabs(A[p * k + q]);
+        if (sm == 0.0)
             break;
+        const double tresh = sweep < 4 ? 0.2 * sm / (k * k) : 0.0;
         for (int p = 0; p < k - 1; ++p)
             for (int q = p + 1; q < k; ++q)
             {
                 const double apq = A[p * k + q];
-                if (apq == 0.0) continue;
                 const double app = A[p * k + p], aqq = A[q * k + q];
+                const double g = 100.0 * std::fabs(apq);
+                if (sweep > 4 && std::fabs(app) + g == std::fabs(app) && std::fabs(aqq) + g == std::fabs(aqq))
+                {
+                    A[p * k + q] = A[q * k + p] = 0.0;
+                    continue;
+                }
+                if (std::fabs(apq) <= tresh || apq == 0.0)
+                    continue;
                 const double theta = (aqq - app) / (2.0 * apq);
                 const double t = (theta >= 0.0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
                 const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
@@ -348,6 +358,7 @@ inline void jacobi_eigen(const double* H, int k, double* lambda, double* V)
                     A[p * k + r] = c * apr - s * aqr;
                     A[q * k + r] = s * apr + c * aqr;
                 }
+                A[p * k + q] = A[q * k + p] = 0.0;  // annihilated in exact arithmetic
                 for (int r = 0; r < k; ++r)
                 {
                     const double vrp = V[r * k + p], vrq = V[r * k + q];
@@ -538,31 +549,51 @@ inline void eval_second_order(const Function& fn, const double* x, bool project,
 }
 
 // Triplets -> CSR with duplicates summed in insertion order.
+// Linear-time triplet -> CSR with duplicates summed in triplet order (the
+// semantics of Eigen's setFromTriplets, ScalarFunctionImpl.hh:398): bucket the
+// triplets per row in input order, merge equal columns of a row through a
+// column marker, then sort each row's columns.
 inline void triplets_to_csr(long n, const std::vector<Triplet>& trip,
                             std::vector<int>& row_ptr, std::vector<int>& col_idx, std::vector<double>& values)
 {
+    std::vector<long> rstart(n + 1, 0);
+    for (const Triplet& t : trip) rstart[t.r + 1]++;
+    for (long r = 0; r < n; ++r) rstart[r + 1] += rstart[r];
     std::vector<long> order(trip.size());
-    std::iota(order.begin(), order.end(), 0L);
-    std::stable_sort(order.begin(), order.end(), [&](long a, long b) {
-        if (trip[a].r != trip[b].r) return trip[a].r < trip[b].r;
-        return trip[a].c < trip[b].c;
-    });
-    row_ptr.assign(n + 1, 0);
-    col_idx.clear(); values.clear();
-    long i = 0;
-    while (i < (long)order.size())
     {
-        const Triplet& t0 = trip[order[i]];
-        double s = 0.0;
-        long j = i;
-        while (j < (long)order.size() && trip[order[j]].r == t0.r && trip[order[j]].c == t0.c)
-            s += trip[order[j++]].v;
-        col_idx.push_back(t0.c);
-        values.push_back(s);
-        row_ptr[t0.r + 1]++;
-        i = j;
+        std::vector<long> fill(rstart.begin(), rstart.end() - 1);
+        for (long i = 0; i < (long)trip.size(); ++i) order[fill[trip[i].r]++] = i;
     }
-    for (long r = 0; r < n; ++r) row_ptr[r + 1] += row_ptr[r];
+    row_ptr.assign(n + 1, 0);
+    col_idx.clear();
+    values.clear();
+    std::vector<long> marker(n, -1);
+    std::vector<std::pair<int, double>> row;
+    for (long r = 0; r < n; ++r)
+    {
+        row.clear();
+        for (long i = rstart[r]; i < rstart[r + 1]; ++i)
+        {
+            const Triplet& t = trip[order[i]];
+            if (marker[t.c] < 0)
+            {
+                marker[t.c] = (long)row.size();
+                row.emplace_back(t.c, t.v);
+            }
+            else
+                row[marker[t.c]].second += t.v;
+        }
+        for (const auto& e : row) marker[e.first] = -1;
+        std::sort(row.begin(), row.end(), [](const std::pair<int, double>& a, const std::pair<int, double>& b) {
+            return a.first < b.first;
+        });
+        for (const auto& e : row)
+        {
+            col_idx.push_back(e.first);
+            values.push_back(e.second);
+        }
+        row_ptr[r + 1] = (int)col_idx.size();
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -312,6 +312,26 @@ def test_projection_matches_eigh(case, eps):
         assert np.array_equal(Hp, A)  # unchanged bits when nothing is clamped
 
 
+@pytest.mark.parametrize("eps", [1e-9, -1.0])
+def test_tet_projection_matches_eigh(eps):
+    """12 x 12 tet Hessians (three null directions, mixed signs) at strongly
+    perturbed positions: the projected element Hessian is the LAPACK spectral
+    projection of the unprojected one."""
+    P = tet_problem()
+    fo = oracle.Function.from_problem(P)
+    x = perturbed(P, 0.35, 9)
+    clamped = 0
+    for e in range(0, 48, 3):
+        val, _, H, _ = fo.element(0, e, x, 12)
+        if not np.isfinite(val):
+            continue
+        _, _, Hp, _ = fo.element(0, e, x, 12, project=True, eps=eps)
+        ref = spectral_projection(0.5 * (H + H.T), eps)
+        assert np.abs(Hp - ref).max() <= 1e-12 * np.abs(ref).max(), e
+        clamped += np.linalg.eigvalsh(H).min() < -1e-9 * np.abs(H).max()
+    assert clamped >= 3  # indefinite elements were among the cases
+
+
 # ---------------------------------------------------------------------------
 # Assembly
 # ---------------------------------------------------------------------------
