@@ -66,6 +66,9 @@ def main():
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
                     help="pick the element-kernel launch shape first (untimed; default on)")
     ap.add_argument("--pcg-tol", type=float, default=1e-10)
+    ap.add_argument("--coarse-aggregate", type=int, default=None,
+                    help="handles per aggregate of the two-level preconditioner (0: block-Jacobi only; "
+                         "default: the library's)")
     ap.add_argument("--cpu-sample", type=int, default=400000, help="tets in the CPU baseline sample (0: skip)")
     ap.add_argument("--no-newton", action="store_true")
     ap.add_argument("--newton-iters", type=int, default=30,
@@ -74,6 +77,7 @@ def main():
                     help="first run 3 device-to-device copies of this many MB (a plain copy kernel of known "
                          "size, used to scale the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    coarse_kw = {} if args.coarse_aggregate is None else {"solve_coarse_aggregate": args.coarse_aggregate}
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -160,7 +164,7 @@ def main():
     newton = None
     statsB = {}
     if not args.no_newton:
-        opts = tad.newton_default_opts(solve_tol=args.pcg_tol)
+        opts = tad.newton_default_opts(solve_tol=args.pcg_tol, **coarse_kw)
         x = x0.clone()
         for i in range(args.warmup):
             st = fn.newton_step(x, opts)
@@ -191,15 +195,25 @@ def main():
             log(f"[bench] newton: f {st['f']:.10g} -> {st['f_new']:.10g} cg {st['cg_iters']} ls {st['ls_trials']} "
                 f"step {st['step']:.3g}")
         statsB = {k: ctx.kernel_stats(k) for k in ["element", "assemble_gradient", "assemble_hessian", "energy_sum",
-                                                   "spmv", "pcg_update", "pcg_setup", "pcg_init",
-                                                   "energy"]}
+                                                   "spmv", "pcg_update", "pcg_coarse", "pcg_setup",
+                                                   "coarse_setup", "coarse_galerkin", "coarse_potrf",
+                                                   "coarse_potri", "pcg_init", "energy"]}
         n_cg = sum(s["cg_iters"] for s in sts)
         n_ls = sum(s["ls_trials"] for s in sts)
         s_r = 8
         B_eval = n_tets * (4 * 4 + 10 * s_r) + 16 * n + 8 + 8 * nnz
         B_energy = n_tets * (4 * 4 + 10 * s_r) + 16 * n
         B_cg = 12 * nnz + 4 * (n + 1) + 104 * n
-        B_step_total = args.steps * B_eval + n_cg * B_cg + n_ls * B_energy
+        # two-level preconditioner: per CG iteration the restriction (r, lever
+        # arms, member list), the dense coarse product and the prolongation (z
+        # read and written, lever arms, aggregate ids); per solve the Galerkin
+        # sums (H, block indices) and the dense coarse matrix written, factored
+        # and inverted in place (counted as four passes over it)
+        nc = sts[-1]["coarse_unknowns"]
+        nb_blocks = nnz // 9
+        B_coarse = (40 * n + 8 * P.n_handles + 8 * nc * nc + 16 * nc) if nc else 0
+        B_coarse_setup = (8 * nnz + 24 * nb_blocks + 32 * nc * nc) if nc else 0
+        B_step_total = (args.steps * (B_eval + B_coarse_setup) + n_cg * (B_cg + B_coarse) + n_ls * B_energy)
         newton = {
             "iters_per_s": args.steps / tB,
             "ms_per_iter": tB / args.steps * 1e3,
@@ -207,7 +221,9 @@ def main():
             "ls_trials_total": n_ls,
             "f_final": sts[-1]["f_new"],
             "step_roofline_frac": B_step_total / tB / (HBM_PEAK_GBS * 1e9),
-            "step_bytes_formula": "steps*B_eval_hess + n_cg*B_cg + n_ls*B_energy (scalar-CSR B_cg)",
+            "step_bytes_formula": ("steps*(B_eval_hess + B_coarse_setup) + n_cg*(B_cg + B_coarse) + n_ls*B_energy "
+                                   "(scalar-CSR B_cg)"),
+            "coarse_unknowns": nc,
         }
         log(f"[bench] phase B: {newton['ms_per_iter']:.2f} ms per Newton iteration, cg {n_cg}, ls {n_ls}")
 
@@ -307,7 +323,7 @@ def main():
                              "tests)",
                 "f_rel": abs(fm - fd) / abs(fd), "g": errs(gm, gd), "H_proj": errs(Hm, Hd)}
         if newton is not None:
-            opts64 = tad.newton_default_opts(solve_tol=args.pcg_tol)
+            opts64 = tad.newton_default_opts(solve_tol=args.pcg_tol, **coarse_kw)
             x64 = x0.clone()
             for _ in range(args.warmup):
                 f64.newton_step(x64, opts64)
@@ -351,6 +367,9 @@ def main():
                    (f"disk_batch: {args.disks} radius-clipped height-field disks, {n_tets} triangles, "
                     f"{P.n_handles} vertices, nnz={nnz}, one launch per term, per-disk energies"),
                    "precision": args.precision, "pcg_tol": args.pcg_tol,
+                   "coarse_aggregate": tad.solve_default_opts(**({} if args.coarse_aggregate is None else
+                                                                 {"coarse_aggregate": args.coarse_aggregate})
+                                                              ).coarse_aggregate,
                    "autotuned": bool(args.autotune),
                    "launch_shape": list(fn.launch_shape(0)), "assembly_shape": fn.assembly_shape},
         "roofline": roof,

@@ -219,6 +219,8 @@ typedef struct tad_solve_opts
     double tol;         /* relative residual ||r|| <= tol ||rhs||                        */
     int max_iters;
     int check_every;    /* host polls convergence every this many iterations             */
+    int coarse_aggregate; /* two-level preconditioner: handles per aggregate of the      */
+                        /* coarse space (default 1024); 0: block-Jacobi only            */
 } tad_solve_opts;
 
 typedef struct tad_solve_stats
@@ -226,13 +228,18 @@ typedef struct tad_solve_stats
     int iters;
     int converged;
     double rel_residual;
+    int coarse_unknowns;    /* size of the coarse space used (0: block-Jacobi only)      */
 } tad_solve_stats;
 
 tad_status tad_solve_default_opts(tad_solve_opts* opts);
 
-/* Solves (w I + H) sol = rhs by block-Jacobi preconditioned CG (FP64) on the
- * function's pattern; replaces the SimplicialLDLT factorize/solve of
- * newton_direction (NewtonDirection.hh:25-48, LinearSolver.hh:13-19).
+/* Solves (w I + H) sol = rhs by preconditioned CG (FP64) on the function's
+ * pattern; replaces the SimplicialLDLT factorize/solve of newton_direction
+ * (NewtonDirection.hh:25-48, LinearSolver.hh:13-19).  The preconditioner is
+ * block-Jacobi (d x d diagonal blocks) plus, with coarse_aggregate > 0, an
+ * additive coarse correction on aggregates of about that many handles
+ * (translations, and rotations about the positions of the last second-order
+ * evaluation when there was one; Galerkin coarse matrix inverted in FP64).
  * H_values == NULL uses the function's buffer from the last second-order eval. */
 tad_status tad_fn_solve(tad_fn fn, const double* H_values, const double* rhs, double* sol, tad_mem mem,
                         const tad_solve_opts* opts, tad_solve_stats* stats);
@@ -260,6 +267,7 @@ typedef struct tad_newton_stats
     int ls_trials;          /* energies evaluated in the line search                    */
     int ls_failed;          /* 1: no step accepted, x unchanged (LineSearch.hh:62-64)   */
     int solve_converged;
+    int coarse_unknowns;    /* coarse space of the direction solve (0: none)            */
 } tad_newton_stats;
 
 tad_status tad_newton_default_opts(tad_newton_opts* opts);

@@ -17,6 +17,8 @@
 #include "tad_internal.hpp"
 
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -165,6 +167,7 @@ struct tad_ctx_s
     hipStream_t own_stream = nullptr;
     KernelTimer timer;
     double* pinned = nullptr;  // small pinned staging buffer (256 doubles)
+    rocblas_handle blas = nullptr;  // dense factorization of coarse matrices (created on first use)
     // one reference held by the creator plus one per live function: the
     // context is released when the last of them is gone, whatever the order
     // of tad_ctx_destroy and tad_fn_destroy calls
@@ -178,6 +181,7 @@ static void ctx_release(tad_ctx ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->timer.collect();
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
 }
@@ -212,6 +216,7 @@ struct tad_fn_s
     DevBuf<int64_t> uslot_ptr;
     DevBuf<int32_t> uplace;        // 4 per upper block
     int asm_shape = 0;             // Hessian assembly launch shape (tad_fn_autotune)
+    std::vector<int32_t> h_brow_ptr, h_bcol;  // host copy of the block pattern (coarse aggregation)
 
     // evaluation buffers
     DevBuf<unsigned char> gc_buf, hc_buf;  // contributions (FP64 or FP32 by precision)
@@ -231,6 +236,29 @@ struct tad_fn_s
     DevBuf<double> seg_f;
     // adjoint products (FP64 gradient slots when the contributions are FP32)
     DevBuf<double> adj_g, adj_rest, adj_lam;
+    // two-level preconditioner: handle aggregates, coarse block pairs with
+    // their runs of fine CSR blocks, the coarse matrix and its inverse
+    struct Coarse
+    {
+        int target = 0;               // aggregate size the pattern was built for (0: none)
+        int64_t n_agg = 0, n_pairs = 0, n_chunks = 0, n_gchunks = 0;
+        DevBuf<int32_t> agg_of, agg_ptr, agg_vert, pair_blocks, pair_ab, chunk_ptr, agg_chunk, pair_chunk;
+        DevBuf<int64_t> gchunk_ptr;
+        DevBuf<double> dvec, Ac, rc, yc, cparts, chunk_part, gpart;
+        DevBuf<int> info;
+        void release()
+        {
+            agg_of.release(); agg_ptr.release(); agg_vert.release(); pair_blocks.release(); pair_ab.release();
+            chunk_ptr.release(); agg_chunk.release(); pair_chunk.release(); gchunk_ptr.release();
+            dvec.release(); Ac.release(); rc.release(); yc.release(); cparts.release(); chunk_part.release();
+            gpart.release(); info.release();
+            target = 0;
+            n_agg = n_pairs = n_chunks = n_gchunks = 0;
+        }
+    } co;
+    // positions of the last second-order evaluation (rotation modes of the coarse space)
+    DevBuf<double> x_hess;
+    bool x_hess_valid = false;
 
     hipStream_t s() const { return ctx->stream; }
     HessAsm hess_asm() const
@@ -262,6 +290,8 @@ void free_pattern(tad_fn fn)
     fn->brow_ptr.release(); fn->bcol.release(); fn->block_row.release(); fn->diag_block.release();
     fn->row_ptr.release(); fn->col_idx.release(); fn->hslot_ptr.release(); fn->gslot_ptr.release();
     fn->block_mirror.release(); fn->uslot_ptr.release(); fn->uplace.release();
+    fn->co.release();
+    fn->h_brow_ptr.clear(); fn->h_bcol.clear();
     for (TermDev& t : fn->terms)
     {
         if (t.gpos) (void)hipFree(t.gpos);
@@ -472,12 +502,252 @@ tad_status build_pattern(tad_fn fn)
         TAD_HIP(hipMemcpyAsync(td.hpos, hpos[ti].data(), hpos[ti].size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     }
     TAD_HIP(hipStreamSynchronize(s));
+    fn->h_brow_ptr = std::move(brow_ptr);
+    fn->h_bcol = std::move(bcol);
     fn->n_hslots = hptr[nb];
     fn->n_gslots = gptr[nh];
     fn->n_blocks = nb;
     fn->n_upper = (int64_t)uptr.size() - 1;
     fn->nnz = nnz;
     fn->pattern_ready = true;
+    return TAD_OK;
+}
+
+// Aggregates of handles for the two-level preconditioner, grown on the block
+// graph: the lowest unaggregated handle starts an aggregate, which takes
+// unaggregated neighbours breadth first until it holds `target` handles;
+// aggregates below target/4 are then merged into the neighbouring aggregate
+// they share the most couplings with (lowest index on ties), and aggregates
+// are renumbered in order.  Coarse block pairs (a, b) with their runs of fine
+// CSR blocks (CSR order) give the Galerkin sums.  Deterministic; once per
+// function and target.
+tad_status build_coarse(tad_fn fn, int target)
+{
+    const int64_t nh = fn->n_handles;
+    const std::vector<int32_t>& rp = fn->h_brow_ptr;
+    const std::vector<int32_t>& cb = fn->h_bcol;
+    std::vector<int32_t> agg(nh, -1);
+    int32_t na = 0;
+    std::vector<int32_t> queue;
+    queue.reserve(target);
+    for (int64_t root = 0; root < nh; ++root)
+    {
+        if (agg[root] >= 0) continue;
+        queue.clear();
+        queue.push_back((int32_t)root);
+        agg[root] = na;
+        for (size_t head = 0; head < queue.size() && (int)queue.size() < target; ++head)
+        {
+            const int32_t u = queue[head];
+            for (int32_t k = rp[u]; k < rp[u + 1] && (int)queue.size() < target; ++k)
+            {
+                const int32_t v = cb[k];
+                if (agg[v] < 0)
+                {
+                    agg[v] = na;
+                    queue.push_back(v);
+                }
+            }
+        }
+        ++na;
+    }
+    std::vector<int64_t> size(na, 0);
+    for (int64_t v = 0; v < nh; ++v) size[agg[v]]++;
+    const int64_t small_lim = std::max(1, target / 4);
+    std::vector<std::vector<int32_t>> members(na);
+    for (int64_t v = 0; v < nh; ++v) members[agg[v]].push_back((int32_t)v);
+    std::vector<int32_t> merged(na);
+    for (int32_t a = 0; a < na; ++a) merged[a] = a;
+    for (int32_t a = 0; a < na; ++a)
+    {
+        if (size[a] >= small_lim) continue;
+        std::map<int32_t, int64_t> links;  // neighbouring large aggregate -> couplings
+        for (int32_t u : members[a])
+            for (int32_t k = rp[u]; k < rp[u + 1]; ++k)
+            {
+                const int32_t b = agg[cb[k]];
+                if (b != a && size[b] >= small_lim) links[b]++;
+            }
+        int32_t best = -1;
+        int64_t bl = 0;
+        for (const auto& kv : links)
+            if (kv.second > bl)
+            {
+                best = kv.first;
+                bl = kv.second;
+            }
+        if (best >= 0) merged[a] = best;  // a small component on its own stays an aggregate
+    }
+    std::vector<int32_t> renum(na, -1);
+    int32_t nn = 0;
+    for (int32_t a = 0; a < na; ++a)
+        if (merged[a] == a) renum[a] = nn++;
+    std::vector<int32_t> agg_of(nh);
+    for (int64_t v = 0; v < nh; ++v) agg_of[v] = renum[merged[agg[v]]];
+    std::vector<int32_t> agg_ptr(nn + 1, 0), agg_vert(nh);
+    for (int64_t v = 0; v < nh; ++v) agg_ptr[agg_of[v] + 1]++;
+    for (int32_t a = 0; a < nn; ++a) agg_ptr[a + 1] += agg_ptr[a];
+    {
+        std::vector<int32_t> fill(agg_ptr.begin(), agg_ptr.end() - 1);
+        for (int64_t v = 0; v < nh; ++v) agg_vert[fill[agg_of[v]]++] = (int32_t)v;  // ascending per aggregate
+    }
+    // coarse block pairs and their runs of fine blocks
+    const int64_t nb = (int64_t)cb.size();
+    std::vector<std::pair<int64_t, int32_t>> keyed(nb);
+    for (int64_t u = 0; u < nh; ++u)
+        for (int32_t k = rp[u]; k < rp[u + 1]; ++k)
+            keyed[k] = { (int64_t)agg_of[u] * nn + agg_of[cb[k]], k };
+    std::sort(keyed.begin(), keyed.end());
+    std::vector<int64_t> pair_ptr(1, 0);
+    std::vector<int32_t> pair_ab, pair_blocks(nb);
+    for (int64_t i = 0; i < nb; ++i)
+    {
+        if (i == 0 || keyed[i].first != keyed[i - 1].first)
+        {
+            if (i > 0) pair_ptr.push_back(i);
+            pair_ab.push_back((int32_t)(keyed[i].first / nn));
+            pair_ab.push_back((int32_t)(keyed[i].first % nn));
+        }
+        pair_blocks[i] = keyed[i].second;
+    }
+    pair_ptr.push_back(nb);
+    // chunks of at most kCoarseChunk handles / fine blocks (parallel sums)
+    std::vector<int32_t> chunk_ptr, agg_chunk(nn + 1, 0);
+    for (int32_t a = 0; a < nn; ++a)
+    {
+        agg_chunk[a] = (int32_t)chunk_ptr.size();
+        for (int32_t o = agg_ptr[a]; o < agg_ptr[a + 1]; o += kCoarseChunk) chunk_ptr.push_back(o);
+    }
+    agg_chunk[nn] = (int32_t)chunk_ptr.size();
+    chunk_ptr.push_back((int32_t)nh);
+    const int64_t n_pairs = (int64_t)pair_ab.size() / 2;
+    std::vector<int64_t> gchunk_ptr;
+    std::vector<int32_t> pair_chunk(n_pairs + 1, 0);
+    for (int64_t q = 0; q < n_pairs; ++q)
+    {
+        pair_chunk[q] = (int32_t)gchunk_ptr.size();
+        for (int64_t o = pair_ptr[q]; o < pair_ptr[q + 1]; o += kCoarseChunk) gchunk_ptr.push_back(o);
+    }
+    pair_chunk[n_pairs] = (int32_t)gchunk_ptr.size();
+    gchunk_ptr.push_back(nb);
+    auto& c = fn->co;
+    c.release();
+    hipStream_t s = fn->s();
+    auto up = [&](auto& buf, const auto& vec) -> hipError_t {
+        hipError_t e = buf.ensure(vec.size());
+        if (e != hipSuccess || vec.empty()) return e;
+        return hipMemcpyAsync(buf.p, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice, s);
+    };
+    TAD_HIP(up(c.agg_of, agg_of));
+    TAD_HIP(up(c.agg_ptr, agg_ptr));
+    TAD_HIP(up(c.agg_vert, agg_vert));
+    TAD_HIP(up(c.pair_blocks, pair_blocks));
+    TAD_HIP(up(c.pair_ab, pair_ab));
+    TAD_HIP(up(c.chunk_ptr, chunk_ptr));
+    TAD_HIP(up(c.agg_chunk, agg_chunk));
+    TAD_HIP(up(c.gchunk_ptr, gchunk_ptr));
+    TAD_HIP(up(c.pair_chunk, pair_chunk));
+    TAD_HIP(hipStreamSynchronize(s));
+    c.n_agg = nn;
+    c.n_pairs = n_pairs;
+    c.n_chunks = (int64_t)chunk_ptr.size() - 1;
+    c.n_gchunks = (int64_t)gchunk_ptr.size() - 1;
+    c.target = target;
+    return TAD_OK;
+}
+
+// Largest coarse matrix the two-level preconditioner builds (dense FP64 inverse).
+constexpr int64_t kMaxCoarse = 8192;
+
+// Per solve: coarse unknowns per aggregate, lever arms, A_c = P^T (w I + H) P,
+// its Cholesky factorization and inverse (rocSOLVER potrf / potri, lower
+// triangle, rocBLAS atomics off so the result is deterministic), symmetrized.
+// *use = false when the coarse space is off or A_c is not numerically
+// positive definite (block-Jacobi alone then).
+tad_status setup_coarse(tad_fn fn, const double* H_dev, const tad_solve_opts& o, CoarseDev* cd, bool* use)
+{
+    *use = false;
+    const int d = fn->d;
+    const int64_t nh = fn->n_handles;
+    if (o.coarse_aggregate <= 0 || nh == 0 || fn->n_blocks == 0) return TAD_OK;
+    const bool rot = fn->x_hess_valid && d >= 2;
+    const int kc = rot ? (d == 3 ? 6 : 3) : d;
+    // keep the dense coarse matrix bounded: larger aggregates for large meshes
+    const int target = (int)std::max<int64_t>(o.coarse_aggregate, (nh * kc + kMaxCoarse - 1) / kMaxCoarse);
+    if (fn->co.target != target) TAD_TRY(build_coarse(fn, target));
+    auto& c = fn->co;
+    if (c.n_agg < 2) return TAD_OK;  // one aggregate: nothing a coarse space adds at this size
+    const int64_t nc = kc * c.n_agg;
+    if (nc > kMaxCoarse) return TAD_OK;
+    hipStream_t s = fn->s();
+    KernelTimer& tm = fn->ctx->timer;
+    TAD_HIP(c.dvec.ensure((size_t)nh * d));
+    TAD_HIP(c.Ac.ensure((size_t)(nc * nc)));
+    TAD_HIP(c.rc.ensure((size_t)nc));
+    TAD_HIP(c.yc.ensure((size_t)nc));
+    const int Gc = coarse_apply_groups(nc);
+    TAD_HIP(c.cparts.ensure((size_t)2 * Gc));
+    TAD_HIP(c.info.ensure(2));
+    TAD_HIP(c.chunk_part.ensure((size_t)c.n_chunks * kc));
+    TAD_HIP(c.gpart.ensure((size_t)c.n_gchunks * kc * kc));
+    cd->kc = kc;
+    cd->n_agg = c.n_agg;
+    cd->nc = nc;
+    cd->agg_of = c.agg_of.p;
+    cd->agg_ptr = c.agg_ptr.p;
+    cd->agg_vert = c.agg_vert.p;
+    cd->n_chunks = c.n_chunks;
+    cd->chunk_ptr = c.chunk_ptr.p;
+    cd->agg_chunk = c.agg_chunk.p;
+    cd->chunk_part = c.chunk_part.p;
+    cd->dvec = c.dvec.p;
+    cd->Ainv = c.Ac.p;
+    cd->rc = c.rc.p;
+    cd->yc = c.yc.p;
+    cd->cparts = c.cparts.p;
+    cd->Gc = Gc;
+    tm.begin("coarse_setup", s);
+    tm.begin("coarse_galerkin", s);
+    if (rot) TAD_HIP(launch_coarse_dvec(d, nh, fn->x_hess.p, *cd, c.dvec.p, s));
+    else TAD_HIP(launch_fill(c.dvec.p, nh * d, 0.0, s));
+    TAD_HIP(launch_fill(c.Ac.p, nc * nc, 0.0, s));
+    TAD_HIP(launch_coarse_galerkin(d, c.n_pairs, c.n_gchunks, c.gchunk_ptr.p, c.pair_chunk.p, c.pair_blocks.p,
+                                   c.pair_ab.p, fn->block_row.p, fn->brow_ptr.p, fn->bcol.p, fn->row_ptr.p, H_dev,
+                                   o.w_identity, *cd, c.gpart.p, c.Ac.p, s));
+    tm.end("coarse_galerkin", s);
+    tad_ctx ctx = fn->ctx;
+    if (!ctx->blas)
+    {
+        if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
+        {
+            ctx->blas = nullptr;
+            return fail(TAD_ERR_HIP, "rocblas_create_handle failed");
+        }
+        (void)rocblas_set_atomics_mode(ctx->blas, rocblas_atomics_not_allowed);
+    }
+    if (rocblas_set_stream(ctx->blas, s) != rocblas_status_success) return fail(TAD_ERR_HIP, "rocblas_set_stream failed");
+    tm.begin("coarse_potrf", s);
+    if (rocsolver_dpotrf(ctx->blas, rocblas_fill_lower, (rocblas_int)nc, c.Ac.p, (rocblas_int)nc, c.info.p) !=
+        rocblas_status_success)
+        return fail(TAD_ERR_HIP, "rocsolver_dpotrf failed");
+    tm.end("coarse_potrf", s);
+    tm.begin("coarse_potri", s);
+    if (rocsolver_dpotri(ctx->blas, rocblas_fill_lower, (rocblas_int)nc, c.Ac.p, (rocblas_int)nc, c.info.p + 1) !=
+        rocblas_status_success)
+        return fail(TAD_ERR_HIP, "rocsolver_dpotri failed");
+    tm.end("coarse_potri", s);
+    int info[2] = { 0, 0 };
+    TAD_HIP(hipMemcpyAsync(info, c.info.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    TAD_HIP(hipStreamSynchronize(s));
+    if (info[0] != 0 || info[1] != 0)
+    {
+        tm.end("coarse_setup", s);
+        cd->kc = 0;
+        return TAD_OK;  // not positive definite at working precision: block-Jacobi alone
+    }
+    TAD_HIP(launch_symmetrize_lower(nc, c.Ac.p, s));
+    tm.end("coarse_setup", s);
+    *use = true;
     return TAD_OK;
 }
 
@@ -533,6 +803,10 @@ tad_status eval_core(tad_fn fn, const double* x_dev, int mode, double eps, doubl
         tm.begin("assemble_hessian", s);
         TAD_HIP(launch_assemble_hessian(fn->d, fn->precision, fn->asm_shape, fn->hess_asm(), fn->hc_buf.p, H_dev, s));
         tm.end("assemble_hessian", s);
+        // positions of this Hessian: lever arms of the coarse space's rotations
+        TAD_HIP(fn->x_hess.ensure((size_t)fn->n_vars));
+        TAD_HIP(hipMemcpyAsync(fn->x_hess.p, x_dev, fn->n_vars * sizeof(double), hipMemcpyDeviceToDevice, s));
+        fn->x_hess_valid = true;
     }
     tm.begin("energy_sum", s);
     TAD_HIP(launch_sum(fn->elem_f.p, fn->n_elem_total, 1, fn->n_elem_total, fn->partials.p, fn->results.p, s));
@@ -610,6 +884,10 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
     tm.begin("pcg_setup", s);
     TAD_HIP(launch_pcg_setup(d, nh, fn->brow_ptr.p, fn->diag_block.p, fn->row_ptr.p, H_dev, o.w_identity, fn->Dinv.p, s));
     tm.end("pcg_setup", s);
+    CoarseDev cd;
+    bool use_coarse = false;
+    TAD_TRY(setup_coarse(fn, H_dev, o, &cd, &use_coarse));
+    const CoarseDev* co = use_coarse ? &cd : nullptr;
     PcgScalars hs;
     std::memset(&hs, 0, sizeof(hs));
     hs.tol2 = o.tol * o.tol;
@@ -618,7 +896,7 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
     std::memcpy(pin, &hs, sizeof(hs));
     TAD_HIP(hipMemcpyAsync(fn->scal.p, pin, sizeof(hs), hipMemcpyHostToDevice, s));
     tm.begin("pcg_init", s);
-    TAD_HIP(launch_pcg_init(d, nh, rhs_dev, fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->pcg_parts.p, fn->scal.p, s));
+    TAD_HIP(launch_pcg_init(d, nh, rhs_dev, fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->pcg_parts.p, fn->scal.p, co, s));
     tm.end("pcg_init", s);
     const int every = std::max(1, o.check_every);
     Timer* timer = fn->ctx->timer.enabled ? &fn->ctx->timer : nullptr;
@@ -632,7 +910,7 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
         for (int c = 0; c < every && k <= o.max_iters; ++c, ++k)
             TAD_HIP(launch_pcg_iteration(d, nh, fn->brow_ptr.p, fn->bcol.p, fn->row_ptr.p, H_dev, o.w_identity,
                                          fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->p.p, fn->q.p, fn->pcg_parts.p,
-                                         fn->scal.p, k, k < o.max_iters, s, timer));
+                                         fn->scal.p, k, k < o.max_iters, co, s, timer));
         TAD_HIP(hipMemcpyAsync(pin, fn->scal.p, sizeof(hs), hipMemcpyDeviceToHost, s));
         TAD_HIP(hipStreamSynchronize(s));
         if (pin->done) break;
@@ -645,6 +923,7 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
         st->iters = hs.iters;
         st->rel_residual = rel;
         st->converged = (!hs.breakdown && hs.rr <= hs.tol2 * hs.bb) ? 1 : 0;
+        st->coarse_unknowns = use_coarse ? (int)cd.nc : 0;
     }
     if (hs.breakdown) return fail(TAD_ERR_SOLVE, "Linear solve failed: matrix not positive definite");
     if (!std::isfinite(rel)) return fail(TAD_ERR_SOLVE, "Linear solve failed: non-finite residual");
@@ -821,6 +1100,7 @@ tad_status tad_fn_destroy(tad_fn fn)
     fn->Dinv.release(); fn->r.release(); fn->z.release(); fn->p.release(); fn->q.release(); fn->sol.release();
     fn->scal.release(); fn->pcg_parts.release(); fn->seg_ranges.release(); fn->seg_f.release();
     fn->adj_g.release(); fn->adj_rest.release(); fn->adj_lam.release();
+    fn->co.release(); fn->x_hess.release();
     tad_ctx ctx = fn->ctx;
     delete fn;
     ctx_release(ctx);
@@ -1008,6 +1288,7 @@ tad_status tad_solve_default_opts(tad_solve_opts* o)
     o->tol = 1e-10;
     o->max_iters = 20000;
     o->check_every = 16;
+    o->coarse_aggregate = 1024;
     return TAD_OK;
 }
 
@@ -1125,6 +1406,7 @@ tad_status tad_newton_step(tad_fn fn, double* x, tad_mem mem, const tad_newton_o
     TAD_TRY(solve_core(fn, fn->H_buf.p, fn->g_buf.p, fn->sol.p, o.solve, &ss));
     st.cg_iters = ss.iters;
     st.solve_converged = ss.converged;
+    st.coarse_unknowns = ss.coarse_unknowns;
     double dpg = 0.0;
     TAD_TRY(dot_dev(fn, fn->sol.p, fn->g_buf.p, &dpg));
     const double dg = -dpg;  // d^T g

@@ -112,19 +112,73 @@ hipError_t launch_axpy_point(const double* x0, const double* dir, const double* 
 hipError_t launch_dot(const double* a, const double* b, int64_t n, double* partials, double* out, hipStream_t s);
 
 // --- PCG -------------------------------------------------------------------
+// Coarse space of the two-level preconditioner (additive correction
+// z += P A_c^-1 P^T r on top of block-Jacobi).  P maps kc unknowns per
+// aggregate to its handles: d translations and, when kc > d, the rotations
+// about the aggregate's first handle (lever arms dvec[v] = x_v - x_root).
+struct CoarseDev
+{
+    int kc = 0;                      // unknowns per aggregate; 0: no coarse correction
+    int64_t n_agg = 0, nc = 0;       // aggregates, coarse unknowns (kc * n_agg)
+    const int32_t* agg_of = nullptr; // [n_handles]
+    const int32_t* agg_ptr = nullptr;  // [n_agg + 1] members agg_vert[agg_ptr[a] .. agg_ptr[a+1]), ascending
+    const int32_t* agg_vert = nullptr; // [n_handles]
+    // member lists cut into chunks of at most kCoarseChunk handles, chunks of
+    // one aggregate consecutive: chunk ch covers agg_vert[chunk_ptr[ch] ..
+    // chunk_ptr[ch+1]); aggregate a owns chunks agg_chunk[a] .. agg_chunk[a+1]
+    int64_t n_chunks = 0;
+    const int32_t* chunk_ptr = nullptr;  // [n_chunks + 1]
+    const int32_t* agg_chunk = nullptr;  // [n_agg + 1]
+    double* chunk_part = nullptr;        // [n_chunks * kc] restriction partials
+    const double* dvec = nullptr;    // [n_handles * d]
+    const double* Ainv = nullptr;    // [nc * nc] inverse Galerkin matrix (symmetric, both triangles)
+    double* rc = nullptr;            // [nc] restricted residual
+    double* yc = nullptr;            // [nc] coarse correction
+    double* cparts = nullptr;        // [2 * Gc] partials of rc . yc, slot k & 1
+    int Gc = 0;                      // workgroups of the coarse product
+};
+
+constexpr int kCoarseChunk = 256;  // handles per restriction chunk, fine blocks per Galerkin chunk
+
+// Workgroups of the coarse product for nc coarse unknowns.
+int coarse_apply_groups(int64_t nc);
+
+// Lever arms dvec[v] = x_v - x_(first member of v's aggregate).
+hipError_t launch_coarse_dvec(int d, int64_t nh, const double* x, const CoarseDev& c, double* dvec, hipStream_t s);
+
+// A_c (column-major nc x nc, zeroed by the caller) = P^T (w I + H) P.  The
+// fine CSR blocks of each coarse block pair (a, b) (pair_blocks, CSR order)
+// are cut into chunks of at most kCoarseChunk blocks (gchunk_ptr; the chunks
+// of pair p are pair_chunk[p] .. pair_chunk[p+1]); one wave sums a chunk,
+// then each pair sums its chunk partials (gpart, n_gchunks * kc^2) in order.
+hipError_t launch_coarse_galerkin(int d, int64_t n_pairs, int64_t n_gchunks, const int64_t* gchunk_ptr,
+                                  const int32_t* pair_chunk, const int32_t* pair_blocks, const int32_t* pair_ab,
+                                  const int32_t* block_row, const int32_t* brow_ptr, const int32_t* bcol,
+                                  const int32_t* row_ptr, const double* H, double w, const CoarseDev& c,
+                                  double* gpart, double* Ac, hipStream_t s);
+
+// Copies the lower triangle of the column-major n x n matrix A to the upper.
+hipError_t launch_symmetrize_lower(int64_t n, double* A, hipStream_t s);
+
+// z += P A_c^-1 P^T r (restriction, coarse product with partials of rc . yc
+// into slot `slot`, prolongation); every kernel returns at once once the
+// solve has terminated.
+hipError_t launch_coarse_correction(int d, int64_t nh, const CoarseDev& c, const double* r, double* z, int slot,
+                                    PcgScalars* sc, hipStream_t s, Timer* timer);
 hipError_t launch_pcg_setup(int d, int64_t n_handles, const int32_t* brow_ptr, const int32_t* diag_block,
                             const int32_t* row_ptr, const double* H, double w, double* Dinv, hipStream_t s);
 
 hipError_t launch_pcg_init(int d, int64_t n_handles, const double* b, const double* Dinv, double* x, double* r,
-                           double* z, double* partials, PcgScalars* sc, hipStream_t s);
+                           double* z, double* partials, PcgScalars* sc, const CoarseDev* coarse, hipStream_t s);
 
 // One CG iteration k: SpMV on z with the termination test, then (with_update)
-// the scalar recurrences and the p / q / x / r / z update.  pbuf holds 2 * n
-// values (p, then s = (w I + H) z).
+// the scalar recurrences and the p / q / x / r / z update, and the coarse
+// correction of z when `coarse` has kc > 0.  pbuf holds 2 * n values (p, then
+// s = (w I + H) z).
 hipError_t launch_pcg_iteration(int d, int64_t n_handles, const int32_t* brow_ptr, const int32_t* bcol,
                                 const int32_t* row_ptr, const double* H, double w, const double* Dinv, double* x,
                                 double* r, double* z, double* pbuf, double* q, double* partials, PcgScalars* sc, int k,
-                                bool with_update, hipStream_t s, Timer* timer);
+                                bool with_update, const CoarseDev* coarse, hipStream_t s, Timer* timer);
 
 // Size (doubles) of the partial-sum buffer the PCG kernels need.
 int64_t pcg_parts_size(int d, int64_t n_handles);

@@ -659,6 +659,7 @@ __global__ __launch_bounds__(kUpdVerts * D) void k_pcg_update(int64_t nh, const 
                                                               double* __restrict__ q,
                                                               const double* __restrict__ sv,
                                                               double* __restrict__ parts, int Gs, int Gv, int k,
+                                                              const double* __restrict__ cparts, int Gc,
                                                               PcgScalars* sc)
 {
     constexpr int NT = kUpdVerts * D;
@@ -694,7 +695,9 @@ __global__ __launch_bounds__(kUpdVerts * D) void k_pcg_update(int64_t nh, const 
     int64_t c0 = (int64_t)blockIdx.x * kUpdVerts;
     if (c0 < nh) load(c0);  // in flight during the scalar re-sums
     const double delta = sum_parts<NT>(parts, Gs, sh);
-    const double rho = sum_parts<NT>(parts + Gs + ((k + 1) & 1) * Gv, Gv, sh);
+    double rho = sum_parts<NT>(parts + Gs + ((k + 1) & 1) * Gv, Gv, sh);
+    // r.z with the coarse correction: r.(D^-1 r) + (P^T r).(A_c^-1 P^T r)
+    if (Gc > 0) rho += sum_parts<NT>(cparts + ((k + 1) & 1) * Gc, Gc, sh);
     double beta = 0.0, den = delta;
     if (k > 0)
     {
@@ -769,6 +772,356 @@ __global__ __launch_bounds__(kUpdVerts * D) void k_pcg_update(int64_t nh, const 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Two-level preconditioner: additive coarse correction on handle aggregates
+// ---------------------------------------------------------------------------
+// z = D^-1 r + P A_c^-1 P^T r with A_c = P^T (w I + H) P.  P has KC columns per
+// aggregate; its rows for handle v are R_v (D x KC): the identity on the
+// translations and, for KC > D, the rotations omega about the aggregate's
+// first handle, delta x_v = omega x dv (D = 3) or omega (-dv_1, dv_0) (D = 2).
+// Every sum below runs in a fixed order (fixed lane assignment and a fixed
+// shuffle tree), so the correction is bitwise reproducible.
+template <int D, int KC>
+__device__ __forceinline__ void coarse_rows(const double* __restrict__ dv, double R[D][KC])
+{
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int c = 0; c < KC; ++c) R[i][c] = (c == i) ? 1.0 : 0.0;
+    if constexpr (KC > D)
+    {
+        if constexpr (D == 3)
+        {
+            R[0][4] = dv[2];
+            R[0][5] = -dv[1];
+            R[1][3] = -dv[2];
+            R[1][5] = dv[0];
+            R[2][3] = dv[1];
+            R[2][4] = -dv[0];
+        }
+        else if constexpr (D == 2)
+        {
+            R[0][2] = -dv[1];
+            R[1][2] = dv[0];
+        }
+    }
+}
+
+// sum over the 64 lanes of a wave, fixed xor tree (every lane gets the total)
+template <int L>
+__device__ __forceinline__ void wave_sum(double* v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int m = 0; m < L; ++m) v[m] += __shfl_xor(v[m], off, 64);
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_coarse_dvec(int64_t nh, const double* __restrict__ x,
+                                                        const int32_t* __restrict__ agg_of,
+                                                        const int32_t* __restrict__ agg_ptr,
+                                                        const int32_t* __restrict__ agg_vert, double* __restrict__ dvec)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nh) return;
+    const int64_t root = agg_vert[agg_ptr[agg_of[v]]];
+#pragma unroll
+    for (int i = 0; i < D; ++i) dvec[v * D + i] = x[v * D + i] - x[root * D + i];
+}
+
+// one wave per chunk of a coarse block pair's fine blocks (u, v): sum of
+// R_u^T (H_uv + w I [u == v]) R_v over the chunk
+template <int D, int KC>
+__global__ __launch_bounds__(kBlock) void k_coarse_galerkin_chunk(int64_t n_gchunks,
+                                                                  const int64_t* __restrict__ gchunk_ptr,
+                                                                  const int32_t* __restrict__ pair_blocks,
+                                                                  const int32_t* __restrict__ block_row,
+                                                                  const int32_t* __restrict__ brow_ptr,
+                                                                  const int32_t* __restrict__ bcol,
+                                                                  const int32_t* __restrict__ row_ptr,
+                                                                  const double* __restrict__ H, double w,
+                                                                  const double* __restrict__ dvec,
+                                                                  double* __restrict__ gpart)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t ch = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (ch >= n_gchunks) return;
+    double acc[KC * KC];
+#pragma unroll
+    for (int m = 0; m < KC * KC; ++m) acc[m] = 0.0;
+    const int64_t c1 = gchunk_ptr[ch + 1];
+    for (int64_t c = gchunk_ptr[ch] + lane; c < c1; c += 64)
+    {
+        const int32_t b = pair_blocks[c];
+        const int u = block_row[b];
+        const int v = bcol[b];
+        const int b0 = brow_ptr[u];
+        const int nbu = brow_ptr[u + 1] - b0;
+        const int64_t base = (int64_t)row_ptr[D * u] + (int64_t)D * (b - b0);
+        double Hb[D][D];
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j)
+                Hb[i][j] = H[base + (int64_t)i * D * nbu + j] + ((u == v && i == j) ? w : 0.0);
+        double Ru[D][KC], Rv[D][KC];
+        coarse_rows<D, KC>(dvec + (int64_t)u * D, Ru);
+        coarse_rows<D, KC>(dvec + (int64_t)v * D, Rv);
+        double T[D][KC];
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int c2 = 0; c2 < KC; ++c2)
+            {
+                double t = 0.0;
+#pragma unroll
+                for (int j = 0; j < D; ++j) t += Hb[i][j] * Rv[j][c2];
+                T[i][c2] = t;
+            }
+#pragma unroll
+        for (int ca = 0; ca < KC; ++ca)
+#pragma unroll
+            for (int c2 = 0; c2 < KC; ++c2)
+            {
+                double t = 0.0;
+#pragma unroll
+                for (int i = 0; i < D; ++i) t += Ru[i][ca] * T[i][c2];
+                acc[ca * KC + c2] += t;
+            }
+    }
+    wave_sum<KC * KC>(acc);
+#pragma unroll
+    for (int m = 0; m < KC * KC; ++m)
+        if (lane == m) gpart[ch * (KC * KC) + m] = acc[m];
+}
+
+// one thread per (pair, entry): chunk partials in order into A_c's (a, b) block
+template <int KC>
+__global__ __launch_bounds__(kBlock) void k_coarse_galerkin_pairs(int64_t n_pairs, const int32_t* __restrict__ pair_chunk,
+                                                                  const int32_t* __restrict__ pair_ab,
+                                                                  const double* __restrict__ gpart, int64_t nc,
+                                                                  double* __restrict__ Ac)
+{
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n_pairs * KC * KC) return;
+    const int64_t pr = id / (KC * KC);
+    const int m = (int)(id % (KC * KC));
+    double t = 0.0;
+    for (int32_t ch = pair_chunk[pr]; ch < pair_chunk[pr + 1]; ++ch) t += gpart[(int64_t)ch * (KC * KC) + m];
+    const int64_t a = pair_ab[2 * pr], bb = pair_ab[2 * pr + 1];
+    Ac[(a * KC + m / KC) + (bb * KC + m % KC) * nc] = t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_symmetrize_lower(int64_t n, double* __restrict__ A)
+{
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n * n) return;
+    const int64_t i = id % n, j = id / n;  // entry (i, j), column-major
+    if (i < j) A[i + j * n] = A[j + i * n];
+}
+
+// r_c = P^T r: one wave per chunk of an aggregate's handles, then one
+// thread per coarse unknown sums its aggregate's chunk partials in order
+template <int D, int KC>
+__global__ __launch_bounds__(kBlock) void k_coarse_restrict_chunk(int64_t n_chunks,
+                                                                  const int32_t* __restrict__ chunk_ptr,
+                                                                  const int32_t* __restrict__ agg_vert,
+                                                                  const double* __restrict__ dvec,
+                                                                  const double* __restrict__ r,
+                                                                  double* __restrict__ part, PcgScalars* sc)
+{
+    if (block_done(sc)) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t ch = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (ch >= n_chunks) return;
+    double acc[KC];
+#pragma unroll
+    for (int c = 0; c < KC; ++c) acc[c] = 0.0;
+    const int64_t m1 = chunk_ptr[ch + 1];
+    for (int64_t m = chunk_ptr[ch] + lane; m < m1; m += 64)
+    {
+        const int64_t v = agg_vert[m];
+        double R[D][KC];
+        coarse_rows<D, KC>(dvec + v * D, R);
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+        {
+            const double ri = r[v * D + i];
+#pragma unroll
+            for (int c = 0; c < KC; ++c) acc[c] += R[i][c] * ri;
+        }
+    }
+    wave_sum<KC>(acc);
+#pragma unroll
+    for (int c = 0; c < KC; ++c)
+        if (lane == c) part[ch * KC + c] = acc[c];
+}
+
+template <int KC>
+__global__ __launch_bounds__(kBlock) void k_coarse_restrict_sum(int64_t n_agg, const int32_t* __restrict__ agg_chunk,
+                                                                const double* __restrict__ part,
+                                                                double* __restrict__ rc, PcgScalars* sc)
+{
+    if (block_done(sc)) return;
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n_agg * KC) return;
+    const int64_t a = id / KC;
+    const int c = (int)(id % KC);
+    double t = 0.0;
+    for (int32_t ch = agg_chunk[a]; ch < agg_chunk[a + 1]; ++ch) t += part[(int64_t)ch * KC + c];
+    rc[id] = t;
+}
+
+// y_c = A_c^-1 r_c, one wave per row (row i of the symmetric inverse is its
+// contiguous column i); one partial of r_c . y_c per workgroup
+constexpr int kCoarseRows = kBlock / 64;
+
+__global__ __launch_bounds__(kBlock) void k_coarse_apply(int64_t nc, const double* __restrict__ Ainv,
+                                                         const double* __restrict__ rc, double* __restrict__ yc,
+                                                         double* __restrict__ cparts, PcgScalars* sc)
+{
+    if (block_done(sc)) return;
+    __shared__ double wpart[kCoarseRows];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t i = (int64_t)blockIdx.x * kCoarseRows + wv;
+    double prod = 0.0;
+    if (i < nc)
+    {
+        const double* __restrict__ col = Ainv + i * nc;
+        double acc[1] = { 0.0 };
+        int64_t j = lane;
+        for (; j + 3 * 64 < nc; j += 4 * 64)
+        {
+            const double a0 = col[j], a1 = col[j + 64], a2 = col[j + 128], a3 = col[j + 192];
+            const double r0 = rc[j], r1 = rc[j + 64], r2 = rc[j + 128], r3 = rc[j + 192];
+            acc[0] += a0 * r0;
+            acc[0] += a1 * r1;
+            acc[0] += a2 * r2;
+            acc[0] += a3 * r3;
+        }
+        for (; j < nc; j += 64) acc[0] += col[j] * rc[j];
+        wave_sum<1>(acc);
+        if (lane == 0) yc[i] = acc[0];
+        prod = rc[i] * acc[0];
+    }
+    if (lane == 0) wpart[wv] = prod;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < kCoarseRows; ++k) t += wpart[k];
+        cparts[blockIdx.x] = t;
+    }
+}
+
+// z_v += R_v y_c[a(v)]
+template <int D, int KC>
+__global__ __launch_bounds__(kBlock) void k_coarse_prolong(int64_t nh, const int32_t* __restrict__ agg_of,
+                                                           const double* __restrict__ dvec,
+                                                           const double* __restrict__ yc, double* __restrict__ z,
+                                                           PcgScalars* sc)
+{
+    if (block_done(sc)) return;
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nh) return;
+    const int64_t a = agg_of[v];
+    double R[D][KC];
+    coarse_rows<D, KC>(dvec + v * D, R);
+    double y[KC];
+#pragma unroll
+    for (int c = 0; c < KC; ++c) y[c] = yc[a * KC + c];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+    {
+        double t = z[v * D + i];
+#pragma unroll
+        for (int c = 0; c < KC; ++c) t += R[i][c] * y[c];
+        z[v * D + i] = t;
+    }
+}
+
+int coarse_apply_groups(int64_t nc) { return (int)std::max<int64_t>(1, (nc + kCoarseRows - 1) / kCoarseRows); }
+
+hipError_t launch_coarse_dvec(int d, int64_t nh, const double* x, const CoarseDev& c, double* dvec, hipStream_t s)
+{
+    if (nh == 0) return hipSuccess;
+    const unsigned g = (unsigned)((nh + kBlock - 1) / kBlock);
+    if (d == 1) k_coarse_dvec<1><<<g, kBlock, 0, s>>>(nh, x, c.agg_of, c.agg_ptr, c.agg_vert, dvec);
+    else if (d == 2) k_coarse_dvec<2><<<g, kBlock, 0, s>>>(nh, x, c.agg_of, c.agg_ptr, c.agg_vert, dvec);
+    else k_coarse_dvec<3><<<g, kBlock, 0, s>>>(nh, x, c.agg_of, c.agg_ptr, c.agg_vert, dvec);
+    return hipGetLastError();
+}
+
+template <int D, int KC>
+static void galerkin_t(int64_t n_pairs, int64_t n_gchunks, const int64_t* gchunk_ptr, const int32_t* pair_chunk,
+                       const int32_t* pair_blocks, const int32_t* pair_ab, const int32_t* block_row,
+                       const int32_t* brow_ptr, const int32_t* bcol, const int32_t* row_ptr, const double* H, double w,
+                       const CoarseDev& c, double* gpart, double* Ac, hipStream_t s)
+{
+    const unsigned g1 = (unsigned)((n_gchunks * 64 + kBlock - 1) / kBlock);
+    k_coarse_galerkin_chunk<D, KC><<<g1, kBlock, 0, s>>>(n_gchunks, gchunk_ptr, pair_blocks, block_row, brow_ptr,
+                                                          bcol, row_ptr, H, w, c.dvec, gpart);
+    const unsigned g2 = (unsigned)((n_pairs * KC * KC + kBlock - 1) / kBlock);
+    k_coarse_galerkin_pairs<KC><<<g2, kBlock, 0, s>>>(n_pairs, pair_chunk, pair_ab, gpart, c.nc, Ac);
+}
+
+hipError_t launch_coarse_galerkin(int d, int64_t n_pairs, int64_t n_gchunks, const int64_t* gchunk_ptr,
+                                  const int32_t* pair_chunk, const int32_t* pair_blocks, const int32_t* pair_ab,
+                                  const int32_t* block_row, const int32_t* brow_ptr, const int32_t* bcol,
+                                  const int32_t* row_ptr, const double* H, double w, const CoarseDev& c,
+                                  double* gpart, double* Ac, hipStream_t s)
+{
+    if (n_pairs == 0) return hipSuccess;
+#define TAD_GALERKIN(DD, KK) galerkin_t<DD, KK>(n_pairs, n_gchunks, gchunk_ptr, pair_chunk, pair_blocks, pair_ab, \
+                                                 block_row, brow_ptr, bcol, row_ptr, H, w, c, gpart, Ac, s)
+    if (d == 3 && c.kc == 6) TAD_GALERKIN(3, 6);
+    else if (d == 3 && c.kc == 3) TAD_GALERKIN(3, 3);
+    else if (d == 2 && c.kc == 3) TAD_GALERKIN(2, 3);
+    else if (d == 2 && c.kc == 2) TAD_GALERKIN(2, 2);
+    else if (d == 1 && c.kc == 1) TAD_GALERKIN(1, 1);
+    else return hipErrorInvalidValue;
+#undef TAD_GALERKIN
+    return hipGetLastError();
+}
+
+hipError_t launch_symmetrize_lower(int64_t n, double* A, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    k_symmetrize_lower<<<(unsigned)((n * n + kBlock - 1) / kBlock), kBlock, 0, s>>>(n, A);
+    return hipGetLastError();
+}
+
+template <int D, int KC>
+static void coarse_correction_t(int64_t nh, const CoarseDev& c, const double* r, double* z, int slot, PcgScalars* sc,
+                                hipStream_t s)
+{
+    const unsigned gch = (unsigned)((c.n_chunks * 64 + kBlock - 1) / kBlock);
+    k_coarse_restrict_chunk<D, KC><<<gch, kBlock, 0, s>>>(c.n_chunks, c.chunk_ptr, c.agg_vert, c.dvec, r,
+                                                           c.chunk_part, sc);
+    const unsigned gs = (unsigned)((c.n_agg * KC + kBlock - 1) / kBlock);
+    k_coarse_restrict_sum<KC><<<gs, kBlock, 0, s>>>(c.n_agg, c.agg_chunk, c.chunk_part, c.rc, sc);
+    k_coarse_apply<<<c.Gc, kBlock, 0, s>>>(c.nc, c.Ainv, c.rc, c.yc, c.cparts + (int64_t)slot * c.Gc, sc);
+    k_coarse_prolong<D, KC><<<(unsigned)((nh + kBlock - 1) / kBlock), kBlock, 0, s>>>(nh, c.agg_of, c.dvec, c.yc, z,
+                                                                                       sc);
+}
+
+hipError_t launch_coarse_correction(int d, int64_t nh, const CoarseDev& c, const double* r, double* z, int slot,
+                                    PcgScalars* sc, hipStream_t s, Timer* timer)
+{
+    if (c.kc == 0 || c.n_agg == 0) return hipSuccess;
+    if (timer) timer->begin("pcg_coarse", s);
+    if (d == 3 && c.kc == 6) coarse_correction_t<3, 6>(nh, c, r, z, slot, sc, s);
+    else if (d == 3 && c.kc == 3) coarse_correction_t<3, 3>(nh, c, r, z, slot, sc, s);
+    else if (d == 2 && c.kc == 3) coarse_correction_t<2, 3>(nh, c, r, z, slot, sc, s);
+    else if (d == 2 && c.kc == 2) coarse_correction_t<2, 2>(nh, c, r, z, slot, sc, s);
+    else if (d == 1 && c.kc == 1) coarse_correction_t<1, 1>(nh, c, r, z, slot, sc, s);
+    else return hipErrorInvalidValue;
+    if (timer) timer->end("pcg_coarse", s);
+    return hipGetLastError();
+}
+
 void pcg_grids(int d, int64_t nh, int* Gs, int* Gv)
 {
     const int lpb = d == 3 ? 16 : 8;
@@ -797,21 +1150,31 @@ hipError_t launch_pcg_setup(int d, int64_t nh, const int32_t* brow_ptr, const in
 }
 
 hipError_t launch_pcg_init(int d, int64_t nh, const double* b, const double* Dinv, double* x, double* r, double* z,
-                           double* parts, PcgScalars* sc, hipStream_t s)
+                           double* parts, PcgScalars* sc, const CoarseDev* co, hipStream_t s)
 {
     int Gs, Gv;
     pcg_grids(d, nh, &Gs, &Gv);
     if (d == 1) k_pcg_init<1><<<Gv, kBlock, 0, s>>>(nh, b, Dinv, x, r, z, parts, Gs, Gv, sc);
     else if (d == 2) k_pcg_init<2><<<Gv, kBlock, 0, s>>>(nh, b, Dinv, x, r, z, parts, Gs, Gv, sc);
     else k_pcg_init<3><<<Gv, kBlock, 0, s>>>(nh, b, Dinv, x, r, z, parts, Gs, Gv, sc);
+    // z_0 gets the coarse correction; its r.z partials join rho_0 in slot 1
+    if (co && co->kc > 0 && co->n_agg > 0)
+    {
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        return launch_coarse_correction(d, nh, *co, r, z, 1, sc, s, nullptr);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_pcg_iteration(int d, int64_t nh, const int32_t* brow_ptr, const int32_t* bcol,
                                 const int32_t* row_ptr, const double* H, double w, const double* Dinv, double* x,
                                 double* r, double* z, double* pbuf, double* q, double* parts, PcgScalars* sc, int k,
-                                bool with_update, hipStream_t s, Timer* timer)
+                                bool with_update, const CoarseDev* co, hipStream_t s, Timer* timer)
 {
+    const bool coarse = co && co->kc > 0 && co->n_agg > 0;
+    const double* cparts = coarse ? co->cparts : nullptr;
+    const int Gc = coarse ? co->Gc : 0;
     int Gs, Gv;
     pcg_grids(d, nh, &Gs, &Gv);
     double* p = pbuf;
@@ -823,10 +1186,21 @@ hipError_t launch_pcg_iteration(int d, int64_t nh, const int32_t* brow_ptr, cons
     if (timer) timer->end("spmv", s);
     if (!with_update) return hipGetLastError();
     if (timer) timer->begin("pcg_update", s);
-    if (d == 1) k_pcg_update<1><<<Gv, kUpdVerts, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, sc);
-    else if (d == 2) k_pcg_update<2><<<Gv, 2 * kUpdVerts, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, sc);
-    else k_pcg_update<3><<<Gv, 3 * kUpdVerts, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, sc);
+    if (d == 1)
+        k_pcg_update<1><<<Gv, kUpdVerts, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, cparts, Gc, sc);
+    else if (d == 2)
+        k_pcg_update<2><<<Gv, 2 * kUpdVerts, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, cparts, Gc, sc);
+    else
+        k_pcg_update<3><<<Gv, 3 * kUpdVerts, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, cparts, Gc, sc);
     if (timer) timer->end("pcg_update", s);
+    // z of iteration k + 1 gets the coarse correction; its r.z partials go to
+    // slot k & 1 next to the update's
+    if (coarse)
+    {
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        return launch_coarse_correction(d, nh, *co, r, z, k & 1, sc, s, timer);
+    }
     return hipGetLastError();
 }
 
