@@ -191,6 +191,65 @@ static bool test_segments_autotune_params()
     both.set_params(0, rows);
     auto fresh = build(2, 2.0);
     EXPECT(both.eval(x) == fresh.eval(x));
+    // launch shapes: store, change, restore; results keep their bits
+    const auto tuned = both.launch_shape(0);
+    const int tuned_asm = both.assembly_shape();
+    auto [f3, g3, H3] = both.eval_with_hessian_proj(x);
+    both.set_assembly_shape((tuned_asm + 1) % 4);
+    both.set_launch_shape(0, { 64, 8, 1 });
+    auto [f4, g4, H4] = both.eval_with_hessian_proj(x);
+    EXPECT(f4 == f3 && g4 == g3 && H4.values == H3.values);
+    both.set_launch_shape(0, tuned);
+    both.set_assembly_shape(tuned_asm);
+    EXPECT(both.launch_shape(0).block == tuned.block && both.assembly_shape() == tuned_asm);
+    return ok;
+}
+
+// Newton direction with and without the two-level preconditioner on a pinned
+// 24 x 24 grid of triangles: same direction, the coarse space reported.
+static bool test_two_level_solve()
+{
+    bool ok = true;
+    const int n = 24;
+    auto vid = [&](int i, int j) { return i * n + j; };
+    std::vector<std::array<int, 3>> F;
+    for (int i = 0; i + 1 < n; ++i)
+        for (int j = 0; j + 1 < n; ++j)
+        {
+            F.push_back({ vid(i, j), vid(i + 1, j), vid(i + 1, j + 1) });
+            F.push_back({ vid(i, j), vid(i + 1, j + 1), vid(i, j + 1) });
+        }
+    const double h = 1.0 / (n - 1);
+    auto func = tinyad_amd::scalar_function<2>(tinyad_amd::range(n * n));
+    func.add_elements<3>(tinyad_amd::range((int)F.size()), TAD_E_SYMDIR_TRI, [&](int f) { return F[f]; },
+                         [&](int) { return std::vector<double>{ 1 / h, 0, 0, 1 / h, 1e-3 }; });
+    const int pins[3] = { vid(0, 0), vid(n - 1, 0), vid(0, n - 1) };
+    func.add_elements<1>(tinyad_amd::range(3), TAD_E_PENALTY, [&](int k) { return std::array<int, 1>{ pins[k] }; },
+                         [&](int k) {
+                             const int v = pins[k];
+                             return std::vector<double>{ (v / n) * h, (v % n) * h, 1.0 };
+                         });
+    VectorXd x = func.x_from_data([&](int v) {
+        const double a = (v / n) * h, b = (v % n) * h;
+        return std::array<double, 2>{ 1.1 * a + 0.001 * std::sin(7.0 * v), 0.9 * b + 0.001 * std::cos(5.0 * v) };
+    });
+    auto [f, g, H] = func.eval_with_hessian_proj(x);
+    tinyad_amd::LinearSolver plain, two;
+    plain.opts.coarse_aggregate = 0;
+    plain.opts.tol = two.opts.tol = 1e-12;
+    two.opts.coarse_aggregate = 32;
+    VectorXd d0 = tinyad_amd::newton_direction(func, g, H, plain);
+    VectorXd d1 = tinyad_amd::newton_direction(func, g, H, two);
+    double dmax = 0, diff = 0;
+    for (size_t i = 0; i < d0.size(); ++i)
+    {
+        dmax = std::fmax(dmax, std::fabs(d0[i]));
+        diff = std::fmax(diff, std::fabs(d0[i] - d1[i]));
+    }
+    EXPECT(plain.last_stats.coarse_unknowns == 0 && two.last_stats.coarse_unknowns % 3 == 0 &&
+           two.last_stats.coarse_unknowns >= 30);
+    EXPECT(two.last_stats.converged && two.last_stats.iters < plain.last_stats.iters);
+    EXPECT(diff <= 1e-8 * dmax);
     return ok;
 }
 
@@ -204,6 +263,7 @@ int main()
         { "NewtonTest.Deterministic", test_newton_deterministic },
         { "ExceptionTest", test_errors_and_recovery },
         { "Batching.SegmentsAutotuneParams", test_segments_autotune_params },
+        { "Solver.TwoLevel", test_two_level_solve },
     };
     for (const Case& c : cases)
     {

@@ -357,6 +357,19 @@ def test_two_level_preconditioner(ctx):
     assert s4.coarse_unknowns * 2 == s1.coarse_unknowns  # 3 instead of 6 unknowns per aggregate
 
 
+def test_two_level_coarse_size_cap(ctx):
+    """A requested aggregate size that would give more than 8,192 coarse
+    unknowns is raised until the dense coarse inverse fits that bound."""
+    P = configs.twist_cube(m=30, seed=4)  # 29,791 handles
+    fg = tad.ScalarFunction.from_problem(P, ctx)
+    f, g, H = fg.eval_with_hessian_proj(P.x0)
+    d, st = fg.solve(g, tol=1e-10, coarse_aggregate=4)
+    res = np.linalg.norm(H @ d - g) / np.linalg.norm(g)
+    print(f"coarse size {st.coarse_unknowns}, {st.iters} iterations, residual {res:.2e}")
+    assert st.converged and res <= 2e-10
+    assert 6 * 1000 <= st.coarse_unknowns <= 8192 and st.coarse_unknowns % 6 == 0
+
+
 def test_two_level_preconditioner_2d_and_batches(ctx):
     """Triangle meshes (2 translations + 1 rotation per aggregate) and a batch
     of independent disks (aggregates never span two meshes).  Without pinned

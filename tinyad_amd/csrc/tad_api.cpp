@@ -240,7 +240,8 @@ struct tad_fn_s
     // their runs of fine CSR blocks, the coarse matrix and its inverse
     struct Coarse
     {
-        int target = 0;               // aggregate size the pattern was built for (0: none)
+        int requested = 0;            // aggregate size asked for when the pattern was built (0: none)
+        int target = 0;               // aggregate size the pattern was built with
         int64_t n_agg = 0, n_pairs = 0, n_chunks = 0, n_gchunks = 0;
         DevBuf<int32_t> agg_of, agg_ptr, agg_vert, pair_blocks, pair_ab, chunk_ptr, agg_chunk, pair_chunk;
         DevBuf<int64_t> gchunk_ptr;
@@ -252,7 +253,7 @@ struct tad_fn_s
             chunk_ptr.release(); agg_chunk.release(); pair_chunk.release(); gchunk_ptr.release();
             dvec.release(); Ac.release(); rc.release(); yc.release(); cparts.release(); chunk_part.release();
             gpart.release(); info.release();
-            target = 0;
+            requested = target = 0;
             n_agg = n_pairs = n_chunks = n_gchunks = 0;
         }
     } co;
@@ -673,9 +674,20 @@ tad_status setup_coarse(tad_fn fn, const double* H_dev, const tad_solve_opts& o,
     const bool rot = fn->x_hess_valid && d >= 2;
     const int kc = rot ? (d == 3 ? 6 : 3) : d;
     // keep the dense coarse matrix bounded: larger aggregates for large meshes
-    const int target = (int)std::max<int64_t>(o.coarse_aggregate, (nh * kc + kMaxCoarse - 1) / kMaxCoarse);
-    if (fn->co.target != target) TAD_TRY(build_coarse(fn, target));
+    // (grown further while breadth-first aggregates come out too small)
+    const int want = (int)std::max<int64_t>(o.coarse_aggregate, (nh * kc + kMaxCoarse - 1) / kMaxCoarse);
     auto& c = fn->co;
+    if (c.requested != want)
+    {
+        int target = want;
+        for (;;)
+        {
+            TAD_TRY(build_coarse(fn, target));
+            if (kc * c.n_agg <= kMaxCoarse || target >= nh) break;
+            target = (int)std::min<int64_t>(nh, (int64_t)target + target / 2 + 1);
+        }
+        c.requested = want;
+    }
     if (c.n_agg < 2) return TAD_OK;  // one aggregate: nothing a coarse space adds at this size
     const int64_t nc = kc * c.n_agg;
     if (nc > kMaxCoarse) return TAD_OK;
