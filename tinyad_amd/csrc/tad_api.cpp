@@ -123,7 +123,15 @@ struct KernelTimer : Timer
         if (!enabled) return;
         hipEvent_t e = get();
         (void)hipEventRecord(e, s);
-        open[name] = e;
+        auto it = open.find(name);
+        if (it != open.end())
+        {
+            // a begin left open by an error path: recycle its event
+            pool.push_back(it->second);
+            it->second = e;
+        }
+        else
+            open[name] = e;
     }
     void end(const char* name, hipStream_t s) override
     {
@@ -154,6 +162,7 @@ struct KernelTimer : Timer
     ~KernelTimer() override
     {
         collect();
+        for (auto& kv : open) (void)hipEventDestroy(kv.second);
         for (auto e : pool) (void)hipEventDestroy(e);
     }
 };
