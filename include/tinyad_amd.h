@@ -244,6 +244,16 @@ tad_status tad_solve_default_opts(tad_solve_opts* opts);
 tad_status tad_fn_solve(tad_fn fn, const double* H_values, const double* rhs, double* sol, tad_mem mem,
                         const tad_solve_opts* opts, tad_solve_stats* stats);
 
+/* The solve restricted to a subspace whose basis B selects variables
+ * (newton_direction_reduced_basis, NewtonDirection.hh:76-105, for B made of
+ * columns of the identity, the pinning case): sol = B (B^T H B + w I)^-1 B^T
+ * rhs, i.e. CG on the free variables, and sol = 0 on the fixed ones.
+ * free_vars: n_vars bytes, nonzero = free, in the memory space `mem` like
+ * the vectors.  Same preconditioner as tad_fn_solve, restricted to the free
+ * variables. */
+tad_status tad_fn_solve_reduced(tad_fn fn, const double* H_values, const double* rhs, double* sol, tad_mem mem,
+                                const unsigned char* free_vars, const tad_solve_opts* opts, tad_solve_stats* stats);
+
 typedef struct tad_newton_opts
 {
     double projection_eps;  /* eval_with_hessian_proj eps (default 1e-9); NaN: unprojected */

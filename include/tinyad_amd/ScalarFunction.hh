@@ -369,6 +369,19 @@ struct LinearSolver
     tad_solve_stats last_stats {};
 };
 
+// identity(n) (Helpers.hh:82-89): the n x n sparse identity.
+inline SparseMatrix identity(int64_t n)
+{
+    SparseMatrix I;
+    I.n_rows = I.n_cols = n;
+    I.outer.resize((size_t)n + 1);
+    I.inner.resize((size_t)n);
+    I.values.assign((size_t)n, 1.0);
+    for (int64_t i = 0; i <= n; ++i) I.outer[i] = (int32_t)i;
+    for (int64_t i = 0; i < n; ++i) I.inner[i] = (int32_t)i;
+    return I;
+}
+
 // newton_direction(g, H_proj, solver, w) (NewtonDirection.hh:25-48):
 // solves (w I + H) d = -g; throws if the solve fails.
 template <int dd>
@@ -384,6 +397,51 @@ VectorXd newton_direction(const ScalarFunction<dd>& func, const VectorXd& g, con
                        &solver.last_stats));
     if (!solver.last_stats.converged) throw std::runtime_error("[tinyad_amd] Linear solve failed.");
     return dvec;
+}
+
+// newton_direction_reduced_basis(g, H_proj, B, solver, w) (NewtonDirection.hh:76-105):
+// d = B (B^T H B + w I)^-1 (-B^T g) for an n x m basis B that selects
+// variables (every column one entry 1, distinct rows: the pinning case);
+// other bases throw std::runtime_error.  d is 0 on the variables B omits.
+template <int dd>
+VectorXd newton_direction_reduced_basis(const ScalarFunction<dd>& func, const VectorXd& g,
+                                        const SparseMatrix& H_proj, const SparseMatrix& B, LinearSolver& solver,
+                                        double w_identity = 0.0)
+{
+    const int64_t n = (int64_t)g.size();
+    if (B.rows() != n || H_proj.rows() != n || H_proj.cols() != n)
+        throw std::runtime_error("[tinyad_amd] newton_direction_reduced_basis: size mismatch");
+    std::vector<unsigned char> free_vars((size_t)n, 0);
+    std::vector<int> per_col((size_t)B.cols(), 0);
+    for (int64_t r = 0; r < B.rows(); ++r)
+        for (int32_t k = B.outer[r]; k < B.outer[r + 1]; ++k)
+        {
+            if (B.values[k] == 0.0) continue;
+            if (B.values[k] != 1.0 || free_vars[r])
+                throw std::runtime_error("[tinyad_amd] newton_direction_reduced_basis: B must select variables");
+            free_vars[r] = 1;
+            per_col[B.inner[k]]++;
+        }
+    for (int c : per_col)
+        if (c != 1) throw std::runtime_error("[tinyad_amd] newton_direction_reduced_basis: B must select variables");
+    VectorXd minus_g(g.size());
+    for (size_t i = 0; i < g.size(); ++i) minus_g[i] = -g[i];
+    VectorXd dvec(g.size());
+    tad_solve_opts o = solver.opts;
+    o.w_identity = w_identity;
+    check(tad_fn_solve_reduced(func.handle(), H_proj.values.data(), minus_g.data(), dvec.data(), TAD_MEM_HOST,
+                               free_vars.data(), &o, &solver.last_stats));
+    if (!solver.last_stats.converged) throw std::runtime_error("[tinyad_amd] Linear solve failed.");
+    return dvec;
+}
+
+// newton_direction_reduced_basis(g, H_proj, B, w) with a fresh solver (NewtonDirection.hh:114-123).
+template <int dd>
+VectorXd newton_direction_reduced_basis(const ScalarFunction<dd>& func, const VectorXd& g,
+                                        const SparseMatrix& H_proj, const SparseMatrix& B, double w_identity = 0.0)
+{
+    LinearSolver solver;
+    return newton_direction_reduced_basis(func, g, H_proj, B, solver, w_identity);
 }
 
 // line_search(x0, d, f, g, eval, s_max, shrink, max_iters, armijo) (LineSearch.hh:26-65),

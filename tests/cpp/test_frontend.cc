@@ -253,6 +253,41 @@ static bool test_two_level_solve()
     return ok;
 }
 
+// SwitchSolverTest (SwitchSolverTest.cc:16-47): x^2 + y^2 at (1, 1), the Newton
+// direction and the reduced-basis one with B = I are (-1, -1); a basis that
+// keeps only x gives (-1, 0); a basis that does not select variables throws.
+static bool test_reduced_basis()
+{
+    bool ok = true;
+    auto func = tinyad_amd::scalar_function<2>(tinyad_amd::range(1));
+    func.add_elements<1>(tinyad_amd::range(1), TAD_E_QUADRATIC, [](int) { return std::array<int, 1>{ 0 }; },
+                         [](int) { return std::vector<double>{ 2, 0, 0, 2, 0, 0, 0 }; });
+    VectorXd x = { 1.0, 1.0 };
+    auto [f, g, H] = func.eval_with_hessian_proj(x);
+    EXPECT(g[0] == 2.0 && g[1] == 2.0);
+    tinyad_amd::LinearSolver solver;
+    VectorXd d = tinyad_amd::newton_direction(func, g, H, solver);
+    EXPECT(std::fabs(d[0] + 1.0) <= 1e-12 && std::fabs(d[1] + 1.0) <= 1e-12);
+    d = tinyad_amd::newton_direction_reduced_basis(func, g, H, tinyad_amd::identity(2), solver);
+    EXPECT(std::fabs(d[0] + 1.0) <= 1e-12 && std::fabs(d[1] + 1.0) <= 1e-12);
+    tinyad_amd::SparseMatrix Bx;
+    Bx.n_rows = 2;
+    Bx.n_cols = 1;
+    Bx.outer = { 0, 1, 1 };
+    Bx.inner = { 0 };
+    Bx.values = { 1.0 };
+    d = tinyad_amd::newton_direction_reduced_basis(func, g, H, Bx, solver);
+    EXPECT(std::fabs(d[0] + 1.0) <= 1e-12 && d[1] == 0.0);
+    d = tinyad_amd::newton_direction_reduced_basis(func, g, H, Bx);
+    EXPECT(std::fabs(d[0] + 1.0) <= 1e-12 && d[1] == 0.0);
+    Bx.values = { 2.0 };
+    bool threw = false;
+    try { tinyad_amd::newton_direction_reduced_basis(func, g, H, Bx, solver); }
+    catch (const std::runtime_error&) { threw = true; }
+    EXPECT(threw);
+    return ok;
+}
+
 int main()
 {
     struct Case { const char* name; bool (*fn)(); };
@@ -264,6 +299,7 @@ int main()
         { "ExceptionTest", test_errors_and_recovery },
         { "Batching.SegmentsAutotuneParams", test_segments_autotune_params },
         { "Solver.TwoLevel", test_two_level_solve },
+        { "SwitchSolverTest.ReducedBasis", test_reduced_basis },
     };
     for (const Case& c : cases)
     {

@@ -357,6 +357,69 @@ def test_two_level_preconditioner(ctx):
     assert s4.coarse_unknowns * 2 == s1.coarse_unknowns  # 3 instead of 6 unknowns per aggregate
 
 
+def pinned_cube_mask(P):
+    """Free-variable mask of a cube with its bottom and top layers held fixed."""
+    rest = P.info["rest"].reshape(-1, 3)
+    h = P.info["h"]
+    fixed = (rest[:, 2] < 0.5 * h) | (rest[:, 2] > 1.0 - 0.5 * h)
+    return np.repeat(~fixed, 3).astype(np.uint8)
+
+
+def test_reduced_basis_solve(ctx):
+    """newton_direction_reduced_basis with a selection basis B (NewtonDirection.hh:76-105):
+    sol = B (B^T H B + w I)^-1 B^T rhs against a direct solve of the free block,
+    with and without the coarse correction, regularized, host and device."""
+    import scipy.sparse.linalg as spla
+    import torch
+    P = configs.twist_cube(m=12, seed=6)
+    free = pinned_cube_mask(P)
+    fi = np.nonzero(free)[0]
+    fg = tad.ScalarFunction.from_problem(P, ctx)
+    f, g, H = fg.eval_with_hessian_proj(P.x0)
+    Hff = H.tocsr()[fi][:, fi]
+    for w in [0.0, 0.3]:
+        ref = np.zeros_like(g)
+        ref[fi] = spla.spsolve((Hff + w * sp.identity(fi.size)).tocsc(), -g[fi])
+        for coarse in [0, 64]:
+            d, st = fg.solve(-g, free=free, tol=1e-12, w_identity=w, coarse_aggregate=coarse)
+            print(f"w={w} coarse={coarse}: {st.iters} iterations, coarse size {st.coarse_unknowns}")
+            assert st.converged and nrel(d, ref) <= 1e-8
+            assert np.all(d[free == 0] == 0.0)
+            assert (st.coarse_unknowns > 0) == (coarse > 0)
+    dd, st = fg.solve(torch.tensor(-g, device="cuda:0"), free=torch.tensor(free, device="cuda:0"), tol=1e-12)
+    assert st.converged and nrel(dd.cpu().numpy(), fg.solve(-g, free=free, tol=1e-12)[0]) <= 1e-12
+    with pytest.raises(tad.TadError):
+        fg.solve(-g, free=free[:-1])
+
+
+def test_reduced_basis_newton_keeps_pins(ctx):
+    """Projected Newton with hard pins (reduced-basis directions and the
+    library's line search): pinned coordinates never move, the energy
+    decreases monotonically, and the free gradient drops to the solver level."""
+    import torch
+    P = configs.twist_cube(m=6, seed=8)
+    free = pinned_cube_mask(P)
+    fg = tad.ScalarFunction(3, P.n_handles, ctx)
+    T = P.terms[0]
+    fg.add_elements(4, tad.SYMDIR_TET, T.conn, T.params)
+    x = torch.tensor(P.x0, device="cuda:0")
+    x0 = x.clone()
+    fd = torch.tensor(free, device="cuda:0")
+    fprev = float("inf")
+    for it in range(12):
+        f, g, H = fg.eval_with_hessian_proj(x)
+        d, st = fg.solve(-g, H=H, free=fd, tol=1e-12)
+        assert st.converged
+        x, f_new, step, trials = fg.line_search(x, d, f, g)
+        assert f_new <= f <= fprev
+        fprev = f_new
+    f, g, H = fg.eval_with_hessian_proj(x)
+    gf = g.cpu().numpy()[free == 1]
+    print(f"reduced Newton: f {f:.12g}, |g_free| {np.abs(gf).max():.2e}")
+    assert torch.equal(x[fd == 0], x0[fd == 0])
+    assert np.abs(gf).max() <= 1e-8 * max(1.0, abs(f))
+
+
 def test_two_level_coarse_size_cap(ctx):
     """A requested aggregate size that would give more than 8,192 coarse
     unknowns is raised until the dense coarse inverse fits that bound."""

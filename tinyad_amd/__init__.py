@@ -357,8 +357,12 @@ class ScalarFunction:
         check(lib().tad_fn_set_assembly_shape(self._h, int(shape)))
 
     # -- solver ------------------------------------------------------------
-    def solve(self, rhs, H=None, **opts):
-        """Solves (w I + H) sol = rhs with the library's PCG; H=None uses the last evaluated Hessian."""
+    def solve(self, rhs, H=None, free=None, **opts):
+        """Solves (w I + H) sol = rhs with the library's PCG; H=None uses the last evaluated Hessian.
+
+        free: optional per-variable mask (nonzero = free) of a reduced-basis
+        solve (newton_direction_reduced_basis with a selection basis): CG on
+        the free variables, sol = 0 on the fixed ones."""
         o = solve_default_opts(**opts)
         st = SolveStats()
         if _is_torch(rhs):
@@ -366,8 +370,15 @@ class ScalarFunction:
             _order_after_torch(self.ctx, rhs.device)
             sol = torch.empty_like(rhs)
             hp = H.data_ptr() if H is not None else None
-            check(lib().tad_fn_solve(self._h, hp, rhs.data_ptr(), sol.data_ptr(), _lib.MEM_DEVICE,
-                                     ctypes.byref(o), ctypes.byref(st)))
+            if free is None:
+                check(lib().tad_fn_solve(self._h, hp, rhs.data_ptr(), sol.data_ptr(), _lib.MEM_DEVICE,
+                                         ctypes.byref(o), ctypes.byref(st)))
+            else:
+                fm = torch.as_tensor(free, device=rhs.device).to(torch.uint8).contiguous()
+                if fm.numel() != rhs.numel():
+                    raise TadError(1, "free mask must have one entry per variable")
+                check(lib().tad_fn_solve_reduced(self._h, hp, rhs.data_ptr(), sol.data_ptr(), _lib.MEM_DEVICE,
+                                                 fm.data_ptr(), ctypes.byref(o), ctypes.byref(st)))
             return sol, st
         rhs = np.ascontiguousarray(rhs, dtype=np.float64)
         sol = np.zeros_like(rhs)
@@ -375,8 +386,16 @@ class ScalarFunction:
         if H is not None:
             hv = np.ascontiguousarray(H.data if hasattr(H, "data") and not isinstance(H, np.ndarray) else H,
                                       dtype=np.float64)
-        check(lib().tad_fn_solve(self._h, hv.ctypes.data if hv is not None else None, rhs.ctypes.data,
-                                 sol.ctypes.data, _lib.MEM_HOST, ctypes.byref(o), ctypes.byref(st)))
+        hptr = hv.ctypes.data if hv is not None else None
+        if free is None:
+            check(lib().tad_fn_solve(self._h, hptr, rhs.ctypes.data, sol.ctypes.data, _lib.MEM_HOST,
+                                     ctypes.byref(o), ctypes.byref(st)))
+        else:
+            fm = np.ascontiguousarray(np.asarray(free) != 0, dtype=np.uint8)
+            if fm.size != rhs.size:
+                raise TadError(1, "free mask must have one entry per variable")
+            check(lib().tad_fn_solve_reduced(self._h, hptr, rhs.ctypes.data, sol.ctypes.data, _lib.MEM_HOST,
+                                             fm.ctypes.data, ctypes.byref(o), ctypes.byref(st)))
         return sol, st
 
     def newton_step(self, x, opts: NewtonOpts | None = None) -> dict:

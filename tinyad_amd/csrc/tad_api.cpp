@@ -268,6 +268,7 @@ struct tad_fn_s
     } co;
     // positions of the last second-order evaluation (rotation modes of the coarse space)
     DevBuf<double> x_hess;
+    DevBuf<unsigned char> free_buf;  // staged free-variable mask of a reduced-basis solve
     bool x_hess_valid = false;
 
     hipStream_t s() const { return ctx->stream; }
@@ -674,7 +675,8 @@ constexpr int64_t kMaxCoarse = 8192;
 // triangle, rocBLAS atomics off so the result is deterministic), symmetrized.
 // *use = false when the coarse space is off or A_c is not numerically
 // positive definite (block-Jacobi alone then).
-tad_status setup_coarse(tad_fn fn, const double* H_dev, const tad_solve_opts& o, CoarseDev* cd, bool* use)
+tad_status setup_coarse(tad_fn fn, const double* H_dev, const tad_solve_opts& o, const unsigned char* free_dev,
+                        CoarseDev* cd, bool* use)
 {
     *use = false;
     const int d = fn->d;
@@ -727,6 +729,7 @@ tad_status setup_coarse(tad_fn fn, const double* H_dev, const tad_solve_opts& o,
     cd->yc = c.yc.p;
     cd->cparts = c.cparts.p;
     cd->Gc = Gc;
+    cd->free_vars = free_dev;
     tm.begin("coarse_setup", s);
     tm.begin("coarse_galerkin", s);
     if (rot) TAD_HIP(launch_coarse_dvec(d, nh, fn->x_hess.p, *cd, c.dvec.p, s));
@@ -735,6 +738,10 @@ tad_status setup_coarse(tad_fn fn, const double* H_dev, const tad_solve_opts& o,
     TAD_HIP(launch_coarse_galerkin(d, c.n_pairs, c.n_gchunks, c.gchunk_ptr.p, c.pair_chunk.p, c.pair_blocks.p,
                                    c.pair_ab.p, fn->block_row.p, fn->brow_ptr.p, fn->bcol.p, fn->row_ptr.p, H_dev,
                                    o.w_identity, *cd, c.gpart.p, c.Ac.p, s));
+    // reduced basis: coarse directions without free variables leave A_c only
+    // semidefinite; a relative diagonal shift makes it definite (those
+    // directions prolong to zero)
+    if (free_dev) TAD_HIP(launch_shift_diagonal(nc, c.Ac.p, 1e-10, s));
     tm.end("coarse_galerkin", s);
     tad_ctx ctx = fn->ctx;
     if (!ctx->blas)
@@ -888,7 +895,7 @@ tad_status dot_dev(tad_fn fn, const double* a, const double* b, double* out)
 }
 
 tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, double* sol_dev,
-                      const tad_solve_opts& o, tad_solve_stats* st)
+                      const tad_solve_opts& o, tad_solve_stats* st, const unsigned char* free_dev = nullptr)
 {
     const int d = fn->d;
     const int64_t nh = fn->n_handles, n = fn->n_vars;
@@ -903,11 +910,12 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
     TAD_HIP(fn->pcg_parts.ensure((size_t)pcg_parts_size(d, nh)));
     if (o.max_iters <= 0) return fail(TAD_ERR_INVALID, "max_iters must be positive");
     tm.begin("pcg_setup", s);
-    TAD_HIP(launch_pcg_setup(d, nh, fn->brow_ptr.p, fn->diag_block.p, fn->row_ptr.p, H_dev, o.w_identity, fn->Dinv.p, s));
+    TAD_HIP(launch_pcg_setup(d, nh, fn->brow_ptr.p, fn->diag_block.p, fn->row_ptr.p, H_dev, o.w_identity, free_dev,
+                             fn->Dinv.p, s));
     tm.end("pcg_setup", s);
     CoarseDev cd;
     bool use_coarse = false;
-    TAD_TRY(setup_coarse(fn, H_dev, o, &cd, &use_coarse));
+    TAD_TRY(setup_coarse(fn, H_dev, o, free_dev, &cd, &use_coarse));
     const CoarseDev* co = use_coarse ? &cd : nullptr;
     PcgScalars hs;
     std::memset(&hs, 0, sizeof(hs));
@@ -917,7 +925,8 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
     std::memcpy(pin, &hs, sizeof(hs));
     TAD_HIP(hipMemcpyAsync(fn->scal.p, pin, sizeof(hs), hipMemcpyHostToDevice, s));
     tm.begin("pcg_init", s);
-    TAD_HIP(launch_pcg_init(d, nh, rhs_dev, fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->pcg_parts.p, fn->scal.p, co, s));
+    TAD_HIP(launch_pcg_init(d, nh, rhs_dev, free_dev, fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->pcg_parts.p,
+                            fn->scal.p, co, s));
     tm.end("pcg_init", s);
     const int every = std::max(1, o.check_every);
     Timer* timer = fn->ctx->timer.enabled ? &fn->ctx->timer : nullptr;
@@ -931,7 +940,7 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
         for (int c = 0; c < every && k <= o.max_iters; ++c, ++k)
             TAD_HIP(launch_pcg_iteration(d, nh, fn->brow_ptr.p, fn->bcol.p, fn->row_ptr.p, H_dev, o.w_identity,
                                          fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->p.p, fn->q.p, fn->pcg_parts.p,
-                                         fn->scal.p, k, k < o.max_iters, co, s, timer));
+                                         fn->scal.p, k, k < o.max_iters, free_dev, co, s, timer));
         TAD_HIP(hipMemcpyAsync(pin, fn->scal.p, sizeof(hs), hipMemcpyDeviceToHost, s));
         TAD_HIP(hipStreamSynchronize(s));
         if (pin->done) break;
@@ -1121,7 +1130,7 @@ tad_status tad_fn_destroy(tad_fn fn)
     fn->Dinv.release(); fn->r.release(); fn->z.release(); fn->p.release(); fn->q.release(); fn->sol.release();
     fn->scal.release(); fn->pcg_parts.release(); fn->seg_ranges.release(); fn->seg_f.release();
     fn->adj_g.release(); fn->adj_rest.release(); fn->adj_lam.release();
-    fn->co.release(); fn->x_hess.release();
+    fn->co.release(); fn->x_hess.release(); fn->free_buf.release();
     tad_ctx ctx = fn->ctx;
     delete fn;
     ctx_release(ctx);
@@ -1313,8 +1322,8 @@ tad_status tad_solve_default_opts(tad_solve_opts* o)
     return TAD_OK;
 }
 
-tad_status tad_fn_solve(tad_fn fn, const double* H_values, const double* rhs, double* sol, tad_mem mem,
-                        const tad_solve_opts* opts, tad_solve_stats* stats)
+static tad_status solve_entry(tad_fn fn, const double* H_values, const double* rhs, double* sol, tad_mem mem,
+                              const unsigned char* free_vars, const tad_solve_opts* opts, tad_solve_stats* stats)
 {
     TAD_TRY(check_fn(fn));
     if (!rhs || !sol) return fail(TAD_ERR_INVALID, "null argument");
@@ -1339,6 +1348,13 @@ tad_status tad_fn_solve(tad_fn fn, const double* H_values, const double* rhs, do
         fn->h_valid = true;
         Hd = fn->H_buf.p;
     }
+    const unsigned char* fd = free_vars;
+    if (free_vars && mem == TAD_MEM_HOST)
+    {
+        TAD_HIP(fn->free_buf.ensure(n));
+        TAD_HIP(hipMemcpyAsync(fn->free_buf.p, free_vars, n, hipMemcpyHostToDevice, s));
+        fd = fn->free_buf.p;
+    }
     TAD_HIP(fn->sol.ensure(n));
     const double* bd = rhs;
     double* xd = sol;
@@ -1349,13 +1365,26 @@ tad_status tad_fn_solve(tad_fn fn, const double* H_values, const double* rhs, do
         bd = fn->x_tmp.p;
         xd = fn->sol.p;
     }
-    tad_status st = solve_core(fn, Hd, bd, xd, o, stats);
+    tad_status st = solve_core(fn, Hd, bd, xd, o, stats, fd);
     if (mem == TAD_MEM_HOST)
     {
         TAD_HIP(hipMemcpyAsync(sol, xd, n * sizeof(double), hipMemcpyDeviceToHost, s));
         TAD_HIP(hipStreamSynchronize(s));
     }
     return st;
+}
+
+tad_status tad_fn_solve(tad_fn fn, const double* H_values, const double* rhs, double* sol, tad_mem mem,
+                        const tad_solve_opts* opts, tad_solve_stats* stats)
+{
+    return solve_entry(fn, H_values, rhs, sol, mem, nullptr, opts, stats);
+}
+
+tad_status tad_fn_solve_reduced(tad_fn fn, const double* H_values, const double* rhs, double* sol, tad_mem mem,
+                                const unsigned char* free_vars, const tad_solve_opts* opts, tad_solve_stats* stats)
+{
+    if (!free_vars) return fail(TAD_ERR_INVALID, "null free-variable mask");
+    return solve_entry(fn, H_values, rhs, sol, mem, free_vars, opts, stats);
 }
 
 tad_status tad_newton_default_opts(tad_newton_opts* o)

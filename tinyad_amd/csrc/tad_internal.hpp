@@ -136,6 +136,7 @@ struct CoarseDev
     double* yc = nullptr;            // [nc] coarse correction
     double* cparts = nullptr;        // [2 * Gc] partials of rc . yc, slot k & 1
     int Gc = 0;                      // workgroups of the coarse product
+    const unsigned char* free_vars = nullptr;  // [n_vars] reduced basis (null: every variable free)
 };
 
 constexpr int kCoarseChunk = 256;  // handles per restriction chunk, fine blocks per Galerkin chunk
@@ -157,6 +158,9 @@ hipError_t launch_coarse_galerkin(int d, int64_t n_pairs, int64_t n_gchunks, con
                                   const int32_t* row_ptr, const double* H, double w, const CoarseDev& c,
                                   double* gpart, double* Ac, hipStream_t s);
 
+// A_jj <- A_jj (1 + rel), 1 where A_jj = 0 (column-major n x n).
+hipError_t launch_shift_diagonal(int64_t n, double* A, double rel, hipStream_t s);
+
 // Copies the lower triangle of the column-major n x n matrix A to the upper.
 hipError_t launch_symmetrize_lower(int64_t n, double* A, hipStream_t s);
 
@@ -165,11 +169,16 @@ hipError_t launch_symmetrize_lower(int64_t n, double* A, hipStream_t s);
 // solve has terminated.
 hipError_t launch_coarse_correction(int d, int64_t nh, const CoarseDev& c, const double* r, double* z, int slot,
                                     PcgScalars* sc, hipStream_t s, Timer* timer);
+// free_vars (optional, n bytes, 0 = fixed): CG on the free variables only, the
+// reduced basis of newton_direction_reduced_basis (NewtonDirection.hh:76-105)
+// for a basis B that selects variables; fixed variables of the solution are 0.
 hipError_t launch_pcg_setup(int d, int64_t n_handles, const int32_t* brow_ptr, const int32_t* diag_block,
-                            const int32_t* row_ptr, const double* H, double w, double* Dinv, hipStream_t s);
+                            const int32_t* row_ptr, const double* H, double w, const unsigned char* free_vars,
+                            double* Dinv, hipStream_t s);
 
-hipError_t launch_pcg_init(int d, int64_t n_handles, const double* b, const double* Dinv, double* x, double* r,
-                           double* z, double* partials, PcgScalars* sc, const CoarseDev* coarse, hipStream_t s);
+hipError_t launch_pcg_init(int d, int64_t n_handles, const double* b, const unsigned char* free_vars,
+                           const double* Dinv, double* x, double* r, double* z, double* partials, PcgScalars* sc,
+                           const CoarseDev* coarse, hipStream_t s);
 
 // One CG iteration k: SpMV on z with the termination test, then (with_update)
 // the scalar recurrences and the p / q / x / r / z update, and the coarse
@@ -178,7 +187,8 @@ hipError_t launch_pcg_init(int d, int64_t n_handles, const double* b, const doub
 hipError_t launch_pcg_iteration(int d, int64_t n_handles, const int32_t* brow_ptr, const int32_t* bcol,
                                 const int32_t* row_ptr, const double* H, double w, const double* Dinv, double* x,
                                 double* r, double* z, double* pbuf, double* q, double* partials, PcgScalars* sc, int k,
-                                bool with_update, const CoarseDev* coarse, hipStream_t s, Timer* timer);
+                                bool with_update, const unsigned char* free_vars, const CoarseDev* coarse,
+                                hipStream_t s, Timer* timer);
 
 // Size (doubles) of the partial-sum buffer the PCG kernels need.
 int64_t pcg_parts_size(int d, int64_t n_handles);

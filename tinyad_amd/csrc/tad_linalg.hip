@@ -487,6 +487,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_setup(int64_t nh, const int32_t*
                                                       const int32_t* __restrict__ diag_block,
                                                       const int32_t* __restrict__ row_ptr,
                                                       const double* __restrict__ H, double w,
+                                                      const unsigned char* __restrict__ free_vars,
                                                       double* __restrict__ Dinv)
 {
     const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -494,20 +495,32 @@ __global__ __launch_bounds__(kBlock) void k_pcg_setup(int64_t nh, const int32_t*
     const int nbu = brow_ptr[u + 1] - brow_ptr[u];
     const int k = diag_block[u];
     const int64_t base = row_ptr[D * u];
+    // reduced basis: a fixed component takes a unit row / column here and a
+    // zero row / column in the inverse (the free sub-block's inverse)
+    bool fr[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) fr[i] = free_vars ? free_vars[u * D + i] != 0 : true;
     double B[D * D], o[D * D];
     if (k < 0)
     {
         // handle used by no element: its rows are empty, (w I + H) is w I there
 #pragma unroll
-        for (int m = 0; m < D * D; ++m) Dinv[u * D * D + m] = (m % (D + 1) == 0) ? (w > 0.0 ? 1.0 / w : 1.0) : 0.0;
+        for (int m = 0; m < D * D; ++m)
+            Dinv[u * D * D + m] = (m % (D + 1) == 0 && fr[m / D]) ? (w > 0.0 ? 1.0 / w : 1.0) : 0.0;
         return;
     }
 #pragma unroll
     for (int i = 0; i < D; ++i)
 #pragma unroll
         for (int j = 0; j < D; ++j)
-            B[i * D + j] = H[base + (int64_t)i * D * nbu + D * k + j] + (i == j ? w : 0.0);
+            B[i * D + j] = (fr[i] && fr[j]) ? H[base + (int64_t)i * D * nbu + D * k + j] + (i == j ? w : 0.0)
+                                            : (i == j ? 1.0 : 0.0);
     invert_block<D>(B, o);
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j)
+            if (!(fr[i] && fr[j])) o[i * D + j] = 0.0;
 #pragma unroll
     for (int m = 0; m < D * D; ++m) Dinv[u * D * D + m] = o[m];
 }
@@ -515,6 +528,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_setup(int64_t nh, const int32_t*
 // x = 0, r = b, z = M^-1 r; partials rho_0 = r.z (slot 1) and ||b||^2 (rr0)
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t nh, const double* __restrict__ b,
+                                                     const unsigned char* __restrict__ free_vars,
                                                      const double* __restrict__ Dinv, double* __restrict__ x,
                                                      double* __restrict__ r, double* __restrict__ z,
                                                      double* __restrict__ parts, int Gs, int Gv, PcgScalars* sc)
@@ -527,7 +541,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t nh, const double* _
 #pragma unroll
         for (int i = 0; i < D; ++i)
         {
-            rv[i] = b[u * D + i];
+            rv[i] = (free_vars && !free_vars[u * D + i]) ? 0.0 : b[u * D + i];
             x[u * D + i] = 0.0;
             r[u * D + i] = rv[i];
         }
@@ -660,6 +674,7 @@ __global__ __launch_bounds__(kUpdVerts * D) void k_pcg_update(int64_t nh, const 
                                                               const double* __restrict__ sv,
                                                               double* __restrict__ parts, int Gs, int Gv, int k,
                                                               const double* __restrict__ cparts, int Gc,
+                                                              const unsigned char* __restrict__ free_vars,
                                                               PcgScalars* sc)
 {
     constexpr int NT = kUpdVerts * D;
@@ -744,6 +759,7 @@ __global__ __launch_bounds__(kUpdVerts * D) void k_pcg_update(int64_t nh, const 
             q[iv] = qi;
             x[iv] = xc + alpha * pi;
             rv = rc - alpha * qi;
+            if (free_vars && !free_vars[iv]) rv = 0.0;  // reduced basis: no residual on fixed variables
             r[iv] = rv;
         }
         __syncthreads();  // the previous chunk's LDS reads are complete
@@ -842,6 +858,7 @@ __global__ __launch_bounds__(kBlock) void k_coarse_galerkin_chunk(int64_t n_gchu
                                                                   const int32_t* __restrict__ row_ptr,
                                                                   const double* __restrict__ H, double w,
                                                                   const double* __restrict__ dvec,
+                                                                  const unsigned char* __restrict__ free_vars,
                                                                   double* __restrict__ gpart)
 {
     const int lane = threadIdx.x & 63;
@@ -868,6 +885,17 @@ __global__ __launch_bounds__(kBlock) void k_coarse_galerkin_chunk(int64_t n_gchu
         double Ru[D][KC], Rv[D][KC];
         coarse_rows<D, KC>(dvec + (int64_t)u * D, Ru);
         coarse_rows<D, KC>(dvec + (int64_t)v * D, Rv);
+        if (free_vars)  // reduced basis: P restricted to the free variables
+        {
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+#pragma unroll
+                for (int c2 = 0; c2 < KC; ++c2)
+                {
+                    if (!free_vars[(int64_t)u * D + i]) Ru[i][c2] = 0.0;
+                    if (!free_vars[(int64_t)v * D + i]) Rv[i][c2] = 0.0;
+                }
+        }
         double T[D][KC];
 #pragma unroll
         for (int i = 0; i < D; ++i)
@@ -911,6 +939,17 @@ __global__ __launch_bounds__(kBlock) void k_coarse_galerkin_pairs(int64_t n_pair
     for (int32_t ch = pair_chunk[pr]; ch < pair_chunk[pr + 1]; ++ch) t += gpart[(int64_t)ch * (KC * KC) + m];
     const int64_t a = pair_ab[2 * pr], bb = pair_ab[2 * pr + 1];
     Ac[(a * KC + m / KC) + (bb * KC + m % KC) * nc] = t;
+}
+
+// A_jj <- A_jj (1 + rel), and 1 where A_jj = 0: a positive semidefinite
+// Galerkin matrix of a reduced basis (coarse directions with no free
+// variable) becomes positive definite; the added directions prolong to zero.
+__global__ __launch_bounds__(kBlock) void k_shift_diagonal(int64_t n, double* __restrict__ A, double rel)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double a = A[j + j * n];
+    A[j + j * n] = a == 0.0 ? 1.0 : a * (1.0 + rel);
 }
 
 __global__ __launch_bounds__(kBlock) void k_symmetrize_lower(int64_t n, double* __restrict__ A)
@@ -1020,8 +1059,9 @@ __global__ __launch_bounds__(kBlock) void k_coarse_apply(int64_t nc, const doubl
 template <int D, int KC>
 __global__ __launch_bounds__(kBlock) void k_coarse_prolong(int64_t nh, const int32_t* __restrict__ agg_of,
                                                            const double* __restrict__ dvec,
-                                                           const double* __restrict__ yc, double* __restrict__ z,
-                                                           PcgScalars* sc)
+                                                           const double* __restrict__ yc,
+                                                           const unsigned char* __restrict__ free_vars,
+                                                           double* __restrict__ z, PcgScalars* sc)
 {
     if (block_done(sc)) return;
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1035,6 +1075,7 @@ __global__ __launch_bounds__(kBlock) void k_coarse_prolong(int64_t nh, const int
 #pragma unroll
     for (int i = 0; i < D; ++i)
     {
+        if (free_vars && !free_vars[v * D + i]) continue;
         double t = z[v * D + i];
 #pragma unroll
         for (int c = 0; c < KC; ++c) t += R[i][c] * y[c];
@@ -1062,7 +1103,7 @@ static void galerkin_t(int64_t n_pairs, int64_t n_gchunks, const int64_t* gchunk
 {
     const unsigned g1 = (unsigned)((n_gchunks * 64 + kBlock - 1) / kBlock);
     k_coarse_galerkin_chunk<D, KC><<<g1, kBlock, 0, s>>>(n_gchunks, gchunk_ptr, pair_blocks, block_row, brow_ptr,
-                                                          bcol, row_ptr, H, w, c.dvec, gpart);
+                                                          bcol, row_ptr, H, w, c.dvec, c.free_vars, gpart);
     const unsigned g2 = (unsigned)((n_pairs * KC * KC + kBlock - 1) / kBlock);
     k_coarse_galerkin_pairs<KC><<<g2, kBlock, 0, s>>>(n_pairs, pair_chunk, pair_ab, gpart, c.nc, Ac);
 }
@@ -1086,6 +1127,13 @@ hipError_t launch_coarse_galerkin(int d, int64_t n_pairs, int64_t n_gchunks, con
     return hipGetLastError();
 }
 
+hipError_t launch_shift_diagonal(int64_t n, double* A, double rel, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    k_shift_diagonal<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(n, A, rel);
+    return hipGetLastError();
+}
+
 hipError_t launch_symmetrize_lower(int64_t n, double* A, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
@@ -1103,8 +1151,8 @@ static void coarse_correction_t(int64_t nh, const CoarseDev& c, const double* r,
     const unsigned gs = (unsigned)((c.n_agg * KC + kBlock - 1) / kBlock);
     k_coarse_restrict_sum<KC><<<gs, kBlock, 0, s>>>(c.n_agg, c.agg_chunk, c.chunk_part, c.rc, sc);
     k_coarse_apply<<<c.Gc, kBlock, 0, s>>>(c.nc, c.Ainv, c.rc, c.yc, c.cparts + (int64_t)slot * c.Gc, sc);
-    k_coarse_prolong<D, KC><<<(unsigned)((nh + kBlock - 1) / kBlock), kBlock, 0, s>>>(nh, c.agg_of, c.dvec, c.yc, z,
-                                                                                       sc);
+    k_coarse_prolong<D, KC><<<(unsigned)((nh + kBlock - 1) / kBlock), kBlock, 0, s>>>(nh, c.agg_of, c.dvec, c.yc,
+                                                                                       c.free_vars, z, sc);
 }
 
 hipError_t launch_coarse_correction(int d, int64_t nh, const CoarseDev& c, const double* r, double* z, int slot,
@@ -1140,23 +1188,25 @@ int64_t pcg_parts_size(int d, int64_t nh)
 }
 
 hipError_t launch_pcg_setup(int d, int64_t nh, const int32_t* brow_ptr, const int32_t* diag_block,
-                            const int32_t* row_ptr, const double* H, double w, double* Dinv, hipStream_t s)
+                            const int32_t* row_ptr, const double* H, double w, const unsigned char* free_vars,
+                            double* Dinv, hipStream_t s)
 {
     const unsigned grid = (unsigned)((nh + kBlock - 1) / kBlock);
-    if (d == 1) k_pcg_setup<1><<<grid, kBlock, 0, s>>>(nh, brow_ptr, diag_block, row_ptr, H, w, Dinv);
-    else if (d == 2) k_pcg_setup<2><<<grid, kBlock, 0, s>>>(nh, brow_ptr, diag_block, row_ptr, H, w, Dinv);
-    else k_pcg_setup<3><<<grid, kBlock, 0, s>>>(nh, brow_ptr, diag_block, row_ptr, H, w, Dinv);
+    if (d == 1) k_pcg_setup<1><<<grid, kBlock, 0, s>>>(nh, brow_ptr, diag_block, row_ptr, H, w, free_vars, Dinv);
+    else if (d == 2) k_pcg_setup<2><<<grid, kBlock, 0, s>>>(nh, brow_ptr, diag_block, row_ptr, H, w, free_vars, Dinv);
+    else k_pcg_setup<3><<<grid, kBlock, 0, s>>>(nh, brow_ptr, diag_block, row_ptr, H, w, free_vars, Dinv);
     return hipGetLastError();
 }
 
-hipError_t launch_pcg_init(int d, int64_t nh, const double* b, const double* Dinv, double* x, double* r, double* z,
-                           double* parts, PcgScalars* sc, const CoarseDev* co, hipStream_t s)
+hipError_t launch_pcg_init(int d, int64_t nh, const double* b, const unsigned char* free_vars, const double* Dinv,
+                           double* x, double* r, double* z, double* parts, PcgScalars* sc, const CoarseDev* co,
+                           hipStream_t s)
 {
     int Gs, Gv;
     pcg_grids(d, nh, &Gs, &Gv);
-    if (d == 1) k_pcg_init<1><<<Gv, kBlock, 0, s>>>(nh, b, Dinv, x, r, z, parts, Gs, Gv, sc);
-    else if (d == 2) k_pcg_init<2><<<Gv, kBlock, 0, s>>>(nh, b, Dinv, x, r, z, parts, Gs, Gv, sc);
-    else k_pcg_init<3><<<Gv, kBlock, 0, s>>>(nh, b, Dinv, x, r, z, parts, Gs, Gv, sc);
+    if (d == 1) k_pcg_init<1><<<Gv, kBlock, 0, s>>>(nh, b, free_vars, Dinv, x, r, z, parts, Gs, Gv, sc);
+    else if (d == 2) k_pcg_init<2><<<Gv, kBlock, 0, s>>>(nh, b, free_vars, Dinv, x, r, z, parts, Gs, Gv, sc);
+    else k_pcg_init<3><<<Gv, kBlock, 0, s>>>(nh, b, free_vars, Dinv, x, r, z, parts, Gs, Gv, sc);
     // z_0 gets the coarse correction; its r.z partials join rho_0 in slot 1
     if (co && co->kc > 0 && co->n_agg > 0)
     {
@@ -1170,7 +1220,8 @@ hipError_t launch_pcg_init(int d, int64_t nh, const double* b, const double* Din
 hipError_t launch_pcg_iteration(int d, int64_t nh, const int32_t* brow_ptr, const int32_t* bcol,
                                 const int32_t* row_ptr, const double* H, double w, const double* Dinv, double* x,
                                 double* r, double* z, double* pbuf, double* q, double* parts, PcgScalars* sc, int k,
-                                bool with_update, const CoarseDev* co, hipStream_t s, Timer* timer)
+                                bool with_update, const unsigned char* free_vars, const CoarseDev* co, hipStream_t s,
+                                Timer* timer)
 {
     const bool coarse = co && co->kc > 0 && co->n_agg > 0;
     const double* cparts = coarse ? co->cparts : nullptr;
@@ -1187,11 +1238,14 @@ hipError_t launch_pcg_iteration(int d, int64_t nh, const int32_t* brow_ptr, cons
     if (!with_update) return hipGetLastError();
     if (timer) timer->begin("pcg_update", s);
     if (d == 1)
-        k_pcg_update<1><<<Gv, kUpdVerts, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, cparts, Gc, sc);
+        k_pcg_update<1><<<Gv, kUpdVerts, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, cparts, Gc,
+                                                  free_vars, sc);
     else if (d == 2)
-        k_pcg_update<2><<<Gv, 2 * kUpdVerts, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, cparts, Gc, sc);
+        k_pcg_update<2><<<Gv, 2 * kUpdVerts, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, cparts, Gc,
+                                                      free_vars, sc);
     else
-        k_pcg_update<3><<<Gv, 3 * kUpdVerts, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, cparts, Gc, sc);
+        k_pcg_update<3><<<Gv, 3 * kUpdVerts, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, cparts, Gc,
+                                                      free_vars, sc);
     if (timer) timer->end("pcg_update", s);
     // z of iteration k + 1 gets the coarse correction; its r.z partials go to
     // slot k & 1 next to the update's
