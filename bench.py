@@ -69,6 +69,9 @@ def main():
     ap.add_argument("--coarse-aggregate", type=int, default=None,
                     help="handles per aggregate of the two-level preconditioner (0: block-Jacobi only; "
                          "default: the library's)")
+    ap.add_argument("--coarse-refresh", type=int, default=5,
+                    help="rebuild the coarse space of the direction solve every this many Newton iterations "
+                         "(reusing the previous inverse coarse matrix in between; 1: every iteration)")
     ap.add_argument("--cpu-sample", type=int, default=400000, help="tets in the CPU baseline sample (0: skip)")
     ap.add_argument("--no-newton", action="store_true")
     ap.add_argument("--newton-iters", type=int, default=30,
@@ -78,6 +81,17 @@ def main():
                          "size, used to scale the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     coarse_kw = {} if args.coarse_aggregate is None else {"solve_coarse_aggregate": args.coarse_aggregate}
+    if args.coarse_refresh < 1:
+        ap.error("--coarse-refresh must be at least 1")
+
+    def newton_steps(f, x, opts, count):
+        """count Newton iterations on x; the coarse space is rebuilt on every
+        coarse_refresh-th one (counted from the first of this call)."""
+        out = []
+        for i in range(count):
+            opts.solve.coarse_reuse = 0 if i % args.coarse_refresh == 0 else 1
+            out.append(f.newton_step(x, opts))
+        return out
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -166,8 +180,7 @@ def main():
     if not args.no_newton:
         opts = tad.newton_default_opts(solve_tol=args.pcg_tol, **coarse_kw)
         x = x0.clone()
-        for i in range(args.warmup):
-            st = fn.newton_step(x, opts)
+        for i, st in enumerate(newton_steps(fn, x, opts, args.warmup)):
             log(f"[bench] warmup newton {i}: f {st['f']:.10g} -> {st['f_new']:.10g} cg {st['cg_iters']} "
                 f"ls {st['ls_trials']} step {st['step']:.3g}")
         ctx.set_stream(None)
@@ -175,9 +188,7 @@ def main():
         torch.cuda.synchronize()
         ctx.synchronize()
         t0 = time.perf_counter()
-        sts = []
-        for i in range(args.steps):
-            sts.append(fn.newton_step(x, opts))
+        sts = newton_steps(fn, x, opts, args.steps)
         ctx.synchronize()
         torch.cuda.synchronize()
         tB = time.perf_counter() - t0
@@ -186,7 +197,7 @@ def main():
         xi = x_start.clone()
         ctx.reset_stats()
         ctx.enable_timing(True)
-        sts_i = [fn.newton_step(xi, opts) for _ in range(args.steps)]
+        sts_i = newton_steps(fn, xi, opts, args.steps)
         ctx.synchronize()
         ctx.enable_timing(False)
         if [s["cg_iters"] for s in sts_i] != [s["cg_iters"] for s in sts] or not torch.equal(xi, x):
@@ -213,7 +224,9 @@ def main():
         nb_blocks = nnz // 9
         B_coarse = (40 * n + 8 * P.n_handles + 8 * nc * nc + 16 * nc) if nc else 0
         B_coarse_setup = (8 * nnz + 24 * nb_blocks + 32 * nc * nc) if nc else 0
-        B_step_total = (args.steps * (B_eval + B_coarse_setup) + n_cg * (B_cg + B_coarse) + n_ls * B_energy)
+        n_rebuilt = sum(1 for s in sts if not s["coarse_reused"])
+        B_step_total = (args.steps * B_eval + n_rebuilt * B_coarse_setup + n_cg * (B_cg + B_coarse)
+                        + n_ls * B_energy)
         newton = {
             "iters_per_s": args.steps / tB,
             "ms_per_iter": tB / args.steps * 1e3,
@@ -221,9 +234,12 @@ def main():
             "ls_trials_total": n_ls,
             "f_final": sts[-1]["f_new"],
             "step_roofline_frac": B_step_total / tB / (HBM_PEAK_GBS * 1e9),
-            "step_bytes_formula": ("steps*(B_eval_hess + B_coarse_setup) + n_cg*(B_cg + B_coarse) + n_ls*B_energy "
-                                   "(scalar-CSR B_cg)"),
+            "step_bytes_formula": ("steps*B_eval_hess + coarse_rebuilds*B_coarse_setup + n_cg*(B_cg + B_coarse) "
+                                   "+ n_ls*B_energy (scalar-CSR B_cg)"),
             "coarse_unknowns": nc,
+            "coarse_refresh": args.coarse_refresh,
+            "coarse_rebuilds": n_rebuilt,
+            "cg_iters_per_step": [s["cg_iters"] for s in sts],
         }
         log(f"[bench] phase B: {newton['ms_per_iter']:.2f} ms per Newton iteration, cg {n_cg}, ls {n_ls}")
 
@@ -325,11 +341,10 @@ def main():
         if newton is not None:
             opts64 = tad.newton_default_opts(solve_tol=args.pcg_tol, **coarse_kw)
             x64 = x0.clone()
-            for _ in range(args.warmup):
-                f64.newton_step(x64, opts64)
+            newton_steps(f64, x64, opts64, args.warmup)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            st64 = [f64.newton_step(x64, opts64) for _ in range(args.steps)]
+            st64 = newton_steps(f64, x64, opts64, args.steps)
             ctx.synchronize()
             t64 = time.perf_counter() - t0
             prec["newton_ms_per_iter_fp64"] = t64 / args.steps * 1e3
@@ -337,9 +352,8 @@ def main():
             # converged energies: the Newton run of the workload (30 iterations
             # from x0) in both precisions, untimed
             xa, xb = x0.clone(), x0.clone()
-            for _ in range(args.newton_iters):
-                f64.newton_step(xa, opts64)
-                fn.newton_step(xb, opts)
+            newton_steps(f64, xa, opts64, args.newton_iters)
+            newton_steps(fn, xb, opts, args.newton_iters)
             fa, fb = f64.eval(xa), f64.eval(xb)
             prec["converged_newton_iters"] = args.newton_iters
             prec["converged_f_fp64"] = fa
@@ -370,6 +384,7 @@ def main():
                    "coarse_aggregate": tad.solve_default_opts(**({} if args.coarse_aggregate is None else
                                                                  {"coarse_aggregate": args.coarse_aggregate})
                                                               ).coarse_aggregate,
+                   "coarse_refresh": args.coarse_refresh,
                    "autotuned": bool(args.autotune),
                    "launch_shape": list(fn.launch_shape(0)), "assembly_shape": fn.assembly_shape},
         "roofline": roof,

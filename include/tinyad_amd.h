@@ -221,6 +221,13 @@ typedef struct tad_solve_opts
     int check_every;    /* host polls convergence every this many iterations             */
     int coarse_aggregate; /* two-level preconditioner: handles per aggregate of the      */
                         /* coarse space (default 1024); 0: block-Jacobi only            */
+    int coarse_reuse;   /* 1: keep the coarse space and inverse coarse matrix of the     */
+                        /* function's previous solve when it is compatible (same         */
+                        /* aggregates, unknowns per aggregate and w, neither solve       */
+                        /* reduced) instead of rebuilding them from this H; 0 (default): */
+                        /* rebuild every solve.  A lagged coarse operator is still       */
+                        /* symmetric positive definite, so CG converges to the same      */
+                        /* tolerance; only the iteration count changes.                 */
 } tad_solve_opts;
 
 typedef struct tad_solve_stats
@@ -229,6 +236,7 @@ typedef struct tad_solve_stats
     int converged;
     double rel_residual;
     int coarse_unknowns;    /* size of the coarse space used (0: block-Jacobi only)      */
+    int coarse_reused;      /* 1: the coarse inverse of an earlier solve was used        */
 } tad_solve_stats;
 
 tad_status tad_solve_default_opts(tad_solve_opts* opts);
@@ -278,6 +286,7 @@ typedef struct tad_newton_stats
     int ls_failed;          /* 1: no step accepted, x unchanged (LineSearch.hh:62-64)   */
     int solve_converged;
     int coarse_unknowns;    /* coarse space of the direction solve (0: none)            */
+    int coarse_reused;      /* 1: that solve reused an earlier coarse inverse           */
 } tad_newton_stats;
 
 tad_status tad_newton_default_opts(tad_newton_opts* opts);

@@ -420,6 +420,60 @@ def test_reduced_basis_newton_keeps_pins(ctx):
     assert np.abs(gf).max() <= 1e-8 * max(1.0, abs(f))
 
 
+def test_coarse_reuse(ctx):
+    """coarse_reuse keeps the previous compatible coarse inverse: the solve
+    still converges to tolerance, incompatible solves (other w, reduced)
+    rebuild, and a refresh schedule replayed from the same point gives the
+    same bits."""
+    import torch
+    P = configs.twist_cube(m=12, seed=4)
+    fg = tad.ScalarFunction.from_problem(P, ctx)
+    x0 = torch.tensor(P.x0, device="cuda:0")
+
+    def run(k, steps=5):
+        x = x0.clone()
+        o = tad.newton_default_opts(solve_coarse_aggregate=128)
+        sts = []
+        for i in range(steps):
+            o.solve.coarse_reuse = 0 if i % k == 0 else 1
+            sts.append(fg.newton_step(x, o))
+        return x, sts
+
+    xa, sa = run(3)
+    assert [s["coarse_reused"] for s in sa] == [0, 1, 1, 0, 1]
+    assert all(s["solve_converged"] and s["coarse_unknowns"] > 0 for s in sa)
+    xb, sb = run(3)
+    assert torch.equal(xa, xb) and [s["cg_iters"] for s in sa] == [s["cg_iters"] for s in sb]
+    x1, s1 = run(1)
+    assert not any(s["coarse_reused"] for s in s1)
+    f1, fa = s1[-1]["f_new"], sa[-1]["f_new"]
+    print(f"refresh 1: f {f1:.15g}, cg {[s['cg_iters'] for s in s1]}; refresh 3: f {fa:.15g}, "
+          f"cg {[s['cg_iters'] for s in sa]}")
+    assert abs(f1 - fa) <= 1e-8 * abs(f1)
+
+    f, g, H = fg.eval_with_hessian_proj(P.x0)
+    ref = spla_solve(H, g)
+    _, st = fg.solve(g, tol=1e-12, coarse_aggregate=128)
+    assert st.coarse_reused == 0
+    d, st = fg.solve(g, tol=1e-12, coarse_aggregate=128, coarse_reuse=1)
+    assert st.coarse_reused == 1 and st.converged and nrel(d, ref) <= 1e-8
+    _, st = fg.solve(g, tol=1e-12, coarse_aggregate=128, coarse_reuse=1, w_identity=0.5)
+    assert st.coarse_reused == 0
+    free = np.ones(g.size, dtype=np.uint8)
+    free[:3] = 0
+    _, st = fg.solve(g, tol=1e-12, coarse_aggregate=128, coarse_reuse=1, free=free)
+    assert st.coarse_reused == 0 and st.coarse_unknowns > 0
+    _, st = fg.solve(g, tol=1e-12, coarse_aggregate=128, coarse_reuse=1)
+    assert st.coarse_reused == 0  # the reduced solve's factor is not kept
+    _, st = fg.solve(g, tol=1e-12, coarse_reuse=1, coarse_aggregate=32)
+    assert st.coarse_reused == 0
+
+
+def spla_solve(H, b):
+    import scipy.sparse.linalg as spla
+    return spla.spsolve(H.tocsc(), b)
+
+
 def test_two_level_coarse_size_cap(ctx):
     """A requested aggregate size that would give more than 8,192 coarse
     unknowns is raised until the dense coarse inverse fits that bound."""

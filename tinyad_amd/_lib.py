@@ -42,12 +42,12 @@ class TadError(RuntimeError):
 
 class SolveOpts(ctypes.Structure):
     _fields_ = [("w_identity", ctypes.c_double), ("tol", ctypes.c_double), ("max_iters", ctypes.c_int),
-                ("check_every", ctypes.c_int), ("coarse_aggregate", ctypes.c_int)]
+                ("check_every", ctypes.c_int), ("coarse_aggregate", ctypes.c_int), ("coarse_reuse", ctypes.c_int)]
 
 
 class SolveStats(ctypes.Structure):
     _fields_ = [("iters", ctypes.c_int), ("converged", ctypes.c_int), ("rel_residual", ctypes.c_double),
-                ("coarse_unknowns", ctypes.c_int)]
+                ("coarse_unknowns", ctypes.c_int), ("coarse_reused", ctypes.c_int)]
 
 
 class NewtonOpts(ctypes.Structure):
@@ -60,7 +60,7 @@ class NewtonStats(ctypes.Structure):
     _fields_ = [("f", ctypes.c_double), ("f_new", ctypes.c_double), ("step", ctypes.c_double),
                 ("newton_decrement", ctypes.c_double), ("grad_norm", ctypes.c_double), ("cg_iters", ctypes.c_int),
                 ("ls_trials", ctypes.c_int), ("ls_failed", ctypes.c_int), ("solve_converged", ctypes.c_int),
-                ("coarse_unknowns", ctypes.c_int)]
+                ("coarse_unknowns", ctypes.c_int), ("coarse_reused", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -256,6 +256,12 @@ struct tad_fn_s
         DevBuf<int64_t> gchunk_ptr;
         DevBuf<double> dvec, Ac, rc, yc, cparts, chunk_part, gpart;
         DevBuf<int> info;
+        // the inverse coarse matrix in Ac and the lever arms in dvec belong
+        // together and stay usable by a later solve that asks for reuse when
+        // it needs the same aggregates, kc and w and neither solve is reduced
+        bool factor_valid = false;
+        int f_kc = 0;
+        double f_w = 0.0;
         void release()
         {
             agg_of.release(); agg_ptr.release(); agg_vert.release(); pair_blocks.release(); pair_ab.release();
@@ -264,6 +270,7 @@ struct tad_fn_s
             gpart.release(); info.release();
             requested = target = 0;
             n_agg = n_pairs = n_chunks = n_gchunks = 0;
+            factor_valid = false;
         }
     } co;
     // positions of the last second-order evaluation (rotation modes of the coarse space)
@@ -674,11 +681,13 @@ constexpr int64_t kMaxCoarse = 8192;
 // its Cholesky factorization and inverse (rocSOLVER potrf / potri, lower
 // triangle, rocBLAS atomics off so the result is deterministic), symmetrized.
 // *use = false when the coarse space is off or A_c is not numerically
-// positive definite (block-Jacobi alone then).
+// positive definite (block-Jacobi alone then).  With o.coarse_reuse the
+// inverse and lever arms of the previous compatible solve are kept (*reused).
 tad_status setup_coarse(tad_fn fn, const double* H_dev, const tad_solve_opts& o, const unsigned char* free_dev,
-                        CoarseDev* cd, bool* use)
+                        CoarseDev* cd, bool* use, bool* reused)
 {
     *use = false;
+    *reused = false;
     const int d = fn->d;
     const int64_t nh = fn->n_handles;
     if (o.coarse_aggregate <= 0 || nh == 0 || fn->n_blocks == 0) return TAD_OK;
@@ -690,6 +699,7 @@ tad_status setup_coarse(tad_fn fn, const double* H_dev, const tad_solve_opts& o,
     auto& c = fn->co;
     if (c.requested != want)
     {
+        c.factor_valid = false;
         int target = want;
         for (;;)
         {
@@ -730,6 +740,13 @@ tad_status setup_coarse(tad_fn fn, const double* H_dev, const tad_solve_opts& o,
     cd->cparts = c.cparts.p;
     cd->Gc = Gc;
     cd->free_vars = free_dev;
+    if (o.coarse_reuse && c.factor_valid && !free_dev && c.f_kc == kc && c.f_w == o.w_identity)
+    {
+        *use = true;
+        *reused = true;
+        return TAD_OK;
+    }
+    c.factor_valid = false;
     tm.begin("coarse_setup", s);
     tm.begin("coarse_galerkin", s);
     if (rot) TAD_HIP(launch_coarse_dvec(d, nh, fn->x_hess.p, *cd, c.dvec.p, s));
@@ -776,6 +793,9 @@ tad_status setup_coarse(tad_fn fn, const double* H_dev, const tad_solve_opts& o,
     TAD_HIP(launch_symmetrize_lower(nc, c.Ac.p, s));
     tm.end("coarse_setup", s);
     *use = true;
+    c.factor_valid = !free_dev;
+    c.f_kc = kc;
+    c.f_w = o.w_identity;
     return TAD_OK;
 }
 
@@ -914,8 +934,8 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
                              fn->Dinv.p, s));
     tm.end("pcg_setup", s);
     CoarseDev cd;
-    bool use_coarse = false;
-    TAD_TRY(setup_coarse(fn, H_dev, o, free_dev, &cd, &use_coarse));
+    bool use_coarse = false, coarse_reused = false;
+    TAD_TRY(setup_coarse(fn, H_dev, o, free_dev, &cd, &use_coarse, &coarse_reused));
     const CoarseDev* co = use_coarse ? &cd : nullptr;
     PcgScalars hs;
     std::memset(&hs, 0, sizeof(hs));
@@ -954,6 +974,7 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
         st->rel_residual = rel;
         st->converged = (!hs.breakdown && hs.rr <= hs.tol2 * hs.bb) ? 1 : 0;
         st->coarse_unknowns = use_coarse ? (int)cd.nc : 0;
+        st->coarse_reused = coarse_reused ? 1 : 0;
     }
     if (hs.breakdown) return fail(TAD_ERR_SOLVE, "Linear solve failed: matrix not positive definite");
     if (!std::isfinite(rel)) return fail(TAD_ERR_SOLVE, "Linear solve failed: non-finite residual");
@@ -1319,6 +1340,7 @@ tad_status tad_solve_default_opts(tad_solve_opts* o)
     o->max_iters = 20000;
     o->check_every = 16;
     o->coarse_aggregate = 1024;
+    o->coarse_reuse = 0;
     return TAD_OK;
 }
 
@@ -1457,6 +1479,7 @@ tad_status tad_newton_step(tad_fn fn, double* x, tad_mem mem, const tad_newton_o
     st.cg_iters = ss.iters;
     st.solve_converged = ss.converged;
     st.coarse_unknowns = ss.coarse_unknowns;
+    st.coarse_reused = ss.coarse_reused;
     double dpg = 0.0;
     TAD_TRY(dot_dev(fn, fn->sol.p, fn->g_buf.p, &dpg));
     const double dg = -dpg;  // d^T g
