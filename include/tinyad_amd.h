@@ -78,7 +78,8 @@ typedef enum tad_energy
     TAD_E_SYMDIR_TRI = 1,
     TAD_E_SYMDIR_TET = 2,
     TAD_E_PENALTY = 3,
-    TAD_E_QUADRATIC = 4
+    TAD_E_QUADRATIC = 4,
+    TAD_E_DEVICE = 100      /* terms added by tad_fn_add_elements_device */
 } tad_energy;
 
 typedef struct tad_ctx_s* tad_ctx;
@@ -126,6 +127,71 @@ tad_status tad_fn_destroy(tad_fn fn);
  * reference the same handle twice.  Invalidates the Hessian pattern. */
 tad_status tad_fn_add_elements(tad_fn fn, tad_energy energy, int valence, int64_t n_elem,
                                const int32_t* conn, const double* params, int params_per_elem);
+
+/* --- elements with caller-compiled device energies ---------------------------
+ * The reference's add_elements takes any generic lambda over the element's
+ * active scalars (ScalarFunction.hh:72-78, type-erased in
+ * ScalarObjectiveTerm.hh:75-134).  Here such an energy is a functor compiled
+ * by the caller with hipcc into its own code object; the header
+ * include/tinyad_amd/DeviceElements.hh instantiates its element kernels
+ * (value + gradient, + Hessian, + projected Hessian, each with FP64 and FP32
+ * contributions) and its line-search energy kernel, and fills a
+ * tad_device_ops with host functions that launch them.  The library owns the
+ * pattern, the scatter maps, assembly, solve and line search as for its own
+ * energies, and calls these functions with the arguments below. */
+#define TAD_DEVICE_OPS_VERSION 1
+
+typedef struct tad_device_element_args
+{
+    int64_t n_elem;
+    const int32_t* conn;    /* device, SoA: conn[a * n_elem + e]                       */
+    const double* params;   /* device, SoA: params[k * n_elem + e]                     */
+    const double* x;        /* device, n_vars                                           */
+    double eps;             /* projection eps (mode 3)                                  */
+    double* elem_f;         /* device: energy of element e at elem_f[e]                 */
+    void* G;                /* gradient-part slots, double or float by precision       */
+    void* Hc;               /* Hessian-block slots, double or float by precision       */
+    const uint32_t* gpos;   /* slot of (vertex a, element e): gpos[a * n_elem + e]      */
+    const uint32_t* hpos;   /* slot of upper block (a <= b) of element e; top bit set:  */
+                            /* the slot holds the transposed block                      */
+    int* err;               /* set to 1 when a derivative is not finite                 */
+    int mode;               /* 1 gradient, 2 Hessian, 3 projected Hessian               */
+    int precision;          /* 0: FP64 contributions, 1: FP32                           */
+    void* stream;           /* hipStream_t to launch on                                 */
+} tad_device_element_args;
+
+typedef struct tad_device_energy_args
+{
+    int64_t n_elem;
+    const int32_t* conn;
+    const double* params;
+    const double* x;
+    const double* dir;      /* NULL: energies at x; else at x + steps[c] * dir          */
+    const double* steps;    /* device, ncand step sizes                                 */
+    int ncand;              /* 1..8                                                     */
+    double* out;            /* out[c * stride + e]                                      */
+    int64_t stride;
+    void* stream;
+} tad_device_energy_args;
+
+typedef struct tad_device_ops
+{
+    int version;            /* TAD_DEVICE_OPS_VERSION of the header that built them    */
+    int valence;            /* handles per element                                      */
+    int dim;                /* d: must equal the function's                             */
+    int n_params;           /* parameters per element (>= 0)                            */
+    /* both return a hipError_t value (0: launched) */
+    int (*element)(const tad_device_element_args* args);
+    int (*energy)(const tad_device_energy_args* args);
+} tad_device_ops;
+
+/* add_elements<N>(range(n_elem), lambda) for a device energy: conn (n_elem x
+ * valence) and params (n_elem x n_params), row-major host memory, as for
+ * tad_fn_add_elements.  ops is copied; its functions must stay loaded while the
+ * function exists.  Terms added this way take the plain one-lane-per-element
+ * launch (no autotuned shapes, no rest-shape products). */
+tad_status tad_fn_add_elements_device(tad_fn fn, const tad_device_ops* ops, int64_t n_elem, const int32_t* conn,
+                                      const double* params);
 
 tad_status tad_fn_n_vars(tad_fn fn, int64_t* n_vars);
 tad_status tad_fn_n_elements(tad_fn fn, int64_t* n_elements);

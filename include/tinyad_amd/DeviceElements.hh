@@ -1,0 +1,137 @@
+// Element energies written once as generic device functors, the MI355X form of
+// the reference's generic element lambdas (ScalarFunction.hh:72-78,
+// tests/NewtonTest.cc:28-44).  Compile the including file with hipcc for
+// gfx950 (with -Xclang -target-feature -Xclang -packed-fp32-ops, like the
+// library).
+//
+//   struct MyEnergy
+//   {
+//       // x: the element's valence * d coordinates, vertex-major (x[a * d + i]),
+//       // p: its parameter row; S is double (passive value) or tad::Dual<K, T>
+//       template <class S>
+//       __device__ S operator()(const S* x, const double* p) const { ... }
+//   };
+//   func.add_elements<3>(range(n_faces), tinyad_amd::device_element<MyEnergy, 3, 2, 5>(),
+//                        conn_of, params_of);
+//
+// Inside the functor use +, -, *, / (with doubles or ints), comparisons and
+// tad::sqr, tad::sqrt, tad::exp, tad::log, tad::sin, tad::cos, tad::pow(S, double);
+// return S(INFINITY) (or INFINITY) for an infeasible element, which then
+// contributes a zero gradient and, projected, eps I (as an infinite Double<k>
+// does in the reference).  The functor must be stateless (it is
+// default-constructed on the device); per-element data comes in p.
+//
+// For each energy, device_element<> instantiates the library's element kernel
+// (forward mode over the element's K = valence * d variables with
+// tad::Dual<K, T>, the dense K x K eigen-projection, scatter through the
+// library's slot maps) for FP64 and FP32 contributions and the three
+// derivative modes, and the line-search energy kernel, all in the caller's
+// code object.  The value returned by every mode is the functor evaluated in
+// FP64 with S = double, the same function the line search evaluates, so both
+// give the same bits.
+#pragma once
+
+#include <tinyad_amd.h>
+#include <tinyad_amd/device/Kernels.hpp>
+
+#include <hip/hip_runtime.h>
+
+namespace tinyad_amd {
+
+template <class F, int NN, int DD, int NPAR>
+struct DeviceElement
+{
+    static constexpr int N = NN, D = DD, NP = NPAR, K = NN * DD;
+    static_assert(NN >= 1 && DD >= 1 && DD <= 3 && K <= 12, "valence * d must be in 1..12, d in 1..3");
+    static_assert(NPAR >= 0, "negative parameter count");
+
+    template <class T, int MODE, class Sink>
+    __device__ static double eval(const double* X, const double* p, T eps, const Sink& out, bool& nonfinite)
+    {
+        const double f = F{}(X, p);
+        tad::Dual<K, T> v[K];
+#pragma unroll
+        for (int i = 0; i < K; ++i) v[i] = tad::dual_var<K, T>(T(X[i]), i);
+        const tad::Dual<K, T> E = F{}(static_cast<const tad::Dual<K, T>*>(v), p);
+        T g[K], Hp[tad::Sym<K>::n];
+#pragma unroll
+        for (int i = 0; i < K; ++i) g[i] = E.g[i];
+#pragma unroll
+        for (int i = 0; i < tad::Sym<K>::n; ++i) Hp[i] = E.h[i];
+        tad::write_dense_record<NN, DD, T, MODE>(g, Hp, eps, out, nonfinite);
+        return f;
+    }
+
+    __device__ static double energy(const double* X, const double* p) { return F{}(X, p); }
+};
+
+namespace detail {
+
+template <class E, class T>
+inline hipError_t launch_device_element_t(const tad_device_element_args* a, dim3 grid, hipStream_t s)
+{
+    const tad::ElemOut<T> o{ static_cast<T*>(a->G), static_cast<T*>(a->Hc), a->gpos, a->hpos };
+    switch (a->mode)
+    {
+        case tad::MODE_GRAD:
+            tad::k_element<E, T, tad::MODE_GRAD><<<grid, tad::kBlock, 0, s>>>(a->n_elem, a->conn, a->params, a->x,
+                                                                             a->eps, a->elem_f, o, a->err);
+            break;
+        case tad::MODE_HESS:
+            tad::k_element<E, T, tad::MODE_HESS><<<grid, tad::kBlock, 0, s>>>(a->n_elem, a->conn, a->params, a->x,
+                                                                             a->eps, a->elem_f, o, a->err);
+            break;
+        case tad::MODE_HESS_PROJ:
+            tad::k_element<E, T, tad::MODE_HESS_PROJ><<<grid, tad::kBlock, 0, s>>>(
+                a->n_elem, a->conn, a->params, a->x, a->eps, a->elem_f, o, a->err);
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <class E>
+int launch_device_element(const tad_device_element_args* a)
+{
+    if (!a || a->n_elem < 0) return (int)hipErrorInvalidValue;
+    if (a->n_elem == 0) return (int)hipSuccess;
+    const int64_t tiles = (a->n_elem + tad::kBlock - 1) / tad::kBlock;
+    if (tiles > 0x7fffffff) return (int)hipErrorInvalidValue;
+    const dim3 grid((unsigned)tiles);
+    hipStream_t s = static_cast<hipStream_t>(a->stream);
+    if (a->precision == 0) return (int)launch_device_element_t<E, double>(a, grid, s);
+    if (a->precision == 1) return (int)launch_device_element_t<E, float>(a, grid, s);
+    return (int)hipErrorInvalidValue;
+}
+
+template <class E>
+int launch_device_energy(const tad_device_energy_args* a)
+{
+    if (!a || a->n_elem < 0 || a->ncand < 1 || a->ncand > tad::kMaxCand) return (int)hipErrorInvalidValue;
+    if (a->n_elem == 0) return (int)hipSuccess;
+    const int64_t tiles = (a->n_elem + tad::kBlock - 1) / tad::kBlock;
+    if (tiles > 0x7fffffff) return (int)hipErrorInvalidValue;
+    tad::k_energy<E><<<dim3((unsigned)tiles), tad::kBlock, 0, static_cast<hipStream_t>(a->stream)>>>(
+        a->n_elem, a->conn, a->params, a->x, a->dir, a->steps, a->ncand, a->out, a->stride);
+    return (int)hipGetLastError();
+}
+
+} // namespace detail
+
+// Launch table of energy F on elements of `valence` handles in dimension d
+// with NP parameters per element, for tad_fn_add_elements_device.
+template <class F, int valence, int d, int NP>
+tad_device_ops device_element()
+{
+    using E = DeviceElement<F, valence, d, NP>;
+    tad_device_ops o{};
+    o.version = TAD_DEVICE_OPS_VERSION;
+    o.valence = valence;
+    o.dim = d;
+    o.n_params = NP;
+    o.element = &detail::launch_device_element<E>;
+    o.energy = &detail::launch_device_energy<E>;
+    return o;
+}
+
+} // namespace tinyad_amd

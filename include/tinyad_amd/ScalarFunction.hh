@@ -6,7 +6,8 @@
 // (Utils/Out.hh:64-157).
 //
 // Differences from the reference, by design:
-//   * element functions are entries of the library's energy catalog plus
+//   * element functions are entries of the library's energy catalog, or
+//     device functors compiled by the caller (DeviceElements.hh), plus
 //     per-element parameter rows (the reference instantiates a user lambda
 //     for three scalar types, ScalarObjectiveTerm.hh:54-115);
 //   * Eigen is optional: vectors are std::vector<double>, the Hessian is a
@@ -170,6 +171,32 @@ public:
         }
         check(tad_fn_add_elements(fn_.get(), energy, element_valence, n, conn.data(), params.data(),
                                   np < 0 ? 0 : np));
+        n_elements += n;
+    }
+
+    // add_elements<N>(range, lambda) for an energy compiled on the device:
+    // ops from tinyad_amd::device_element<F, N, d, NP>() (DeviceElements.hh),
+    // conn_of / params_of as above (params_of rows must have NP entries).
+    template <int element_valence, class Range, class ConnFn, class ParamFn>
+    void add_elements(const Range& element_range, const tad_device_ops& ops, ConnFn conn_of, ParamFn params_of)
+    {
+        require_fn();
+        if (ops.valence != element_valence)
+            throw std::runtime_error("[tinyad_amd] device energy valence differs from add_elements<N>");
+        std::vector<int32_t> conn;
+        std::vector<double> params;
+        int64_t n = 0;
+        for (auto e : element_range)
+        {
+            const std::array<int, element_valence> c = conn_of(e);
+            for (int a = 0; a < element_valence; ++a) conn.push_back(c[a]);
+            const auto p = params_of(e);
+            if ((int)p.size() != ops.n_params)
+                throw std::runtime_error("[tinyad_amd] parameter row length differs from the device energy's");
+            for (int k = 0; k < ops.n_params; ++k) params.push_back(p[k]);
+            ++n;
+        }
+        check(tad_fn_add_elements_device(fn_.get(), &ops, n, conn.data(), params.data()));
         n_elements += n;
     }
 

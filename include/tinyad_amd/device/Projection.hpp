@@ -17,7 +17,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
-#include "tad_dual.hpp"
+#include "Dual.hpp"
 
 namespace tad {
 

@@ -1,0 +1,175 @@
+// Scatter of per-element derivative records into the library's destination-
+// sorted contribution slots, shared by the library's built-in element kernels
+// and by element kernels compiled from user energies (DeviceElements.hh).
+// The slot maps (gpos, hpos) are built by the library for every term; their
+// layout is fixed by tad_device_ops version TAD_DEVICE_OPS_VERSION.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "Dual.hpp"
+#include "Modes.hpp"
+#include "Projection.hpp"
+
+namespace tad {
+
+__host__ __device__ constexpr int n_blocks(int N) { return N * (N + 1) / 2; }
+__host__ __device__ constexpr int block_index(int N, int v, int u) { return v * N - (v * (v - 1)) / 2 + (u - v); }
+
+// Contiguous run of L values.  FP32 slots are only 4-byte aligned;
+// gfx9-family compute queues run in unaligned mode, where dword-aligned
+// vector stores are legal, so a 3 x 3 block goes out as two 16-byte stores
+// and one 4-byte store instead of nine (the scattered per-lane writes of the
+// element kernel are bound by L2 write requests: the FP32 tet kernel runs
+// 1.35x faster).  FP64 slots stay with 8-byte stores: 16-byte stores at
+// 8-byte-aligned slots measured slower there, in every evaluation mode.
+typedef float tad_f4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef float tad_f2u __attribute__((ext_vector_type(2), aligned(4)));
+
+template <int L>
+__device__ __forceinline__ void store_run(double* __restrict__ dst, const double* v)
+{
+#pragma unroll
+    for (int m = 0; m < L; ++m) dst[m] = v[m];
+}
+
+template <int L>
+__device__ __forceinline__ void store_run(float* __restrict__ dst, const float* v)
+{
+#pragma unroll
+    for (int m = 0; m + 4 <= L; m += 4)
+    {
+        const tad_f4u w = { v[m], v[m + 1], v[m + 2], v[m + 3] };
+        *reinterpret_cast<tad_f4u*>(dst + m) = w;
+    }
+    constexpr int r = L % 4;
+    if constexpr (r >= 2)
+    {
+        const tad_f2u w = { v[L - r], v[L - r + 1] };
+        *reinterpret_cast<tad_f2u*>(dst + L - r) = w;
+    }
+    if constexpr (r % 2 == 1) dst[L - 1] = v[L - 1];
+}
+
+// Scatter of one element's derivative parts through the precomputed maps:
+// gpos[a n + e] is the slot of vertex a's gradient part in G (d values per
+// slot); hpos[block_index(a, b) n + e] (a <= b) the slot of block (a, b) in Hc
+// (d*d values per slot), top bit set when the slot belongs to the transposed
+// (upper) CSR block.  Slots are sorted by destination (vertex, upper CSR
+// block) and, within a destination, by term and element, so the assembly sums
+// contiguous runs in the reference's element order.
+template <int N, int Dm, class T>
+struct ScatterSink
+{
+    T* __restrict__ G;
+    T* __restrict__ Hc;
+    const uint32_t* __restrict__ gpos;
+    const uint32_t* __restrict__ hpos;
+    int64_t e, n;
+
+    __device__ __forceinline__ void grad(int v, const T* g) const
+    {
+        store_run<Dm>(G + (int64_t)gpos[v * n + e] * Dm, g);
+    }
+    // block (v, u), v <= u, in the orientation of its slot
+    __device__ __forceinline__ void block(int v, int u, const T* b) const
+    {
+        const uint32_t pos = hpos[block_index(N, v, u) * n + e];
+        T* dst = Hc + (int64_t)(pos & 0x7fffffffu) * (Dm * Dm);
+        if constexpr (sizeof(T) == 8)
+        {
+            if (pos >> 31)
+            {
+#pragma unroll
+                for (int i = 0; i < Dm; ++i)
+#pragma unroll
+                    for (int j = 0; j < Dm; ++j) dst[j * Dm + i] = b[i * Dm + j];
+            }
+            else
+            {
+#pragma unroll
+                for (int m = 0; m < Dm * Dm; ++m) dst[m] = b[m];
+            }
+        }
+        else
+        {
+            // the transpose as a register select: every lane stores the same
+            // contiguous pattern of wide stores
+            const bool tr = (pos >> 31) != 0;
+            T o[Dm * Dm];
+#pragma unroll
+            for (int i = 0; i < Dm; ++i)
+#pragma unroll
+                for (int j = 0; j < Dm; ++j) o[i * Dm + j] = tr ? b[j * Dm + i] : b[i * Dm + j];
+            store_run<Dm * Dm>(dst, o);
+        }
+    }
+};
+
+__device__ __forceinline__ double tad_inf() { return __longlong_as_double(0x7ff0000000000000LL); }
+
+template <class T>
+__device__ __forceinline__ bool finite_val(T x) { return isfinite(x); }
+
+// positive_diagonally_dominant (HessianProjection.hh:23-42) on a packed
+// symmetric K x K matrix: H_ii >= sum_{j != i} |H_ij| + eps for every row.
+template <int K, class T>
+__device__ __forceinline__ bool positive_diagonally_dominant(const T* Hp, T eps)
+{
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+    {
+        T off = T(0);
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (j != i) off += fabs(Hp[Sym<K>::at(i, j)]);
+        ok = ok && !(Hp[Sym<K>::idx(i, i)] < off + eps);
+    }
+    return ok;
+}
+
+// Writes the N(N+1)/2 upper d x d blocks of a packed (N d) x (N d) matrix.
+template <int N, int Dm, class T, class Sink>
+__device__ __forceinline__ void write_blocks(const T* Hp, const Sink& out, bool& nonfinite)
+{
+    constexpr int K = N * Dm;
+#pragma unroll
+    for (int v = 0; v < N; ++v)
+#pragma unroll
+        for (int u = v; u < N; ++u)
+        {
+            T blk[Dm * Dm];
+#pragma unroll
+            for (int i = 0; i < Dm; ++i)
+#pragma unroll
+                for (int ip = 0; ip < Dm; ++ip)
+                {
+                    const T val = Hp[Sym<K>::at(v * Dm + i, u * Dm + ip)];
+                    blk[i * Dm + ip] = val;
+                    nonfinite = nonfinite || !finite_val(val);
+                }
+            out.block(v, u, blk);
+        }
+}
+
+// Generic small dense terms (penalty, quadratic): full K x K Hessian in
+// packed form, projected without deflation.
+template <int N, int Dm, class T, int MODE, class Sink>
+__device__ __forceinline__ void write_dense_record(const T* g, T* Hp, T eps, const Sink& out, bool& nonfinite)
+{
+    constexpr int K = N * Dm;
+#pragma unroll
+    for (int i = 0; i < K; ++i) nonfinite = nonfinite || !finite_val(g[i]);
+#pragma unroll
+    for (int v = 0; v < N; ++v) out.grad(v, g + v * Dm);
+    if constexpr (MODE >= MODE_HESS)
+    {
+        if constexpr (MODE == MODE_HESS_PROJ)
+            if (!positive_diagonally_dominant<K, T>(Hp, eps))
+                project_psd<K, T>(Hp, eps);
+        write_blocks<N, Dm, T>(Hp, out, nonfinite);
+    }
+}
+
+} // namespace tad

@@ -1,0 +1,106 @@
+// Energies written as generic device functors, compiled into
+// tests/cpp/bin/libtad_test_device_energies.so and handed to the library
+// through tad_fn_add_elements_device.  The parity tests compare them with the
+// library's built-in catalog (itself pinned to the oracle), with the oracle
+// directly, and (for the transcendental one) with finite differences.
+#include <tinyad_amd/DeviceElements.hh>
+
+#include <cmath>
+
+namespace {
+
+// Symmetric Dirichlet on a triangle as the reference's test lambda states it
+// (tests/NewtonTest.cc:28-44): M = col_mat(b - a, c - a), +inf if det M <= 0,
+// else w (||M Mr^-1||^2 + ||Mr M^-1||^2).  p = [Mr^-1 (2 x 2 row-major), w].
+// For 2 x 2 matrices ||J^-1||_F = ||J||_F / |det J|.
+struct SymdirTri
+{
+    template <class S>
+    __device__ S operator()(const S* x, const double* p) const
+    {
+        const S m00 = x[2] - x[0], m10 = x[3] - x[1];
+        const S m01 = x[4] - x[0], m11 = x[5] - x[1];
+        if (m00 * m11 - m01 * m10 <= 0.0) return S(INFINITY);
+        const S j00 = m00 * p[0] + m01 * p[2], j01 = m00 * p[1] + m01 * p[3];
+        const S j10 = m10 * p[0] + m11 * p[2], j11 = m10 * p[1] + m11 * p[3];
+        const S n2 = tad::sqr(j00) + tad::sqr(j01) + tad::sqr(j10) + tad::sqr(j11);
+        return p[4] * (n2 + n2 / tad::sqr(j00 * j11 - j01 * j10));
+    }
+};
+
+// The same energy on a tetrahedron; ||J^-1||_F = ||cof J||_F / |det J|.
+// p = [Mr^-1 (3 x 3 row-major), w].
+struct SymdirTet
+{
+    template <class S>
+    __device__ S operator()(const S* x, const double* p) const
+    {
+        S m[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int c = 0; c < 3; ++c) m[i][c] = x[3 * (c + 1) + i] - x[i];
+        const S detm = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+                       m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                       m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+        if (detm <= 0.0) return S(INFINITY);
+        S j[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int c = 0; c < 3; ++c) j[i][c] = m[i][0] * p[0 * 3 + c] + m[i][1] * p[1 * 3 + c] + m[i][2] * p[2 * 3 + c];
+        S nj = S(0.0), nc = S(0.0);
+        for (int i = 0; i < 3; ++i)
+            for (int c = 0; c < 3; ++c)
+            {
+                nj += tad::sqr(j[i][c]);
+                const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+                nc += tad::sqr(j[i1][c1] * j[i2][c2] - j[i1][c2] * j[i2][c1]);
+            }
+        const S detj = j[0][0] * (j[1][1] * j[2][2] - j[1][2] * j[2][1]) -
+                       j[0][1] * (j[1][0] * j[2][2] - j[1][2] * j[2][0]) +
+                       j[0][2] * (j[1][0] * j[2][1] - j[1][1] * j[2][0]);
+        return p[9] * (nj + nc / tad::sqr(detj));
+    }
+};
+
+// 0.5 x^T A x + b^T x + c over two 2D handles (K = 4): p = [A (4 x 4), b (4), c],
+// the parameter row of the catalog's quadratic element.
+struct Quadratic4
+{
+    template <class S>
+    __device__ S operator()(const S* x, const double* p) const
+    {
+        S e = S(p[20]);
+        for (int i = 0; i < 4; ++i)
+        {
+            e += p[16 + i] * x[i];
+            for (int k = 0; k < 4; ++k) e += 0.5 * p[4 * i + k] * (x[i] * x[k]);
+        }
+        return e;
+    }
+};
+
+// A smooth energy that uses every transcendental helper, on two 2D handles:
+// p0 exp(0.3 x0 x3) + log(1 + x1^2) + sqrt(1 + x0^2 + x2^2)
+//   + sin(x1) cos(x2) + p1 (1 + x3^2)^1.5 + 1 / (2 + x0^2).
+struct Smooth4
+{
+    template <class S>
+    __device__ S operator()(const S* x, const double* p) const
+    {
+        return p[0] * tad::exp(0.3 * x[0] * x[3]) + tad::log(1 + x[1] * x[1]) +
+               tad::sqrt(1.0 + x[0] * x[0] + x[2] * x[2]) + tad::sin(x[1]) * tad::cos(x[2]) +
+               p[1] * tad::pow(1.0 + x[3] * x[3], 1.5) + 1.0 / (2 + x[0] * x[0]);
+    }
+};
+
+const tad_device_ops g_tri = tinyad_amd::device_element<SymdirTri, 3, 2, 5>();
+const tad_device_ops g_tet = tinyad_amd::device_element<SymdirTet, 4, 3, 10>();
+const tad_device_ops g_quad = tinyad_amd::device_element<Quadratic4, 2, 2, 21>();
+const tad_device_ops g_smooth = tinyad_amd::device_element<Smooth4, 2, 2, 2>();
+
+} // namespace
+
+extern "C" {
+const tad_device_ops* tad_test_symdir_tri_ops() { return &g_tri; }
+const tad_device_ops* tad_test_symdir_tet_ops() { return &g_tet; }
+const tad_device_ops* tad_test_quadratic4_ops() { return &g_quad; }
+const tad_device_ops* tad_test_smooth4_ops() { return &g_smooth; }
+}

@@ -24,6 +24,9 @@ static int g_failed = 0;
 
 using tinyad_amd::VectorXd;
 
+// symmetric Dirichlet on triangles as a device functor (tests/cpp/device_energies.hip)
+extern "C" const tad_device_ops* tad_test_symdir_tri_ops();
+
 static std::vector<double> quad_params(const std::vector<double>& A, const std::vector<double>& b, double c)
 {
     std::vector<double> p = A;
@@ -73,7 +76,9 @@ static bool test_2d_nonconvex()
 }
 
 // tests/NewtonTest.cc:12-95 on the 6-vertex planar fixture (tests/Meshes.hh:12-42)
-static bool test_newton(VectorXd* x_out)
+// NewtonTest.cc:14-80; the triangle energy from the catalog, or with ops
+// given, from a device functor written like the reference's lambda
+static bool test_newton(VectorXd* x_out, const tad_device_ops* ops = nullptr)
 {
     bool ok = true;
     const double V_rest[6][2] = { { 0, 0 }, { 1, 0 }, { 0, 1 }, { 1, 1 }, { 0, 2 }, { 1, 2 } };
@@ -81,14 +86,15 @@ static bool test_newton(VectorXd* x_out)
     const int b[2] = { 0, 4 };
     const double bc[2][2] = { { 0.0, 0.0 }, { -2.0, 0.0 } };
     auto func = tinyad_amd::scalar_function<2>(tinyad_amd::range(6));
-    func.add_elements<3>(tinyad_amd::range(4), TAD_E_SYMDIR_TRI,
-        [&](int f) { return std::array<int, 3>{ F[f][0], F[f][1], F[f][2] }; },
-        [&](int f) {
-            const double m00 = V_rest[F[f][1]][0] - V_rest[F[f][0]][0], m10 = V_rest[F[f][1]][1] - V_rest[F[f][0]][1];
-            const double m01 = V_rest[F[f][2]][0] - V_rest[F[f][0]][0], m11 = V_rest[F[f][2]][1] - V_rest[F[f][0]][1];
-            const double det = m00 * m11 - m01 * m10;
-            return std::vector<double>{ m11 / det, -m01 / det, -m10 / det, m00 / det, 1.0 / 4.0 };
-        });
+    auto conn_of = [&](int f) { return std::array<int, 3>{ F[f][0], F[f][1], F[f][2] }; };
+    auto params_of = [&](int f) {
+        const double m00 = V_rest[F[f][1]][0] - V_rest[F[f][0]][0], m10 = V_rest[F[f][1]][1] - V_rest[F[f][0]][1];
+        const double m01 = V_rest[F[f][2]][0] - V_rest[F[f][0]][0], m11 = V_rest[F[f][2]][1] - V_rest[F[f][0]][1];
+        const double det = m00 * m11 - m01 * m10;
+        return std::vector<double>{ m11 / det, -m01 / det, -m10 / det, m00 / det, 1.0 / 4.0 };
+    };
+    if (ops) func.add_elements<3>(tinyad_amd::range(4), *ops, conn_of, params_of);
+    else func.add_elements<3>(tinyad_amd::range(4), TAD_E_SYMDIR_TRI, conn_of, params_of);
     func.add_elements<1>(tinyad_amd::range(2), TAD_E_PENALTY, [&](int i) { return std::array<int, 1>{ b[i] }; },
                          [&](int i) { return std::vector<double>{ bc[i][0], bc[i][1], 1.0 }; });
     VectorXd x = func.x_from_data([&](int v) { return std::array<double, 2>{ 0.5 * V_rest[v][0], 0.25 * V_rest[v][1] }; });
@@ -296,6 +302,7 @@ int main()
         { "ScalarFunctionTest.2DNonConvex", test_2d_nonconvex },
         { "NewtonTest.2DDeformation", []() { return test_newton(nullptr); } },
         { "NewtonTest.Deterministic", test_newton_deterministic },
+        { "NewtonTest.2DDeformation.DeviceFunctor", []() { return test_newton(nullptr, tad_test_symdir_tri_ops()); } },
         { "ExceptionTest", test_errors_and_recovery },
         { "Batching.SegmentsAutotuneParams", test_segments_autotune_params },
         { "Solver.TwoLevel", test_two_level_solve },

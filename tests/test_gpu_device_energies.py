@@ -1,0 +1,228 @@
+"""Energies compiled as generic device functors (tad_fn_add_elements_device,
+include/tinyad_amd/DeviceElements.hh), the form the reference gives every
+energy: a generic lambda over the element's active scalars
+(ScalarFunction.hh:72-78, tests/NewtonTest.cc:28-44).
+
+The functors live in tests/cpp/device_energies.hip, built into
+tests/cpp/bin/libtad_test_device_energies.so.  Pins:
+  * symmetric Dirichlet (triangles, tets) against the oracle and against the
+    library's built-in catalog at the FP64 bar (1e-10 normwise);
+  * a dense quadratic against the catalog's quadratic element;
+  * an energy using sqrt / exp / log / sin / cos / pow against central
+    differences of its own passive energy, and its projected Hessian against
+    numpy's eigen-projection of its Hessian (HessianProjection.hh:48-101);
+  * mixed precision within the tolerance of the catalog's mixed mode;
+  * a Newton trajectory equal to the catalog's to the converged-energy bar.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import tinyad_amd as tad
+from tinyad_amd import configs, _lib
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+PLUGIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "bin", "libtad_test_device_energies.so")
+
+
+def nrel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.abs(a - b).max(initial=0.0) / max(np.abs(b).max(initial=0.0), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def plugin():
+    if not os.path.exists(PLUGIN):
+        raise RuntimeError(f"{PLUGIN} is missing: run python -m tinyad_amd.build first")
+    L = ctypes.CDLL(PLUGIN)
+    ops = {}
+    for name in ["symdir_tri", "symdir_tet", "quadratic4", "smooth4"]:
+        f = getattr(L, f"tad_test_{name}_ops")
+        f.restype = ctypes.c_void_p
+        ops[name] = f()
+    ops["_lib"] = L
+    return ops
+
+
+def device_function(P, plugin, ctx, precision="fp64"):
+    """P with its symmetric-Dirichlet terms as device functors, other terms from the catalog."""
+    fn = tad.ScalarFunction(P.d, P.n_handles, ctx, precision)
+    for t in P.terms:
+        if t.energy == tad.SYMDIR_TRI:
+            fn.add_elements_device(plugin["symdir_tri"], t.conn, t.params)
+        elif t.energy == tad.SYMDIR_TET:
+            fn.add_elements_device(plugin["symdir_tet"], t.conn, t.params)
+        else:
+            fn.add_elements(t.valence, t.energy, t.conn, t.params)
+    return fn
+
+
+def perturbed_cube(m, sigma, seed):
+    P = configs.twist_cube(m=m, seed=seed)
+    rest = P.info["rest"]
+    P.x0 = rest + sigma * P.info["h"] * configs.normal(seed + 1, rest.size, stream=9)
+    return P
+
+
+CASES = {
+    "grid_param_12": lambda: configs.grid_param(12, seed=5),
+    "newton_test": configs.newton_test_problem,
+    "twist_cube_3": lambda: configs.twist_cube(m=3, seed=5),
+    "cube_perturbed": lambda: perturbed_cube(3, 0.15, 13),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("eps", [1e-9, -1.0])
+def test_device_symdir_matches_oracle_and_catalog(ctx, plugin, name, eps):
+    P = CASES[name]()
+    fo = oracle.Function.from_problem(P)
+    fc = tad.ScalarFunction.from_problem(P, ctx)
+    fd = device_function(P, plugin, ctx)
+    f1, g1, H1 = fo.eval_with_hessian_proj(P.x0, eps)
+    f2, g2, H2 = fd.eval_with_hessian_proj(P.x0, eps)
+    f3, g3, H3 = fc.eval_with_hessian_proj(P.x0, eps)
+    assert np.array_equal(H2.indptr, H3.indptr) and np.array_equal(H2.indices, H3.indices)
+    assert abs(f2 - f1) <= 1e-12 * abs(f1)
+    assert nrel(g2, g1) <= FP64_TOL and nrel(H2.toarray(), H1.toarray()) <= FP64_TOL
+    assert nrel(g2, g3) <= FP64_TOL and nrel(H2.data, H3.data) <= FP64_TOL
+    assert abs(fd.eval(P.x0) - f2) <= 1e-14 * abs(f2)
+    fa, ga = fd.eval_with_gradient(P.x0)
+    fb, gb, Hb = fd.eval_with_derivatives(P.x0)
+    fo2, go2, Ho2 = fo.eval_with_derivatives(P.x0)
+    assert fa == f2 and fb == f2
+    assert nrel(ga, g1) <= FP64_TOL and nrel(gb, go2) <= FP64_TOL
+    assert nrel(Hb.toarray(), Ho2.toarray()) <= FP64_TOL
+
+
+def test_device_inverted_element(ctx, plugin):
+    """+inf energy, zero gradient and eps I for an inverted tet (NewtonTest.cc:40-41)."""
+    P = configs.twist_cube(m=2, seed=3)
+    X = P.x0.reshape(-1, 3).copy()
+    a, b = P.terms[0].conn[0, 1], P.terms[0].conn[0, 2]
+    X[[a, b]] = X[[b, a]]
+    x = X.ravel()
+    fo = oracle.Function.from_problem(P)
+    fd = device_function(P, plugin, ctx)
+    f1, g1, H1 = fo.eval_with_hessian_proj(x)
+    f2, g2, H2 = fd.eval_with_hessian_proj(x)
+    assert f1 == np.inf and f2 == np.inf and fd.eval(x) == np.inf
+    assert nrel(g2, g1) <= FP64_TOL and nrel(H2.toarray(), H1.toarray()) <= FP64_TOL
+
+
+def quad_problem(seed, n_handles=7, n_elem=12):
+    rng = np.random.default_rng(seed)
+    conn = np.array([rng.choice(n_handles, 2, replace=False) for _ in range(n_elem)], dtype=np.int32)
+    A = rng.standard_normal((n_elem, 4, 4))  # not symmetric, indefinite
+    b = rng.standard_normal((n_elem, 4))
+    c = rng.standard_normal((n_elem, 1))
+    params = np.concatenate([A.reshape(n_elem, 16), b, c], axis=1)
+    x = rng.standard_normal(2 * n_handles)
+    return conn, params, x
+
+
+@pytest.mark.parametrize("eps", [1e-9, -1.0, 0.3])
+def test_device_quadratic_matches_catalog(ctx, plugin, eps):
+    conn, params, x = quad_problem(4)
+    fc = tad.ScalarFunction(2, 7, ctx)
+    fc.add_elements(2, tad.QUADRATIC, conn, params)
+    fd = tad.ScalarFunction(2, 7, ctx)
+    fd.add_elements_device(plugin["quadratic4"], conn, params)
+    r1 = fc.eval_with_hessian_proj(x, eps)
+    r2 = fd.eval_with_hessian_proj(x, eps)
+    assert abs(r2[0] - r1[0]) <= 1e-13 * max(1.0, abs(r1[0]))
+    assert nrel(r2[1], r1[1]) <= 1e-13
+    assert nrel(r2[2].data, r1[2].data) <= 1e-12
+
+
+def smooth_energy(x, p):
+    """The passive energy of Smooth4 (tests/cpp/device_energies.hip) in numpy."""
+    return (p[0] * np.exp(0.3 * x[0] * x[3]) + np.log(1 + x[1] ** 2) + np.sqrt(1 + x[0] ** 2 + x[2] ** 2)
+            + np.sin(x[1]) * np.cos(x[2]) + p[1] * (1 + x[3] ** 2) ** 1.5 + 1 / (2 + x[0] ** 2))
+
+
+def reference_projection(H, eps):
+    """project_positive_definite (HessianProjection.hh:48-101) with numpy:
+    unchanged when diagonally dominant with H_ii >= sum_j |H_ij| + eps (the
+    early out, which for eps < 0 also accepts some indefinite matrices) or
+    when no eigenvalue is clamped; else V clamp(lambda) V^T."""
+    off = np.abs(H).sum(axis=1) - np.abs(np.diag(H))
+    if not np.any(np.diag(H) < off + eps):
+        return H
+    lam, V = np.linalg.eigh(H)
+    lc = np.maximum(lam, eps) if eps >= 0 else np.abs(lam)
+    return H if np.array_equal(lc, lam) else (V * lc) @ V.T
+
+
+def test_device_transcendental_derivatives(ctx, plugin):
+    """One Smooth4 element: value against numpy, gradient and Hessian against
+    central differences (steps 1e-5: error ~1e-10 relative), projection
+    against numpy's eigendecomposition."""
+    rng = np.random.default_rng(9)
+    p = np.array([0.7, 0.4])
+    fd = tad.ScalarFunction(2, 2, ctx)
+    fd.add_elements_device(plugin["smooth4"], [[0, 1]], p)
+    for trial in range(4):
+        x = rng.uniform(-1.2, 1.2, 4)
+        f, g, H = fd.eval_with_derivatives(x)
+        assert abs(f - smooth_energy(x, p)) <= 1e-14 * abs(f)
+        h = 1e-5
+        gfd = np.array([(smooth_energy(x + h * e, p) - smooth_energy(x - h * e, p)) / (2 * h) for e in np.eye(4)])
+        assert nrel(g, gfd) <= 1e-9
+        Hd = H.toarray()
+        Hfd = np.array([(fd.eval_with_gradient(x + h * e)[1] - fd.eval_with_gradient(x - h * e)[1]) / (2 * h)
+                        for e in np.eye(4)])
+        assert np.array_equal(Hd, Hd.T)
+        assert nrel(Hd, Hfd) <= 1e-8
+        for eps in [1e-9, -1.0, -0.01, 0.5]:
+            _, _, Hp = fd.eval_with_hessian_proj(x, eps)
+            assert nrel(Hp.toarray(), reference_projection(Hd, eps)) <= 1e-12, (trial, eps, np.linalg.eigvalsh(Hd))
+
+
+def test_device_mixed_precision(ctx, plugin):
+    P = configs.grid_param(16, seed=2)
+    f64 = device_function(P, plugin, ctx)
+    fmx = device_function(P, plugin, ctx, precision="mixed")
+    fa, ga, Ha = f64.eval_with_hessian_proj(P.x0)
+    fb, gb, Hb = fmx.eval_with_hessian_proj(P.x0)
+    assert fa == fb  # the value is the passive FP64 energy in both modes
+    assert nrel(gb, ga) <= 2e-5 and nrel(Hb.data, Ha.data) <= 2e-5
+
+
+def test_device_newton_matches_catalog(ctx, plugin):
+    """20 projected-Newton iterations (the 2D parametrization workload, smaller
+    grid) with the device functor and with the catalog reach the same energy."""
+    import torch
+    P = configs.grid_param(30, seed=1)
+    fc = tad.ScalarFunction.from_problem(P, ctx)
+    fd = device_function(P, plugin, ctx)
+    xc = torch.tensor(P.x0, device="cuda:0")
+    xd = xc.clone()
+    for _ in range(20):
+        sc = fc.newton_step(xc)
+        sd = fd.newton_step(xd)
+    fc_, fd_ = fc.eval(xc), fd.eval(xd)
+    print(f"catalog f {fc_:.15g} device functor f {fd_:.15g}")
+    assert abs(fd_ - fc_) <= 1e-8 * abs(fc_)
+    assert sd["solve_converged"] and sc["solve_converged"]
+
+
+def test_device_ops_validation(ctx, plugin):
+    fn = tad.ScalarFunction(3, 5, ctx)
+    with pytest.raises(tad.TadError):  # 2D energy on a 3D function
+        fn.add_elements_device(plugin["symdir_tri"], [[0, 1, 2]], np.zeros(5))
+    fn2 = tad.ScalarFunction(2, 5, ctx)
+    with pytest.raises(tad.TadError):  # parameter row of the wrong length
+        fn2.add_elements_device(plugin["symdir_tri"], [[0, 1, 2]], np.zeros(4))
+    with pytest.raises(tad.TadError):  # repeated handle
+        fn2.add_elements_device(plugin["symdir_tri"], [[0, 1, 1]], np.zeros(5))
+    bad = _lib.DeviceOps.from_buffer_copy(_lib.DeviceOps.from_address(plugin["symdir_tri"]))
+    bad.version = 999
+    with pytest.raises(tad.TadError, match="TAD_DEVICE_OPS_VERSION"):
+        fn2.add_elements_device(ctypes.addressof(bad), [[0, 1, 2]], np.zeros(5))

@@ -166,6 +166,33 @@ class ScalarFunction:
         self.n_elements += conn.shape[0]
         self._pattern = None
 
+    def add_elements_device(self, ops, conn, params=None):
+        """add_elements<N>(range, lambda) for an energy compiled on the device.
+
+        ops: address of a tad_device_ops (an int or ctypes pointer), e.g. the
+        return value of a function in the caller's hipcc-built library that
+        returns &ops for tinyad_amd::device_element<F, N, d, NP>()
+        (include/tinyad_amd/DeviceElements.hh).  conn: n_elem x N handles;
+        params: n_elem x NP rows (one row is broadcast)."""
+        addr = ctypes.cast(ops, ctypes.c_void_p).value if not isinstance(ops, int) else ops
+        if not addr:
+            raise TadError(1, "null device ops")
+        o = _lib.DeviceOps.from_address(addr)
+        conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1, o.valence)
+        n = conn.shape[0]
+        if o.n_params > 0:
+            params = np.ascontiguousarray(params, dtype=np.float64)
+            if params.ndim == 1:
+                params = np.ascontiguousarray(np.broadcast_to(params, (n, params.size)))
+            if params.shape != (n, o.n_params):
+                raise TadError(1, f"device energy expects {o.n_params} parameters per element")
+            pp = params.ctypes.data
+        else:
+            pp = None
+        check(lib().tad_fn_add_elements_device(self._h, addr, n, conn.ctypes.data, pp))
+        self.n_elements += n
+        self._pattern = None
+
     @classmethod
     def from_problem(cls, problem, ctx=None, precision="fp64"):
         f = cls(problem.d, problem.n_handles, ctx, precision)

@@ -22,7 +22,8 @@ EXPORTED = [
     "tad_last_error", "tad_version",
     "tad_ctx_create", "tad_ctx_destroy", "tad_ctx_set_stream", "tad_ctx_get_stream", "tad_ctx_synchronize",
     "tad_ctx_enable_timing", "tad_ctx_kernel_stats", "tad_ctx_reset_stats",
-    "tad_fn_create", "tad_fn_destroy", "tad_fn_add_elements", "tad_fn_n_vars", "tad_fn_n_elements",
+    "tad_fn_create", "tad_fn_destroy", "tad_fn_add_elements", "tad_fn_add_elements_device", "tad_fn_n_vars",
+    "tad_fn_n_elements",
     "tad_fn_pattern", "tad_fn_eval", "tad_fn_eval_with_gradient", "tad_fn_eval_with_derivatives",
     "tad_fn_eval_with_hessian_proj", "tad_fn_device_buffers",
     "tad_solve_default_opts", "tad_fn_solve", "tad_fn_solve_reduced", "tad_newton_default_opts", "tad_newton_step",
@@ -48,6 +49,12 @@ class SolveOpts(ctypes.Structure):
 class SolveStats(ctypes.Structure):
     _fields_ = [("iters", ctypes.c_int), ("converged", ctypes.c_int), ("rel_residual", ctypes.c_double),
                 ("coarse_unknowns", ctypes.c_int), ("coarse_reused", ctypes.c_int)]
+
+
+class DeviceOps(ctypes.Structure):
+    """tad_device_ops (read-only view; the function pointers are opaque)."""
+    _fields_ = [("version", ctypes.c_int), ("valence", ctypes.c_int), ("dim", ctypes.c_int),
+                ("n_params", ctypes.c_int), ("element", ctypes.c_void_p), ("energy", ctypes.c_void_p)]
 
 
 class NewtonOpts(ctypes.Structure):
@@ -103,6 +110,7 @@ def lib():
         "tad_fn_create": [vp, i32, i64, i32, pvp],
         "tad_fn_destroy": [vp],
         "tad_fn_add_elements": [vp, i32, i32, i64, vp, vp, i32],
+        "tad_fn_add_elements_device": [vp, vp, i64, vp, vp],
         "tad_fn_n_vars": [vp, pi64],
         "tad_fn_n_elements": [vp, pi64],
         "tad_fn_pattern": [vp, pi64, pvp, pvp, vp, vp],

@@ -16,7 +16,11 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(HERE, "libtinyad_amd.so")
 SOURCES = ["tad_element.hip", "tad_energy.hip", "tad_linalg.hip", "tad_adjoint.hip", "tad_api.cpp"]
-HEADERS = ["tad_dual.hpp", "tad_eigen.hpp", "tad_energies.hpp", "tad_elements.hpp", "tad_internal.hpp"]
+HEADERS = ["tad_energies.hpp", "tad_elements.hpp", "tad_internal.hpp"]
+# public headers the library's sources also compile (device code shared with
+# element kernels built from user energies)
+PUBLIC_HEADERS = ["tinyad_amd.h", "tinyad_amd/device/Dual.hpp", "tinyad_amd/device/Projection.hpp",
+                  "tinyad_amd/device/Scatter.hpp", "tinyad_amd/device/Modes.hpp", "tinyad_amd/device/Kernels.hpp"]
 ARCH = os.environ.get("TAD_OFFLOAD_ARCH", "gfx950")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
@@ -28,7 +32,8 @@ FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
 def _compile(src: str, force: bool) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
     srcp = os.path.join(CSRC, src)
-    deps = [srcp] + [os.path.join(CSRC, h) for h in HEADERS] + [os.path.join(ROOT, "include", "tinyad_amd.h")]
+    deps = ([srcp] + [os.path.join(CSRC, h) for h in HEADERS]
+            + [os.path.join(ROOT, "include", h) for h in PUBLIC_HEADERS])
     if not force and os.path.exists(obj) and os.path.getmtime(obj) >= max(os.path.getmtime(p) for p in deps):
         return obj
     lang = ["-x", "hip"] if src.endswith(".hip") else []
@@ -54,9 +59,32 @@ def build(force: bool = False, jobs: int = 4) -> str:
     return LIB
 
 
+def build_device_energy_libs(force: bool = False) -> list:
+    """Compiles the device-energy test plugins (tests/cpp/*.hip: energies as
+    generic functors through include/tinyad_amd/DeviceElements.hh) with hipcc
+    into tests/cpp/bin/libtad_test_<name>.so."""
+    src_dir = os.path.join(ROOT, "tests", "cpp")
+    out_dir = os.path.join(src_dir, "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    deps = [os.path.join(ROOT, "include", h) for h in PUBLIC_HEADERS + ["tinyad_amd/DeviceElements.hh"]]
+    outs = []
+    for src in sorted(f for f in os.listdir(src_dir) if f.endswith(".hip")):
+        srcp = os.path.join(src_dir, src)
+        lib = os.path.join(out_dir, "libtad_test_" + os.path.splitext(src)[0] + ".so")
+        if force or not os.path.exists(lib) or os.path.getmtime(lib) < max(os.path.getmtime(p) for p in deps + [srcp]):
+            cmd = ["hipcc"] + FLAGS + ["-shared", "-x", "hip", srcp, "-o", lib + ".tmp"]
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-8000:]}")
+            os.replace(lib + ".tmp", lib)
+        outs.append(lib)
+    return outs
+
+
 def build_cpp_tests() -> list:
     """Compiles the C++ front-end test programs (tests/cpp/*.cc) with g++
-    against include/ and the library; binaries go to tests/cpp/bin/."""
+    against include/, the library and the device-energy test plugin; binaries
+    go to tests/cpp/bin/."""
     src_dir = os.path.join(ROOT, "tests", "cpp")
     out_dir = os.path.join(src_dir, "bin")
     os.makedirs(out_dir, exist_ok=True)
@@ -64,7 +92,8 @@ def build_cpp_tests() -> list:
     for src in sorted(f for f in os.listdir(src_dir) if f.endswith(".cc")):
         exe = os.path.join(out_dir, os.path.splitext(src)[0])
         cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), os.path.join(src_dir, src),
-               "-L" + HERE, "-ltinyad_amd", "-Wl,-rpath,$ORIGIN/../../../tinyad_amd", "-o", exe]
+               "-L" + HERE, "-ltinyad_amd", "-L" + out_dir, "-ltad_test_device_energies",
+               "-Wl,-rpath,$ORIGIN/../../../tinyad_amd", "-Wl,-rpath,$ORIGIN", "-o", exe]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"g++ failed for {src}:\n{r.stderr[-4000:]}")
@@ -74,4 +103,5 @@ def build_cpp_tests() -> list:
 
 if __name__ == "__main__":
     print(build(force="--force" in sys.argv))
+    print(build_device_energy_libs(force="--force" in sys.argv))
     print(build_cpp_tests())

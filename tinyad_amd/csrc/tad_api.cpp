@@ -1158,6 +1158,64 @@ tad_status tad_fn_destroy(tad_fn fn)
     return TAD_OK;
 }
 
+// Validates and uploads one term (shared by the catalog and device-energy
+// entry points): handle range, no repeated handle, finite parameters.
+static tad_status add_term(tad_fn fn, int energy, int N, int64_t n_elem, const int32_t* conn, const double* params,
+                           int np, const tad_device_ops* ops)
+{
+    if (n_elem > 0 && (!conn || (np > 0 && !params)))
+        return fail(TAD_ERR_INVALID, "null connectivity or parameters");
+    for (int64_t e = 0; e < n_elem; ++e)
+        for (int a = 0; a < N; ++a)
+        {
+            const int32_t v = conn[e * N + a];
+            if (v < 0 || v >= fn->n_handles)
+                return fail(TAD_ERR_INVALID, "element " + std::to_string(e) + " references handle out of range");
+            for (int b = 0; b < a; ++b)
+                if (conn[e * N + b] == v)
+                    return fail(TAD_ERR_INVALID, "element " + std::to_string(e) + " references a handle twice");
+        }
+    for (int64_t i = 0; i < n_elem * (int64_t)np; ++i)
+        if (!std::isfinite(params[i])) return fail(TAD_ERR_INVALID, "non-finite element parameter");
+
+    TermDev t;
+    t.energy = energy;
+    if (ops) t.dev_ops = *ops;
+    t.N = N;
+    t.n_params = np;
+    t.n_elem = n_elem;
+    t.f_base = fn->n_elem_total;
+    if (n_elem > 0)
+    {
+        std::vector<int32_t> cs((size_t)n_elem * N);
+        for (int64_t e = 0; e < n_elem; ++e)
+            for (int a = 0; a < N; ++a) cs[a * n_elem + e] = conn[e * N + a];
+        std::vector<double> ps((size_t)n_elem * np);
+        for (int64_t e = 0; e < n_elem; ++e)
+            for (int k = 0; k < np; ++k) ps[k * n_elem + e] = params[e * np + k];
+        TAD_HIP(hipMalloc((void**)&t.conn, cs.size() * sizeof(int32_t)));
+        TAD_HIP(hipMemcpy(t.conn, cs.data(), cs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        if (!ps.empty())
+        {
+            TAD_HIP(hipMalloc((void**)&t.params, ps.size() * sizeof(double)));
+            TAD_HIP(hipMemcpy(t.params, ps.data(), ps.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+    }
+    HostTerm h;
+    h.energy = energy;
+    h.N = N;
+    h.n_params = np;
+    h.n_elem = n_elem;
+    h.conn.assign(conn, conn + n_elem * N);
+    fn->terms.push_back(t);
+    fn->hterms.push_back(std::move(h));
+    fn->n_elem_total += n_elem;
+    free_pattern(fn);
+    fn->h_valid = false;
+    fn->n_seg = 0;  // segment ranges must be declared again
+    return TAD_OK;
+}
+
 tad_status tad_fn_add_elements(tad_fn fn, tad_energy energy, int valence, int64_t n_elem, const int32_t* conn,
                                const double* params, int params_per_elem)
 {
@@ -1179,52 +1237,27 @@ tad_status tad_fn_add_elements(tad_fn fn, tad_energy energy, int valence, int64_
     if (np != params_per_elem)
         return fail(TAD_ERR_INVALID, "energy expects " + std::to_string(np) + " parameters per element, got " +
                                          std::to_string(params_per_elem));
-    if (n_elem > 0 && (!conn || !params)) return fail(TAD_ERR_INVALID, "null connectivity or parameters");
-    for (int64_t e = 0; e < n_elem; ++e)
-        for (int a = 0; a < N; ++a)
-        {
-            const int32_t v = conn[e * N + a];
-            if (v < 0 || v >= fn->n_handles)
-                return fail(TAD_ERR_INVALID, "element " + std::to_string(e) + " references handle out of range");
-            for (int b = 0; b < a; ++b)
-                if (conn[e * N + b] == v)
-                    return fail(TAD_ERR_INVALID, "element " + std::to_string(e) + " references a handle twice");
-        }
-    for (int64_t i = 0; i < n_elem * (int64_t)np; ++i)
-        if (!std::isfinite(params[i])) return fail(TAD_ERR_INVALID, "non-finite element parameter");
+    return add_term(fn, energy, N, n_elem, conn, params, np, nullptr);
+}
 
-    TermDev t;
-    t.energy = energy;
-    t.N = N;
-    t.n_params = np;
-    t.n_elem = n_elem;
-    t.f_base = fn->n_elem_total;
-    if (n_elem > 0)
-    {
-        std::vector<int32_t> cs((size_t)n_elem * N);
-        for (int64_t e = 0; e < n_elem; ++e)
-            for (int a = 0; a < N; ++a) cs[a * n_elem + e] = conn[e * N + a];
-        std::vector<double> ps((size_t)n_elem * np);
-        for (int64_t e = 0; e < n_elem; ++e)
-            for (int k = 0; k < np; ++k) ps[k * n_elem + e] = params[e * np + k];
-        TAD_HIP(hipMalloc((void**)&t.conn, cs.size() * sizeof(int32_t)));
-        TAD_HIP(hipMalloc((void**)&t.params, ps.size() * sizeof(double)));
-        TAD_HIP(hipMemcpy(t.conn, cs.data(), cs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-        TAD_HIP(hipMemcpy(t.params, ps.data(), ps.size() * sizeof(double), hipMemcpyHostToDevice));
-    }
-    HostTerm h;
-    h.energy = energy;
-    h.N = N;
-    h.n_params = np;
-    h.n_elem = n_elem;
-    h.conn.assign(conn, conn + n_elem * N);
-    fn->terms.push_back(t);
-    fn->hterms.push_back(std::move(h));
-    fn->n_elem_total += n_elem;
-    free_pattern(fn);
-    fn->h_valid = false;
-    fn->n_seg = 0;  // segment ranges must be declared again
-    return TAD_OK;
+tad_status tad_fn_add_elements_device(tad_fn fn, const tad_device_ops* ops, int64_t n_elem, const int32_t* conn,
+                                      const double* params)
+{
+    TAD_TRY(check_fn(fn));
+    if (!ops) return fail(TAD_ERR_INVALID, "null device ops");
+    if (ops->version != TAD_DEVICE_OPS_VERSION)
+        return fail(TAD_ERR_UNSUPPORTED, "device ops built against another TAD_DEVICE_OPS_VERSION (" +
+                                             std::to_string(ops->version) + ", library " +
+                                             std::to_string(TAD_DEVICE_OPS_VERSION) + ")");
+    if (!ops->element || !ops->energy) return fail(TAD_ERR_INVALID, "device ops without element or energy function");
+    if (ops->dim != fn->d)
+        return fail(TAD_ERR_INVALID, "device energy has dimension " + std::to_string(ops->dim) + ", function has " +
+                                         std::to_string(fn->d));
+    const int N = ops->valence;
+    if (N < 1 || fn->d * N > 12) return fail(TAD_ERR_INVALID, "element size d * valence must be in 1..12");
+    if (ops->n_params < 0) return fail(TAD_ERR_INVALID, "negative parameter count");
+    if (n_elem < 0) return fail(TAD_ERR_INVALID, "negative element count");
+    return add_term(fn, TAD_E_DEVICE, N, n_elem, conn, params, ops->n_params, ops);
 }
 
 tad_status tad_fn_n_vars(tad_fn fn, int64_t* n_vars)

@@ -8,54 +8,6 @@
 
 namespace tad {
 
-// ---------------------------------------------------------------------------
-// Element kernel
-// ---------------------------------------------------------------------------
-// Scatter targets of one term: contribution arrays and the term's slot maps.
-template <class T>
-struct ElemOut
-{
-    T* G;
-    T* Hc;
-    const uint32_t* gpos;
-    const uint32_t* hpos;
-};
-
-template <class E, class T, int MODE>
-__device__ __forceinline__ void element_body(int64_t e, int64_t n, const int32_t* __restrict__ conn,
-                                             const double* __restrict__ params, const double* __restrict__ x,
-                                             double eps, double* __restrict__ elem_f, const ElemOut<T>& o,
-                                             int* __restrict__ err)
-{
-    double X[E::N * E::D];
-#pragma unroll
-    for (int a = 0; a < E::N; ++a)
-    {
-        const int64_t v = conn[a * n + e];
-#pragma unroll
-        for (int i = 0; i < E::D; ++i) X[a * E::D + i] = x[v * E::D + i];
-    }
-    double p[E::NP];
-#pragma unroll
-    for (int k = 0; k < E::NP; ++k) p[k] = params[k * n + e];
-    bool nf = false;
-    const ScatterSink<E::N, E::D, T> out{o.G, o.Hc, o.gpos, o.hpos, e, n};
-    elem_f[e] = E::template eval<T, MODE>(X, p, T(eps), out, nf);
-    if (nf) *err = 1;
-}
-
-// One lane per element, one tile of blockDim elements per workgroup.
-template <class E, class T, int MODE>
-__global__ __launch_bounds__(kBlock) void k_element(int64_t n, const int32_t* __restrict__ conn,
-                                                    const double* __restrict__ params, const double* __restrict__ x,
-                                                    double eps, double* __restrict__ elem_f, ElemOut<T> o,
-                                                    int* __restrict__ err)
-{
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    element_body<E, T, MODE>(e, n, conn, params, x, eps, elem_f, o, err);
-}
-
 // Persistent variant: a fixed grid walks the tiles.  map_xcd = 1 gives each of
 // the 8 XCDs (workgroups are dispatched to XCDs round-robin by id) one
 // contiguous eighth of the tiles, so the vertex gathers of neighbouring
@@ -160,6 +112,26 @@ static hipError_t launch_element_p(const TermDev& t, int d, int mode, const doub
 hipError_t launch_element(const TermDev& t, int d, int precision, int mode, const double* x, double eps,
                           double* elem_f, void* G, void* Hc, int* err, hipStream_t s)
 {
+    if (t.dev_ops.element)
+    {
+        if (t.n_elem == 0) return hipSuccess;
+        tad_device_element_args a{};
+        a.n_elem = t.n_elem;
+        a.conn = t.conn;
+        a.params = t.params;
+        a.x = x;
+        a.eps = eps;
+        a.elem_f = elem_f + t.f_base;
+        a.G = G;
+        a.Hc = Hc;
+        a.gpos = t.gpos;
+        a.hpos = t.hpos;
+        a.err = err;
+        a.mode = mode;
+        a.precision = precision;
+        a.stream = (void*)s;
+        return (hipError_t)t.dev_ops.element(&a);
+    }
     if (precision == 0)
         return launch_element_p<double>(t, d, mode, x, eps, elem_f, (double*)G, (double*)Hc, err, s);
     return launch_element_p<float>(t, d, mode, x, eps, elem_f, (float*)G, (float*)Hc, err, s);

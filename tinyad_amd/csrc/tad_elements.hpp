@@ -5,10 +5,9 @@
 #include <cstdint>
 #include "tad_internal.hpp"
 #include "tad_energies.hpp"
+#include "tinyad_amd/device/Kernels.hpp"
 
 namespace tad {
-
-constexpr int kBlock = 256;
 
 // ---------------------------------------------------------------------------
 // Element types

@@ -7,50 +7,6 @@
 
 namespace tad {
 
-// ---------------------------------------------------------------------------
-// Energy kernel (line search)
-// ---------------------------------------------------------------------------
-constexpr int kMaxCand = 8;
-
-template <class E>
-__global__ __launch_bounds__(kBlock) void k_energy(int64_t n, const int32_t* __restrict__ conn,
-                                                   const double* __restrict__ params, const double* __restrict__ x,
-                                                   const double* __restrict__ dir, const double* __restrict__ steps,
-                                                   int ncand, double* __restrict__ out, int64_t stride)
-{
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    constexpr int NX = E::N * E::D;
-    double X[NX], DX[NX];
-#pragma unroll
-    for (int a = 0; a < E::N; ++a)
-    {
-        const int64_t v = conn[a * n + e];
-#pragma unroll
-        for (int i = 0; i < E::D; ++i)
-        {
-            X[a * E::D + i] = x[v * E::D + i];
-            DX[a * E::D + i] = dir ? dir[v * E::D + i] : 0.0;
-        }
-    }
-    double p[E::NP];
-#pragma unroll
-    for (int k = 0; k < E::NP; ++k) p[k] = params[k * n + e];
-    if (!dir)
-    {
-        out[e] = E::energy(X, p);
-        return;
-    }
-    for (int c = 0; c < ncand; ++c)
-    {
-        const double sc = steps[c];
-        double Xc[NX];
-#pragma unroll
-        for (int i = 0; i < NX; ++i) Xc[i] = fma(sc, DX[i], X[i]);
-        out[c * stride + e] = E::energy(Xc, p);
-    }
-}
-
 template <class E>
 static hipError_t launch_energy_t(const TermDev& t, const double* x, const double* dir, const double* steps, int ncand,
                                   double* elem_f, int64_t stride, hipStream_t s)
@@ -66,6 +22,22 @@ hipError_t launch_energy(const TermDev& t, int d, const double* x, const double*
                          double* elem_f, int64_t stride, hipStream_t s)
 {
     if (ncand < 1 || ncand > kMaxCand) return hipErrorInvalidValue;
+    if (t.dev_ops.energy)
+    {
+        if (t.n_elem == 0) return hipSuccess;
+        tad_device_energy_args a{};
+        a.n_elem = t.n_elem;
+        a.conn = t.conn;
+        a.params = t.params;
+        a.x = x;
+        a.dir = dir;
+        a.steps = steps;
+        a.ncand = ncand;
+        a.out = elem_f + t.f_base;
+        a.stride = stride;
+        a.stream = (void*)s;
+        return (hipError_t)t.dev_ops.energy(&a);
+    }
     switch (t.energy)
     {
         case E_SYMDIR_TRI: return launch_energy_t<SymdirElem<2>>(t, x, dir, steps, ncand, elem_f, stride, s);

@@ -4,15 +4,10 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include "tinyad_amd.h"
+#include "tinyad_amd/device/Modes.hpp"
 
 namespace tad {
-
-enum EvalMode
-{
-    MODE_GRAD = 1,      // value + gradient
-    MODE_HESS = 2,      // value + gradient + Hessian
-    MODE_HESS_PROJ = 3, // value + gradient + projected Hessian
-};
 
 struct TermDev
 {
@@ -33,6 +28,8 @@ struct TermDev
     int tune_block = 0;
     int tune_grid = 0;
     int tune_map = 0;
+    // caller-compiled element and energy kernels (energy == TAD_E_DEVICE)
+    tad_device_ops dev_ops{};
 };
 
 // Solver bookkeeping written by block 0 of the PCG kernels (see tad_linalg.hip).
