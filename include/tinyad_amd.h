@@ -193,6 +193,25 @@ typedef struct tad_device_ops
 tad_status tad_fn_add_elements_device(tad_fn fn, const tad_device_ops* ops, int64_t n_elem, const int32_t* conn,
                                       const double* params);
 
+/* add_elements_dynamic<V1, V2, ...>(range(n_elem), lambda)
+ * (ScalarFunctionImpl.hh:102-214): elements with different numbers of
+ * handles.  Element e accesses handles[elem_ptr[e] .. elem_ptr[e+1]) (host
+ * memory, access order); it joins the group of the smallest valence among
+ * ops[0..n_ops) that is >= its handle count, its unused trailing slots become
+ * padding (coordinates read as 0, no contributions), and one term per
+ * non-empty group is added in the order of ops.  ops: n_ops distinct
+ * valences of one energy (same dimension and parameter count); params:
+ * n_elem x n_params row-major.  Fails when an element has more handles than
+ * the largest valence. */
+tad_status tad_fn_add_elements_dynamic_device(tad_fn fn, const tad_device_ops* const* ops, int n_ops, int64_t n_elem,
+                                              const int64_t* elem_ptr, const int32_t* handles, const double* params);
+
+/* The objective terms (the reference's objective_terms, ScalarFunction.hh:229):
+ * their number, and per term its energy (TAD_E_DEVICE for device energies),
+ * valence and element count. */
+tad_status tad_fn_n_terms(tad_fn fn, int* n_terms);
+tad_status tad_fn_term_info(tad_fn fn, int term, int* energy, int* valence, int64_t* n_elem);
+
 tad_status tad_fn_n_vars(tad_fn fn, int64_t* n_vars);
 tad_status tad_fn_n_elements(tad_fn fn, int64_t* n_elements);
 

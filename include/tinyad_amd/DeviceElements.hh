@@ -16,7 +16,11 @@
 //
 // Inside the functor use +, -, *, / (with doubles or ints), comparisons and
 // tad::sqr, tad::sqrt, tad::exp, tad::log, tad::sin, tad::cos, tad::pow(S, double);
-// return S(INFINITY) (or INFINITY) for an infeasible element, which then
+// With add_elements_dynamic the same functor is compiled for several
+// valences; an element with fewer handles than its group's valence sees 0 in
+// the unused trailing coordinates (pass the handle count in p when the energy
+// must know it).
+// Return S(INFINITY) (or INFINITY) for an infeasible element, which then
 // contributes a zero gradient and, projected, eps I (as an infinite Double<k>
 // does in the reference).  The functor must be stateless (it is
 // default-constructed on the device); per-element data comes in p.
@@ -42,16 +46,24 @@ template <class F, int NN, int DD, int NPAR>
 struct DeviceElement
 {
     static constexpr int N = NN, D = DD, NP = NPAR, K = NN * DD;
+    // elements of dynamic-valence terms end in padding slots (handle -1):
+    // their coordinates read as 0 and their derivatives are not stored
+    static constexpr bool padded = true;
     static_assert(NN >= 1 && DD >= 1 && DD <= 3 && K <= 12, "valence * d must be in 1..12, d in 1..3");
     static_assert(NPAR >= 0, "negative parameter count");
 
+    // n_active: handles before the padding; the padded coordinates are
+    // passive constants, so their Hessian rows and columns are zero, as for
+    // the local variables a reference element never accesses
     template <class T, int MODE, class Sink>
-    __device__ static double eval(const double* X, const double* p, T eps, const Sink& out, bool& nonfinite)
+    __device__ static double eval(const double* X, const double* p, T eps, const Sink& out, bool& nonfinite,
+                                  int n_active)
     {
         const double f = F{}(X, p);
         tad::Dual<K, T> v[K];
 #pragma unroll
-        for (int i = 0; i < K; ++i) v[i] = tad::dual_var<K, T>(T(X[i]), i);
+        for (int i = 0; i < K; ++i)
+            v[i] = (i < n_active * DD) ? tad::dual_var<K, T>(T(X[i]), i) : tad::Dual<K, T>(X[i]);
         const tad::Dual<K, T> E = F{}(static_cast<const tad::Dual<K, T>*>(v), p);
         T g[K], Hp[tad::Sym<K>::n];
 #pragma unroll

@@ -200,6 +200,50 @@ public:
         n_elements += n;
     }
 
+    // add_elements_dynamic<V1, V2, ...>(range, lambda) (ScalarFunctionImpl.hh:102-214)
+    // for a device energy compiled at each valence of ops: handles_of(e)
+    // returns element e's handles (any container, at most the largest valence).
+    template <class Range, class HandlesFn, class ParamFn>
+    void add_elements_dynamic(const Range& element_range, const std::vector<tad_device_ops>& ops,
+                              HandlesFn handles_of, ParamFn params_of)
+    {
+        require_fn();
+        if (ops.empty()) throw std::runtime_error("[tinyad_amd] add_elements_dynamic: no valences");
+        std::vector<int64_t> ptr{ 0 };
+        std::vector<int32_t> handles;
+        std::vector<double> params;
+        for (auto e : element_range)
+        {
+            const auto h = handles_of(e);
+            for (auto v : h) handles.push_back((int32_t)v);
+            ptr.push_back((int64_t)handles.size());
+            const auto p = params_of(e);
+            if ((int)p.size() != ops[0].n_params)
+                throw std::runtime_error("[tinyad_amd] parameter row length differs from the device energy's");
+            for (int k = 0; k < ops[0].n_params; ++k) params.push_back(p[k]);
+        }
+        std::vector<const tad_device_ops*> op_ptrs;
+        for (const auto& o : ops) op_ptrs.push_back(&o);
+        const int64_t n = (int64_t)ptr.size() - 1;
+        check(tad_fn_add_elements_dynamic_device(fn_.get(), op_ptrs.data(), (int)op_ptrs.size(), n, ptr.data(),
+                                                 handles.data(), params.data()));
+        n_elements += n;
+    }
+
+    // objective terms: number, and (energy, valence, element count) per term
+    int n_terms() const
+    {
+        int n = 0;
+        check(tad_fn_n_terms(fn_.get(), &n));
+        return n;
+    }
+    int64_t term_elements(int term) const
+    {
+        int64_t n = 0;
+        check(tad_fn_term_info(fn_.get(), term, nullptr, nullptr, &n));
+        return n;
+    }
+
     // x_from_data (ScalarFunctionImpl.hh:216-235)
     VectorXd x_from_data(std::function<PassiveVariableVectorType(int)> read_user_data) const
     {

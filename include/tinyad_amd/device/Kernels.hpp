@@ -11,12 +11,25 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 #include "Scatter.hpp"
 
 namespace tad {
 
 constexpr int kBlock = 256;   // lanes per workgroup of the plain element grid
 constexpr int kMaxCand = 8;   // step sizes per energy launch
+
+// E::padded (optional, default false): elements may end in padding entries
+// (handle -1: coordinates read as 0, no slot), the elements of a
+// dynamic-valence term.
+template <class E, class = void>
+struct padded_slots : std::false_type
+{
+};
+template <class E>
+struct padded_slots<E, std::void_t<decltype(E::padded)>> : std::bool_constant<E::padded>
+{
+};
 
 // ---------------------------------------------------------------------------
 // Element kernel
@@ -38,19 +51,32 @@ __device__ __forceinline__ void element_body(int64_t e, int64_t n, const int32_t
                                              int* __restrict__ err)
 {
     double X[E::N * E::D];
+    int n_active = 0;  // handles before the padding (padded terms)
 #pragma unroll
     for (int a = 0; a < E::N; ++a)
     {
         const int64_t v = conn[a * n + e];
+        if constexpr (padded_slots<E>::value)
+        {
+            n_active += v >= 0 ? 1 : 0;
 #pragma unroll
-        for (int i = 0; i < E::D; ++i) X[a * E::D + i] = x[v * E::D + i];
+            for (int i = 0; i < E::D; ++i) X[a * E::D + i] = v >= 0 ? x[v * E::D + i] : 0.0;
+        }
+        else
+        {
+#pragma unroll
+            for (int i = 0; i < E::D; ++i) X[a * E::D + i] = x[v * E::D + i];
+        }
     }
     double p[E::NP > 0 ? E::NP : 1];
 #pragma unroll
     for (int k = 0; k < E::NP; ++k) p[k] = params[k * n + e];
     bool nf = false;
-    const ScatterSink<E::N, E::D, T> out{o.G, o.Hc, o.gpos, o.hpos, e, n};
-    elem_f[e] = E::template eval<T, MODE>(X, p, T(eps), out, nf);
+    const ScatterSink<E::N, E::D, T, padded_slots<E>::value> out{o.G, o.Hc, o.gpos, o.hpos, e, n};
+    if constexpr (padded_slots<E>::value)
+        elem_f[e] = E::template eval<T, MODE>(X, p, T(eps), out, nf, n_active);
+    else
+        elem_f[e] = E::template eval<T, MODE>(X, p, T(eps), out, nf);
     if (nf) *err = 1;
 }
 
@@ -83,11 +109,23 @@ __global__ __launch_bounds__(kBlock) void k_energy(int64_t n, const int32_t* __r
     for (int a = 0; a < E::N; ++a)
     {
         const int64_t v = conn[a * n + e];
-#pragma unroll
-        for (int i = 0; i < E::D; ++i)
+        if constexpr (padded_slots<E>::value)
         {
-            X[a * E::D + i] = x[v * E::D + i];
-            DX[a * E::D + i] = dir ? dir[v * E::D + i] : 0.0;
+#pragma unroll
+            for (int i = 0; i < E::D; ++i)
+            {
+                X[a * E::D + i] = v >= 0 ? x[v * E::D + i] : 0.0;
+                DX[a * E::D + i] = (dir && v >= 0) ? dir[v * E::D + i] : 0.0;
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int i = 0; i < E::D; ++i)
+            {
+                X[a * E::D + i] = x[v * E::D + i];
+                DX[a * E::D + i] = dir ? dir[v * E::D + i] : 0.0;
+            }
         }
     }
     double p[E::NP > 0 ? E::NP : 1];

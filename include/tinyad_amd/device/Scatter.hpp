@@ -58,7 +58,11 @@ __device__ __forceinline__ void store_run(float* __restrict__ dst, const float* 
 // (upper) CSR block.  Slots are sorted by destination (vertex, upper CSR
 // block) and, within a destination, by term and element, so the assembly sums
 // contiguous runs in the reference's element order.
-template <int N, int Dm, class T>
+// Slot of a padding entry (handle -1) of an element of a dynamic-valence
+// term: nothing is written for it.
+constexpr uint32_t kNoSlot = 0xffffffffu;
+
+template <int N, int Dm, class T, bool Padded = false>
 struct ScatterSink
 {
     T* __restrict__ G;
@@ -69,12 +73,17 @@ struct ScatterSink
 
     __device__ __forceinline__ void grad(int v, const T* g) const
     {
-        store_run<Dm>(G + (int64_t)gpos[v * n + e] * Dm, g);
+        const uint32_t pos = gpos[v * n + e];
+        if constexpr (Padded)
+            if (pos == kNoSlot) return;
+        store_run<Dm>(G + (int64_t)pos * Dm, g);
     }
     // block (v, u), v <= u, in the orientation of its slot
     __device__ __forceinline__ void block(int v, int u, const T* b) const
     {
         const uint32_t pos = hpos[block_index(N, v, u) * n + e];
+        if constexpr (Padded)
+            if (pos == kNoSlot) return;
         T* dst = Hc + (int64_t)(pos & 0x7fffffffu) * (Dm * Dm);
         if constexpr (sizeof(T) == 8)
         {

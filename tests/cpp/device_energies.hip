@@ -91,10 +91,55 @@ struct Smooth4
     }
 };
 
+// ||sum of the element's 2D handle values||^2, DynamicElementsTest.cc:9-33
+// (padding slots read as 0, so the sum runs over the real handles only).
+struct SumSquares2
+{
+    template <int N>
+    struct At
+    {
+        template <class S>
+        __device__ S operator()(const S* x, const double*) const
+        {
+            S sx = S(0.0), sy = S(0.0);
+            for (int a = 0; a < N; ++a)
+            {
+                sx += x[2 * a];
+                sy += x[2 * a + 1];
+            }
+            return tad::sqr(sx) + tad::sqr(sy);
+        }
+    };
+};
+
+// One-ring Dirichlet energy of a vertex (DynamicElementsTest.cc:93-131):
+// x[0] the vertex, x[1 .. count) its neighbours, p[0] = count (1D values).
+struct OneRing
+{
+    template <int N>
+    struct At
+    {
+        template <class S>
+        __device__ S operator()(const S* x, const double* p) const
+        {
+            S e = S(0.0);
+            for (int i = 1; i < N; ++i)
+                if (i < p[0]) e += 0.25 * tad::sqr(x[0] - x[i]);
+            return e;
+        }
+    };
+};
+
 const tad_device_ops g_tri = tinyad_amd::device_element<SymdirTri, 3, 2, 5>();
 const tad_device_ops g_tet = tinyad_amd::device_element<SymdirTet, 4, 3, 10>();
 const tad_device_ops g_quad = tinyad_amd::device_element<Quadratic4, 2, 2, 21>();
 const tad_device_ops g_smooth = tinyad_amd::device_element<Smooth4, 2, 2, 2>();
+const tad_device_ops g_sum_sq[2] = { tinyad_amd::device_element<SumSquares2::At<3>, 3, 2, 0>(),
+                                     tinyad_amd::device_element<SumSquares2::At<1>, 1, 2, 0>() };
+const tad_device_ops g_one_ring[4] = { tinyad_amd::device_element<OneRing::At<6>, 6, 1, 1>(),
+                                       tinyad_amd::device_element<OneRing::At<7>, 7, 1, 1>(),
+                                       tinyad_amd::device_element<OneRing::At<8>, 8, 1, 1>(),
+                                       tinyad_amd::device_element<OneRing::At<10>, 10, 1, 1>() };
 
 } // namespace
 
@@ -103,4 +148,8 @@ const tad_device_ops* tad_test_symdir_tri_ops() { return &g_tri; }
 const tad_device_ops* tad_test_symdir_tet_ops() { return &g_tet; }
 const tad_device_ops* tad_test_quadratic4_ops() { return &g_quad; }
 const tad_device_ops* tad_test_smooth4_ops() { return &g_smooth; }
+// valences 3 and 1, in that order (add_elements_dynamic<3, 1>)
+const tad_device_ops* tad_test_sum_squares_ops(int i) { return (i >= 0 && i < 2) ? &g_sum_sq[i] : nullptr; }
+// valences 6, 7, 8, 10 (add_elements_dynamic<6, 7, 8, 10>)
+const tad_device_ops* tad_test_one_ring_ops(int i) { return (i >= 0 && i < 4) ? &g_one_ring[i] : nullptr; }
 }

@@ -26,6 +26,8 @@ using tinyad_amd::VectorXd;
 
 // symmetric Dirichlet on triangles as a device functor (tests/cpp/device_energies.hip)
 extern "C" const tad_device_ops* tad_test_symdir_tri_ops();
+// ||sum of the element's handle values||^2 compiled at valences 3 (i = 0) and 1 (i = 1)
+extern "C" const tad_device_ops* tad_test_sum_squares_ops(int i);
 
 static std::vector<double> quad_params(const std::vector<double>& A, const std::vector<double>& b, double c)
 {
@@ -294,6 +296,31 @@ static bool test_reduced_basis()
     return ok;
 }
 
+// DynamicElementsTest.cc:9-33: element e accesses handles 0 .. e-1; valences
+// <3, 1> give two terms of two elements each
+static bool test_dynamic_elements()
+{
+    bool ok = true;
+    auto func = tinyad_amd::scalar_function<2>(tinyad_amd::range(4));
+    func.add_elements_dynamic(tinyad_amd::range(4), { *tad_test_sum_squares_ops(0), *tad_test_sum_squares_ops(1) },
+                              [](int e) { return tinyad_amd::range(e); }, [](int) { return std::vector<double>{}; });
+    EXPECT(func.n_terms() == 2);
+    EXPECT(func.term_elements(0) == 2 && func.term_elements(1) == 2);
+    VectorXd x(8, 1.0);
+    auto [f, g, H_proj] = func.eval_with_hessian_proj(x);
+    EXPECT(f == 28.0);
+    EXPECT(g[0] == 12.0 && g[2] == 10.0 && g[4] == 6.0 && g[6] == 0.0);
+    bool threw = false;
+    try
+    {
+        func.add_elements_dynamic(tinyad_amd::range(1), { *tad_test_sum_squares_ops(0) },
+                                  [](int) { return tinyad_amd::range(4); }, [](int) { return std::vector<double>{}; });
+    }
+    catch (const std::runtime_error&) { threw = true; }
+    EXPECT(threw && func.n_terms() == 2);
+    return ok;
+}
+
 int main()
 {
     struct Case { const char* name; bool (*fn)(); };
@@ -307,6 +334,7 @@ int main()
         { "Batching.SegmentsAutotuneParams", test_segments_autotune_params },
         { "Solver.TwoLevel", test_two_level_solve },
         { "SwitchSolverTest.ReducedBasis", test_reduced_basis },
+        { "DynamicElementsTest.DynamicElementsTest", test_dynamic_elements },
     };
     for (const Case& c : cases)
     {

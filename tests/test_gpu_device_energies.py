@@ -226,3 +226,133 @@ def test_device_ops_validation(ctx, plugin):
     bad.version = 999
     with pytest.raises(tad.TadError, match="TAD_DEVICE_OPS_VERSION"):
         fn2.add_elements_device(ctypes.addressof(bad), [[0, 1, 2]], np.zeros(5))
+
+
+# ---------------------------------------------------------------------------
+# add_elements_dynamic (ScalarFunctionImpl.hh:102-214, DynamicElementsTest.cc)
+# ---------------------------------------------------------------------------
+
+def dyn_ops(plugin, name, count):
+    f = getattr(plugin["_lib"], f"tad_test_{name}_ops")
+    f.restype = ctypes.c_void_p
+    f.argtypes = [ctypes.c_int]
+    return [f(i) for i in range(count)]
+
+
+def test_dynamic_elements_grouping(ctx, plugin):
+    """DynamicElementsTest.cc:9-33: element e of 4 accesses handles 0..e-1 of a
+    2D function; valences <3, 1> give two terms of two elements each.  With
+    x = 1: f = sum_e ||sum_{v<e} x_v||^2 = 0 + 2 + 8 + 18, and the derivatives
+    in closed form."""
+    fn = tad.ScalarFunction(2, 4, ctx)
+    fn.add_elements_dynamic_device(dyn_ops(plugin, "sum_squares", 2), [list(range(e)) for e in range(4)])
+    assert fn.terms() == [(tad.DEVICE, 3, 2), (tad.DEVICE, 1, 2)]
+    x = np.ones(8)
+    f, g, H = fn.eval_with_derivatives(x)
+    assert f == 28.0
+    assert np.array_equal(g, np.repeat([12.0, 10.0, 6.0, 0.0], 2))
+    Href = np.zeros((8, 8))
+    for v in range(4):
+        for w in range(4):
+            Href[2 * v:2 * v + 2, 2 * w:2 * w + 2] = 2.0 * sum(1 for e in range(4) if e > max(v, w)) * np.eye(2)
+    assert np.array_equal(H.toarray(), Href)
+    assert H.indptr[-1] - H.indptr[-3] == 0  # handle 3 is used by no element: empty rows
+    # projected: each element's K x K Hessian (K = group valence * 2) has zero
+    # rows and columns for the slots it does not access, as in the reference,
+    # and is projected as a whole; only the accessed blocks are kept
+    fp, gp, Hp = fn.eval_with_hessian_proj(x)
+    assert fp == f and np.array_equal(gp, g)
+    Hpref = np.zeros((8, 8))
+    for e in range(4):
+        Vs = 3 if e > 1 else 1
+        He = np.zeros((2 * Vs, 2 * Vs))
+        He[:2 * e, :2 * e] = 2.0 * np.kron(np.ones((e, e)), np.eye(2))
+        Pe = reference_projection(He, 1e-9)
+        Hpref[:2 * e, :2 * e] += Pe[:2 * e, :2 * e]
+    assert nrel(Hp.toarray(), Hpref) <= 1e-12
+
+
+def bunny_one_ring():
+    import json
+    d = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "bunny_closed.json")))
+    V = np.array(d["V"], dtype=np.float64)
+    F = np.array(d["F"], dtype=np.int32)
+    nbrs = [[] for _ in range(V.shape[0])]
+    for f in F:
+        for i in range(3):
+            nbrs[f[i]].append(int(f[(i + 1) % 3]))
+    return V, F, nbrs
+
+
+def reference_laplace(n, F):
+    """DynamicElementsTest.cc:37-56: L(v1, v1) += 1, L(v1, v2) -= 1 per directed face edge."""
+    import scipy.sparse as sp
+    rows, cols, vals = [], [], []
+    for f in F:
+        for i in range(3):
+            v1, v2 = int(f[i]), int(f[(i + 1) % 3])
+            rows += [v1, v1]
+            cols += [v1, v2]
+            vals += [1.0, -1.0]
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+
+
+def test_dynamic_one_ring_laplacian(ctx, plugin):
+    """DynamicElementsTest.cc:58-135: the Hessian of the one-ring Dirichlet
+    energy with elements of valence 6, 7, 8 and 10 (the bunny's vertex
+    valences plus one, padded up) equals the cotangent-free Laplacian of its
+    faces; the pattern holds extra explicit zeros (neighbour-neighbour pairs),
+    as the reference notes."""
+    V, F, nbrs = bunny_one_ring()
+    n = V.shape[0]
+    counts = np.array([1 + len(nb) for nb in nbrs], dtype=np.float64)
+    assert counts.max() <= 10
+    fn = tad.ScalarFunction(1, n, ctx)
+    fn.add_elements_dynamic_device(dyn_ops(plugin, "one_ring", 4), [[v] + nbrs[v] for v in range(n)],
+                                   counts[:, None])
+    got = {va: ne for _, va, ne in fn.terms()}
+    hist = {}
+    for c in counts.astype(int):
+        g = min(va for va in (6, 7, 8, 10) if va >= c)
+        hist[g] = hist.get(g, 0) + 1
+    assert got == hist
+    L = fn.eval_hessian_of_quadratic()
+    Lref = reference_laplace(n, F)
+    assert L.shape == Lref.shape and L.nnz >= Lref.nnz
+    assert abs(L - Lref).max() <= 1e-12
+
+
+def test_dynamic_newton_step(ctx, plugin):
+    """A Newton step on the one-ring energy plus a pin (catalog penalty)
+    reaches the minimum (every value at the pin's target): exercises the
+    padded energy kernel of the line search and the solve."""
+    import torch
+    V, F, nbrs = bunny_one_ring()
+    n = V.shape[0]
+    counts = np.array([1 + len(nb) for nb in nbrs], dtype=np.float64)
+    fn = tad.ScalarFunction(1, n, ctx)
+    fn.add_elements_dynamic_device(dyn_ops(plugin, "one_ring", 4), [[v] + nbrs[v] for v in range(n)],
+                                   counts[:, None])
+    fn.add_elements(1, tad.PENALTY, [[0]], [0.3, 1.0])
+    x = torch.tensor(np.random.default_rng(5).standard_normal(n), device="cuda:0")
+    # unprojected Hessian (projection_eps = NaN): the exact Newton step of a quadratic
+    st = fn.newton_step(x, tad.newton_default_opts(projection_eps=float("nan"), solve_tol=1e-13))
+    assert st["step"] == 1.0 and st["ls_failed"] == 0 and st["f"] > 1.0
+    print(f"one-ring Newton: f {st['f']:.6g} -> {st['f_new']:.3g}, max |x - 0.3| "
+          f"{float((x - 0.3).abs().max()):.2e}")
+    assert torch.allclose(x, torch.full_like(x, 0.3), atol=1e-6, rtol=0)
+    assert fn.eval(x) <= 1e-10
+
+
+def test_dynamic_validation(ctx, plugin):
+    ops = dyn_ops(plugin, "sum_squares", 2)
+    fn = tad.ScalarFunction(2, 6, ctx)
+    with pytest.raises(tad.TadError, match="exceeds"):
+        fn.add_elements_dynamic_device(ops, [[0, 1], [0, 1, 2, 3]])
+    with pytest.raises(tad.TadError, match="unique"):
+        fn.add_elements_dynamic_device([ops[0], ops[0]], [[0, 1]])
+    with pytest.raises(tad.TadError, match="out of range"):
+        fn.add_elements_dynamic_device(ops, [[0, -1]])
+    with pytest.raises(tad.TadError, match="twice"):
+        fn.add_elements_dynamic_device(ops, [[1], [2, 2]])
+    assert fn.terms() == []  # failed calls add nothing

@@ -27,6 +27,7 @@ SYMDIR_TRI = 1
 SYMDIR_TET = 2
 PENALTY = 3
 QUADRATIC = 4
+DEVICE = 100  # terms added from device energies
 
 DEFAULT_HESSIAN_PROJECTION_EPS = 1e-9  # HessianProjection.hh:16
 
@@ -192,6 +193,51 @@ class ScalarFunction:
         check(lib().tad_fn_add_elements_device(self._h, addr, n, conn.ctypes.data, pp))
         self.n_elements += n
         self._pattern = None
+
+    def add_elements_dynamic_device(self, ops_list, elements, params=None):
+        """add_elements_dynamic<V1, V2, ...>(range, lambda) (ScalarFunctionImpl.hh:102-214)
+        for a device energy compiled at several valences.
+
+        ops_list: tad_device_ops addresses, one per valence; elements: a
+        sequence of handle lists (any lengths up to the largest valence);
+        params: n_elem x NP rows (one row is broadcast).  Each element joins the
+        smallest valence that holds it; one term per non-empty group."""
+        addrs = [ctypes.cast(o, ctypes.c_void_p).value if not isinstance(o, int) else o for o in ops_list]
+        if not addrs or not all(addrs):
+            raise TadError(1, "null device ops")
+        np_ = _lib.DeviceOps.from_address(addrs[0]).n_params
+        lens = np.array([len(e) for e in elements], dtype=np.int64)
+        ptr = np.zeros(lens.size + 1, dtype=np.int64)
+        np.cumsum(lens, out=ptr[1:])
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(e, dtype=np.int32).ravel() for e in elements])
+                                    if lens.sum() else np.zeros(1, dtype=np.int32), dtype=np.int32)
+        n = lens.size
+        if np_ > 0:
+            params = np.ascontiguousarray(params, dtype=np.float64)
+            if params.ndim == 1:
+                params = np.ascontiguousarray(np.broadcast_to(params, (n, params.size)))
+            if params.shape != (n, np_):
+                raise TadError(1, f"device energy expects {np_} parameters per element")
+            pp = params.ctypes.data
+        else:
+            pp = None
+        arr = (ctypes.c_void_p * len(addrs))(*addrs)
+        check(lib().tad_fn_add_elements_dynamic_device(self._h, arr, len(addrs), n, ptr.ctypes.data,
+                                                       flat.ctypes.data, pp))
+        self.n_elements += n
+        self._pattern = None
+
+    def terms(self):
+        """(energy, valence, n_elements) per objective term, in insertion order
+        (the reference's objective_terms, ScalarFunction.hh:229)."""
+        n = ctypes.c_int()
+        check(lib().tad_fn_n_terms(self._h, ctypes.byref(n)))
+        out = []
+        for t in range(n.value):
+            en, va, ne = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()
+            check(lib().tad_fn_term_info(self._h, t, ctypes.byref(en), ctypes.byref(va), ctypes.byref(ne)))
+            out.append((en.value, va.value, ne.value))
+        return out
 
     @classmethod
     def from_problem(cls, problem, ctx=None, precision="fp64"):

@@ -22,7 +22,8 @@ EXPORTED = [
     "tad_last_error", "tad_version",
     "tad_ctx_create", "tad_ctx_destroy", "tad_ctx_set_stream", "tad_ctx_get_stream", "tad_ctx_synchronize",
     "tad_ctx_enable_timing", "tad_ctx_kernel_stats", "tad_ctx_reset_stats",
-    "tad_fn_create", "tad_fn_destroy", "tad_fn_add_elements", "tad_fn_add_elements_device", "tad_fn_n_vars",
+    "tad_fn_create", "tad_fn_destroy", "tad_fn_add_elements", "tad_fn_add_elements_device",
+    "tad_fn_add_elements_dynamic_device", "tad_fn_n_terms", "tad_fn_term_info", "tad_fn_n_vars",
     "tad_fn_n_elements",
     "tad_fn_pattern", "tad_fn_eval", "tad_fn_eval_with_gradient", "tad_fn_eval_with_derivatives",
     "tad_fn_eval_with_hessian_proj", "tad_fn_device_buffers",
@@ -111,6 +112,9 @@ def lib():
         "tad_fn_destroy": [vp],
         "tad_fn_add_elements": [vp, i32, i32, i64, vp, vp, i32],
         "tad_fn_add_elements_device": [vp, vp, i64, vp, vp],
+        "tad_fn_add_elements_dynamic_device": [vp, vp, i32, i64, vp, vp, vp],
+        "tad_fn_n_terms": [vp, vp],
+        "tad_fn_term_info": [vp, i32, vp, vp, vp],
         "tad_fn_n_vars": [vp, pi64],
         "tad_fn_n_elements": [vp, pi64],
         "tad_fn_pattern": [vp, pi64, pvp, pvp, vp, vp],

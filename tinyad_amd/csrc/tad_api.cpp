@@ -55,6 +55,8 @@ tad_status fail(tad_status st, const std::string& msg)
         if (s_ != TAD_OK) return s_;                                                               \
     } while (0)
 
+constexpr uint32_t kNoSlotHost = 0xffffffffu;  // slot of a padding entry (tad::kNoSlot)
+
 int energy_param_count(int energy, int d, int N)
 {
     const int K = d * N;
@@ -329,7 +331,8 @@ tad_status build_pattern(tad_fn fn)
     std::vector<int64_t> cnt(nh + 1, 0);
     for (const HostTerm& t : fn->hterms)
         for (int64_t e = 0; e < t.n_elem; ++e)
-            for (int a = 0; a < t.N; ++a) cnt[t.conn[e * t.N + a] + 1] += t.N;
+            for (int a = 0; a < t.N; ++a)
+                if (t.conn[e * t.N + a] >= 0) cnt[t.conn[e * t.N + a] + 1] += t.N;
     for (int64_t u = 0; u < nh; ++u) cnt[u + 1] += cnt[u];
     std::vector<int32_t> adj(cnt[nh]);
     {
@@ -339,6 +342,7 @@ tad_status build_pattern(tad_fn fn)
                 for (int a = 0; a < t.N; ++a)
                 {
                     const int32_t u = t.conn[e * t.N + a];
+                    if (u < 0) continue;
                     for (int b = 0; b < t.N; ++b) adj[cur[u]++] = t.conn[e * t.N + b];
                 }
     }
@@ -351,6 +355,7 @@ tad_status build_pattern(tad_fn fn)
         auto b = adj.begin() + cnt[u], e = adj.begin() + cnt[u + 1];
         std::sort(b, e);
         e = std::unique(b, e);
+        while (b != e && *b < 0) ++b;  // padding entries of dynamic-valence elements
         for (auto it = b; it != e; ++it) bcol.push_back(*it);
         brow_ptr[u + 1] = (int32_t)bcol.size();
         for (int32_t k = brow_ptr[u]; k < brow_ptr[u + 1]; ++k)
@@ -383,15 +388,18 @@ tad_status build_pattern(tad_fn fn)
             for (int a = 0; a < t.N; ++a)
             {
                 const int32_t u = t.conn[e * t.N + a];
+                if (u < 0) continue;
                 gptr[u + 1] += 1;
                 for (int b = a; b < t.N; ++b)
                 {
                     const int32_t v = t.conn[e * t.N + b];
+                    if (v < 0) continue;
                     hptr[find_block(std::min(u, v), std::max(u, v)) + 1] += 1;
                 }
             }
     for (int64_t k = 0; k < nb; ++k) hptr[k + 1] += hptr[k];
     for (int64_t u = 0; u < nh; ++u) gptr[u + 1] += gptr[u];
+    // slot 2^31 - 1 (with the transpose flag: kNoSlotHost) stays unused
     if (hptr[nb] >= 2147483647LL || gptr[nh] >= 4294967295LL)
         return fail(TAD_ERR_UNSUPPORTED, "more than 2^31 element contributions");
     std::vector<int32_t> mirror(nb);
@@ -412,11 +420,16 @@ tad_status build_pattern(tad_fn fn)
                 for (int a = 0; a < t.N; ++a)
                 {
                     const int32_t u = t.conn[e * t.N + a];
-                    gpos[ti][a * ne + e] = (uint32_t)gc[u]++;
+                    gpos[ti][a * ne + e] = u >= 0 ? (uint32_t)gc[u]++ : kNoSlotHost;
                     for (int b = a; b < t.N; ++b)
                     {
                         const int32_t v = t.conn[e * t.N + b];
                         const int bi = a * t.N - (a * (a - 1)) / 2 + (b - a);
+                        if (u < 0 || v < 0)
+                        {
+                            hpos[ti][(size_t)bi * ne + e] = kNoSlotHost;
+                            continue;
+                        }
                         const uint32_t flag = (u > v) ? 0x80000000u : 0u;
                         hpos[ti][(size_t)bi * ne + e] =
                             (uint32_t)hc[find_block(std::min(u, v), std::max(u, v))]++ | flag;
@@ -1160,8 +1173,9 @@ tad_status tad_fn_destroy(tad_fn fn)
 
 // Validates and uploads one term (shared by the catalog and device-energy
 // entry points): handle range, no repeated handle, finite parameters.
+// allow_pad: elements may end in padding entries -1 (dynamic valences).
 static tad_status add_term(tad_fn fn, int energy, int N, int64_t n_elem, const int32_t* conn, const double* params,
-                           int np, const tad_device_ops* ops)
+                           int np, const tad_device_ops* ops, bool allow_pad = false)
 {
     if (n_elem > 0 && (!conn || (np > 0 && !params)))
         return fail(TAD_ERR_INVALID, "null connectivity or parameters");
@@ -1169,6 +1183,7 @@ static tad_status add_term(tad_fn fn, int energy, int N, int64_t n_elem, const i
         for (int a = 0; a < N; ++a)
         {
             const int32_t v = conn[e * N + a];
+            if (v == -1 && allow_pad && (a + 1 == N || conn[e * N + a + 1] == -1)) continue;
             if (v < 0 || v >= fn->n_handles)
                 return fail(TAD_ERR_INVALID, "element " + std::to_string(e) + " references handle out of range");
             for (int b = 0; b < a; ++b)
@@ -1240,10 +1255,8 @@ tad_status tad_fn_add_elements(tad_fn fn, tad_energy energy, int valence, int64_
     return add_term(fn, energy, N, n_elem, conn, params, np, nullptr);
 }
 
-tad_status tad_fn_add_elements_device(tad_fn fn, const tad_device_ops* ops, int64_t n_elem, const int32_t* conn,
-                                      const double* params)
+static tad_status check_device_ops(tad_fn fn, const tad_device_ops* ops)
 {
-    TAD_TRY(check_fn(fn));
     if (!ops) return fail(TAD_ERR_INVALID, "null device ops");
     if (ops->version != TAD_DEVICE_OPS_VERSION)
         return fail(TAD_ERR_UNSUPPORTED, "device ops built against another TAD_DEVICE_OPS_VERSION (" +
@@ -1253,11 +1266,104 @@ tad_status tad_fn_add_elements_device(tad_fn fn, const tad_device_ops* ops, int6
     if (ops->dim != fn->d)
         return fail(TAD_ERR_INVALID, "device energy has dimension " + std::to_string(ops->dim) + ", function has " +
                                          std::to_string(fn->d));
-    const int N = ops->valence;
-    if (N < 1 || fn->d * N > 12) return fail(TAD_ERR_INVALID, "element size d * valence must be in 1..12");
+    if (ops->valence < 1 || fn->d * ops->valence > 12)
+        return fail(TAD_ERR_INVALID, "element size d * valence must be in 1..12");
     if (ops->n_params < 0) return fail(TAD_ERR_INVALID, "negative parameter count");
+    return TAD_OK;
+}
+
+tad_status tad_fn_add_elements_dynamic_device(tad_fn fn, const tad_device_ops* const* ops, int n_ops, int64_t n_elem,
+                                              const int64_t* elem_ptr, const int32_t* handles, const double* params)
+{
+    TAD_TRY(check_fn(fn));
+    if (!ops || n_ops < 1) return fail(TAD_ERR_INVALID, "no device ops");
     if (n_elem < 0) return fail(TAD_ERR_INVALID, "negative element count");
-    return add_term(fn, TAD_E_DEVICE, N, n_elem, conn, params, ops->n_params, ops);
+    for (int i = 0; i < n_ops; ++i)
+    {
+        TAD_TRY(check_device_ops(fn, ops[i]));
+        if (ops[i]->n_params != ops[0]->n_params)
+            return fail(TAD_ERR_INVALID, "device ops of one dynamic term differ in their parameter count");
+        for (int j = 0; j < i; ++j)
+            if (ops[j]->valence == ops[i]->valence)
+                return fail(TAD_ERR_INVALID, "element valences of add_elements_dynamic are not unique");
+    }
+    if (n_elem > 0 && !elem_ptr) return fail(TAD_ERR_INVALID, "null element offsets");
+    const int np = ops[0]->n_params;
+    // group by the smallest valence that holds the element (reference:
+    // lower_bound over the sorted static valences)
+    std::vector<int> group(n_elem);
+    for (int64_t e = 0; e < n_elem; ++e)
+    {
+        const int64_t c = elem_ptr[e + 1] - elem_ptr[e];
+        if (c < 0 || elem_ptr[e] < 0) return fail(TAD_ERR_INVALID, "element offsets must be non-decreasing");
+        int best = -1;
+        for (int i = 0; i < n_ops; ++i)
+            if (ops[i]->valence >= c && (best < 0 || ops[i]->valence < ops[best]->valence)) best = i;
+        if (best < 0)
+            return fail(TAD_ERR_INVALID, "element valence " + std::to_string(c) +
+                                             " exceeds the largest valence passed to add_elements_dynamic");
+        group[e] = best;
+    }
+    if (n_elem > 0 && elem_ptr[n_elem] > 0 && !handles) return fail(TAD_ERR_INVALID, "null handles");
+    if (n_elem > 0 && np > 0 && !params) return fail(TAD_ERR_INVALID, "null parameters");
+    // validate everything before the first term is added
+    for (int64_t e = 0; e < n_elem; ++e)
+        for (int64_t a = elem_ptr[e]; a < elem_ptr[e + 1]; ++a)
+        {
+            if (handles[a] < 0 || handles[a] >= fn->n_handles)
+                return fail(TAD_ERR_INVALID, "element " + std::to_string(e) + " references handle out of range");
+            for (int64_t b = elem_ptr[e]; b < a; ++b)
+                if (handles[b] == handles[a])
+                    return fail(TAD_ERR_INVALID, "element " + std::to_string(e) + " references a handle twice");
+        }
+    for (int64_t i = 0; i < n_elem * (int64_t)np; ++i)
+        if (!std::isfinite(params[i])) return fail(TAD_ERR_INVALID, "non-finite element parameter");
+    for (int i = 0; i < n_ops; ++i)
+    {
+        const int N = ops[i]->valence;
+        std::vector<int32_t> conn;
+        std::vector<double> ps;
+        int64_t ng = 0;
+        for (int64_t e = 0; e < n_elem; ++e)
+        {
+            if (group[e] != i) continue;
+            const int64_t c = elem_ptr[e + 1] - elem_ptr[e];
+            for (int a = 0; a < N; ++a) conn.push_back(a < c ? handles[elem_ptr[e] + a] : -1);
+            for (int k = 0; k < np; ++k) ps.push_back(params[e * np + k]);
+            ++ng;
+        }
+        if (ng == 0) continue;
+        TAD_TRY(add_term(fn, TAD_E_DEVICE, N, ng, conn.data(), ps.data(), np, ops[i], true));
+    }
+    return TAD_OK;
+}
+
+tad_status tad_fn_add_elements_device(tad_fn fn, const tad_device_ops* ops, int64_t n_elem, const int32_t* conn,
+                                      const double* params)
+{
+    TAD_TRY(check_fn(fn));
+    TAD_TRY(check_device_ops(fn, ops));
+    if (n_elem < 0) return fail(TAD_ERR_INVALID, "negative element count");
+    return add_term(fn, TAD_E_DEVICE, ops->valence, n_elem, conn, params, ops->n_params, ops);
+}
+
+tad_status tad_fn_n_terms(tad_fn fn, int* n_terms)
+{
+    TAD_TRY(check_fn(fn));
+    if (!n_terms) return fail(TAD_ERR_INVALID, "null argument");
+    *n_terms = (int)fn->hterms.size();
+    return TAD_OK;
+}
+
+tad_status tad_fn_term_info(tad_fn fn, int term, int* energy, int* valence, int64_t* n_elem)
+{
+    TAD_TRY(check_fn(fn));
+    if (term < 0 || term >= (int)fn->hterms.size()) return fail(TAD_ERR_INVALID, "term index out of range");
+    const HostTerm& t = fn->hterms[term];
+    if (energy) *energy = t.energy;
+    if (valence) *valence = t.N;
+    if (n_elem) *n_elem = t.n_elem;
+    return TAD_OK;
 }
 
 tad_status tad_fn_n_vars(tad_fn fn, int64_t* n_vars)
@@ -1559,9 +1665,12 @@ tad_status tad_fn_set_segments(tad_fn fn, int64_t n_segments, const int64_t* off
         int64_t prev = -1;
         for (int64_t e = 0; e < t.n_elem; ++e)
         {
-            const int64_t sg = seg_of(t.conn[e * t.N]);
+            // padding entries (-1, dynamic-valence terms) belong to no
+            // segment; an element without handles stays in the current one
+            const int32_t* ce = t.conn.data() + e * t.N;
+            const int64_t sg = ce[0] >= 0 ? seg_of(ce[0]) : std::max<int64_t>(prev, 0);
             for (int a = 1; a < t.N; ++a)
-                if (seg_of(t.conn[e * t.N + a]) != sg)
+                if (ce[a] >= 0 && seg_of(ce[a]) != sg)
                     return fail(TAD_ERR_INVALID, "element " + std::to_string(e) + " spans two segments");
             if (sg < prev) return fail(TAD_ERR_INVALID, "elements of a term must be ordered by segment");
             if (sg != prev)
