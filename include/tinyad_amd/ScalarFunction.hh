@@ -31,6 +31,8 @@
 #include <cstring>
 #include <string>
 #include <tuple>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace tinyad_amd {
@@ -52,6 +54,37 @@ inline std::vector<int> range(int n)
     std::vector<int> r(n);
     for (int i = 0; i < n; ++i) r[i] = i;
     return r;
+}
+
+// Handle types (Support/Common.hh, Support/*.hh): functions are built over a
+// range of variable handles, and every handle maps to an index through
+// idx_from_handle().  Integers map to themselves; handles of OpenMesh and PMP
+// (h.idx()), geometry-central (h.getIndex()) and polymesh (h.idx.value) are
+// recognised by those members, so no mesh-library header is needed here.
+// For another handle type, overload idx_from_handle(const MyHandle&) in
+// namespace tinyad_amd before including this header.
+namespace detail {
+template <class H, class = void> struct has_idx_fn : std::false_type {};
+template <class H> struct has_idx_fn<H, std::void_t<decltype(std::declval<const H&>().idx() + 0)>> : std::true_type {};
+template <class H, class = void> struct has_get_index : std::false_type {};
+template <class H> struct has_get_index<H, std::void_t<decltype(std::declval<const H&>().getIndex() + 0)>> : std::true_type {};
+template <class H, class = void> struct has_idx_value : std::false_type {};
+template <class H> struct has_idx_value<H, std::void_t<decltype(std::declval<const H&>().idx.value + 0)>> : std::true_type {};
+} // namespace detail
+
+template <class H>
+int64_t idx_from_handle(const H& h)
+{
+    if constexpr (std::is_integral<H>::value) return (int64_t)h;
+    else if constexpr (detail::has_idx_fn<H>::value) return (int64_t)h.idx();
+    else if constexpr (detail::has_get_index<H>::value) return (int64_t)h.getIndex();
+    else if constexpr (detail::has_idx_value<H>::value) return (int64_t)h.idx.value;
+    else
+    {
+        static_assert(sizeof(H) == 0, "Handle type not supported: overload tinyad_amd::idx_from_handle() for it "
+                                      "(TinyAD Support/Common.hh)");
+        return -1;
+    }
 }
 
 // CSR matrix (row-major).  For the symmetric Hessians returned here it equals
@@ -114,8 +147,9 @@ private:
     tad_ctx h_ = nullptr;
 };
 
-// ScalarFunction<d> (ScalarFunction.hh:36-236).  Movable, not copyable.
-template <int variable_dimension>
+// ScalarFunction<d, double, VariableHandleT> (ScalarFunction.hh:36-236).
+// Movable, not copyable.
+template <int variable_dimension, class VariableHandleT = int>
 class ScalarFunction
 {
     static_assert(variable_dimension >= 1 && variable_dimension <= 3, "variable dimension must be 1, 2 or 3");
@@ -125,14 +159,16 @@ public:
     using PassiveVariableVectorType = std::array<double, d>;
 
     ScalarFunction() = default;
-    ScalarFunction(std::vector<int> variable_handles, const EvalSettings& settings_ = {})
+    ScalarFunction(std::vector<VariableHandleT> variable_handles, const EvalSettings& settings_ = {})
         : settings(settings_), variable_handles(std::move(variable_handles))
     {
-        // handle indices must be a permutation of 0..n-1 (Element.hh:176-194)
+        // handle indices must be a permutation of 0..n-1 (variable_indices_compact,
+        // ScalarFunctionImpl.hh:47-61, Element.hh:176-194)
         std::vector<char> used(this->variable_handles.size(), 0);
-        for (int v : this->variable_handles)
+        for (const VariableHandleT& h : this->variable_handles)
         {
-            if (v < 0 || v >= (int)used.size() || used[v])
+            const int64_t v = idx_from_handle(h);
+            if (v < 0 || v >= (int64_t)used.size() || used[v])
                 throw std::runtime_error("[tinyad_amd] variable handles are not a permutation of 0..n-1");
             used[v] = 1;
         }
@@ -161,8 +197,10 @@ public:
         int np = -1;
         for (auto e : element_range)
         {
-            const std::array<int, element_valence> c = conn_of(e);
-            for (int a = 0; a < element_valence; ++a) conn.push_back(c[a]);
+            const auto c = conn_of(e);
+            static_assert(std::tuple_size<std::decay_t<decltype(c)>>::value == element_valence,
+                          "conn_of must return std::array<handle, N>");
+            for (int a = 0; a < element_valence; ++a) conn.push_back((int32_t)idx_from_handle(c[a]));
             const auto p = params_of(e);
             if (np < 0) np = (int)p.size();
             if ((int)p.size() != np) throw std::runtime_error("[tinyad_amd] parameter rows differ in length");
@@ -188,8 +226,10 @@ public:
         int64_t n = 0;
         for (auto e : element_range)
         {
-            const std::array<int, element_valence> c = conn_of(e);
-            for (int a = 0; a < element_valence; ++a) conn.push_back(c[a]);
+            const auto c = conn_of(e);
+            static_assert(std::tuple_size<std::decay_t<decltype(c)>>::value == element_valence,
+                          "conn_of must return std::array<handle, N>");
+            for (int a = 0; a < element_valence; ++a) conn.push_back((int32_t)idx_from_handle(c[a]));
             const auto p = params_of(e);
             if ((int)p.size() != ops.n_params)
                 throw std::runtime_error("[tinyad_amd] parameter row length differs from the device energy's");
@@ -215,7 +255,7 @@ public:
         for (auto e : element_range)
         {
             const auto h = handles_of(e);
-            for (auto v : h) handles.push_back((int32_t)v);
+            for (const auto& v : h) handles.push_back((int32_t)idx_from_handle(v));
             ptr.push_back((int64_t)handles.size());
             const auto p = params_of(e);
             if ((int)p.size() != ops[0].n_params)
@@ -245,12 +285,13 @@ public:
     }
 
     // x_from_data (ScalarFunctionImpl.hh:216-235)
-    VectorXd x_from_data(std::function<PassiveVariableVectorType(int)> read_user_data) const
+    VectorXd x_from_data(std::function<PassiveVariableVectorType(VariableHandleT)> read_user_data) const
     {
         VectorXd x(n_vars);
-        for (int v : variable_handles)
+        for (const VariableHandleT& h : variable_handles)
         {
-            const PassiveVariableVectorType p = read_user_data(v);
+            const int64_t v = idx_from_handle(h);
+            const PassiveVariableVectorType p = read_user_data(h);
             for (int i = 0; i < d; ++i) x[d * v + i] = p[i];
         }
         for (double xi : x)
@@ -260,14 +301,16 @@ public:
     }
 
     // x_to_data (ScalarFunctionImpl.hh:237-254)
-    void x_to_data(const VectorXd& x, std::function<void(int, const PassiveVariableVectorType&)> write_user_data) const
+    void x_to_data(const VectorXd& x,
+                   std::function<void(VariableHandleT, const PassiveVariableVectorType&)> write_user_data) const
     {
         if ((int64_t)x.size() != n_vars) throw std::runtime_error("[tinyad_amd] x_to_data: size mismatch");
-        for (int v : variable_handles)
+        for (const VariableHandleT& h : variable_handles)
         {
+            const int64_t v = idx_from_handle(h);
             PassiveVariableVectorType p;
             for (int i = 0; i < d; ++i) p[i] = x[d * v + i];
-            write_user_data(v, p);
+            write_user_data(h, p);
         }
     }
 
@@ -392,7 +435,7 @@ public:
     EvalSettings settings;
     int64_t n_vars = 0;
     int64_t n_elements = 0;
-    std::vector<int> variable_handles;
+    std::vector<VariableHandleT> variable_handles;
 
 private:
     int64_t n_segments_ = 0;
@@ -425,11 +468,14 @@ private:
 };
 
 // scalar_function<d>(range) (ScalarFunctionImpl.hh:418-442)
-template <int variable_dimension>
-ScalarFunction<variable_dimension> scalar_function(const std::vector<int>& variable_range,
-                                                   const EvalSettings& settings = {})
+// over any range of handles (a std::vector, mesh.vertices(), ...; Support/*.hh)
+template <int variable_dimension, class HandleRange>
+auto scalar_function(const HandleRange& variable_range, const EvalSettings& settings = {})
 {
-    return ScalarFunction<variable_dimension>(variable_range, settings);
+    using H = std::decay_t<decltype(*std::begin(variable_range))>;
+    std::vector<H> handles;
+    for (const auto& h : variable_range) handles.push_back(h);
+    return ScalarFunction<variable_dimension, H>(std::move(handles), settings);
 }
 
 // LinearSolver (LinearSolver.hh:13-19): solver options kept across Newton
@@ -455,8 +501,8 @@ inline SparseMatrix identity(int64_t n)
 
 // newton_direction(g, H_proj, solver, w) (NewtonDirection.hh:25-48):
 // solves (w I + H) d = -g; throws if the solve fails.
-template <int dd>
-VectorXd newton_direction(const ScalarFunction<dd>& func, const VectorXd& g, const SparseMatrix& H_proj,
+template <int dd, class H>
+VectorXd newton_direction(const ScalarFunction<dd, H>& func, const VectorXd& g, const SparseMatrix& H_proj,
                           LinearSolver& solver, double w_identity = 0.0)
 {
     VectorXd minus_g(g.size());
@@ -474,8 +520,8 @@ VectorXd newton_direction(const ScalarFunction<dd>& func, const VectorXd& g, con
 // d = B (B^T H B + w I)^-1 (-B^T g) for an n x m basis B that selects
 // variables (every column one entry 1, distinct rows: the pinning case);
 // other bases throw std::runtime_error.  d is 0 on the variables B omits.
-template <int dd>
-VectorXd newton_direction_reduced_basis(const ScalarFunction<dd>& func, const VectorXd& g,
+template <int dd, class H>
+VectorXd newton_direction_reduced_basis(const ScalarFunction<dd, H>& func, const VectorXd& g,
                                         const SparseMatrix& H_proj, const SparseMatrix& B, LinearSolver& solver,
                                         double w_identity = 0.0)
 {
@@ -507,8 +553,8 @@ VectorXd newton_direction_reduced_basis(const ScalarFunction<dd>& func, const Ve
 }
 
 // newton_direction_reduced_basis(g, H_proj, B, w) with a fresh solver (NewtonDirection.hh:114-123).
-template <int dd>
-VectorXd newton_direction_reduced_basis(const ScalarFunction<dd>& func, const VectorXd& g,
+template <int dd, class H>
+VectorXd newton_direction_reduced_basis(const ScalarFunction<dd, H>& func, const VectorXd& g,
                                         const SparseMatrix& H_proj, const SparseMatrix& B, double w_identity = 0.0)
 {
     LinearSolver solver;
@@ -517,9 +563,9 @@ VectorXd newton_direction_reduced_basis(const ScalarFunction<dd>& func, const Ve
 
 // line_search(x0, d, f, g, eval, s_max, shrink, max_iters, armijo) (LineSearch.hh:26-65),
 // evaluated on the host through func.eval (one energy launch per trial).
-template <int dd>
+template <int dd, class H>
 VectorXd line_search(const VectorXd& x0, const VectorXd& dvec, double f, const VectorXd& g,
-                     const ScalarFunction<dd>& func, double s_max = 1.0, double shrink = 0.8, int max_iters = 64,
+                     const ScalarFunction<dd, H>& func, double s_max = 1.0, double shrink = 0.8, int max_iters = 64,
                      double armijo_const = 1e-4)
 {
     if (x0.size() != g.size()) throw std::runtime_error("[tinyad_amd] line_search: size mismatch");
@@ -550,8 +596,8 @@ inline double newton_decrement(const VectorXd& dvec, const VectorXd& g)
 }
 
 // Whole projected-Newton iteration on the device (eval -> PCG -> batched line search).
-template <int dd>
-tad_newton_stats newton_step(const ScalarFunction<dd>& func, VectorXd& x, const tad_newton_opts* opts = nullptr)
+template <int dd, class H>
+tad_newton_stats newton_step(const ScalarFunction<dd, H>& func, VectorXd& x, const tad_newton_opts* opts = nullptr)
 {
     tad_newton_stats st {};
     check(tad_newton_step(func.handle(), x.data(), TAD_MEM_HOST, opts, &st));

@@ -6,6 +6,7 @@
 
 #include <array>
 #include <cmath>
+#include <memory>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -321,6 +322,69 @@ static bool test_dynamic_elements()
     return ok;
 }
 
+// Handle types of mesh libraries (Support/OpenMesh.hh, PMP.hh:
+// h.idx(); GeometryCentral.hh: h.getIndex(); Polymesh.hh: h.idx.value),
+// mocked here without the libraries
+struct OmVertex { int i; int idx() const { return i; } };
+struct GcVertex { size_t k; size_t getIndex() const { return k; } };
+struct PmVertex { struct { int value; } idx; };
+
+template <class H, class MakeH>
+static bool check_handles(MakeH make)
+{
+    bool ok = true;
+    const double V_rest[6][2] = { { 0, 0 }, { 1, 0 }, { 0, 1 }, { 1, 1 }, { 0, 2 }, { 1, 2 } };
+    const int F[4][3] = { { 0, 1, 2 }, { 1, 3, 2 }, { 2, 3, 5 }, { 2, 5, 4 } };
+    auto quad = [](int) { return quad_params({ 2, 0.5, 0.5, 1 }, { 0.1, -0.2 }, 0.0); };
+    // the same energy over integer handles and over handle objects listed in reverse
+    auto fi = tinyad_amd::scalar_function<2>(tinyad_amd::range(6));
+    std::vector<H> hs;
+    for (int v = 5; v >= 0; --v) hs.push_back(make(v));
+    auto fh = tinyad_amd::scalar_function<2>(hs);
+    fi.add_elements<1>(tinyad_amd::range(6), TAD_E_QUADRATIC, [](int v) { return std::array<int, 1>{ v }; }, quad);
+    fh.template add_elements<1>(tinyad_amd::range(6), TAD_E_QUADRATIC, [&](int v) { return std::array<H, 1>{ make(v) }; },
+                       quad);
+    auto rest = [&](int f) {
+        const double m00 = V_rest[F[f][1]][0] - V_rest[F[f][0]][0], m10 = V_rest[F[f][1]][1] - V_rest[F[f][0]][1];
+        const double m01 = V_rest[F[f][2]][0] - V_rest[F[f][0]][0], m11 = V_rest[F[f][2]][1] - V_rest[F[f][0]][1];
+        const double det = m00 * m11 - m01 * m10;
+        return std::vector<double>{ m11 / det, -m01 / det, -m10 / det, m00 / det, 0.25 };
+    };
+    fi.add_elements<3>(tinyad_amd::range(4), TAD_E_SYMDIR_TRI,
+                       [&](int f) { return std::array<int, 3>{ F[f][0], F[f][1], F[f][2] }; }, rest);
+    fh.template add_elements<3>(tinyad_amd::range(4), TAD_E_SYMDIR_TRI,
+                       [&](int f) { return std::array<H, 3>{ make(F[f][0]), make(F[f][1]), make(F[f][2]) }; }, rest);
+    auto pos = [&](int v) { return std::array<double, 2>{ 0.9 * V_rest[v][0] + 0.01 * v, 1.1 * V_rest[v][1] }; };
+    const VectorXd xi = fi.x_from_data([&](int v) { return pos(v); });
+    const VectorXd xh = fh.x_from_data([&](H h) { return pos((int)tinyad_amd::idx_from_handle(h)); });
+    EXPECT(xi == xh);
+    auto [f1, g1, H1] = fi.eval_with_hessian_proj(xi);
+    auto [f2, g2, H2] = fh.eval_with_hessian_proj(xh);
+    EXPECT(f1 == f2 && g1 == g2 && H1.values == H2.values);
+    int written = 0;
+    fh.x_to_data(xh, [&](H h, const std::array<double, 2>& p) {
+        const int v = (int)tinyad_amd::idx_from_handle(h);
+        if (p[0] == xh[2 * v] && p[1] == xh[2 * v + 1]) ++written;
+    });
+    EXPECT(written == 6);
+    auto d = tinyad_amd::newton_direction(fh, g2, H2, *std::make_unique<tinyad_amd::LinearSolver>());
+    EXPECT(d.size() == 12);
+    return ok;
+}
+
+static bool test_handle_types()
+{
+    bool ok = true;
+    ok = check_handles<OmVertex>([](int v) { return OmVertex{ v }; }) && ok;
+    ok = check_handles<GcVertex>([](int v) { return GcVertex{ (size_t)v }; }) && ok;
+    ok = check_handles<PmVertex>([](int v) { PmVertex h; h.idx.value = v; return h; }) && ok;
+    bool threw = false;
+    try { tinyad_amd::scalar_function<2>(std::vector<OmVertex>{ { 0 }, { 0 } }); }
+    catch (const std::runtime_error&) { threw = true; }
+    EXPECT(threw);  // not a permutation of 0..n-1
+    return ok;
+}
+
 int main()
 {
     struct Case { const char* name; bool (*fn)(); };
@@ -335,6 +399,7 @@ int main()
         { "Solver.TwoLevel", test_two_level_solve },
         { "SwitchSolverTest.ReducedBasis", test_reduced_basis },
         { "DynamicElementsTest.DynamicElementsTest", test_dynamic_elements },
+        { "HandleTypeTest.MeshLibraryHandles", test_handle_types },
     };
     for (const Case& c : cases)
     {
