@@ -356,3 +356,23 @@ def test_dynamic_validation(ctx, plugin):
     with pytest.raises(tad.TadError, match="twice"):
         fn.add_elements_dynamic_device(ops, [[1], [2, 2]])
     assert fn.terms() == []  # failed calls add nothing
+
+
+def test_device_empty_terms(ctx, plugin):
+    """Device terms without elements (and dynamic calls without elements) add
+    nothing to f, g or the pattern; they sit beside catalog terms."""
+    fn = tad.ScalarFunction(2, 3, ctx)
+    fn.add_elements_device(plugin["symdir_tri"], np.zeros((0, 3), dtype=np.int32), np.zeros((0, 5)))
+    fn.add_elements_dynamic_device(dyn_ops(plugin, "sum_squares", 2), [])
+    fn.add_elements(1, tad.PENALTY, [[1]], [0.5, -0.25, 2.0])
+    assert fn.terms() == [(tad.DEVICE, 3, 0), (tad.PENALTY, 1, 1)]
+    x = np.array([0.1, 0.2, 0.3, 0.4, 0.5, 0.6])
+    f, g, H = fn.eval_with_hessian_proj(x)
+    assert abs(f - 2.0 * ((0.3 - 0.5) ** 2 + (0.4 + 0.25) ** 2)) <= 1e-15
+    assert H.nnz == 4 and np.count_nonzero(g) == 2
+    fd = tad.ScalarFunction(2, 3, ctx)
+    fd.add_elements_dynamic_device(dyn_ops(plugin, "sum_squares", 2), [[], [2]])  # valence 0 and 1
+    assert fd.terms() == [(tad.DEVICE, 1, 2)]
+    f, g, H = fd.eval_with_derivatives(x)
+    assert f == 0.5 ** 2 + 0.6 ** 2 and np.array_equal(g, [0, 0, 0, 0, 1.0, 1.2])
+    assert np.array_equal(H.toarray()[4:, 4:], 2 * np.eye(2)) and H.nnz == 4
