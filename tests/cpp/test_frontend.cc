@@ -2,6 +2,27 @@
 // (tests/NewtonTest.cc, tests/ScalarFunctionTest.cc, tests/ExceptionTest.cc)
 // but against include/tinyad_amd/ScalarFunction.hh.  Prints one line per
 // case; exit status 0 when every case passes.  Needs a GPU.
+#include <cstdint>
+
+// HandleTypeTest.cc:8-45: custom handle types made known by idx_from_handle()
+// overloads declared before the header is included
+struct CustomVariableHandle
+{
+    CustomVariableHandle() = default;
+    CustomVariableHandle(int i) : idx(i) {}
+    int idx = -1;
+};
+struct CustomElementHandle
+{
+    CustomElementHandle() = default;
+    CustomElementHandle(int i) : idx(i) {}
+    int idx = -1;
+};
+namespace tinyad_amd {
+inline int64_t idx_from_handle(const CustomVariableHandle& h) { return h.idx; }
+inline int64_t idx_from_handle(const CustomElementHandle& h) { return h.idx; }
+} // namespace tinyad_amd
+
 #include <tinyad_amd/ScalarFunction.hh>
 
 #include <array>
@@ -372,9 +393,31 @@ static bool check_handles(MakeH make)
     return ok;
 }
 
+// HandleTypeTest.cc:47-71: 5 variables, 10 elements, element e reads variable e / 2
+template <class VH, class EH>
+static bool check_reference_handle_test()
+{
+    bool ok = true;
+    std::vector<VH> vhs;
+    for (int i = 0; i < 5; ++i) vhs.push_back(VH(i));
+    std::vector<EH> ehs;
+    for (int i = 0; i < 10; ++i) ehs.push_back(EH(i));
+    auto func = tinyad_amd::scalar_function<1>(vhs);
+    func.template add_elements<1>(ehs, TAD_E_QUADRATIC,
+        [](const EH& e) { return std::array<VH, 1>{ VH((int)tinyad_amd::idx_from_handle(e) / 2) }; },
+        [](const EH&) { return quad_params({ 0.0 }, { 1.0 }, 0.0); });
+    auto [f, g, H] = func.eval_with_hessian_proj(VectorXd(func.n_vars, 0.0));
+    EXPECT(f == 0.0);
+    for (double gi : g) EXPECT(gi == 2.0);
+    return ok;
+}
+
 static bool test_handle_types()
 {
     bool ok = true;
+    ok = check_reference_handle_test<int, int>() && ok;
+    ok = check_reference_handle_test<long int, long int>() && ok;
+    ok = check_reference_handle_test<CustomVariableHandle, CustomElementHandle>() && ok;
     ok = check_handles<OmVertex>([](int v) { return OmVertex{ v }; }) && ok;
     ok = check_handles<GcVertex>([](int v) { return GcVertex{ (size_t)v }; }) && ok;
     ok = check_handles<PmVertex>([](int v) { PmVertex h; h.idx.value = v; return h; }) && ok;
