@@ -14,6 +14,8 @@
 //   func.add_elements<3>(range(n_faces), tinyad_amd::device_element<MyEnergy, 3, 2, 5>(),
 //                        conn_of, params_of);
 //
+// S is tad::Dual<K, T> (one pass) for K = valence * d <= 6 and
+// tad::HyperDual<T> (one pass per Hessian entry i <= j) above.
 // Inside the functor use +, -, *, / (with doubles or ints), comparisons and
 // tad::sqr, tad::sqrt, tad::exp, tad::log, tad::sin, tad::cos, tad::pow(S, double);
 // With add_elements_dynamic the same functor is compiled for several
@@ -36,6 +38,7 @@
 #pragma once
 
 #include <tinyad_amd.h>
+#include <tinyad_amd/device/HyperDual.hpp>
 #include <tinyad_amd/device/Kernels.hpp>
 
 #include <hip/hip_runtime.h>
@@ -55,21 +58,65 @@ struct DeviceElement
     // n_active: handles before the padding; the padded coordinates are
     // passive constants, so their Hessian rows and columns are zero, as for
     // the local variables a reference element never accesses
+    // up to this many element variables one pass with a full Dual<K> per
+    // intermediate; above, one hyper-dual pass per Hessian entry
+    static constexpr int kFullDualMaxK = 6;
+
     template <class T, int MODE, class Sink>
     __device__ static double eval(const double* X, const double* p, T eps, const Sink& out, bool& nonfinite,
                                   int n_active)
     {
         const double f = F{}(X, p);
-        tad::Dual<K, T> v[K];
-#pragma unroll
-        for (int i = 0; i < K; ++i)
-            v[i] = (i < n_active * DD) ? tad::dual_var<K, T>(T(X[i]), i) : tad::Dual<K, T>(X[i]);
-        const tad::Dual<K, T> E = F{}(static_cast<const tad::Dual<K, T>*>(v), p);
+        const int na = n_active * DD;  // active variables (the rest is padding)
         T g[K], Hp[tad::Sym<K>::n];
+        if constexpr (K <= kFullDualMaxK)
+        {
+            tad::Dual<K, T> v[K];
 #pragma unroll
-        for (int i = 0; i < K; ++i) g[i] = E.g[i];
+            for (int i = 0; i < K; ++i) v[i] = (i < na) ? tad::dual_var<K, T>(T(X[i]), i) : tad::Dual<K, T>(X[i]);
+            const tad::Dual<K, T> E = F{}(static_cast<const tad::Dual<K, T>*>(v), p);
 #pragma unroll
-        for (int i = 0; i < tad::Sym<K>::n; ++i) Hp[i] = E.h[i];
+            for (int i = 0; i < K; ++i) g[i] = E.g[i];
+#pragma unroll
+            for (int i = 0; i < tad::Sym<K>::n; ++i) Hp[i] = E.h[i];
+        }
+        else
+        {
+            // pass (i, j), i <= j, seeds e1 on x_i and e2 on x_j: .a = g_i,
+            // .ab = H_ij.  Results enter g and Hp through selects on
+            // compile-time indices, so both stay in registers while the pass
+            // loop itself is not unrolled.  The gradient alone needs the
+            // passes i = j.
+#pragma unroll
+            for (int k = 0; k < K; ++k) g[k] = T(0);
+#pragma unroll
+            for (int s = 0; s < tad::Sym<K>::n; ++s) Hp[s] = T(0);
+#pragma unroll 1
+            for (int i = 0; i < na; ++i)
+            {
+                const int j_end = (MODE >= tad::MODE_HESS) ? na : i + 1;
+#pragma unroll 1
+                for (int j = i; j < j_end; ++j)
+                {
+                    tad::HyperDual<T> hx[K];
+#pragma unroll
+                    for (int k = 0; k < K; ++k)
+                        hx[k] = tad::HyperDual<T>(T(X[k]), k == i ? T(1) : T(0), k == j ? T(1) : T(0), T(0));
+                    const tad::HyperDual<T> r = F{}(static_cast<const tad::HyperDual<T>*>(hx), p);
+                    if (j == i)
+                    {
+#pragma unroll
+                        for (int k = 0; k < K; ++k) g[k] = (k == i) ? r.a : g[k];
+                    }
+                    if constexpr (MODE >= tad::MODE_HESS)
+                    {
+                        const int at = tad::Sym<K>::idx(i, j);
+#pragma unroll
+                        for (int s = 0; s < tad::Sym<K>::n; ++s) Hp[s] = (s == at) ? r.ab : Hp[s];
+                    }
+                }
+            }
+        }
         tad::write_dense_record<NN, DD, T, MODE>(g, Hp, eps, out, nonfinite);
         return f;
     }

@@ -66,7 +66,8 @@ def build_device_energy_libs(force: bool = False) -> list:
     src_dir = os.path.join(ROOT, "tests", "cpp")
     out_dir = os.path.join(src_dir, "bin")
     os.makedirs(out_dir, exist_ok=True)
-    deps = [os.path.join(ROOT, "include", h) for h in PUBLIC_HEADERS + ["tinyad_amd/DeviceElements.hh"]]
+    deps = [os.path.join(ROOT, "include", h)
+            for h in PUBLIC_HEADERS + ["tinyad_amd/DeviceElements.hh", "tinyad_amd/device/HyperDual.hpp"]]
     outs = []
     for src in sorted(f for f in os.listdir(src_dir) if f.endswith(".hip")):
         srcp = os.path.join(src_dir, src)
