@@ -95,7 +95,7 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1 "
+        print(json.dumps({"error": "bench.py measures one GPU; --gpus must be 1 "
                                    "(the multi-GPU sharding is a design note, not implemented)"}), flush=True)
         sys.exit(2)
 
