@@ -376,3 +376,22 @@ def test_device_empty_terms(ctx, plugin):
     f, g, H = fd.eval_with_derivatives(x)
     assert f == 0.5 ** 2 + 0.6 ** 2 and np.array_equal(g, [0, 0, 0, 0, 1.0, 1.2])
     assert np.array_equal(H.toarray()[4:, 4:], 2 * np.eye(2)) and H.nnz == 4
+
+
+def test_device_tet_newton_matches_catalog(ctx, plugin):
+    """Projected Newton on a twisted cube with the tet energy as a device
+    functor (K = 12: hyper-dual passes) and from the catalog: same energies
+    along the way and at the end (converged-energy bar 1e-8)."""
+    import torch
+    P = configs.twist_cube(m=8, seed=2)
+    fc = tad.ScalarFunction.from_problem(P, ctx)
+    fd = device_function(P, plugin, ctx)
+    xc = torch.tensor(P.x0, device="cuda:0")
+    xd = xc.clone()
+    for it in range(12):
+        sc = fc.newton_step(xc)
+        sd = fd.newton_step(xd)
+        assert abs(sd["f"] - sc["f"]) <= 1e-10 * abs(sc["f"]), it
+    fc_, fd_ = fc.eval(xc), fd.eval(xd)
+    print(f"tets: catalog f {fc_:.15g} device functor f {fd_:.15g}")
+    assert abs(fd_ - fc_) <= 1e-8 * abs(fc_)
