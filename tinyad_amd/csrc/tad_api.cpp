@@ -254,6 +254,7 @@ struct tad_fn_s
         int requested = 0;            // aggregate size asked for when the pattern was built (0: none)
         int target = 0;               // aggregate size the pattern was built with
         int64_t n_agg = 0, n_pairs = 0, n_chunks = 0, n_gchunks = 0;
+        int max_agg_chunks = 0;
         DevBuf<int32_t> agg_of, agg_ptr, agg_vert, pair_blocks, pair_ab, chunk_ptr, agg_chunk, pair_chunk;
         DevBuf<int64_t> gchunk_ptr;
         DevBuf<double> dvec, Ac, rc, yc, cparts, chunk_part, gpart;
@@ -272,6 +273,7 @@ struct tad_fn_s
             gpart.release(); info.release();
             requested = target = 0;
             n_agg = n_pairs = n_chunks = n_gchunks = 0;
+            max_agg_chunks = 0;
             factor_valid = false;
         }
     } co;
@@ -683,6 +685,7 @@ tad_status build_coarse(tad_fn fn, int target)
     c.n_pairs = n_pairs;
     c.n_chunks = (int64_t)chunk_ptr.size() - 1;
     c.n_gchunks = (int64_t)gchunk_ptr.size() - 1;
+    for (int32_t a = 0; a < nn; ++a) c.max_agg_chunks = std::max(c.max_agg_chunks, agg_chunk[a + 1] - agg_chunk[a]);
     c.target = target;
     return TAD_OK;
 }
@@ -745,6 +748,7 @@ tad_status setup_coarse(tad_fn fn, const double* H_dev, const tad_solve_opts& o,
     cd->n_chunks = c.n_chunks;
     cd->chunk_ptr = c.chunk_ptr.p;
     cd->agg_chunk = c.agg_chunk.p;
+    cd->max_agg_chunks = c.max_agg_chunks;
     cd->chunk_part = c.chunk_part.p;
     cd->dvec = c.dvec.p;
     cd->Ainv = c.Ac.p;

@@ -126,6 +126,7 @@ struct CoarseDev
     int64_t n_chunks = 0;
     const int32_t* chunk_ptr = nullptr;  // [n_chunks + 1]
     const int32_t* agg_chunk = nullptr;  // [n_agg + 1]
+    int max_agg_chunks = 0;              // most chunks of one aggregate
     double* chunk_part = nullptr;        // [n_chunks * kc] restriction partials
     const double* dvec = nullptr;    // [n_handles * d]
     const double* Ainv = nullptr;    // [nc * nc] inverse Galerkin matrix (symmetric, both triangles)

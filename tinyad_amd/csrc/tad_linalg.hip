@@ -1012,6 +1012,59 @@ __global__ __launch_bounds__(kBlock) void k_coarse_restrict_sum(int64_t n_agg, c
     rc[id] = t;
 }
 
+// Both restriction stages in one launch: one workgroup per aggregate, its
+// waves take the aggregate's chunks in turn (each chunk reduced exactly as by
+// k_coarse_restrict_chunk), the chunk partials go to LDS and are summed in
+// chunk order as by k_coarse_restrict_sum, so r_c has the same bits.  For
+// aggregates of at most kRestrictMaxChunks chunks.
+constexpr int kRestrictMaxChunks = 64;
+
+template <int D, int KC>
+__global__ __launch_bounds__(kBlock) void k_coarse_restrict(const int32_t* __restrict__ agg_chunk,
+                                                            const int32_t* __restrict__ chunk_ptr,
+                                                            const int32_t* __restrict__ agg_vert,
+                                                            const double* __restrict__ dvec,
+                                                            const double* __restrict__ r, double* __restrict__ rc,
+                                                            PcgScalars* sc)
+{
+    if (block_done(sc)) return;
+    __shared__ double cpart[kRestrictMaxChunks * KC];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t a = blockIdx.x;
+    const int32_t c0 = agg_chunk[a], c1 = agg_chunk[a + 1];
+    for (int32_t ch = c0 + wv; ch < c1; ch += kBlock / 64)
+    {
+        double acc[KC];
+#pragma unroll
+        for (int c = 0; c < KC; ++c) acc[c] = 0.0;
+        const int64_t m1 = chunk_ptr[ch + 1];
+        for (int64_t m = chunk_ptr[ch] + lane; m < m1; m += 64)
+        {
+            const int64_t v = agg_vert[m];
+            double R[D][KC];
+            coarse_rows<D, KC>(dvec + v * D, R);
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+            {
+                const double ri = r[v * D + i];
+#pragma unroll
+                for (int c = 0; c < KC; ++c) acc[c] += R[i][c] * ri;
+            }
+        }
+        wave_sum<KC>(acc);
+#pragma unroll
+        for (int c = 0; c < KC; ++c)
+            if (lane == c) cpart[(ch - c0) * KC + c] = acc[c];
+    }
+    __syncthreads();
+    if (threadIdx.x < KC)
+    {
+        double t = 0.0;
+        for (int32_t k = 0; k < c1 - c0; ++k) t += cpart[k * KC + threadIdx.x];
+        rc[a * KC + threadIdx.x] = t;
+    }
+}
+
 // y_c = A_c^-1 r_c, one wave per row (row i of the symmetric inverse is its
 // contiguous column i); one partial of r_c . y_c per workgroup
 constexpr int kCoarseRows = kBlock / 64;
@@ -1145,11 +1198,19 @@ template <int D, int KC>
 static void coarse_correction_t(int64_t nh, const CoarseDev& c, const double* r, double* z, int slot, PcgScalars* sc,
                                 hipStream_t s)
 {
-    const unsigned gch = (unsigned)((c.n_chunks * 64 + kBlock - 1) / kBlock);
-    k_coarse_restrict_chunk<D, KC><<<gch, kBlock, 0, s>>>(c.n_chunks, c.chunk_ptr, c.agg_vert, c.dvec, r,
-                                                           c.chunk_part, sc);
-    const unsigned gs = (unsigned)((c.n_agg * KC + kBlock - 1) / kBlock);
-    k_coarse_restrict_sum<KC><<<gs, kBlock, 0, s>>>(c.n_agg, c.agg_chunk, c.chunk_part, c.rc, sc);
+    if (c.max_agg_chunks <= kRestrictMaxChunks)
+    {
+        k_coarse_restrict<D, KC><<<(unsigned)c.n_agg, kBlock, 0, s>>>(c.agg_chunk, c.chunk_ptr, c.agg_vert, c.dvec, r,
+                                                                      c.rc, sc);
+    }
+    else
+    {
+        const unsigned gch = (unsigned)((c.n_chunks * 64 + kBlock - 1) / kBlock);
+        k_coarse_restrict_chunk<D, KC><<<gch, kBlock, 0, s>>>(c.n_chunks, c.chunk_ptr, c.agg_vert, c.dvec, r,
+                                                               c.chunk_part, sc);
+        const unsigned gs = (unsigned)((c.n_agg * KC + kBlock - 1) / kBlock);
+        k_coarse_restrict_sum<KC><<<gs, kBlock, 0, s>>>(c.n_agg, c.agg_chunk, c.chunk_part, c.rc, sc);
+    }
     k_coarse_apply<<<c.Gc, kBlock, 0, s>>>(c.nc, c.Ainv, c.rc, c.yc, c.cparts + (int64_t)slot * c.Gc, sc);
     k_coarse_prolong<D, KC><<<(unsigned)((nh + kBlock - 1) / kBlock), kBlock, 0, s>>>(nh, c.agg_of, c.dvec, c.yc,
                                                                                        c.free_vars, z, sc);
