@@ -281,6 +281,7 @@ struct tad_fn_s
     DevBuf<double> x_hess;
     DevBuf<unsigned char> free_buf;  // staged free-variable mask of a reduced-basis solve
     bool x_hess_valid = false;
+    int last_solve_iters = 0;  // iterations of the previous solve (first poll of the next one)
 
     hipStream_t s() const { return ctx->stream; }
     HessAsm hess_asm() const
@@ -971,19 +972,25 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
     // the device has recorded termination, so overshooting a chunk is cheap.
     // Iteration k = max_iters launches only the SpMV, whose termination test
     // then always fires.
+    // The first poll waits for 3/4 of the previous solve's iteration count
+    // (Newton steps need similar counts), later ones come every `every`:
+    // each poll stalls the device for a host round trip.
     int k = 0;
+    int chunk = std::max(every, fn->last_solve_iters * 3 / 4);
     while (k <= o.max_iters)
     {
-        for (int c = 0; c < every && k <= o.max_iters; ++c, ++k)
+        for (int c = 0; c < chunk && k <= o.max_iters; ++c, ++k)
             TAD_HIP(launch_pcg_iteration(d, nh, fn->brow_ptr.p, fn->bcol.p, fn->row_ptr.p, H_dev, o.w_identity,
                                          fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->p.p, fn->q.p, fn->pcg_parts.p,
                                          fn->scal.p, k, k < o.max_iters, free_dev, co, s, timer));
         TAD_HIP(hipMemcpyAsync(pin, fn->scal.p, sizeof(hs), hipMemcpyDeviceToHost, s));
         TAD_HIP(hipStreamSynchronize(s));
         if (pin->done) break;
+        chunk = every;
     }
     if (!pin->done) return fail(TAD_ERR_SOLVE, "PCG did not report termination");
     std::memcpy(&hs, pin, sizeof(hs));
+    fn->last_solve_iters = hs.iters;
     const double rel = hs.bb > 0 ? std::sqrt(hs.rr / hs.bb) : 0.0;
     if (st)
     {
