@@ -68,7 +68,7 @@ def main():
     ap.add_argument("--pcg-tol", type=float, default=1e-10)
     ap.add_argument("--coarse-aggregate", type=int, default=None,
                     help="handles per aggregate of the two-level preconditioner (0: block-Jacobi only; "
-                         "default: the library's)")
+                         "default: the library's automatic size)")
     ap.add_argument("--coarse-refresh", type=int, default=5,
                     help="rebuild the coarse space of the direction solve every this many Newton iterations "
                          "(reusing the previous inverse coarse matrix in between; 1: every iteration)")
@@ -381,9 +381,10 @@ def main():
                    (f"disk_batch: {args.disks} radius-clipped height-field disks, {n_tets} triangles, "
                     f"{P.n_handles} vertices, nnz={nnz}, one launch per term, per-disk energies"),
                    "precision": args.precision, "pcg_tol": args.pcg_tol,
-                   "coarse_aggregate": tad.solve_default_opts(**({} if args.coarse_aggregate is None else
-                                                                 {"coarse_aggregate": args.coarse_aggregate})
-                                                              ).coarse_aggregate,
+                   "coarse_aggregate": ("auto" if args.coarse_aggregate is None
+                                        and tad.solve_default_opts().coarse_aggregate < 0
+                                        else (args.coarse_aggregate if args.coarse_aggregate is not None
+                                              else tad.solve_default_opts().coarse_aggregate)),
                    "coarse_refresh": args.coarse_refresh,
                    "autotuned": bool(args.autotune),
                    "launch_shape": list(fn.launch_shape(0)), "assembly_shape": fn.assembly_shape},

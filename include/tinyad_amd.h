@@ -305,7 +305,9 @@ typedef struct tad_solve_opts
     int max_iters;
     int check_every;    /* host polls convergence every this many iterations             */
     int coarse_aggregate; /* two-level preconditioner: handles per aggregate of the      */
-                        /* coarse space (default 1024); 0: block-Jacobi only            */
+                        /* coarse space; 0: block-Jacobi only; < 0 (default): automatic, */
+                        /* about 1,400 coarse unknowns with at least 32 handles per      */
+                        /* aggregate                                                     */
     int coarse_reuse;   /* 1: keep the coarse space and inverse coarse matrix of the     */
                         /* function's previous solve when it is compatible (same         */
                         /* aggregates, unknowns per aggregate and w, neither solve       */
@@ -329,7 +331,7 @@ tad_status tad_solve_default_opts(tad_solve_opts* opts);
 /* Solves (w I + H) sol = rhs by preconditioned CG (FP64) on the function's
  * pattern; replaces the SimplicialLDLT factorize/solve of newton_direction
  * (NewtonDirection.hh:25-48, LinearSolver.hh:13-19).  The preconditioner is
- * block-Jacobi (d x d diagonal blocks) plus, with coarse_aggregate > 0, an
+ * block-Jacobi (d x d diagonal blocks) plus, unless coarse_aggregate is 0, an
  * additive coarse correction on aggregates of about that many handles
  * (translations, and rotations about the positions of the last second-order
  * evaluation when there was one; Galerkin coarse matrix inverted in FP64).

@@ -693,6 +693,9 @@ tad_status build_coarse(tad_fn fn, int target)
 
 // Largest coarse matrix the two-level preconditioner builds (dense FP64 inverse).
 constexpr int64_t kMaxCoarse = 8192;
+// Automatic aggregate size (tad_solve_opts.coarse_aggregate < 0).
+constexpr int64_t kAutoCoarse = 1400;
+constexpr int64_t kAutoMinAggregate = 32;
 
 // Per solve: coarse unknowns per aggregate, lever arms, A_c = P^T (w I + H) P,
 // its Cholesky factorization and inverse (rocSOLVER potrf / potri, lower
@@ -707,12 +710,16 @@ tad_status setup_coarse(tad_fn fn, const double* H_dev, const tad_solve_opts& o,
     *reused = false;
     const int d = fn->d;
     const int64_t nh = fn->n_handles;
-    if (o.coarse_aggregate <= 0 || nh == 0 || fn->n_blocks == 0) return TAD_OK;
+    if (o.coarse_aggregate == 0 || nh == 0 || fn->n_blocks == 0) return TAD_OK;
     const bool rot = fn->x_hess_valid && d >= 2;
     const int kc = rot ? (d == 3 ? 6 : 3) : d;
+    // automatic size: about kAutoCoarse coarse unknowns (measured optimum on
+    // 2D grids and tet cubes of 10K to 227K handles), at least 32 handles
+    const int64_t agg = o.coarse_aggregate > 0 ? o.coarse_aggregate
+                                               : std::max<int64_t>(kAutoMinAggregate, (nh * kc + kAutoCoarse - 1) / kAutoCoarse);
     // keep the dense coarse matrix bounded: larger aggregates for large meshes
     // (grown further while breadth-first aggregates come out too small)
-    const int want = (int)std::max<int64_t>(o.coarse_aggregate, (nh * kc + kMaxCoarse - 1) / kMaxCoarse);
+    const int want = (int)std::max<int64_t>(agg, (nh * kc + kMaxCoarse - 1) / kMaxCoarse);
     auto& c = fn->co;
     if (c.requested != want)
     {
@@ -1489,7 +1496,7 @@ tad_status tad_solve_default_opts(tad_solve_opts* o)
     o->tol = 1e-10;
     o->max_iters = 20000;
     o->check_every = 16;
-    o->coarse_aggregate = 1024;
+    o->coarse_aggregate = -1;
     o->coarse_reuse = 0;
     return TAD_OK;
 }
