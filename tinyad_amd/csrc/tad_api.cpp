@@ -939,6 +939,21 @@ tad_status dot_dev(tad_fn fn, const double* a, const double* b, double* out)
     return TAD_OK;
 }
 
+// a1 . b1 and a2 . b2 with one host synchronization
+tad_status dot2_dev(tad_fn fn, const double* a1, const double* b1, const double* a2, const double* b2, double* out1,
+                    double* out2)
+{
+    hipStream_t s = fn->s();
+    TAD_HIP(launch_dot(a1, b1, fn->n_vars, fn->partials.p, fn->results.p + 8, s));
+    TAD_HIP(launch_dot(a2, b2, fn->n_vars, fn->partials.p, fn->results.p + 9, s));
+    double* pin = fn->ctx->pinned;
+    TAD_HIP(hipMemcpyAsync(pin + 48, fn->results.p + 8, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    TAD_HIP(hipStreamSynchronize(s));
+    *out1 = pin[48];
+    *out2 = pin[49];
+    return TAD_OK;
+}
+
 tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, double* sol_dev,
                       const tad_solve_opts& o, tad_solve_stats* st, const unsigned char* free_dev = nullptr)
 {
@@ -1637,11 +1652,9 @@ tad_status tad_newton_step(tad_fn fn, double* x, tad_mem mem, const tad_newton_o
     st.solve_converged = ss.converged;
     st.coarse_unknowns = ss.coarse_unknowns;
     st.coarse_reused = ss.coarse_reused;
-    double dpg = 0.0;
-    TAD_TRY(dot_dev(fn, fn->sol.p, fn->g_buf.p, &dpg));
+    double dpg = 0.0, gg = 0.0;
+    TAD_TRY(dot2_dev(fn, fn->sol.p, fn->g_buf.p, fn->g_buf.p, fn->g_buf.p, &dpg, &gg));
     const double dg = -dpg;  // d^T g
-    double gg = 0.0;
-    TAD_TRY(dot_dev(fn, fn->g_buf.p, fn->g_buf.p, &gg));
     st.grad_norm = std::sqrt(gg);
     st.newton_decrement = -0.5 * dg;
     double fo = 0, so = 0;
