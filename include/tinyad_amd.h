@@ -359,6 +359,10 @@ typedef struct tad_newton_opts
     int ls_max_iters;
     double ls_armijo;
     int ls_batch;           /* step sizes evaluated per energy launch (1..8)            */
+    int coarse_refresh;     /* k > 1: the direction solve reuses the function's coarse   */
+                            /* space (solve.coarse_reuse) until k solves have used it,  */
+                            /* i.e. rebuilds it on every k-th Newton step.  0 or 1      */
+                            /* (default): as solve.coarse_reuse says.                   */
 } tad_newton_opts;
 
 typedef struct tad_newton_stats

@@ -444,6 +444,14 @@ def test_coarse_reuse(ctx):
     assert all(s["solve_converged"] and s["coarse_unknowns"] > 0 for s in sa)
     xb, sb = run(3)
     assert torch.equal(xa, xb) and [s["cg_iters"] for s in sa] == [s["cg_iters"] for s in sb]
+    # the same schedule kept by the library (tad_newton_opts.coarse_refresh) on
+    # a fresh function gives the same bits
+    fg2 = tad.ScalarFunction.from_problem(P, ctx)
+    x2 = x0.clone()
+    o2 = tad.newton_default_opts(solve_coarse_aggregate=128, coarse_refresh=3)
+    s2 = [fg2.newton_step(x2, o2) for _ in range(5)]
+    assert [s["coarse_reused"] for s in s2] == [0, 1, 1, 0, 1]
+    assert torch.equal(x2, xa) and [s["cg_iters"] for s in s2] == [s["cg_iters"] for s in sa]
     x1, s1 = run(1)
     assert not any(s["coarse_reused"] for s in s1)
     f1, fa = s1[-1]["f_new"], sa[-1]["f_new"]

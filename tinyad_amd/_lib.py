@@ -61,7 +61,7 @@ class DeviceOps(ctypes.Structure):
 class NewtonOpts(ctypes.Structure):
     _fields_ = [("projection_eps", ctypes.c_double), ("solve", SolveOpts), ("ls_s_max", ctypes.c_double),
                 ("ls_shrink", ctypes.c_double), ("ls_max_iters", ctypes.c_int), ("ls_armijo", ctypes.c_double),
-                ("ls_batch", ctypes.c_int)]
+                ("ls_batch", ctypes.c_int), ("coarse_refresh", ctypes.c_int)]
 
 
 class NewtonStats(ctypes.Structure):

@@ -282,6 +282,7 @@ struct tad_fn_s
     DevBuf<unsigned char> free_buf;  // staged free-variable mask of a reduced-basis solve
     bool x_hess_valid = false;
     int last_solve_iters = 0;  // iterations of the previous solve (first poll of the next one)
+    int solves_since_rebuild = 0;  // solves that reused the coarse space since it was built
 
     hipStream_t s() const { return ctx->stream; }
     HessAsm hess_asm() const
@@ -1013,6 +1014,7 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
     if (!pin->done) return fail(TAD_ERR_SOLVE, "PCG did not report termination");
     std::memcpy(&hs, pin, sizeof(hs));
     fn->last_solve_iters = hs.iters;
+    fn->solves_since_rebuild = coarse_reused ? fn->solves_since_rebuild + 1 : 0;
     const double rel = hs.bb > 0 ? std::sqrt(hs.rr / hs.bb) : 0.0;
     if (st)
     {
@@ -1591,6 +1593,7 @@ tad_status tad_newton_default_opts(tad_newton_opts* o)
     o->ls_max_iters = 64;
     o->ls_armijo = 1e-4;
     o->ls_batch = 4;
+    o->coarse_refresh = 1;
     return TAD_OK;
 }
 
@@ -1647,7 +1650,9 @@ tad_status tad_newton_step(tad_fn fn, double* x, tad_mem mem, const tad_newton_o
     TAD_HIP(fn->sol.ensure(n));
     tad_solve_stats ss;
     std::memset(&ss, 0, sizeof(ss));
-    TAD_TRY(solve_core(fn, fn->H_buf.p, fn->g_buf.p, fn->sol.p, o.solve, &ss));
+    tad_solve_opts sopt = o.solve;
+    if (o.coarse_refresh > 1 && fn->solves_since_rebuild + 1 < o.coarse_refresh) sopt.coarse_reuse = 1;
+    TAD_TRY(solve_core(fn, fn->H_buf.p, fn->g_buf.p, fn->sol.p, sopt, &ss));
     st.cg_iters = ss.iters;
     st.solve_converged = ss.converged;
     st.coarse_unknowns = ss.coarse_unknowns;
