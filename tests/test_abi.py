@@ -36,3 +36,30 @@ def test_null_arguments_are_rejected_without_a_device():
     assert L.tad_ctx_create(0, None) == 1
     assert L.tad_fn_destroy(None) == 0
     assert L.tad_fn_eval(None, None, 0, None) == 1
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every struct the Python binding mirrors has, field by field, the size
+    and offsets the C compiler gives it from include/tinyad_amd.h."""
+    import subprocess
+    mirrors = {"tad_solve_opts": _lib.SolveOpts, "tad_solve_stats": _lib.SolveStats,
+               "tad_newton_opts": _lib.NewtonOpts, "tad_newton_stats": _lib.NewtonStats,
+               "tad_device_ops": _lib.DeviceOps}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "tinyad_amd.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in py._fields_:
+            lines.append(f'printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        a, b, v = line.split()
+        got[(a, b)] = int(v)
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for field, _ in py._fields_:
+            assert got[(cname, field)] == getattr(py, field).offset, (cname, field)
