@@ -1802,6 +1802,7 @@ tad_status tad_fn_autotune(tad_fn fn, const double* x, tad_mem mem, double eps, 
     // shapes are committed only when every measurement succeeded
     struct Guard
     {
+        explicit Guard(tad_fn f) : fn(f) {}
         tad_fn fn;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         std::vector<TermDev> saved;

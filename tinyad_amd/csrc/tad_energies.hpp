@@ -361,8 +361,6 @@ __device__ __forceinline__ double symdir_element(const double* X, const double* 
 {
     constexpr int N = Dm + 1;
     constexpr int KJ = Dm * Dm;
-    constexpr int KX = N * Dm;
-    constexpr int NB = n_blocks(N);
     const double f = energy_symdir<Dm>(X, p);
 
     if (!(f < tad_inf()))
