@@ -7,7 +7,8 @@
 //   struct MyEnergy
 //   {
 //       // x: the element's valence * d coordinates, vertex-major (x[a * d + i]),
-//       // p: its parameter row; S is double (passive value) or tad::Dual<K, T>
+//       // p: its parameter row; S is double (passive value), tad::Dual<K, T>
+//       // or tad::HyperDual<T> (derivatives)
 //       template <class S>
 //       __device__ S operator()(const S* x, const double* p) const { ... }
 //   };
@@ -17,7 +18,7 @@
 // S is tad::Dual<K, T> (one pass) for K = valence * d <= 6 and
 // tad::HyperDual<T> (one pass per Hessian entry i <= j) above.
 // Inside the functor use +, -, *, / (with doubles or ints), comparisons and
-// tad::sqr, tad::sqrt, tad::exp, tad::log, tad::sin, tad::cos, tad::pow(S, double);
+// tad::sqr, tad::sqrt, tad::exp, tad::log, tad::sin, tad::cos, tad::pow(S, double).
 // With add_elements_dynamic the same functor is compiled for several
 // valences; an element with fewer handles than its group's valence sees 0 in
 // the unused trailing coordinates (pass the handle count in p when the energy
@@ -28,8 +29,8 @@
 // default-constructed on the device); per-element data comes in p.
 //
 // For each energy, device_element<> instantiates the library's element kernel
-// (forward mode over the element's K = valence * d variables with
-// tad::Dual<K, T>, the dense K x K eigen-projection, scatter through the
+// (forward mode over the element's K = valence * d variables, one Dual<K, T>
+// pass or hyper-dual passes, the dense K x K eigen-projection, scatter through the
 // library's slot maps) for FP64 and FP32 contributions and the three
 // derivative modes, and the line-search energy kernel, all in the caller's
 // code object.  The value returned by every mode is the functor evaluated in
