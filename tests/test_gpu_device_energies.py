@@ -395,3 +395,19 @@ def test_device_tet_newton_matches_catalog(ctx, plugin):
     fc_, fd_ = fc.eval(xc), fd.eval(xd)
     print(f"tets: catalog f {fc_:.15g} device functor f {fd_:.15g}")
     assert abs(fd_ - fc_) <= 1e-8 * abs(fc_)
+
+
+def test_device_tet_mixed_precision(ctx, plugin):
+    """Tets as device functors with FP32 contributions (hyper-dual passes in
+    FP32) against the FP64 catalog: f identical (passive FP64 value), g and
+    H_proj within the mixed-mode bar 2e-5, the Hessian unprojected too."""
+    P = perturbed_cube(4, 0.1, 17)
+    fc = tad.ScalarFunction.from_problem(P, ctx)
+    fm = device_function(P, plugin, ctx, precision="mixed")
+    fa, ga, Ha = fc.eval_with_hessian_proj(P.x0)
+    fb, gb, Hb = fm.eval_with_hessian_proj(P.x0)
+    assert fa == fb
+    assert nrel(gb, ga) <= 2e-5 and nrel(Hb.data, Ha.data) <= 2e-5
+    fa, ga, Ha = fc.eval_with_derivatives(P.x0)
+    fb, gb, Hb = fm.eval_with_derivatives(P.x0)
+    assert nrel(gb, ga) <= 2e-5 and nrel(Hb.data, Ha.data) <= 2e-5
