@@ -480,10 +480,15 @@ auto scalar_function(const HandleRange& variable_range, const EvalSettings& sett
 
 // LinearSolver (LinearSolver.hh:13-19): solver options kept across Newton
 // iterations (the pattern is fixed per function, so nothing is re-analyzed).
+// coarse_refresh = k > 1: newton_direction rebuilds the coarse space of the
+// two-level preconditioner on every k-th call and reuses it in between (the
+// role the reference's cached symbolic analysis plays across Newton steps).
 struct LinearSolver
 {
     tad_solve_opts opts = []() { tad_solve_opts o; tad_solve_default_opts(&o); return o; }();
     tad_solve_stats last_stats {};
+    int coarse_refresh = 1;
+    int reused_in_row = 0;  // calls that reused the coarse space since it was built
 };
 
 // identity(n) (Helpers.hh:82-89): the n x n sparse identity.
@@ -510,8 +515,10 @@ VectorXd newton_direction(const ScalarFunction<dd, H>& func, const VectorXd& g, 
     VectorXd dvec(g.size());
     tad_solve_opts o = solver.opts;
     o.w_identity = w_identity;
+    if (solver.coarse_refresh > 1 && solver.reused_in_row + 1 < solver.coarse_refresh) o.coarse_reuse = 1;
     check(tad_fn_solve(func.handle(), H_proj.values.data(), minus_g.data(), dvec.data(), TAD_MEM_HOST, &o,
                        &solver.last_stats));
+    solver.reused_in_row = solver.last_stats.coarse_reused ? solver.reused_in_row + 1 : 0;
     if (!solver.last_stats.converged) throw std::runtime_error("[tinyad_amd] Linear solve failed.");
     return dvec;
 }

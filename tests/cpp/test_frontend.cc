@@ -269,6 +269,23 @@ static bool test_two_level_solve()
     plain.opts.tol = two.opts.tol = 1e-12;
     two.opts.coarse_aggregate = 32;
     VectorXd d0 = tinyad_amd::newton_direction(func, g, H, plain);
+    {
+        // the reference's Newton loop with a solver that refreshes the coarse
+        // space on every 3rd direction
+        tinyad_amd::LinearSolver lag;
+        lag.coarse_refresh = 3;
+        lag.opts.coarse_aggregate = 32;
+        VectorXd xl = x;
+        std::vector<int> reused;
+        for (int it = 0; it < 5; ++it)
+        {
+            auto [fl, gl, Hl] = func.eval_with_hessian_proj(xl);
+            VectorXd dl = tinyad_amd::newton_direction(func, gl, Hl, lag);
+            reused.push_back(lag.last_stats.coarse_reused);
+            xl = tinyad_amd::line_search(xl, dl, fl, gl, func);
+        }
+        EXPECT((reused == std::vector<int>{ 0, 1, 1, 0, 1 }));
+    }
     VectorXd d1 = tinyad_amd::newton_direction(func, g, H, two);
     double dmax = 0, diff = 0;
     for (size_t i = 0; i < d0.size(); ++i)
