@@ -392,6 +392,12 @@ tad_status tad_line_search(tad_fn fn, const double* x0, const double* dir, doubl
                            double* x_out, tad_mem mem, const tad_newton_opts* opts, double* f_out,
                            double* step_out, int* trials_out);
 
+/* newton_decrement(d, g) = -0.5 d^T g (NewtonDecrement.hh:20-26): the
+ * stopping criterion, with the fixed-order FP64 device dot product of
+ * tad_newton_step (whose tad_newton_stats.newton_decrement it reproduces for
+ * the step's direction).  dir, g: n_vars values in memory space `mem`. */
+tad_status tad_newton_decrement(tad_fn fn, const double* dir, const double* g, tad_mem mem, double* out);
+
 /* --- self test -------------------------------------------------------------- */
 
 /* Runs the device Dual<2> operators on a reference known-answer expression

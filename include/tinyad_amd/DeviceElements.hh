@@ -7,19 +7,24 @@
 //   struct MyEnergy
 //   {
 //       // x: the element's valence * d coordinates, vertex-major (x[a * d + i]),
-//       // p: its parameter row; S is double (passive value), tad::Dual<K, T>
-//       // or tad::HyperDualM<T, M> (derivatives)
+//       // p: its parameter row; S is double (passive value), tad::Dual<K, T, double>
+//       // or tad::HyperDualM<T, M, double> (derivatives)
 //       template <class S>
 //       __device__ S operator()(const S* x, const double* p) const { ... }
 //   };
 //   func.add_elements<3>(range(n_faces), tinyad_amd::device_element<MyEnergy, 3, 2, 5>(),
 //                        conn_of, params_of);
 //
-// S is tad::Dual<K, T> (one pass) for K = valence * d <= 6 and
-// tad::HyperDualM<T, M> above (one pass per M = 2 (FP64) or 1 (FP32) Hessian
-// entries of a row).
+// S is tad::Dual<K, T, double> (one pass) for K = valence * d <= 6 and
+// tad::HyperDualM<T, M, double> above (one pass per M = 2 (FP64) or 1 (FP32)
+// Hessian entries of a row).  Values are FP64 in both precisions (T is the
+// derivative type: double, or float for FP32 contributions), so coordinate
+// differences and every value the functor forms keep FP64 accuracy and only
+// the derivative parts are FP32; an element far from the origin (|x| >> its
+// size) keeps the derivative accuracy of one at the origin.
 // Inside the functor use +, -, *, / (with doubles or ints), comparisons and
-// tad::sqr, tad::sqrt, tad::exp, tad::log, tad::sin, tad::cos, tad::pow(S, double).
+// tad::sqr, tad::sqrt, tad::exp, tad::log, tad::sin, tad::cos, tad::pow(S, double),
+// tad::pow(S, int).
 // With add_elements_dynamic the same functor is compiled for several
 // valences; an element with fewer handles than its group's valence sees 0 in
 // the unused trailing coordinates (pass the handle count in p when the energy
@@ -83,16 +88,17 @@ struct DeviceElement
 #pragma unroll 1
             for (int j0 = i; j0 < j_end; j0 += M)
             {
-                tad::HyperDualM<T, M> hx[K];
+                tad::HyperDualM<T, M, double> hx[K];
 #pragma unroll
                 for (int k = 0; k < K; ++k)
                 {
-                    hx[k] = tad::HyperDualM<T, M>(X[k]);
+                    hx[k] = tad::HyperDualM<T, M, double>(X[k]);
                     hx[k].a = (k == i) ? T(1) : T(0);
 #pragma unroll
                     for (int m = 0; m < M; ++m) hx[k].b[m] = (k == j0 + m) ? T(1) : T(0);
                 }
-                const tad::HyperDualM<T, M> r = F{}(static_cast<const tad::HyperDualM<T, M>*>(hx), p);
+                const tad::HyperDualM<T, M, double> r =
+                    F{}(static_cast<const tad::HyperDualM<T, M, double>*>(hx), p);
                 if (j0 == i)
                 {
 #pragma unroll
@@ -122,10 +128,11 @@ struct DeviceElement
         T g[K], Hp[tad::Sym<K>::n];
         if constexpr (K <= kFullDualMaxK)
         {
-            tad::Dual<K, T> v[K];
+            using S = tad::Dual<K, T, double>;
+            S v[K];
 #pragma unroll
-            for (int i = 0; i < K; ++i) v[i] = (i < na) ? tad::dual_var<K, T>(T(X[i]), i) : tad::Dual<K, T>(X[i]);
-            const tad::Dual<K, T> E = F{}(static_cast<const tad::Dual<K, T>*>(v), p);
+            for (int i = 0; i < K; ++i) v[i] = (i < na) ? tad::dual_var<K, T, double>(X[i], i) : S(X[i]);
+            const S E = F{}(static_cast<const S*>(v), p);
 #pragma unroll
             for (int i = 0; i < K; ++i) g[i] = E.g[i];
 #pragma unroll

@@ -432,6 +432,15 @@ public:
 
     tad_fn handle() const { return fn_.get(); }
 
+    // Non-zeros of the Hessian pattern (explicit zeros included).
+    int64_t nnz() const
+    {
+        require_fn();
+        int64_t nz = 0;
+        check(tad_fn_pattern(fn_.get(), &nz, nullptr, nullptr, nullptr, nullptr));
+        return nz;
+    }
+
     EvalSettings settings;
     int64_t n_vars = 0;
     int64_t n_elements = 0;
@@ -489,6 +498,10 @@ struct LinearSolver
     tad_solve_stats last_stats {};
     int coarse_refresh = 1;
     int reused_in_row = 0;  // calls that reused the coarse space since it was built
+    // set once this solver's own call has built the coarse space: the first
+    // call of a solver always builds it, whatever an earlier solver on the
+    // same function left behind
+    bool coarse_built = false;
 };
 
 // identity(n) (Helpers.hh:82-89): the n x n sparse identity.
@@ -510,15 +523,23 @@ template <int dd, class H>
 VectorXd newton_direction(const ScalarFunction<dd, H>& func, const VectorXd& g, const SparseMatrix& H_proj,
                           LinearSolver& solver, double w_identity = 0.0)
 {
+    const int64_t n = func.n_vars;
+    if ((int64_t)g.size() != n || H_proj.rows() != n || H_proj.cols() != n ||
+        (int64_t)H_proj.values.size() != func.nnz())
+        throw std::runtime_error("[tinyad_amd] newton_direction: size mismatch");
     VectorXd minus_g(g.size());
     for (size_t i = 0; i < g.size(); ++i) minus_g[i] = -g[i];
     VectorXd dvec(g.size());
     tad_solve_opts o = solver.opts;
     o.w_identity = w_identity;
-    if (solver.coarse_refresh > 1 && solver.reused_in_row + 1 < solver.coarse_refresh) o.coarse_reuse = 1;
+    if (!solver.coarse_built)
+        o.coarse_reuse = 0;
+    else if (solver.coarse_refresh > 1 && solver.reused_in_row + 1 < solver.coarse_refresh)
+        o.coarse_reuse = 1;
     check(tad_fn_solve(func.handle(), H_proj.values.data(), minus_g.data(), dvec.data(), TAD_MEM_HOST, &o,
                        &solver.last_stats));
     solver.reused_in_row = solver.last_stats.coarse_reused ? solver.reused_in_row + 1 : 0;
+    if (solver.last_stats.coarse_unknowns > 0 && !solver.last_stats.coarse_reused) solver.coarse_built = true;
     if (!solver.last_stats.converged) throw std::runtime_error("[tinyad_amd] Linear solve failed.");
     return dvec;
 }

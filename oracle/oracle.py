@@ -50,7 +50,7 @@ def lib():
         L.or_time_eval_with_hessian_proj.restype = ctypes.c_double
         L.or_time_eval_with_hessian_proj.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, ctypes.c_int]
         L.or_pcg.argtypes = [ctypes.c_long, ctypes.c_int, _ip, _ip, _dp, ctypes.c_double, _dp, _dp,
-                             ctypes.c_double, ctypes.c_int, _dp]
+                             ctypes.c_double, ctypes.c_int, _dp, ctypes.c_int]
         L.or_line_search.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, _dp, ctypes.c_double,
                                      ctypes.c_double, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
         L.or_dual_known_answer.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp]
@@ -172,8 +172,11 @@ class Function:
         return xo, float(fo[0]), float(so[0]), n
 
 
-def pcg(H, b, w: float = 0.0, block: int = 1, tol: float = 1e-10, max_iters: int = 10000):
-    """Block-Jacobi PCG on (w I + H) x = b with x0 = 0 (same algorithm as the GPU solve)."""
+def pcg(H, b, w: float = 0.0, block: int = 1, tol: float = 1e-10, max_iters: int = 10000, n_threads: int = 0):
+    """Block-Jacobi PCG on (w I + H) x = b with x0 = 0 (the GPU solve's
+    recurrences without its coarse correction).  n_threads: OpenMP threads
+    (0: the reference's default, available - 1); the result does not depend
+    on it."""
     H = H.tocsr()
     rp = np.ascontiguousarray(H.indptr, dtype=np.int32)
     ci = np.ascontiguousarray(H.indices, dtype=np.int32)
@@ -181,7 +184,7 @@ def pcg(H, b, w: float = 0.0, block: int = 1, tol: float = 1e-10, max_iters: int
     b = np.ascontiguousarray(b, dtype=np.float64)
     x = np.zeros_like(b)
     rel = np.zeros(1)
-    it = lib().or_pcg(b.size, block, _i(rp), _i(ci), _d(v), w, _d(b), _d(x), tol, max_iters, _d(rel))
+    it = lib().or_pcg(b.size, block, _i(rp), _i(ci), _d(v), w, _d(b), _d(x), tol, max_iters, _d(rel), n_threads)
     return x, it, float(rel[0])
 
 

@@ -617,6 +617,26 @@ inline double dot(long n, const double* a, const double* b)
     return s;
 }
 
+// The same sum taken in chunks of 8192 terms (each chunk in order, the chunk
+// sums in order): the result does not depend on the thread count.
+inline double dot_chunked(long n, const double* a, const double* b)
+{
+    constexpr long kChunk = 8192;
+    const long nc = (n + kChunk - 1) / kChunk;
+    std::vector<double> part(nc);
+    #pragma omp parallel for schedule(static)
+    for (long c = 0; c < nc; ++c)
+    {
+        double s = 0.0;
+        const long e = std::min(n, (c + 1) * kChunk);
+        for (long i = c * kChunk; i < e; ++i) s += a[i] * b[i];
+        part[c] = s;
+    }
+    double s = 0.0;
+    for (long c = 0; c < nc; ++c) s += part[c];
+    return s;
+}
+
 // Invert the bs x bs diagonal blocks of (w I + A) (bs = 1, 2 or 3).
 inline void block_jacobi_inverse(long n, int bs, const int* rp, const int* ci, const double* v,
                                  double w, std::vector<double>& Dinv)
@@ -668,8 +688,10 @@ inline int pcg(long n, int bs, const int* rp, const int* ci, const double* v, do
     std::vector<double> Dinv;
     block_jacobi_inverse(n, bs, rp, ci, v, w, Dinv);
     std::vector<double> r(b, b + n), z(n), p(n), q(n);
+    #pragma omp parallel for schedule(static)
     for (long i = 0; i < n; ++i) x[i] = 0.0;
     auto apply_prec = [&](const double* in, double* out) {
+        #pragma omp parallel for schedule(static)
         for (long blk = 0; blk < n / bs; ++blk)
             for (int i = 0; i < bs; ++i)
             {
@@ -678,26 +700,32 @@ inline int pcg(long n, int bs, const int* rp, const int* ci, const double* v, do
                 out[blk * bs + i] = s;
             }
     };
-    const double bnorm = std::sqrt(dot(n, b, b));
+    const double bnorm = std::sqrt(dot_chunked(n, b, b));
     if (bnorm == 0.0) { if (rel_res) *rel_res = 0.0; return 0; }
     apply_prec(r.data(), z.data());
     p = z;
-    double rho = dot(n, r.data(), z.data());
+    double rho = dot_chunked(n, r.data(), z.data());
     int it = 0;
     double rr = bnorm * bnorm;
     while (it < max_iters && std::sqrt(rr) > tol * bnorm)
     {
         spmv(n, rp, ci, v, p.data(), q.data());
-        if (w != 0.0) for (long i = 0; i < n; ++i) q[i] += w * p[i];
-        const double pq = dot(n, p.data(), q.data());
+        if (w != 0.0)
+        {
+            #pragma omp parallel for schedule(static)
+            for (long i = 0; i < n; ++i) q[i] += w * p[i];
+        }
+        const double pq = dot_chunked(n, p.data(), q.data());
         if (!(pq > 0.0)) return -1;
         const double alpha = rho / pq;
+        #pragma omp parallel for schedule(static)
         for (long i = 0; i < n; ++i) { x[i] += alpha * p[i]; r[i] -= alpha * q[i]; }
         apply_prec(r.data(), z.data());
-        const double rho_new = dot(n, r.data(), z.data());
-        rr = dot(n, r.data(), r.data());
+        const double rho_new = dot_chunked(n, r.data(), z.data());
+        rr = dot_chunked(n, r.data(), r.data());
         const double beta = rho_new / rho;
         rho = rho_new;
+        #pragma omp parallel for schedule(static)
         for (long i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
         ++it;
     }
@@ -826,10 +854,23 @@ double or_time_eval_with_hessian_proj(void* fnp, const double* x, double eps, in
     return ts[ts.size() / 2];
 }
 
+// Block-Jacobi PCG on (w I + A) x = b.  n_threads: OpenMP threads of its
+// products and vector updates (0: the reference's default, max(1,
+// omp_get_max_threads() - 1), Parallel.hh:20-31); the iterates do not depend
+// on it (chunked, ordered dot products).
 int or_pcg(long n, int bs, const int* rp, const int* ci, const double* v, double w,
-           const double* b, double* x, double tol, int max_iters, double* rel_res)
+           const double* b, double* x, double tol, int max_iters, double* rel_res, int n_threads)
 {
+#ifdef _OPENMP
+    const int prev = omp_get_max_threads();
+    omp_set_num_threads(n_threads >= 1 ? n_threads : std::max(1, prev - 1));
+    const int it = pcg(n, bs, rp, ci, v, w, b, x, tol, max_iters, rel_res);
+    omp_set_num_threads(prev);
+    return it;
+#else
+    (void)n_threads;
     return pcg(n, bs, rp, ci, v, w, b, x, tol, max_iters, rel_res);
+#endif
 }
 
 // Armijo backtracking line search (LineSearch.hh:26-65).  Writes the accepted

@@ -91,6 +91,39 @@ struct Smooth4
     }
 };
 
+// The same helpers on four 2D handles (K = 8, hyper-dual passes), with
+// active / active division and an integer power:
+// p0 exp(0.3 x0 x7) + log(1 + x1^2) + sqrt(1 + x2^2 + x4^2) + sin(x3) cos(x5)
+//   + p1 (1 + x6^2)^1.5 + 1 / (2 + x0^2) + x5 / (1.5 + x7^2) + (x2 - x6)^3.
+struct Smooth8
+{
+    template <class S>
+    __device__ S operator()(const S* x, const double* p) const
+    {
+        return p[0] * tad::exp(0.3 * x[0] * x[7]) + tad::log(1 + x[1] * x[1]) +
+               tad::sqrt(1.0 + x[2] * x[2] + x[4] * x[4]) + tad::sin(x[3]) * tad::cos(x[5]) +
+               p[1] * tad::pow(1.0 + x[6] * x[6], 1.5) + 1.0 / (2 + x[0] * x[0]) + x[5] / (1.5 + x[7] * x[7]) +
+               tad::pow(x[2] - x[6], 3);
+    }
+};
+
+// Powers with the base at 0 allowed (Scalar.hh:266-306): x0^2 and x1^3 with
+// double exponents, x2^1, x3^2, x4^3 and x5^0 with int exponents, and x_k^2
+// (double exponent) for the coordinates of handles past the third; N 2D
+// handles, K = 2N (N = 3: one Dual pass, N = 4: hyper-dual passes).
+template <int N>
+struct Powers
+{
+    template <class S>
+    __device__ S operator()(const S* x, const double*) const
+    {
+        S e = tad::pow(x[0], 2.0) + tad::pow(x[1], 3.0) + tad::pow(x[2], 1) + tad::pow(x[3], 2) +
+              tad::pow(x[4], 3) + tad::pow(x[5], 0);
+        for (int k = 6; k < 2 * N; ++k) e += tad::pow(x[k], 2.0);
+        return e;
+    }
+};
+
 // ||sum of the element's 2D handle values||^2, DynamicElementsTest.cc:9-33
 // (padding slots read as 0, so the sum runs over the real handles only).
 struct SumSquares2
@@ -134,6 +167,9 @@ const tad_device_ops g_tri = tinyad_amd::device_element<SymdirTri, 3, 2, 5>();
 const tad_device_ops g_tet = tinyad_amd::device_element<SymdirTet, 4, 3, 10>();
 const tad_device_ops g_quad = tinyad_amd::device_element<Quadratic4, 2, 2, 21>();
 const tad_device_ops g_smooth = tinyad_amd::device_element<Smooth4, 2, 2, 2>();
+const tad_device_ops g_smooth8 = tinyad_amd::device_element<Smooth8, 4, 2, 2>();
+const tad_device_ops g_powers[2] = { tinyad_amd::device_element<Powers<3>, 3, 2, 0>(),
+                                     tinyad_amd::device_element<Powers<4>, 4, 2, 0>() };
 const tad_device_ops g_sum_sq[2] = { tinyad_amd::device_element<SumSquares2::At<3>, 3, 2, 0>(),
                                      tinyad_amd::device_element<SumSquares2::At<1>, 1, 2, 0>() };
 const tad_device_ops g_one_ring[4] = { tinyad_amd::device_element<OneRing::At<6>, 6, 1, 1>(),
@@ -148,6 +184,9 @@ const tad_device_ops* tad_test_symdir_tri_ops() { return &g_tri; }
 const tad_device_ops* tad_test_symdir_tet_ops() { return &g_tet; }
 const tad_device_ops* tad_test_quadratic4_ops() { return &g_quad; }
 const tad_device_ops* tad_test_smooth4_ops() { return &g_smooth; }
+const tad_device_ops* tad_test_smooth8_ops() { return &g_smooth8; }
+// 3 and 4 handles (K = 6 and 8)
+const tad_device_ops* tad_test_powers_ops(int i) { return (i >= 0 && i < 2) ? &g_powers[i] : nullptr; }
 // valences 3 and 1, in that order (add_elements_dynamic<3, 1>)
 const tad_device_ops* tad_test_sum_squares_ops(int i) { return (i >= 0 && i < 2) ? &g_sum_sq[i] : nullptr; }
 // valences 6, 7, 8, 10 (add_elements_dynamic<6, 7, 8, 10>)

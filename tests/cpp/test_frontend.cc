@@ -285,6 +285,29 @@ static bool test_two_level_solve()
             xl = tinyad_amd::line_search(xl, dl, fl, gl, func);
         }
         EXPECT((reused == std::vector<int>{ 0, 1, 1, 0, 1 }));
+        // a fresh solver on the same function builds its own coarse space on
+        // its first call, although a compatible one is cached on the function
+        tinyad_amd::LinearSolver fresh;
+        fresh.coarse_refresh = 3;
+        fresh.opts.coarse_aggregate = 32;
+        auto [ff, gf, Hf] = func.eval_with_hessian_proj(xl);
+        tinyad_amd::newton_direction(func, gf, Hf, fresh);
+        EXPECT(fresh.last_stats.coarse_reused == 0 && fresh.last_stats.coarse_unknowns > 0);
+        tinyad_amd::newton_direction(func, gf, Hf, fresh);
+        EXPECT(fresh.last_stats.coarse_reused == 1);
+        // a Hessian of the wrong size is rejected before the solve
+        bool threw = false;
+        try
+        {
+            tinyad_amd::SparseMatrix Hbad = Hf;
+            Hbad.values.pop_back();
+            tinyad_amd::newton_direction(func, gf, Hbad, fresh);
+        }
+        catch (const std::runtime_error&)
+        {
+            threw = true;
+        }
+        EXPECT(threw);
     }
     VectorXd d1 = tinyad_amd::newton_direction(func, g, H, two);
     double dmax = 0, diff = 0;

@@ -11,7 +11,12 @@ tests/cpp/bin/libtad_test_device_energies.so.  Pins:
   * an energy using sqrt / exp / log / sin / cos / pow against central
     differences of its own passive energy, and its projected Hessian against
     numpy's eigen-projection of its Hessian (HessianProjection.hh:48-101);
-  * mixed precision within the tolerance of the catalog's mixed mode;
+  * the same helpers on a K = 8 element (hyper-dual passes) in FP64 and with
+    FP32 derivatives;
+  * powers at base 0 with double and int exponents (Scalar.hh:266-306) on
+    both derivative paths, against their closed forms;
+  * mixed precision within the tolerance of the catalog's mixed mode, also
+    for meshes translated far from the origin;
   * a Newton trajectory equal to the catalog's to the converged-energy bar.
 """
 import ctypes
@@ -42,7 +47,7 @@ def plugin():
         raise RuntimeError(f"{PLUGIN} is missing: run python -m tinyad_amd.build first")
     L = ctypes.CDLL(PLUGIN)
     ops = {}
-    for name in ["symdir_tri", "symdir_tet", "quadratic4", "smooth4"]:
+    for name in ["symdir_tri", "symdir_tet", "quadratic4", "smooth4", "smooth8"]:
         f = getattr(L, f"tad_test_{name}_ops")
         f.restype = ctypes.c_void_p
         ops[name] = f()
@@ -183,6 +188,108 @@ def test_device_transcendental_derivatives(ctx, plugin):
         for eps in [1e-9, -1.0, -0.01, 0.5]:
             _, _, Hp = fd.eval_with_hessian_proj(x, eps)
             assert nrel(Hp.toarray(), reference_projection(Hd, eps)) <= 1e-12, (trial, eps, np.linalg.eigvalsh(Hd))
+
+
+def smooth8_energy(x, p):
+    """The passive energy of Smooth8 (tests/cpp/device_energies.hip) in numpy."""
+    return (p[0] * np.exp(0.3 * x[0] * x[7]) + np.log(1 + x[1] ** 2) + np.sqrt(1 + x[2] ** 2 + x[4] ** 2)
+            + np.sin(x[3]) * np.cos(x[5]) + p[1] * (1 + x[6] ** 2) ** 1.5 + 1 / (2 + x[0] ** 2)
+            + x[5] / (1.5 + x[7] ** 2) + (x[2] - x[6]) ** 3)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_device_transcendental_derivatives_k8(ctx, plugin, precision):
+    """One Smooth8 element (K = 8: hyper-dual passes, FP64 or FP32
+    derivative parts): value against numpy (FP64 in both precisions),
+    gradient against central differences of the numpy energy, Hessian
+    against central differences of the FP64 gradient, projection against
+    numpy's eigen-projection of the element's own Hessian.  Bars: FP64 1e-9
+    (g) / 1e-8 (H) / 1e-12 (projection); FP32 derivatives 2e-5 normwise."""
+    rng = np.random.default_rng(19)
+    p = np.array([0.7, 0.4])
+    fd = tad.ScalarFunction(2, 4, ctx, precision)
+    fd.add_elements_device(plugin["smooth8"], [[0, 1, 2, 3]], p)
+    f64 = tad.ScalarFunction(2, 4, ctx)
+    f64.add_elements_device(plugin["smooth8"], [[0, 1, 2, 3]], p)
+    tol_g, tol_h = (1e-9, 1e-8) if precision == "fp64" else (2e-5, 2e-5)
+    for trial in range(4):
+        x = rng.uniform(-1.2, 1.2, 8)
+        f, g, H = fd.eval_with_derivatives(x)
+        assert abs(f - smooth8_energy(x, p)) <= 1e-14 * abs(f)
+        h = 1e-5
+        gfd = np.array([(smooth8_energy(x + h * e, p) - smooth8_energy(x - h * e, p)) / (2 * h) for e in np.eye(8)])
+        assert nrel(g, gfd) <= tol_g
+        Hd = H.toarray()
+        Hfd = np.array([(f64.eval_with_gradient(x + h * e)[1] - f64.eval_with_gradient(x - h * e)[1]) / (2 * h)
+                        for e in np.eye(8)])
+        assert np.array_equal(Hd, Hd.T)
+        assert nrel(Hd, Hfd) <= tol_h
+        for eps in [1e-9, -1.0, 0.5]:
+            _, _, Hp = fd.eval_with_hessian_proj(x, eps)
+            tol_p = 1e-12 if precision == "fp64" else 1e-5
+            assert nrel(Hp.toarray(), reference_projection(Hd, eps)) <= tol_p, (trial, eps)
+
+
+@pytest.mark.parametrize("handles", [3, 4])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_device_pow_base_zero(ctx, plugin, handles, precision):
+    """pow at base 0 (Scalar.hh:266-306): x^2 and x^3 with double exponents
+    and x^1, x^2, x^3, x^0 with int exponents stay finite and exact at x = 0
+    (f = 1, g = e_2, H = diag(2, 0, 0, 2, 0, 0, 2, 2)), on the one-pass Dual
+    path (3 handles, K = 6) and the hyper-dual path (4 handles, K = 8); at a
+    random point they match the closed forms."""
+    L = plugin["_lib"]
+    L.tad_test_powers_ops.restype = ctypes.c_void_p
+    L.tad_test_powers_ops.argtypes = [ctypes.c_int]
+    K = 2 * handles
+    fn = tad.ScalarFunction(2, handles, ctx, precision)
+    fn.add_elements_device(L.tad_test_powers_ops(handles - 3), [list(range(handles))])
+    f, g, H = fn.eval_with_derivatives(np.zeros(K))
+    assert f == 1.0
+    assert np.array_equal(g, np.eye(K)[2])
+    assert np.array_equal(H.toarray(), np.diag([2.0, 0, 0, 2.0, 0, 0, 2.0, 2.0][:K]))
+    fp, gp, Hp = fn.eval_with_hessian_proj(np.zeros(K))
+    assert fp == 1.0 and np.array_equal(gp, g)
+    assert np.allclose(Hp.toarray(), np.diag([2.0, 1e-9, 1e-9, 2.0, 1e-9, 1e-9, 2.0, 2.0][:K]), rtol=1e-6, atol=0)
+    x = np.random.default_rng(handles).uniform(-1.5, 1.5, K)
+    f, g, H = fn.eval_with_derivatives(x)
+    fr = x[0] ** 2 + x[1] ** 3 + x[2] + x[3] ** 2 + x[4] ** 3 + 1.0 + (x[6:] ** 2).sum()
+    gr = np.array([2 * x[0], 3 * x[1] ** 2, 1.0, 2 * x[3], 3 * x[4] ** 2, 0.0] + list(2 * x[6:]))
+    Hr = np.diag([2.0, 6 * x[1], 0.0, 2.0, 6 * x[4], 0.0] + [2.0] * (K - 6))
+    tol = 1e-14 if precision == "fp64" else 1e-6
+    assert abs(f - fr) <= 1e-14 * abs(fr)
+    assert nrel(g, gr) <= tol and nrel(H.toarray(), Hr) <= tol
+
+
+def translated(P, offset):
+    """P moved by `offset` in every coordinate: positions and penalty targets
+    (element rest parameters are translation invariant)."""
+    import copy
+    Q = copy.deepcopy(P)
+    Q.x0 = P.x0 + offset
+    for t in Q.terms:
+        if t.energy == tad.PENALTY:
+            t.params = t.params.copy()
+            t.params[:, :P.d] += offset
+    return Q
+
+
+@pytest.mark.parametrize("kind", ["tets", "triangles"])
+def test_device_mixed_precision_far_from_origin(ctx, plugin, kind):
+    """Device functors with FP32 derivatives on a mesh translated by 1e3
+    (|x| / h up to 4e3 for tets, 1.1e4 for triangles) against the FP64
+    catalog: the functor's values stay FP64, so the mixed bar 2e-5 holds as
+    at the origin.  The catalog's own mixed mode is checked the same way."""
+    P = perturbed_cube(4, 0.1, 17) if kind == "tets" else configs.grid_param(12, seed=5)
+    Q = translated(P, 1000.0)
+    fc = tad.ScalarFunction.from_problem(Q, ctx)
+    fa, ga, Ha = fc.eval_with_hessian_proj(Q.x0)
+    for fm in (device_function(Q, plugin, ctx, precision="mixed"),
+               tad.ScalarFunction.from_problem(Q, ctx, precision="mixed")):
+        fb, gb, Hb = fm.eval_with_hessian_proj(Q.x0)
+        assert abs(fb - fa) <= 1e-12 * abs(fa)
+        print(f"{kind} at 1e3: g {nrel(gb, ga):.2e} H {nrel(Hb.data, Ha.data):.2e}")
+        assert nrel(gb, ga) <= 2e-5 and nrel(Hb.data, Ha.data) <= 2e-5
 
 
 def test_device_mixed_precision(ctx, plugin):

@@ -659,3 +659,160 @@ def test_autotune_keeps_bits(ctx, precision, kind):
         with pytest.raises(tad.TadError):
             fg.set_launch_shape(len(P.terms) - 1, 64, 8, 0)
     assert (fg.launch_shape(0), fg.assembly_shape) == tuned
+
+
+# ---------------------------------------------------------------------------
+# Line search and Newton decrement (LineSearch.hh:14-65, NewtonDecrement.hh:20-26)
+# ---------------------------------------------------------------------------
+
+def quadratic_chain(n=60, seed=3):
+    """A strictly convex quadratic over n 2D handles: SPD 4x4 blocks on
+    consecutive handle pairs plus unit penalties (catalog QUADRATIC and
+    PENALTY elements), so f(x + s d) is an exact parabola in s."""
+    rng = np.random.default_rng(seed)
+    conn = np.stack([np.arange(n - 1), np.arange(1, n)], axis=1).astype(np.int32)
+    rows = []
+    for _ in range(n - 1):
+        Q, _ = np.linalg.qr(rng.standard_normal((4, 4)))
+        A = (Q * rng.uniform(0.5, 3.0, 4)) @ Q.T
+        rows.append(quad_params(0.5 * (A + A.T), rng.standard_normal(4), 0.0))
+    pins = np.arange(n, dtype=np.int32)[:, None]
+    pen = np.concatenate([rng.standard_normal((n, 2)), np.full((n, 1), 0.5)], axis=1)
+    x0 = 3.0 * rng.standard_normal(2 * n)
+    return conn, np.array(rows), pins, pen, x0
+
+
+def quadratic_functions(ctx, n=60, seed=3):
+    conn, rows, pins, pen, x0 = quadratic_chain(n, seed)
+    fg = tad.ScalarFunction(2, n, ctx)
+    fg.add_elements(2, tad.QUADRATIC, conn, rows)
+    fg.add_elements(1, tad.PENALTY, pins, pen)
+    fo = oracle.Function(2, n)
+    fo.add_elements(tad.QUADRATIC, conn, rows)
+    fo.add_elements(tad.PENALTY, pins, pen)
+    return fg, fo, x0
+
+
+def both_line_searches(fg, fo, x0, d, s_max=1.0):
+    """The same (x0, d, f, g) through the GPU search and the oracle's."""
+    import torch
+    f, g, _ = fo.eval_with_hessian_proj(x0)
+    xo, fo_new, so, no = fo.line_search(x0, d, f, g, s_max=s_max)
+    dev = torch.device("cuda", 0)
+    xt, dt, gt = (torch.tensor(a, device=dev) for a in (x0, d, g))
+    xg, fg_new, sg, ng = fg.line_search(xt, dt, f, gt, opts=tad.newton_default_opts(ls_s_max=s_max))
+    return (xo, fo_new, so, no), (xg.cpu().numpy(), fg_new, sg, ng), f, g
+
+
+@pytest.mark.parametrize("alpha,s_max,step,trials", [
+    (1.0, 1.0, 1.0, 1),          # the full Newton step of a quadratic
+    (1.98, 1.0, 1.0, 1),         # s alpha = 1.98 <= 2 (1 - 1e-4): accepted at once
+    (1.98, 2.0, 1.0, 5),         # s_max > 1: 2, 1.6, 1.28, 1.024 rejected, then exactly 1 (LineSearch.hh:40, :55-58)
+    (50.0, 1.0, 15, 16),         # 15 shrinks (4 candidate batches of 4)
+])
+def test_line_search_matches_oracle_on_quadratic(ctx, alpha, s_max, step, trials):
+    """Armijo backtracking on an exact quadratic, direction alpha * d_Newton:
+    f(x + s d) - f = -s alpha q + (s alpha)^2 q / 2 with q = d_N^T H d_N, so
+    the Armijo test holds iff s alpha <= 2 (1 - 1e-4).  The GPU search and the
+    oracle accept the same step after the same number of trials; the step is
+    the closed-form one; f_new to 1e-12, x_new to 1e-14.  An integer `step`
+    means that many shrinks of s = 1 by 0.8, in the search's own arithmetic."""
+    import scipy.sparse.linalg as spla
+    if isinstance(step, int):
+        k, step = step, 1.0
+        for _ in range(k):
+            step *= 0.8
+    fg, fo, x0 = quadratic_functions(ctx)
+    f, g, H = fo.eval_with_hessian_proj(x0)
+    dN = -spla.spsolve(H.tocsc(), g)
+    (xo, fo_new, so, no), (xg, fg_new, sg, ng), f, g = both_line_searches(fg, fo, x0, alpha * dN, s_max)
+    print(f"alpha {alpha} s_max {s_max}: oracle step {so} trials {no}, GPU step {sg} trials {ng}")
+    assert no == trials and ng == trials
+    assert so == step and sg == step
+    assert abs(fg_new - fo_new) <= 1e-12 * abs(fo_new)
+    assert fg_new <= f + 1e-4 * sg * float(alpha * dN @ g)
+    assert nrel(xg, xo) <= 1e-14
+
+
+def test_line_search_failure_and_errors(ctx):
+    """An ascent direction: no step passes Armijo within 64 trials; the search
+    returns x0 bit for bit, step 0 and the energy f (LineSearch.hh:62-64),
+    as the oracle does.  s_max <= 0 is rejected (LineSearch.hh:36-37); a NaN
+    energy is an error (LineSearch.hh:52)."""
+    import scipy.sparse.linalg as spla
+    import torch
+    fg, fo, x0 = quadratic_functions(ctx)
+    f, g, H = fo.eval_with_hessian_proj(x0)
+    dN = -spla.spsolve(H.tocsc(), g)
+    (xo, fo_new, so, no), (xg, fg_new, sg, ng), f, g = both_line_searches(fg, fo, x0, -dN)
+    assert no == -64 and ng == -64
+    assert so == 0.0 and sg == 0.0 and fg_new == f and fo_new == f
+    assert np.array_equal(xg, x0) and np.array_equal(xo, x0)
+    dev = torch.device("cuda", 0)
+    xt, dt, gt = (torch.tensor(a, device=dev) for a in (x0, dN, g))
+    with pytest.raises(tad.TadError) as ei:
+        fg.line_search(xt, dt, f, gt, opts=tad.newton_default_opts(ls_s_max=0.0))
+    assert ei.value.status == 1
+    with pytest.raises(oracle.OracleError):
+        fo.line_search(x0, dN, f, g, s_max=0.0)
+    dnan = dt.clone()
+    dnan[3] = float("nan")
+    with pytest.raises(tad.TadError) as ei:
+        fg.line_search(xt, dnan, f, gt)
+    assert ei.value.status == 5
+    with pytest.raises(oracle.OracleError):
+        fo.line_search(x0, dnan.cpu().numpy(), f, g)
+    with pytest.raises(tad.TadError):  # a direction of the wrong length
+        fg.line_search(xt, dt[:-1].contiguous(), f, gt)
+
+
+def test_line_search_matches_oracle_on_tets(ctx):
+    """The projected Newton direction of a twisted cube (symmetric Dirichlet:
+    not quadratic, steps < 1 at x0): same accepted step and trial count as the
+    oracle, f_new to 1e-12, x_new to 1e-14."""
+    import scipy.sparse.linalg as spla
+    P = configs.twist_cube(m=6, seed=9)
+    fo = oracle.Function.from_problem(P)
+    fg = tad.ScalarFunction.from_problem(P, ctx)
+    f, g, H = fo.eval_with_hessian_proj(P.x0)
+    d = -spla.spsolve(H.tocsc(), g)
+    for scale in (1.0, 3.0):
+        (xo, fo_new, so, no), (xg, fg_new, sg, ng), f, g = both_line_searches(fg, fo, P.x0, scale * d)
+        print(f"tets scale {scale}: oracle step {so} trials {no}, GPU step {sg} trials {ng}")
+        assert no == ng and so == sg and no > 0
+        assert abs(fg_new - fo_new) <= 1e-12 * abs(fo_new)
+        assert nrel(xg, xo) <= 1e-14
+
+
+def test_newton_decrement(ctx):
+    """newton_decrement(d, g) = -0.5 d^T g (NewtonDecrement.hh:20-26): against
+    an exactly rounded sum (math.fsum) to 1e-14; for an exact quadratic and
+    the exact Newton direction it equals f(x) - f(x + d) (the drop of the
+    quadratic model, the reference's definition); and tad_newton_step's
+    reported decrement is the one of its own direction, bit for bit."""
+    import math
+    import scipy.sparse.linalg as spla
+    import torch
+    fg, fo, x0 = quadratic_functions(ctx)
+    rng = np.random.default_rng(1)
+    g = rng.standard_normal(fg.n_vars)
+    d = -g * rng.uniform(0.1, 2.0, fg.n_vars)
+    ref = -0.5 * math.fsum(d * g)
+    assert abs(fg.newton_decrement(d, g) - ref) <= 1e-14 * abs(ref)
+    dev = torch.device("cuda", 0)
+    assert fg.newton_decrement(torch.tensor(d, device=dev), torch.tensor(g, device=dev)) == fg.newton_decrement(d, g)
+    f, g, H = fo.eval_with_hessian_proj(x0)
+    dN = -spla.spsolve(H.tocsc(), g)
+    lam = fg.newton_decrement(dN, g)
+    assert abs(lam - (f - fo.eval(x0 + dN))) <= 1e-10 * abs(lam)
+    # the Newton step's own decrement
+    P = configs.twist_cube(m=5, seed=4)
+    ft = tad.ScalarFunction.from_problem(P, ctx)
+    xt = torch.tensor(P.x0, device=dev)
+    opts = tad.newton_default_opts(solve_tol=1e-12)
+    fv, gv, Hv = ft.eval_with_hessian_proj(xt)
+    sol, st = ft.solve(gv, H=Hv, tol=1e-12)
+    dec = ft.newton_decrement(-sol, gv)
+    stn = ft.newton_step(xt.clone(), opts)
+    print(f"decrement {dec:.17g}, newton_step {stn['newton_decrement']:.17g}")
+    assert stn["newton_decrement"] == dec and dec > 0.0

@@ -430,9 +430,32 @@ class ScalarFunction:
         check(lib().tad_fn_set_assembly_shape(self._h, int(shape)))
 
     # -- solver ------------------------------------------------------------
+    def _dev_vec(self, a, n: int, name: str):
+        """a checked as a contiguous float64 CUDA tensor of n entries on the context's device."""
+        import torch
+        if not (_is_torch(a) and a.device.type == "cuda" and a.dtype == torch.float64 and a.is_contiguous()):
+            raise TadError(1, f"{name} must be a contiguous float64 CUDA tensor")
+        if a.device.index != self.ctx.device:
+            raise TadError(1, f"{name} is on cuda:{a.device.index}, the function on cuda:{self.ctx.device}")
+        if a.numel() != n:
+            raise TadError(1, f"{name} has {a.numel()} entries, expected {n}")
+        return a
+
+    def _host_vec(self, a, n: int, name: str):
+        """a as a contiguous float64 numpy array of n entries (a sparse matrix gives its values)."""
+        if hasattr(a, "tocsr") and not isinstance(a, np.ndarray):
+            a = a.data
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.size != n:
+            raise TadError(1, f"{name} has {a.size} entries, expected {n}")
+        return a.ravel()
+
     def solve(self, rhs, H=None, free=None, **opts):
         """Solves (w I + H) sol = rhs with the library's PCG; H=None uses the last evaluated Hessian.
 
+        rhs: n_vars values (numpy, or a float64 CUDA tensor: then H, if
+        given, is the CUDA tensor of the nnz CSR values); H on the host: the
+        nnz values in the pattern order or a scipy matrix over the pattern.
         free: optional per-variable mask (nonzero = free) of a reduced-basis
         solve (newton_direction_reduced_basis with a selection basis): CG on
         the free variables, sol = 0 on the fixed ones."""
@@ -440,9 +463,10 @@ class ScalarFunction:
         st = SolveStats()
         if _is_torch(rhs):
             import torch
+            self._dev_vec(rhs, self.n_vars, "rhs")
+            hp = self._dev_vec(H, self.nnz, "H").data_ptr() if H is not None else None
             _order_after_torch(self.ctx, rhs.device)
             sol = torch.empty_like(rhs)
-            hp = H.data_ptr() if H is not None else None
             if free is None:
                 check(lib().tad_fn_solve(self._h, hp, rhs.data_ptr(), sol.data_ptr(), _lib.MEM_DEVICE,
                                          ctypes.byref(o), ctypes.byref(st)))
@@ -453,12 +477,9 @@ class ScalarFunction:
                 check(lib().tad_fn_solve_reduced(self._h, hp, rhs.data_ptr(), sol.data_ptr(), _lib.MEM_DEVICE,
                                                  fm.data_ptr(), ctypes.byref(o), ctypes.byref(st)))
             return sol, st
-        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        rhs = self._host_vec(rhs, self.n_vars, "rhs")
         sol = np.zeros_like(rhs)
-        hv = None
-        if H is not None:
-            hv = np.ascontiguousarray(H.data if hasattr(H, "data") and not isinstance(H, np.ndarray) else H,
-                                      dtype=np.float64)
+        hv = self._host_vec(H, self.nnz, "H") if H is not None else None
         hptr = hv.ctypes.data if hv is not None else None
         if free is None:
             check(lib().tad_fn_solve(self._h, hptr, rhs.ctypes.data, sol.ctypes.data, _lib.MEM_HOST,
@@ -486,16 +507,37 @@ class ScalarFunction:
         return st.as_dict()
 
     def line_search(self, x0, d, f, g, out=None, opts: NewtonOpts | None = None):
-        """line_search (LineSearch.hh:26-65) on device tensors; returns (x_new, f_new, step, trials)."""
+        """line_search (LineSearch.hh:26-65) on device tensors; returns (x_new, f_new, step, trials).
+
+        trials is negative when no step was accepted: then x_new equals x0
+        and step is 0 (the reference returns x0, LineSearch.hh:62-64)."""
         import torch
         o = opts or newton_default_opts()
         self._x(x0)
-        out = torch.empty_like(x0) if out is None else out
+        for a, name in ((d, "d"), (g, "g")):
+            self._dev_vec(a, self.n_vars, name)
+        out = torch.empty_like(x0) if out is None else self._dev_vec(out, self.n_vars, "out")
         fo, so, tr = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
         check(lib().tad_line_search(self._h, x0.data_ptr(), d.data_ptr(), float(f), g.data_ptr(), out.data_ptr(),
                                     _lib.MEM_DEVICE, ctypes.byref(o), ctypes.byref(fo), ctypes.byref(so),
                                     ctypes.byref(tr)))
         return out, fo.value, so.value, tr.value
+
+    def newton_decrement(self, d, g) -> float:
+        """newton_decrement(d, g) = -0.5 d^T g (NewtonDecrement.hh:20-26), with the
+        library's fixed-order FP64 dot product (numpy arrays or CUDA tensors)."""
+        out = ctypes.c_double()
+        if _is_torch(d) or _is_torch(g):
+            self._dev_vec(d, self.n_vars, "d")
+            self._dev_vec(g, self.n_vars, "g")
+            _order_after_torch(self.ctx, d.device)
+            check(lib().tad_newton_decrement(self._h, d.data_ptr(), g.data_ptr(), _lib.MEM_DEVICE,
+                                             ctypes.byref(out)))
+            return out.value
+        d = self._host_vec(d, self.n_vars, "d")
+        g = self._host_vec(g, self.n_vars, "g")
+        check(lib().tad_newton_decrement(self._h, d.ctypes.data, g.ctypes.data, _lib.MEM_HOST, ctypes.byref(out)))
+        return out.value
 
 
 def scalar_function(d: int, n_handles: int, ctx: Context | None = None, precision: str = "fp64") -> ScalarFunction:

@@ -28,7 +28,7 @@ EXPORTED = [
     "tad_fn_pattern", "tad_fn_eval", "tad_fn_eval_with_gradient", "tad_fn_eval_with_derivatives",
     "tad_fn_eval_with_hessian_proj", "tad_fn_device_buffers",
     "tad_solve_default_opts", "tad_fn_solve", "tad_fn_solve_reduced", "tad_newton_default_opts", "tad_newton_step",
-    "tad_line_search",
+    "tad_line_search", "tad_newton_decrement",
     "tad_fn_set_segments", "tad_fn_segment_energies", "tad_fn_autotune",
     "tad_fn_get_launch_shape", "tad_fn_set_launch_shape", "tad_fn_get_assembly_shape", "tad_fn_set_assembly_shape",
     "tad_fn_set_params", "tad_fn_rest_vjp",
@@ -130,6 +130,7 @@ def lib():
         "tad_newton_step": [vp, vp, i32, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonStats)],
         "tad_line_search": [vp, vp, vp, dbl, vp, vp, i32, ctypes.POINTER(NewtonOpts), pdbl, pdbl,
                             ctypes.POINTER(ctypes.c_int)],
+        "tad_newton_decrement": [vp, vp, vp, i32, pdbl],
         "tad_fn_set_segments": [vp, i64, vp],
         "tad_fn_segment_energies": [vp, vp, i32],
         "tad_fn_autotune": [vp, vp, i32, dbl, ctypes.c_char_p, ctypes.c_size_t],

@@ -1620,6 +1620,32 @@ tad_status tad_line_search(tad_fn fn, const double* x0, const double* dir, doubl
     return TAD_OK;
 }
 
+tad_status tad_newton_decrement(tad_fn fn, const double* dir, const double* g, tad_mem mem, double* out)
+{
+    TAD_TRY(check_fn(fn));
+    if (!dir || !g || !out) return fail(TAD_ERR_INVALID, "null argument");
+    TAD_HIP(hipSetDevice(fn->ctx->device));
+    TAD_TRY(ensure_ready(fn));
+    const int64_t n = fn->n_vars;
+    const double* dd = dir;
+    const double* gd = g;
+    if (mem == TAD_MEM_HOST)
+    {
+        TAD_HIP(fn->x_tmp.ensure(n));
+        TAD_HIP(fn->sol.ensure(n));
+        TAD_HIP(hipMemcpyAsync(fn->x_tmp.p, dir, n * sizeof(double), hipMemcpyHostToDevice, fn->s()));
+        TAD_HIP(hipMemcpyAsync(fn->sol.p, g, n * sizeof(double), hipMemcpyHostToDevice, fn->s()));
+        dd = fn->x_tmp.p;
+        gd = fn->sol.p;
+    }
+    // the same fixed-order device dot product the Newton step uses, so the
+    // decrement of tad_newton_stats is reproduced bit for bit
+    double dg = 0.0;
+    TAD_TRY(dot_dev(fn, dd, gd, &dg));
+    *out = -0.5 * dg;
+    return TAD_OK;
+}
+
 tad_status tad_newton_step(tad_fn fn, double* x, tad_mem mem, const tad_newton_opts* opts, tad_newton_stats* stats)
 {
     TAD_TRY(check_fn(fn));
