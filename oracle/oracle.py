@@ -55,6 +55,18 @@ def lib():
                                      ctypes.c_double, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
         L.or_dual_known_answer.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp]
         L.or_tet_rest_mixed_vjp.argtypes = [ctypes.c_long, _ip, _dp, _dp, _dp, ctypes.c_double, ctypes.c_long, _dp]
+        L.or_vfn_create.restype = ctypes.c_void_p
+        L.or_vfn_create.argtypes = [ctypes.c_int, ctypes.c_long]
+        L.or_vfn_destroy.argtypes = [ctypes.c_void_p]
+        L.or_vfn_add_elements.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_long, _ip, _dp,
+                                          ctypes.c_int]
+        L.or_vfn_n_outputs.restype = ctypes.c_long
+        L.or_vfn_n_outputs.argtypes = [ctypes.c_void_p]
+        L.or_vfn_eval.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.or_vfn_sum_of_squares.restype = ctypes.c_double
+        L.or_vfn_sum_of_squares.argtypes = [ctypes.c_void_p, _dp]
+        L.or_vfn_eval_with_jacobian.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.POINTER(ctypes.c_long)]
+        L.or_vfn_get_jacobian.argtypes = [ctypes.c_void_p, _ip, _ip, _dp]
         _lib = L
     return _lib
 
@@ -170,6 +182,91 @@ class Function:
         if n == 0:
             raise OracleError(lib().or_last_error().decode())
         return xo, float(fo[0]), float(so[0]), n
+
+
+# residual ids of the vector-function restatement (tad_oracle.cpp)
+R_SYMDIR_TRI = 101
+R_PIN = 102
+
+
+class VectorFunction:
+    """CPU restatement of VectorFunction (FP64, VectorFunction.hh:19-192)
+    over the residuals of tad_oracle.cpp (R_SYMDIR_TRI, R_PIN)."""
+
+    def __init__(self, d: int, n_handles: int):
+        L = lib()
+        self._h = L.or_vfn_create(d, n_handles)
+        if not self._h:
+            raise OracleError(L.or_last_error().decode())
+        self.d = d
+        self.n_vars = d * n_handles
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.or_vfn_destroy(self._h)
+            self._h = None
+
+    @classmethod
+    def from_problem(cls, problem):
+        f = cls(problem.d, problem.n_handles)
+        for t in problem.terms:
+            f.add_elements(t.energy, t.conn, t.params)
+        return f
+
+    @property
+    def n_outputs(self) -> int:
+        return lib().or_vfn_n_outputs(self._h)
+
+    def add_elements(self, rid: int, conn, params):
+        conn = np.ascontiguousarray(conn, dtype=np.int32)
+        params = np.ascontiguousarray(params, dtype=np.float64)
+        if conn.ndim == 1:
+            conn = conn[:, None]
+        rc = lib().or_vfn_add_elements(self._h, rid, conn.shape[1], conn.shape[0], _i(conn), _d(params),
+                                       params.shape[1])
+        if rc:
+            raise OracleError(lib().or_last_error().decode())
+
+    def eval(self, x):
+        """r(x) (VectorFunctionImpl.hh:130-147)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        r = np.zeros(self.n_outputs)
+        lib().or_vfn_eval(self._h, _d(x), _d(r))
+        return r
+
+    def eval_with_jacobian(self, x):
+        """(r, J) with J a scipy CSR matrix n_outputs x n_vars (VectorFunctionImpl.hh:157-176)."""
+        import scipy.sparse as sp
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        m = self.n_outputs
+        r = np.zeros(m)
+        nnz = ctypes.c_long(0)
+        if lib().or_vfn_eval_with_jacobian(self._h, _d(x), _d(r), ctypes.byref(nnz)):
+            raise OracleError(lib().or_last_error().decode())
+        rp = np.zeros(m + 1, dtype=np.int32)
+        ci = np.zeros(nnz.value, dtype=np.int32)
+        v = np.zeros(nnz.value)
+        lib().or_vfn_get_jacobian(self._h, _i(rp), _i(ci), _d(v))
+        return r, sp.csr_matrix((v, ci, rp), shape=(m, self.n_vars))
+
+    def eval_sum_of_squares(self, x) -> float:
+        """sum_e r_e . r_e in element order (VectorFunctionImpl.hh:228-239)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        return lib().or_vfn_sum_of_squares(self._h, _d(x))
+
+    def eval_sum_of_squares_with_derivatives(self, x):
+        """(f, g, r, J): f = r . r, g = 2 J^T r (VectorFunctionImpl.hh:260-275)."""
+        r, J = self.eval_with_jacobian(x)
+        return float(r @ r), 2.0 * (J.T @ r), r, J
+
+
+def gauss_newton_direction(r, J, w_identity: float = 0.0):
+    """d with (w I + J^T J) d = -J^T r, by a direct sparse solve
+    (GaussNewtonDirection.hh:24-47 factorizes with SimplicialLDLT)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    A = (J.T @ J + w_identity * sp.identity(J.shape[1])).tocsc()
+    return spla.spsolve(A, -(J.T @ r))
 
 
 def pcg(H, b, w: float = 0.0, block: int = 1, tol: float = 1e-10, max_iters: int = 10000, n_threads: int = 0):

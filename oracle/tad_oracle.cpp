@@ -454,7 +454,7 @@ void element_second_order(const Function& fn, const Term& t, long e, const doubl
             v[a * d + i] = Dual<K>(x[gi], a * d + i);
             out.idx[a * d + i] = (int)gi;
         }
-    const Dual<K> r = eval_energy<Dual<K>>(t.energy, v, &t.params[e * t.n_params], d, t.N);
+    const Dual<K> r = eval_energy<Dual<K>>(t.energy, v, &t.params[e * t.n_params], std::min(d, K), t.N);
     out.val = r.val;
     for (int i = 0; i < K; ++i) out.grad[i] = r.grad[i];
     for (int i = 0; i < K; ++i) for (int j = 0; j < K; ++j) out.hess[i * K + j] = r.hess[i][j];
@@ -731,6 +731,200 @@ inline int pcg(long n, int bs, const int* rp, const int* ci, const double* v, do
     }
     if (rel_res) *rel_res = std::sqrt(rr) / bnorm;
     return it;
+}
+
+// ---------------------------------------------------------------------------
+// Vector functions (VectorFunction.hh:19-192): elements produce fixed-size
+// segments of a residual vector r(x).  Residuals are written as the reference's
+// GaussNewtonTest writes its element lambdas (tests/GaussNewtonTest.cc:30-74):
+//   R_SYMDIR_TRI (d=2, N=3, 8 outputs): params [Mr^-1 (2x2 row-major), s]:
+//     s * (J00, J01, J10, J11, Ji00, Ji01, Ji10, Ji11), J = M Mr^-1,
+//     Ji = Mr M^-1 (Mr formed from Mr^-1 by the 2x2 inverse), every output
+//     INFINITY if det(M) <= 0;
+//   R_PIN (N=1, d outputs): params [target (d), s]: s * (target - x).
+// The Jacobian follows VectorObjectiveTerm.hh:188-246: per residual, one
+// triplet per local variable in local (connectivity) order, rows numbered in
+// term then element then output order; the sparse matrix sums duplicates.
+// ---------------------------------------------------------------------------
+enum Residual
+{
+    R_SYMDIR_TRI = 101,
+    R_PIN = 102,
+};
+
+inline int residual_outputs(int rid, int d, int N)
+{
+    if (rid == R_SYMDIR_TRI) return (d == 2 && N == 3) ? 8 : -1;
+    if (rid == R_PIN) return N == 1 ? d : -1;
+    return -1;
+}
+
+inline int residual_params(int rid, int d)
+{
+    if (rid == R_SYMDIR_TRI) return 5;
+    if (rid == R_PIN) return d + 1;
+    return -1;
+}
+
+template <class S>
+void residual_symdir_tri(const S* v, const double* p, S* r)
+{
+    const S M00 = v[2] - v[0], M10 = v[3] - v[1];
+    const S M01 = v[4] - v[0], M11 = v[5] - v[1];
+    const S detM = M00 * M11 - M01 * M10;
+    if (value(detM) <= 0.0)
+    {
+        for (int o = 0; o < 8; ++o) r[o] = S((double)INFINITY);
+        return;
+    }
+    const double R00 = p[0], R01 = p[1], R10 = p[2], R11 = p[3], s = p[4];
+    // Mr = (Mr^-1)^-1
+    const double dR = R00 * R11 - R01 * R10;
+    const double Q00 = R11 / dR, Q01 = -R01 / dR, Q10 = -R10 / dR, Q11 = R00 / dR;
+    const S J00 = M00 * R00 + M01 * R10, J01 = M00 * R01 + M01 * R11;
+    const S J10 = M10 * R00 + M11 * R10, J11 = M10 * R01 + M11 * R11;
+    // M^-1 = adj(M) / det(M); Ji = Mr M^-1
+    const S inv = 1.0 / detM;
+    const S N00 = M11 * inv, N01 = -M01 * inv, N10 = -M10 * inv, N11 = M00 * inv;
+    const S I00 = Q00 * N00 + Q01 * N10, I01 = Q00 * N01 + Q01 * N11;
+    const S I10 = Q10 * N00 + Q11 * N10, I11 = Q10 * N01 + Q11 * N11;
+    r[0] = s * J00; r[1] = s * J01; r[2] = s * J10; r[3] = s * J11;
+    r[4] = s * I00; r[5] = s * I01; r[6] = s * I10; r[7] = s * I11;
+}
+
+template <class S>
+void residual_pin(const S* v, const double* p, int d, S* r)
+{
+    for (int i = 0; i < d; ++i) r[i] = p[d] * (p[i] - v[i]);
+}
+
+template <class S>
+void eval_residual(int rid, const S* v, const double* p, int d, S* r)
+{
+    switch (rid)
+    {
+        case R_SYMDIR_TRI: residual_symdir_tri<S>(v, p, r); break;
+        case R_PIN: residual_pin<S>(v, p, d, r); break;
+        default: throw std::runtime_error("unknown residual");
+    }
+}
+
+struct RTerm
+{
+    int rid = 0, N = 0, K = 0, M = 0, n_params = 0;
+    long n_elem = 0;
+    std::vector<int> conn;
+    std::vector<double> params;
+};
+
+struct VFunction
+{
+    int d = 0;
+    long n_handles = 0, n_vars = 0, n_outputs = 0;
+    std::vector<RTerm> terms;
+    std::vector<int> row_ptr, col_idx;  // last Jacobian (CSR)
+    std::vector<double> values;
+};
+
+// r (VectorObjectiveTerm.hh:162-183): element segments in term order.
+inline void veval(const VFunction& fn, const double* x, double* r)
+{
+    long base = 0;
+    for (const RTerm& t : fn.terms)
+    {
+        #pragma omp parallel for schedule(static)
+        for (long e = 0; e < t.n_elem; ++e)
+        {
+            double v[12];
+            for (int a = 0; a < t.N; ++a)
+                for (int i = 0; i < fn.d; ++i) v[a * fn.d + i] = x[(long)fn.d * t.conn[e * t.N + a] + i];
+            eval_residual<double>(t.rid, v, &t.params[e * t.n_params], fn.d, r + base + e * t.M);
+        }
+        base += t.n_elem * t.M;
+    }
+}
+
+// sum of squares (VectorObjectiveTerm.hh:326-348, VectorFunctionImpl.hh:228-239):
+// per element res.dot(res), summed in element order, terms in order.
+inline double vsum_of_squares(const VFunction& fn, const double* x)
+{
+    double f = 0.0;
+    for (const RTerm& t : fn.terms)
+    {
+        std::vector<double> sq(t.n_elem);
+        #pragma omp parallel for schedule(static)
+        for (long e = 0; e < t.n_elem; ++e)
+        {
+            double v[12], r[32];
+            for (int a = 0; a < t.N; ++a)
+                for (int i = 0; i < fn.d; ++i) v[a * fn.d + i] = x[(long)fn.d * t.conn[e * t.N + a] + i];
+            eval_residual<double>(t.rid, v, &t.params[e * t.n_params], fn.d, r);
+            double q = 0.0;
+            for (int o = 0; o < t.M; ++o) q += r[o] * r[o];
+            sq[e] = q;
+        }
+        double s = 0.0;
+        for (long e = 0; e < t.n_elem; ++e) s += sq[e];
+        f += s;
+    }
+    return f;
+}
+
+template <int K>
+void residual_first_order(const VFunction& fn, const RTerm& t, long e, const double* x, double* rv, double* jac,
+                          int* idx)
+{
+    Dual<K> v[K];
+    for (int a = 0; a < t.N; ++a)
+        for (int i = 0; i < fn.d; ++i)
+        {
+            const long gi = (long)fn.d * t.conn[e * t.N + a] + i;
+            v[a * fn.d + i] = Dual<K>(x[gi], a * fn.d + i);
+            idx[a * fn.d + i] = (int)gi;
+        }
+    std::vector<Dual<K>> r(t.M);
+    // d <= K always (K = d * N); the bound lets the compiler see it
+    eval_residual<Dual<K>>(t.rid, v, &t.params[e * t.n_params], std::min(fn.d, K), r.data());
+    for (int o = 0; o < t.M; ++o)
+    {
+        rv[o] = r[o].val;
+        for (int i = 0; i < K; ++i) jac[o * K + i] = r[o].grad[i];
+    }
+}
+
+// r and the Jacobian triplets (VectorObjectiveTerm.hh:188-246).
+inline void veval_with_jacobian(const VFunction& fn, const double* x, double* r, std::vector<Triplet>& trip)
+{
+    trip.clear();
+    long base = 0;
+    for (const RTerm& t : fn.terms)
+    {
+        std::vector<double> jac((size_t)t.n_elem * t.M * t.K);
+        std::vector<int> idx((size_t)t.n_elem * t.K);
+        bool nonfinite = false;
+        #pragma omp parallel for schedule(static) reduction(||:nonfinite)
+        for (long e = 0; e < t.n_elem; ++e)
+        {
+            double* jr = &jac[(size_t)e * t.M * t.K];
+            int* ix = &idx[(size_t)e * t.K];
+            switch (t.K)
+            {
+                case 1: residual_first_order<1>(fn, t, e, x, r + base + e * t.M, jr, ix); break;
+                case 2: residual_first_order<2>(fn, t, e, x, r + base + e * t.M, jr, ix); break;
+                case 3: residual_first_order<3>(fn, t, e, x, r + base + e * t.M, jr, ix); break;
+                case 6: residual_first_order<6>(fn, t, e, x, r + base + e * t.M, jr, ix); break;
+                default: break;
+            }
+            for (int i = 0; i < t.M * t.K; ++i) if (!std::isfinite(jr[i])) nonfinite = true;
+        }
+        if (nonfinite) throw std::runtime_error("non-finite residual gradient");
+        for (long e = 0; e < t.n_elem; ++e)
+            for (int o = 0; o < t.M; ++o)
+                for (int i = 0; i < t.K; ++i)
+                    trip.push_back({ (int)(base + e * t.M + o), idx[(size_t)e * t.K + i],
+                                     jac[((size_t)e * t.M + o) * t.K + i] });
+        base += t.n_elem * t.M;
+    }
 }
 
 } // namespace oracle
@@ -1010,6 +1204,65 @@ int or_tet_rest_mixed_vjp(long n_tets, const int* conn, const double* x, const d
     for (long e = 0; e < n_tets; ++e)
         for (int k = 0; k < 12; ++k) out[3 * conn[4 * e + k / 3] + k % 3] += contrib[(size_t)e * 12 + k];
     return 0;
+}
+
+// --- vector functions ---------------------------------------------------------
+void* or_vfn_create(int d, long n_handles)
+{
+    if (d < 1 || d > 3 || n_handles < 1) { g_err = "invalid dimension or handle count"; return nullptr; }
+    VFunction* f = new VFunction();
+    f->d = d; f->n_handles = n_handles; f->n_vars = (long)d * n_handles;
+    return f;
+}
+
+void or_vfn_destroy(void* fn) { delete (VFunction*)fn; }
+
+int or_vfn_add_elements(void* fnp, int rid, int N, long n_elem, const int* conn, const double* params, int n_params)
+{
+    VFunction& fn = *(VFunction*)fnp;
+    const int M = residual_outputs(rid, fn.d, N);
+    if (M < 0) { g_err = "residual does not match dimension/valence"; return 1; }
+    if (residual_params(rid, fn.d) != n_params) { g_err = "parameter count mismatch"; return 1; }
+    RTerm t;
+    t.rid = rid; t.N = N; t.K = fn.d * N; t.M = M; t.n_params = n_params; t.n_elem = n_elem;
+    if (t.K != 1 && t.K != 2 && t.K != 3 && t.K != 6) { g_err = "unsupported element size"; return 1; }
+    t.conn.assign(conn, conn + n_elem * N);
+    t.params.assign(params, params + n_elem * n_params);
+    for (long i = 0; i < n_elem * N; ++i)
+        if (t.conn[i] < 0 || t.conn[i] >= fn.n_handles) { g_err = "handle out of range"; return 1; }
+    fn.n_outputs += n_elem * M;
+    fn.terms.push_back(std::move(t));
+    return 0;
+}
+
+long or_vfn_n_outputs(void* fnp) { return ((VFunction*)fnp)->n_outputs; }
+
+void or_vfn_eval(void* fnp, const double* x, double* r) { veval(*(VFunction*)fnp, x, r); }
+
+double or_vfn_sum_of_squares(void* fnp, const double* x) { return vsum_of_squares(*(VFunction*)fnp, x); }
+
+// eval_with_jacobian (VectorFunctionImpl.hh:157-176): r, then J (n_outputs x
+// n_vars) from the triplets; fetch it with or_vfn_get_jacobian.
+int or_vfn_eval_with_jacobian(void* fnp, const double* x, double* r, long* nnz)
+{
+    VFunction& fn = *(VFunction*)fnp;
+    try
+    {
+        std::vector<Triplet> trip;
+        veval_with_jacobian(fn, x, r, trip);
+        triplets_to_csr(fn.n_outputs, trip, fn.row_ptr, fn.col_idx, fn.values);
+        *nnz = (long)fn.values.size();
+        return 0;
+    }
+    catch (const std::exception& ex) { g_err = ex.what(); return 3; }
+}
+
+void or_vfn_get_jacobian(void* fnp, int* row_ptr, int* col_idx, double* values)
+{
+    VFunction& fn = *(VFunction*)fnp;
+    std::copy(fn.row_ptr.begin(), fn.row_ptr.end(), row_ptr);
+    std::copy(fn.col_idx.begin(), fn.col_idx.end(), col_idx);
+    std::copy(fn.values.begin(), fn.values.end(), values);
 }
 
 } // extern "C"

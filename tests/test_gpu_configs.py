@@ -4,8 +4,9 @@ oracle (the restated reference path, ScalarFunctionImpl.hh:378-416):
 * twisted Kuhn cube, 60^3 cells x 6 = 1,296,000 tets, FP64:
   eval_with_hessian_proj at x0 and after 30 projected-Newton iterations
   against the oracle (f 1e-12, g and H_proj 1e-10 normwise, identical CSR
-  patterns, CSR value arrays compared directly); the 30 iterations
-  themselves (NewtonTest.cc:69-75 loop body) checked step by step: Armijo
+  patterns, CSR value arrays compared directly; at the converged point the
+  gradient's error is measured against the gradient scale at x0); the 30
+  iterations themselves (NewtonTest.cc:69-75 loop body) checked step by step: Armijo
   with the reported step and decrement (LineSearch.hh:14-24), monotone
   energy, each step starting at the energy the previous one accepted, a
   converged solve, the gradient driven down; two runs bit-identical;
@@ -46,14 +47,19 @@ def log(msg):
     print(msg, flush=True)
 
 
-def compare_with_oracle(fg_res, fo_res, tol, f_tol=1e-12):
-    """f relative, g and the CSR value arrays normwise; patterns identical."""
+def compare_with_oracle(fg_res, fo_res, tol, f_tol=1e-12, g_scale=None):
+    """f relative, g and the CSR value arrays normwise; patterns identical.
+    g_scale: the magnitude g's error is measured against (default max|g|).
+    At a converged point every entry of g is a sum of O(1) element
+    contributions that cancel to ~1e-17, so its error is judged against the
+    gradient scale of the same mesh before convergence instead."""
     f1, g1, H1 = fo_res
     f2, g2, H2 = fg_res
     assert H1.has_sorted_indices and H2.has_sorted_indices
     assert np.array_equal(H2.indptr, H1.indptr) and np.array_equal(H2.indices, H1.indices)
-    ef, eg, eh = abs(f2 - f1) / abs(f1), nrel(g2, g1), nrel(H2.data, H1.data)
-    log(f"  f rel {ef:.2e}  g {eg:.2e}  H_proj {eh:.2e}  (nnz {H1.nnz})")
+    gs = np.abs(g1).max() if g_scale is None else g_scale
+    ef, eg, eh = abs(f2 - f1) / abs(f1), float(np.abs(g2 - g1).max() / gs), nrel(H2.data, H1.data)
+    log(f"  f rel {ef:.2e}  g {eg:.2e} (of {gs:.3e})  H_proj {eh:.2e}  (nnz {H1.nnz})")
     assert ef <= f_tol and eg <= tol and eh <= tol
     return ef, eg, eh
 
@@ -99,6 +105,7 @@ def test_cube60_fp64_eval_matches_oracle(ctx, cube60, cube60_fp64_run):
     ro = fo.eval_with_hessian_proj(P.x0)
     assert fn.nnz == 29845269 == ro[2].nnz
     compare_with_oracle(fn.eval_with_hessian_proj(P.x0), ro, FP64_TOL)
+    g0_scale = float(np.abs(ro[1]).max())
     assert fn.eval(P.x0) == ro[0] or abs(fn.eval(P.x0) - ro[0]) <= 1e-12 * abs(ro[0])
     # the unprojected Hessian and the gradient-only path at x0
     compare_with_oracle(fn.eval_with_derivatives(P.x0), fo.eval_with_derivatives(P.x0), FP64_TOL)
@@ -107,7 +114,7 @@ def test_cube60_fp64_eval_matches_oracle(ctx, cube60, cube60_fp64_run):
     # after the 30 Newton iterations: a twisted, converged state
     x30 = cube60_fp64_run[1].cpu().numpy()
     log("oracle eval_with_hessian_proj after 30 Newton iterations ...")
-    compare_with_oracle(fn.eval_with_hessian_proj(x30), fo.eval_with_hessian_proj(x30), FP64_TOL)
+    compare_with_oracle(fn.eval_with_hessian_proj(x30), fo.eval_with_hessian_proj(x30), FP64_TOL, g_scale=g0_scale)
 
 
 def test_cube60_fp64_newton(ctx, cube60, cube60_fp64_run):
@@ -147,6 +154,7 @@ def test_cube60_mixed(ctx, cube60, cube60_fp64_run):
     rm = fm.eval_with_hessian_proj(P.x0)
     assert rm[0] == f64.eval_with_hessian_proj(P.x0)[0]
     compare_with_oracle(rm, ro, MIXED_TOL)
+    g0_scale = float(np.abs(ro[1]).max())
     x = torch.tensor(P.x0, device="cuda:0")
     sts = newton_run(fm, x, NEWTON_ITERS, cube60_fp64_run[3])
     fa = f64.eval(cube60_fp64_run[1])
@@ -155,7 +163,7 @@ def test_cube60_mixed(ctx, cube60, cube60_fp64_run):
     assert abs(fa - fb) <= 1e-8 * abs(fa)
     xm = x.cpu().numpy()
     log("oracle eval_with_hessian_proj at the mixed run's end point ...")
-    compare_with_oracle(fm.eval_with_hessian_proj(xm), fo.eval_with_hessian_proj(xm), MIXED_TOL)
+    compare_with_oracle(fm.eval_with_hessian_proj(xm), fo.eval_with_hessian_proj(xm), MIXED_TOL, g_scale=g0_scale)
 
 
 @pytest.fixture(scope="module")

@@ -481,3 +481,107 @@ def test_rest_mixed_vjp_inverted_and_translation():
     assert np.array_equal(full, part)
     # the energy depends on theta through edge vectors only: a rest translation is free
     assert np.abs(full.reshape(-1, 3).sum(axis=0)).max() <= 1e-12 * np.abs(full).sum()
+
+
+# ---------------------------------------------------------------------------
+# Vector functions and the Gauss-Newton direction (VectorFunction.hh,
+# VectorObjectiveTerm.hh, GaussNewtonDirection.hh)
+# ---------------------------------------------------------------------------
+
+def armijo(f_of, x, d, f, g, s_max=1.0, shrink=0.8, max_iters=64, c=1e-4):
+    """LineSearch.hh:26-65 on an energy callable (x0 on failure)."""
+    dg = float(d @ g)
+    s = s_max
+    for i in range(max_iters):
+        xn = x + s * d
+        fn = f_of(xn)
+        if fn <= f + c * s * dg:
+            return xn
+        s = 1.0 if (s > 1.0 and s * shrink < 1.0) else s * shrink
+    return x
+
+
+def test_vector_sum_of_squares_matches_scalar_along_gauss_newton():
+    """tests/GaussNewtonTest.cc:113-162: along 20 Gauss-Newton iterations
+    (w = 1e-12, Armijo on eval_sum_of_squares) the residual form's f and
+    g = 2 J^T r equal the scalar form's (symmetric Dirichlet / F.rows() plus
+    unit pins) to 1e-12; the iteration reaches f = 4 +- 0.1 with |g| <= 0.1."""
+    Pv = configs.gauss_newton_test_problem()
+    fv = oracle.VectorFunction.from_problem(Pv)
+    fs = oracle.Function.from_problem(configs.newton_test_problem())
+    assert fv.n_outputs == 4 * 8 + 2 * 2
+    x = Pv.x0.copy()
+    for _ in range(20):
+        f_ref, g_ref, _ = fs.eval_with_hessian_proj(x)
+        assert abs(fs.eval(x) - f_ref) <= 1e-12
+        assert abs(fv.eval_sum_of_squares(x) - f_ref) <= 1e-12
+        f, g, r, J = fv.eval_sum_of_squares_with_derivatives(x)
+        assert abs(f - f_ref) <= 1e-12 and np.abs(g - g_ref).max() <= 1e-12
+        assert np.abs(r - fv.eval(x)).max() <= 1e-15 * np.abs(r).max()
+        d = oracle.gauss_newton_direction(r, J, 1e-12)
+        x = armijo(fv.eval_sum_of_squares, x, d, f, g)
+    f, g, _, _ = fv.eval_sum_of_squares_with_derivatives(x)
+    assert abs(f - 4.0) <= 0.1 and np.abs(g).max() <= 0.1
+
+
+def torch_grid_residuals(P, xt):
+    """Independent restatement of R_SYMDIR_TRI and R_PIN in torch."""
+    import torch
+    tri, pin = P.terms
+    F = torch.as_tensor(tri.conn.astype(np.int64))
+    p = torch.as_tensor(tri.params)
+    X = xt.reshape(-1, 2)
+    a, b, c = X[F[:, 0]], X[F[:, 1]], X[F[:, 2]]
+    M = torch.stack([b - a, c - a], dim=2)
+    Ri = p[:, :4].reshape(-1, 2, 2)
+    J = M @ Ri
+    Ji = torch.linalg.inv(Ri) @ torch.linalg.inv(M)
+    rt = p[:, 4:5] * torch.cat([J.reshape(-1, 4), Ji.reshape(-1, 4)], dim=1)
+    rp = torch.as_tensor(pin.params[:, 2:3]) * (torch.as_tensor(pin.params[:, :2]) - X[torch.as_tensor(
+        pin.conn[:, 0].astype(np.int64))])
+    return torch.cat([rt.reshape(-1), rp.reshape(-1)])
+
+
+def test_vector_jacobian_matches_torch():
+    """r and J of a 7 x 7 grid against torch.func.jacfwd of an independent
+    torch restatement (1e-12); J's pattern is each residual's element
+    variables, columns sorted; J^T J has the scalar function's pattern."""
+    import torch
+    P = configs.grid_residuals(7, seed=4)
+    fv = oracle.VectorFunction.from_problem(P)
+    r, J = fv.eval_with_jacobian(P.x0)
+    xt = torch.tensor(P.x0)
+    rt = torch_grid_residuals(P, xt)
+    Jt = torch.func.jacfwd(lambda z: torch_grid_residuals(P, z))(xt).numpy()
+    assert np.abs(r - rt.numpy()).max() <= 1e-12 * np.abs(rt.numpy()).max()
+    assert np.abs(J.toarray() - Jt).max() <= 1e-12 * np.abs(Jt).max()
+    assert J.shape == (fv.n_outputs, P.n_vars) and J.has_sorted_indices
+    assert np.all(np.diff(J.indptr)[: 8 * P.terms[0].conn.shape[0]] == 6)
+    S = oracle.Function(2, P.n_handles)
+    S.add_elements(configs.E_SYMDIR_TRI, P.terms[0].conn, P.terms[0].params)
+    S.add_elements(configs.E_PENALTY, P.terms[1].conn, P.terms[1].params)
+    H = S.eval_with_derivatives(P.x0)[2]
+    JtJ = (J.T @ J).tocsr()
+    JtJ.sort_indices()
+    assert np.array_equal(JtJ.indptr, H.indptr) and np.array_equal(JtJ.indices, H.indices)
+
+
+def test_vector_inverted_and_pins_only():
+    """An inverted triangle makes all 8 residuals +inf (the reference returns
+    Vector<T, 8>::Constant(INFINITY)); for pin residuals alone J = -s I on the
+    pinned variables and one Gauss-Newton step (w = 0) lands on the targets."""
+    P = configs.gauss_newton_test_problem()
+    fv = oracle.VectorFunction.from_problem(P)
+    x = P.x0.copy()
+    x[2:4], x[4:6] = P.x0[4:6], P.x0[2:4]  # swap vertices 1 and 2: face 0 inverted
+    r = fv.eval(x)
+    assert np.all(np.isinf(r[:8])) and np.isinf(fv.eval_sum_of_squares(x))
+    pins = oracle.VectorFunction(2, 3)
+    tgt = np.array([[1.0, -2.0], [0.5, 0.25], [3.0, 4.0]])
+    pins.add_elements(oracle.R_PIN, np.arange(3, dtype=np.int32)[:, None],
+                      np.concatenate([tgt, np.full((3, 1), 2.0)], axis=1))
+    x0 = np.zeros(6)
+    r, J = pins.eval_with_jacobian(x0)
+    assert np.array_equal(J.toarray(), -2.0 * np.eye(6))
+    d = oracle.gauss_newton_direction(r, J, 0.0)
+    assert np.abs(x0 + d - tgt.ravel()).max() <= 1e-15

@@ -37,6 +37,12 @@ E_SYMDIR_TRI = 1
 E_SYMDIR_TET = 2
 E_PENALTY = 3
 E_QUADRATIC = 4
+# residual elements of vector functions (sum-of-squares / Gauss-Newton):
+#   symmetric Dirichlet triangle residuals: [Mr^-1 (2x2 row-major), s] -> 8 outputs
+#     s (J00, J01, J10, J11, Ji00, Ji01, Ji10, Ji11), J = M Mr^-1, Ji = Mr M^-1
+#   pin residuals:                          [target (d), s] -> d outputs s (target - x)
+R_SYMDIR_TRI = 101
+R_PIN = 102
 
 _GOLDEN = np.uint64(0x9E3779B97F4A7C15)
 
@@ -181,6 +187,33 @@ def newton_test_problem() -> Problem:
     tri = Term(E_SYMDIR_TRI, F, planar_rest_params(V_rest, F, 1.0 / F.shape[0]))
     pen = Term(E_PENALTY, b[:, None].copy(), np.concatenate([bc, np.ones((2, 1))], axis=1))
     return Problem("newton_test", 2, 6, [tri, pen], V_init.ravel().copy())
+
+
+def gauss_newton_test_problem() -> Problem:
+    """The residual form of newton_test_problem, as in the reference's
+    Gauss-Newton test (tests/GaussNewtonTest.cc:30-74): 8 residuals per
+    triangle scaled by 1 / sqrt(F.rows()) and 2 unit pin residuals per pinned
+    vertex, so the sum of squares equals newton_test_problem's energy."""
+    V_rest, V_init, F, b, bc = planar_test_mesh()
+    p = planar_rest_params(V_rest, F, 1.0 / math.sqrt(F.shape[0]))
+    tri = Term(R_SYMDIR_TRI, F, p)
+    pin = Term(R_PIN, b[:, None].copy(), np.concatenate([bc, np.ones((2, 1))], axis=1))
+    return Problem("gauss_newton_test", 2, 6, [tri, pin], V_init.ravel().copy())
+
+
+def grid_residuals(n: int = 100, seed: int = 1, noise: float = 0.2, pins: int = 2) -> Problem:
+    """grid_param's energy as residuals: per triangle s = sqrt(w) times the 8
+    entries of J and J^-1, plus `pins` pinned corner vertices at their noisy
+    start (s = 1)."""
+    P = grid_param(n, seed, noise)
+    t = P.terms[0]
+    p = t.params.copy()
+    p[:, 4] = np.sqrt(p[:, 4])
+    corners = np.array([0, n - 1, n * n - n, n * n - 1][:pins], dtype=np.int32)
+    x0 = P.x0.reshape(-1, 2)
+    pin = Term(R_PIN, corners[:, None].copy(), np.concatenate([x0[corners], np.ones((pins, 1))], axis=1))
+    return Problem(f"grid_residuals_{n}", 2, P.n_handles, [Term(R_SYMDIR_TRI, t.conn, p), pin], P.x0.copy(),
+                   dict(P.info))
 
 
 # ---------------------------------------------------------------------------
