@@ -117,7 +117,7 @@ def kernel_table(ctx, names):
 
 EVAL_KERNELS = ["element", "assemble_gradient", "assemble_hessian", "energy_sum"]
 NEWTON_KERNELS = EVAL_KERNELS + ["spmv", "pcg_update", "pcg_coarse", "pcg_setup", "coarse_setup", "coarse_galerkin",
-                                 "coarse_potrf", "coarse_potri", "pcg_init", "energy"]
+                                 "coarse_inverse", "pcg_init", "energy"]
 
 
 def eval_phase(ctx, fn, x0, steps, warmup):

@@ -206,6 +206,85 @@ tad_status tad_fn_add_elements_device(tad_fn fn, const tad_device_ops* ops, int6
 tad_status tad_fn_add_elements_dynamic_device(tad_fn fn, const tad_device_ops* const* ops, int n_ops, int64_t n_elem,
                                               const int64_t* elem_ptr, const int32_t* handles, const double* params);
 
+/* --- vector functions: residuals, Jacobians, sums of squares -------------------
+ *
+ * VectorFunction (VectorFunction.hh:19-192): each element produces a fixed
+ * number of residuals; r(x) stacks them term by term, element by element
+ * (VectorObjectiveTerm.hh:162-183).  The element functions are device functors
+ * compiled by the caller (include/tinyad_amd/DeviceElements.hh,
+ * tinyad_amd::device_residual<>), the GPU form of the reference's
+ * `(auto& element) -> TINYAD_VECTOR_TYPE(element)` lambdas. */
+
+typedef struct tad_device_residual_args
+{
+    int64_t n_elem;
+    const int32_t* conn;    /* device, SoA: conn[a * n_elem + e]                       */
+    const double* params;   /* device, SoA: params[k * n_elem + e]                     */
+    const double* x;        /* device, n_vars                                           */
+    double* r;              /* device: residual o of element e at r[e * n_outputs + o]  */
+    double* J;              /* device or NULL: row (e, o) of the Jacobian at            */
+                            /* J[(e * n_outputs + o) * K ..], K = valence * d values in  */
+                            /* ascending variable order                                 */
+    void* stream;           /* hipStream_t to launch on                                 */
+} tad_device_residual_args;
+
+typedef struct tad_device_residual_ops
+{
+    int version;            /* TAD_DEVICE_OPS_VERSION                                   */
+    int valence;            /* handles per element                                      */
+    int dim;                /* d: must equal the function's                             */
+    int n_params;           /* parameters per element (>= 0)                            */
+    int n_outputs;          /* residuals per element (1..64)                            */
+    /* sum of squares f_e = r_e . r_e with g_e = 2 J_e^T r_e and, modes 2 and 3,
+     * the Gauss-Newton Hessian 2 J_e^T J_e (no projection: it is positive
+     * semi-definite), through the library's slot maps */
+    int (*element)(const tad_device_element_args* args);
+    /* f_e = r_e . r_e (passive FP64) for eval and the line search */
+    int (*energy)(const tad_device_energy_args* args);
+    /* r_e and, when args->J is set, the rows of J_e */
+    int (*residuals)(const tad_device_residual_args* args);
+} tad_device_residual_ops;
+
+/* vector_function<d>(range(n_handles)) (VectorFunctionImpl.hh:300-325, ctor
+ * :40-56): a function whose terms are residual elements.  Every tad_fn entry
+ * point applies to the sum of squares f(x) = r(x) . r(x): tad_fn_eval is
+ * eval_sum_of_squares, tad_fn_eval_with_gradient gives g = 2 J^T r, and the
+ * second-order entry points, tad_fn_solve, tad_line_search and
+ * tad_newton_step see the Gauss-Newton Hessian 2 J^T J (eps is ignored), so
+ * tad_newton_step is a Gauss-Newton step (solve w = 2 w_identity for the
+ * reference's gauss_newton_direction(r, J, solver, w_identity)). */
+tad_status tad_fn_create_vector(tad_ctx ctx, int d, int64_t n_handles, tad_precision precision, tad_fn* out);
+
+/* add_elements<N, M>(range(n_elem), lambda) (VectorFunctionImpl.hh:58-90):
+ * conn (n_elem x valence) and params (n_elem x n_params), row-major host
+ * memory; the element's residuals follow those of the terms added before. */
+tad_status tad_fn_add_residuals_device(tad_fn fn, const tad_device_residual_ops* ops, int64_t n_elem,
+                                       const int32_t* conn, const double* params);
+
+/* n_outputs (VectorFunction.hh:183): residuals of all terms. */
+tad_status tad_fn_n_outputs(tad_fn fn, int64_t* n_outputs);
+
+/* Builds (once) and returns the Jacobian pattern: CSR of the n_outputs x n_vars
+ * matrix, row (term, element, output) holding the element's valence * d
+ * variables in ascending order (the reference builds it from triplets,
+ * VectorObjectiveTerm.hh:230-240).  Device pointers owned by fn; host copies
+ * when row_ptr_host / col_idx_host are non-NULL. */
+tad_status tad_fn_jacobian_pattern(tad_fn fn, int64_t* nnz, const int32_t** row_ptr_dev, const int32_t** col_idx_dev,
+                                   int32_t* row_ptr_host, int32_t* col_idx_host);
+
+/* eval(x) of a vector function (VectorFunctionImpl.hh:130-147): r, n_outputs values. */
+tad_status tad_fn_eval_residuals(tad_fn fn, const double* x, double* r, tad_mem mem);
+
+/* eval_with_jacobian(x, r, J) (VectorFunctionImpl.hh:157-176): r and the J
+ * values in the order of tad_fn_jacobian_pattern (J_values may be NULL). */
+tad_status tad_fn_eval_with_jacobian(tad_fn fn, const double* x, double* r, double* J_values, tad_mem mem);
+
+/* eval_sum_of_squares_with_derivatives(x, f, g, r, J)
+ * (VectorFunctionImpl.hh:258-275): f = sum_e r_e . r_e, g = 2 J^T r, r and
+ * J (r, J_values may be NULL). */
+tad_status tad_fn_eval_sum_of_squares_with_derivatives(tad_fn fn, const double* x, double* f, double* g, double* r,
+                                                       double* J_values, tad_mem mem);
+
 /* The objective terms (the reference's objective_terms, ScalarFunction.hh:229):
  * their number, and per term its energy (TAD_E_DEVICE for device energies),
  * valence and element count. */
@@ -291,6 +370,17 @@ tad_status tad_fn_get_launch_shape(tad_fn fn, int term, int* block, int* grid, i
 tad_status tad_fn_set_launch_shape(tad_fn fn, int term, int block, int grid, int map);
 tad_status tad_fn_get_assembly_shape(tad_fn fn, int* shape);
 tad_status tad_fn_set_assembly_shape(tad_fn fn, int shape);
+
+/* Pipelined second-order evaluation (no counterpart in the reference): the
+ * largest catalog term's elements run in `chunks` launches and the Hessian and
+ * gradient rows each launch completes are assembled on a second stream while
+ * the next one runs.  Results are identical for every chunk count (each entry
+ * is summed in the same fixed order).  chunks: 0 automatic (off), 1 off, at
+ * most 64 (fewer when the term has fewer 256-element tiles);
+ * tad_fn_autotune picks among 1, 2, 4 and 8.  get: the request and the chunk
+ * count in use. */
+tad_status tad_fn_get_pipeline(tad_fn fn, int* requested, int* used);
+tad_status tad_fn_set_pipeline(tad_fn fn, int chunks);
 
 /* Device pointers of the function's own buffers holding the gradient and CSR
  * values of the last second-order evaluation (valid until the next call). */

@@ -154,6 +154,166 @@ struct DeviceElement
     __device__ static double energy(const double* X, const double* p) { return F{}(X, p); }
 };
 
+// Residual elements of vector functions (VectorObjectiveTerm.hh:46-134): the
+// functor writes the element's M residuals,
+//   template <class S>
+//   __device__ void operator()(const S* x, const double* p, S* r) const;
+// with S = double (values) or tad::HyperDualM<T, M2, double> (derivatives;
+// only the first-order parts .a and .b[] are read).  The element kernel
+// evaluates the sum of squares f_e = r_e . r_e (FP64), g_e = 2 J_e^T r_e and
+// the Gauss-Newton Hessian 2 J_e^T J_e (positive semi-definite: written
+// without projection in modes 2 and 3).  Pass (i, j0) seeds .a on x_i and
+// .b[m] on x_{j0+m}, so one evaluation of the M residuals yields column i of
+// J_e and columns j0 .. j0+M2-1: (J_e^T J_e)(i, j) = sum_o J_oi J_oj.
+template <class F, int NN, int DD, int MM, int NPAR>
+struct DeviceResidual
+{
+    static constexpr int N = NN, D = DD, NP = NPAR, M = MM, K = NN * DD;
+    static_assert(NN >= 1 && DD >= 1 && DD <= 3 && K <= 12, "valence * d must be in 1..12, d in 1..3");
+    static_assert(MM >= 1 && MM <= 64, "residuals per element must be in 1..64");
+    static_assert(NPAR >= 0, "negative parameter count");
+
+    template <class T>
+    static constexpr int second_dirs() { return sizeof(T) == 8 ? 2 : 1; }
+
+    template <class T, bool HESS, int M2>
+    __device__ static void gn_passes(const double* X, const double* p, T* g, T* Hp)
+    {
+#pragma unroll 1
+        for (int i = 0; i < K; ++i)
+        {
+            const int j_end = HESS ? K : i + 1;
+#pragma unroll 1
+            for (int j0 = i; j0 < j_end; j0 += M2)
+            {
+                tad::HyperDualM<T, M2, double> hx[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                {
+                    hx[k] = tad::HyperDualM<T, M2, double>(X[k]);
+                    hx[k].a = (k == i) ? T(1) : T(0);
+#pragma unroll
+                    for (int m = 0; m < M2; ++m) hx[k].b[m] = (k == j0 + m) ? T(1) : T(0);
+                }
+                tad::HyperDualM<T, M2, double> r[M];
+                F{}(static_cast<const tad::HyperDualM<T, M2, double>*>(hx), p, r);
+                if (j0 == i)
+                {
+                    T gi = T(0);
+#pragma unroll
+                    for (int o = 0; o < M; ++o) gi += T(r[o].v) * r[o].a;
+#pragma unroll
+                    for (int k = 0; k < K; ++k) g[k] = (k == i) ? T(2) * gi : g[k];
+                }
+                if constexpr (HESS)
+                {
+#pragma unroll
+                    for (int m = 0; m < M2; ++m)
+                    {
+                        const int j = j0 + m;
+                        T h = T(0);
+#pragma unroll
+                        for (int o = 0; o < M; ++o) h += r[o].a * r[o].b[m];
+                        const int at = j < j_end ? tad::Sym<K>::idx(i, j) : -1;
+#pragma unroll
+                        for (int s = 0; s < tad::Sym<K>::n; ++s) Hp[s] = (s == at) ? T(2) * h : Hp[s];
+                    }
+                }
+            }
+        }
+    }
+
+    template <class T, int MODE, class Sink>
+    __device__ static double eval(const double* X, const double* p, T eps, const Sink& out, bool& nonfinite)
+    {
+        const double f = energy(X, p);
+        T g[K], Hp[tad::Sym<K>::n];
+#pragma unroll
+        for (int k = 0; k < K; ++k) g[k] = T(0);
+#pragma unroll
+        for (int s = 0; s < tad::Sym<K>::n; ++s) Hp[s] = T(0);
+        if constexpr (MODE >= tad::MODE_HESS) gn_passes<T, true, second_dirs<T>()>(X, p, g, Hp);
+        else gn_passes<T, false, 1>(X, p, g, Hp);
+        // 2 J^T J is positive semi-definite: modes 2 and 3 both write it as is
+        constexpr int WMODE = MODE >= tad::MODE_HESS ? tad::MODE_HESS : MODE;
+        tad::write_dense_record<NN, DD, T, WMODE>(g, Hp, eps, out, nonfinite);
+        return f;
+    }
+
+    // r_e . r_e in output order (the reference's res.dot(res),
+    // VectorObjectiveTerm.hh:336-338)
+    __device__ static double energy(const double* X, const double* p)
+    {
+        double r[M];
+        F{}(X, p, r);
+        double f = 0.0;
+#pragma unroll
+        for (int o = 0; o < M; ++o) f += r[o] * r[o];
+        return f;
+    }
+};
+
+// Residuals and Jacobian rows of residual elements, one lane per element.
+// Row (e, o) holds dr_o/dx for the element's K variables in ascending global
+// order (handle rank among the element's handles, then coordinate).
+template <class E, class F>
+__global__ __launch_bounds__(tad::kBlock) void k_residuals(int64_t n, const int32_t* __restrict__ conn,
+                                                           const double* __restrict__ params,
+                                                           const double* __restrict__ x, double* __restrict__ r,
+                                                           double* __restrict__ J)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    constexpr int N = E::N, D = E::D, K = E::K, M = E::M;
+    double X[K];
+    int32_t h[N];
+#pragma unroll
+    for (int a = 0; a < N; ++a)
+    {
+        h[a] = conn[a * n + e];
+#pragma unroll
+        for (int i = 0; i < D; ++i) X[a * D + i] = x[(int64_t)h[a] * D + i];
+    }
+    double p[E::NP > 0 ? E::NP : 1];
+#pragma unroll
+    for (int k = 0; k < E::NP; ++k) p[k] = params[k * n + e];
+    double rv[M];
+    F{}(static_cast<const double*>(X), p, rv);
+#pragma unroll
+    for (int o = 0; o < M; ++o) r[e * M + o] = rv[o];
+    if (!J) return;
+    int rank[N];
+#pragma unroll
+    for (int a = 0; a < N; ++a)
+    {
+        int c = 0;
+#pragma unroll
+        for (int b = 0; b < N; ++b) c += (h[b] < h[a]) ? 1 : 0;
+        rank[a] = c;
+    }
+    double* Je = J + e * (int64_t)(M * K);
+#pragma unroll 1
+    for (int i = 0; i < K; ++i)
+    {
+        tad::HyperDualM<double, 1, double> hx[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+        {
+            hx[k] = tad::HyperDualM<double, 1, double>(X[k]);
+            hx[k].a = (k == i) ? 1.0 : 0.0;
+            hx[k].b[0] = 0.0;
+        }
+        tad::HyperDualM<double, 1, double> ro[M];
+        F{}(static_cast<const tad::HyperDualM<double, 1, double>*>(hx), p, ro);
+        int ra = 0;
+#pragma unroll
+        for (int a = 0; a < N; ++a) ra = (a == i / D) ? rank[a] : ra;
+        const int col = ra * D + i % D;
+#pragma unroll
+        for (int o = 0; o < M; ++o) Je[o * K + col] = ro[o].a;
+    }
+}
+
 namespace detail {
 
 template <class E, class T>
@@ -205,6 +365,18 @@ int launch_device_energy(const tad_device_energy_args* a)
     return (int)hipGetLastError();
 }
 
+template <class E, class F>
+int launch_device_residuals(const tad_device_residual_args* a)
+{
+    if (!a || a->n_elem < 0 || !a->r) return (int)hipErrorInvalidValue;
+    if (a->n_elem == 0) return (int)hipSuccess;
+    const int64_t tiles = (a->n_elem + tad::kBlock - 1) / tad::kBlock;
+    if (tiles > 0x7fffffff) return (int)hipErrorInvalidValue;
+    k_residuals<E, F><<<dim3((unsigned)tiles), tad::kBlock, 0, static_cast<hipStream_t>(a->stream)>>>(
+        a->n_elem, a->conn, a->params, a->x, a->r, a->J);
+    return (int)hipGetLastError();
+}
+
 } // namespace detail
 
 // Launch table of energy F on elements of `valence` handles in dimension d
@@ -220,6 +392,25 @@ tad_device_ops device_element()
     o.n_params = NP;
     o.element = &detail::launch_device_element<E>;
     o.energy = &detail::launch_device_energy<E>;
+    return o;
+}
+
+// Launch table of residual functor F on elements of `valence` handles in
+// dimension d with M residuals and NP parameters per element, for
+// tad_fn_add_residuals_device.
+template <class F, int valence, int d, int M, int NP>
+tad_device_residual_ops device_residual()
+{
+    using E = DeviceResidual<F, valence, d, M, NP>;
+    tad_device_residual_ops o{};
+    o.version = TAD_DEVICE_OPS_VERSION;
+    o.valence = valence;
+    o.dim = d;
+    o.n_params = NP;
+    o.n_outputs = M;
+    o.element = &detail::launch_device_element<E>;
+    o.energy = &detail::launch_device_energy<E>;
+    o.residuals = &detail::launch_device_residuals<E, F>;
     return o;
 }
 

@@ -24,6 +24,7 @@
 #include <tinyad_amd.h>
 
 #include <array>
+#include <cmath>
 #include <cstdint>
 #include <functional>
 #include <memory>
@@ -487,6 +488,225 @@ auto scalar_function(const HandleRange& variable_range, const EvalSettings& sett
     return ScalarFunction<variable_dimension, H>(std::move(handles), settings);
 }
 
+// VectorFunction<d, double, VariableHandleT> (VectorFunction.hh:19-192): a
+// function R^n -> R^m whose elements produce fixed-size residual segments,
+// compiled on the device as residual functors (device_residual<F, N, d, M, NP>()
+// in DeviceElements.hh).  Movable, not copyable.  J is returned as a CSR
+// SparseMatrix (n_outputs x n_vars) with the pattern the reference's triplets
+// give (each residual row holds its element's variables, ascending).
+template <int variable_dimension, class VariableHandleT = int>
+class VectorFunction
+{
+    static_assert(variable_dimension >= 1 && variable_dimension <= 3, "variable dimension must be 1, 2 or 3");
+
+public:
+    static constexpr int d = variable_dimension;
+    using PassiveVariableVectorType = std::array<double, d>;
+
+    VectorFunction() = default;
+    VectorFunction(std::vector<VariableHandleT> variable_handles, const EvalSettings& settings_ = {})
+        : settings(settings_), variable_handles(std::move(variable_handles))
+    {
+        // handle indices must be a permutation of 0..n-1 (VectorFunctionImpl.hh:40-56)
+        std::vector<char> used(this->variable_handles.size(), 0);
+        for (const VariableHandleT& h : this->variable_handles)
+        {
+            const int64_t v = idx_from_handle(h);
+            if (v < 0 || v >= (int64_t)used.size() || used[v])
+                throw std::runtime_error("[tinyad_amd] variable handles are not a permutation of 0..n-1");
+            used[v] = 1;
+        }
+        if (this->variable_handles.empty()) throw std::runtime_error("[tinyad_amd] no variable handles");
+        ctx_ = Context::on_device(settings.device);
+        n_vars = d * (int64_t)this->variable_handles.size();
+        tad_fn h = nullptr;
+        check(tad_fn_create_vector(ctx_->handle(), d, (int64_t)this->variable_handles.size(), settings.precision,
+                                   &h));
+        fn_.reset(h, [](tad_fn p) { tad_fn_destroy(p); });
+    }
+    VectorFunction(const VectorFunction&) = delete;
+    VectorFunction& operator=(const VectorFunction&) = delete;
+    VectorFunction(VectorFunction&&) = default;
+    VectorFunction& operator=(VectorFunction&&) = default;
+
+    // add_elements<N, M>(range, lambda) (VectorFunctionImpl.hh:58-90) for a
+    // residual functor: ops from device_residual<F, N, d, M, NP>(), conn_of(e)
+    // the N handles element e accesses, params_of(e) its NP parameters.
+    template <int element_valence, int outputs_per_element, class Range, class ConnFn, class ParamFn>
+    void add_elements(const Range& element_range, const tad_device_residual_ops& ops, ConnFn conn_of,
+                      ParamFn params_of)
+    {
+        require_fn();
+        if (ops.valence != element_valence || ops.n_outputs != outputs_per_element)
+            throw std::runtime_error("[tinyad_amd] residual ops differ from add_elements<N, M>");
+        std::vector<int32_t> conn;
+        std::vector<double> params;
+        int64_t n = 0;
+        for (auto e : element_range)
+        {
+            const auto c = conn_of(e);
+            static_assert(std::tuple_size<std::decay_t<decltype(c)>>::value == element_valence,
+                          "conn_of must return std::array<handle, N>");
+            for (int a = 0; a < element_valence; ++a) conn.push_back((int32_t)idx_from_handle(c[a]));
+            const auto p = params_of(e);
+            if ((int)p.size() != ops.n_params)
+                throw std::runtime_error("[tinyad_amd] parameter row length differs from the residual's");
+            for (int k = 0; k < ops.n_params; ++k) params.push_back(p[k]);
+            ++n;
+        }
+        check(tad_fn_add_residuals_device(fn_.get(), &ops, n, conn.data(), params.data()));
+        n_elements += n;
+        n_outputs += n * outputs_per_element;
+        jac_.outer.clear();
+    }
+
+    VectorXd x_from_data(std::function<PassiveVariableVectorType(VariableHandleT)> read_user_data) const
+    {
+        VectorXd x((size_t)n_vars, std::nan(""));
+        for (const VariableHandleT& h : variable_handles)
+        {
+            const PassiveVariableVectorType v = read_user_data(h);
+            for (int i = 0; i < d; ++i) x[d * idx_from_handle(h) + i] = v[i];
+        }
+        for (double xi : x)
+            if (!std::isfinite(xi)) throw std::runtime_error("[tinyad_amd] x_from_data: non-finite entry");
+        return x;
+    }
+
+    void x_to_data(const VectorXd& x, std::function<void(VariableHandleT, PassiveVariableVectorType)> write_user_data)
+        const
+    {
+        check_size(x);
+        for (const VariableHandleT& h : variable_handles)
+        {
+            PassiveVariableVectorType v;
+            for (int i = 0; i < d; ++i) v[i] = x[d * idx_from_handle(h) + i];
+            write_user_data(h, v);
+        }
+    }
+
+    // eval(x) / operator() (VectorFunctionImpl.hh:130-147): the residuals.
+    VectorXd eval(const VectorXd& x) const
+    {
+        require_fn();
+        check_size(x);
+        VectorXd r((size_t)n_outputs);
+        check(tad_fn_eval_residuals(fn_.get(), x.data(), r.data(), TAD_MEM_HOST));
+        return r;
+    }
+    VectorXd operator()(const VectorXd& x) const { return eval(x); }
+
+    // eval_with_jacobian (VectorFunctionImpl.hh:157-190)
+    void eval_with_jacobian(const VectorXd& x, VectorXd& r, SparseMatrix& J) const
+    {
+        require_fn();
+        check_size(x);
+        r.assign((size_t)n_outputs, 0.0);
+        J = empty_jacobian();
+        check(tad_fn_eval_with_jacobian(fn_.get(), x.data(), r.data(), J.values.data(), TAD_MEM_HOST));
+    }
+    std::tuple<VectorXd, SparseMatrix> eval_with_jacobian(const VectorXd& x) const
+    {
+        VectorXd r;
+        SparseMatrix J;
+        eval_with_jacobian(x, r, J);
+        return { std::move(r), std::move(J) };
+    }
+
+    // eval_sum_of_squares (VectorFunctionImpl.hh:228-239)
+    double eval_sum_of_squares(const VectorXd& x) const
+    {
+        require_fn();
+        check_size(x);
+        double f = 0.0;
+        check(tad_fn_eval(fn_.get(), x.data(), TAD_MEM_HOST, &f));
+        return f;
+    }
+
+    // eval_sum_of_squares_with_derivatives (VectorFunctionImpl.hh:241-297):
+    // f = r . r, g = 2 J^T r
+    void eval_sum_of_squares_with_derivatives(const VectorXd& x, double& f, VectorXd& g, VectorXd& r,
+                                              SparseMatrix& J) const
+    {
+        require_fn();
+        check_size(x);
+        g.assign((size_t)n_vars, 0.0);
+        r.assign((size_t)n_outputs, 0.0);
+        J = empty_jacobian();
+        check(tad_fn_eval_sum_of_squares_with_derivatives(fn_.get(), x.data(), &f, g.data(), r.data(),
+                                                          J.values.data(), TAD_MEM_HOST));
+    }
+    std::tuple<double, VectorXd, VectorXd, SparseMatrix> eval_sum_of_squares_with_derivatives(const VectorXd& x) const
+    {
+        double f = 0.0;
+        VectorXd g, r;
+        SparseMatrix J;
+        eval_sum_of_squares_with_derivatives(x, f, g, r, J);
+        return { f, std::move(g), std::move(r), std::move(J) };
+    }
+
+    tad_fn handle() const { return fn_.get(); }
+
+    // Non-zeros of the Gauss-Newton matrix pattern (the scalar Hessian pattern).
+    int64_t nnz() const
+    {
+        require_fn();
+        int64_t nz = 0;
+        check(tad_fn_pattern(fn_.get(), &nz, nullptr, nullptr, nullptr, nullptr));
+        return nz;
+    }
+
+    EvalSettings settings;
+    int64_t n_vars = 0;
+    int64_t n_elements = 0;
+    int64_t n_outputs = 0;
+    std::vector<VariableHandleT> variable_handles;
+
+private:
+    void require_fn() const
+    {
+        if (!fn_) throw std::runtime_error("[tinyad_amd] empty (default-constructed or moved-from) VectorFunction");
+    }
+    void check_size(const VectorXd& x) const
+    {
+        if ((int64_t)x.size() != n_vars)
+            throw std::runtime_error("[tinyad_amd] x has " + std::to_string(x.size()) + " entries, expected " +
+                                     std::to_string(n_vars));
+    }
+    SparseMatrix empty_jacobian() const
+    {
+        if (jac_.outer.empty())
+        {
+            int64_t nnz = 0;
+            check(tad_fn_jacobian_pattern(fn_.get(), &nnz, nullptr, nullptr, nullptr, nullptr));
+            jac_.n_rows = n_outputs;
+            jac_.n_cols = n_vars;
+            jac_.outer.resize((size_t)n_outputs + 1);
+            jac_.inner.resize((size_t)(nnz > 0 ? nnz : 1));
+            check(tad_fn_jacobian_pattern(fn_.get(), nullptr, nullptr, nullptr, jac_.outer.data(), jac_.inner.data()));
+            jac_.inner.resize((size_t)nnz);
+        }
+        SparseMatrix J = jac_;
+        J.values.assign(J.inner.size() > 0 ? J.inner.size() : 1, 0.0);
+        J.values.resize(J.inner.size());
+        return J;
+    }
+
+    mutable SparseMatrix jac_;  // pattern cache (values unused)
+    std::shared_ptr<Context> ctx_;
+    std::shared_ptr<tad_fn_s> fn_;
+};
+
+// vector_function<d>(range) (VectorFunctionImpl.hh:300-325)
+template <int variable_dimension, class HandleRange>
+auto vector_function(const HandleRange& variable_range, const EvalSettings& settings = {})
+{
+    using H = std::decay_t<decltype(*std::begin(variable_range))>;
+    std::vector<H> handles;
+    for (const auto& h : variable_range) handles.push_back(h);
+    return VectorFunction<variable_dimension, H>(std::move(handles), settings);
+}
+
 // LinearSolver (LinearSolver.hh:13-19): solver options kept across Newton
 // iterations (the pattern is fixed per function, so nothing is re-analyzed).
 // coarse_refresh = k > 1: newton_direction rebuilds the coarse space of the
@@ -542,6 +762,42 @@ VectorXd newton_direction(const ScalarFunction<dd, H>& func, const VectorXd& g, 
     if (solver.last_stats.coarse_unknowns > 0 && !solver.last_stats.coarse_reused) solver.coarse_built = true;
     if (!solver.last_stats.converged) throw std::runtime_error("[tinyad_amd] Linear solve failed.");
     return dvec;
+}
+
+// gauss_newton_direction(r, J, solver, w) (GaussNewtonDirection.hh:24-47):
+// d with (w I + J^T J) d = -J^T r.  The GPU form takes the function and the
+// point x instead of (r, J): J^T J and J^T r are formed on the device from
+// the element Jacobians at x (J never leaves the device) and solved by the
+// library's PCG as 2 (w I + J^T J) d = -2 J^T r; throws if the solve fails.
+template <int dd, class H>
+VectorXd gauss_newton_direction(const VectorFunction<dd, H>& func, const VectorXd& x, LinearSolver& solver,
+                                double w_identity = 0.0)
+{
+    const int64_t n = func.n_vars;
+    if ((int64_t)x.size() != n) throw std::runtime_error("[tinyad_amd] gauss_newton_direction: size mismatch");
+    double f = 0.0;
+    VectorXd g((size_t)n), JtJ((size_t)func.nnz()), minus_g((size_t)n), dvec((size_t)n);
+    check(tad_fn_eval_with_derivatives(func.handle(), x.data(), &f, g.data(), JtJ.data(), TAD_MEM_HOST));
+    for (int64_t i = 0; i < n; ++i) minus_g[i] = -g[i];
+    tad_solve_opts o = solver.opts;
+    o.w_identity = 2.0 * w_identity;
+    if (!solver.coarse_built)
+        o.coarse_reuse = 0;
+    else if (solver.coarse_refresh > 1 && solver.reused_in_row + 1 < solver.coarse_refresh)
+        o.coarse_reuse = 1;
+    check(tad_fn_solve(func.handle(), JtJ.data(), minus_g.data(), dvec.data(), TAD_MEM_HOST, &o, &solver.last_stats));
+    solver.reused_in_row = solver.last_stats.coarse_reused ? solver.reused_in_row + 1 : 0;
+    if (solver.last_stats.coarse_unknowns > 0 && !solver.last_stats.coarse_reused) solver.coarse_built = true;
+    if (!solver.last_stats.converged) throw std::runtime_error("[tinyad_amd] Linear solve failed.");
+    return dvec;
+}
+
+// gauss_newton_direction with a fresh solver (GaussNewtonDirection.hh:57-64).
+template <int dd, class H>
+VectorXd gauss_newton_direction(const VectorFunction<dd, H>& func, const VectorXd& x, double w_identity = 0.0)
+{
+    LinearSolver solver;
+    return gauss_newton_direction(func, x, solver, w_identity);
 }
 
 // newton_direction_reduced_basis(g, H_proj, B, solver, w) (NewtonDirection.hh:76-105):
@@ -607,6 +863,33 @@ VectorXd line_search(const VectorXd& x0, const VectorXd& dvec, double f, const V
     {
         for (size_t j = 0; j < x0.size(); ++j) x_new[j] = x0[j] + s * dvec[j];
         const double f_new = func.eval(x_new);
+        if (f_new != f_new) throw std::runtime_error("[tinyad_amd] line_search: NaN energy");
+        if (f_new <= f + armijo_const * s * dg) return x_new;
+        if (try_one && s > 1.0 && s * shrink < 1.0) s = 1.0;
+        else s *= shrink;
+    }
+    return x0;
+}
+
+// line_search(x0, d, f, g, eval, ...) (LineSearch.hh:26-65) with any energy
+// callable double(const VectorXd&), e.g. a VectorFunction's
+// eval_sum_of_squares as in GaussNewtonTest.cc:146-149.
+template <class EvalFn, class = std::enable_if_t<std::is_invocable_r<double, EvalFn, const VectorXd&>::value>>
+VectorXd line_search(const VectorXd& x0, const VectorXd& dvec, double f, const VectorXd& g, EvalFn eval,
+                     double s_max = 1.0, double shrink = 0.8, int max_iters = 64, double armijo_const = 1e-4)
+{
+    if (x0.size() != g.size() || x0.size() != dvec.size())
+        throw std::runtime_error("[tinyad_amd] line_search: size mismatch");
+    if (s_max <= 0.0) throw std::runtime_error("[tinyad_amd] Max step size not positive.");
+    const bool try_one = s_max > 1.0;
+    double dg = 0.0;
+    for (size_t i = 0; i < g.size(); ++i) dg += dvec[i] * g[i];
+    VectorXd x_new = x0;
+    double s = s_max;
+    for (int i = 0; i < max_iters; ++i)
+    {
+        for (size_t j = 0; j < x0.size(); ++j) x_new[j] = x0[j] + s * dvec[j];
+        const double f_new = eval(x_new);
         if (f_new != f_new) throw std::runtime_error("[tinyad_amd] line_search: NaN energy");
         if (f_new <= f + armijo_const * s * dg) return x_new;
         if (try_one && s > 1.0 && s * shrink < 1.0) s = 1.0;

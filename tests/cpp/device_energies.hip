@@ -163,6 +163,91 @@ struct OneRing
     };
 };
 
+// Residuals of the reference's Gauss-Newton test (tests/GaussNewtonTest.cc:30-60):
+// s (J00, J01, J10, J11, Ji00, Ji01, Ji10, Ji11) with J = M Mr^-1 and
+// Ji = Mr M^-1 (Mr = (Mr^-1)^-1 from the parameters), every residual +inf if
+// det M <= 0.  p = [Mr^-1 (2 x 2 row-major), s].
+struct SymdirTriResidual
+{
+    template <class S>
+    __device__ void operator()(const S* x, const double* p, S* r) const
+    {
+        const S m00 = x[2] - x[0], m10 = x[3] - x[1];
+        const S m01 = x[4] - x[0], m11 = x[5] - x[1];
+        const S det = m00 * m11 - m01 * m10;
+        if (det <= 0.0)
+        {
+            for (int o = 0; o < 8; ++o) r[o] = S(INFINITY);
+            return;
+        }
+        const double dr = p[0] * p[3] - p[1] * p[2];
+        const double q00 = p[3] / dr, q01 = -p[1] / dr, q10 = -p[2] / dr, q11 = p[0] / dr;
+        const S inv = 1.0 / det;
+        const S n00 = m11 * inv, n01 = -m01 * inv, n10 = -m10 * inv, n11 = m00 * inv;
+        r[0] = p[4] * (m00 * p[0] + m01 * p[2]);
+        r[1] = p[4] * (m00 * p[1] + m01 * p[3]);
+        r[2] = p[4] * (m10 * p[0] + m11 * p[2]);
+        r[3] = p[4] * (m10 * p[1] + m11 * p[3]);
+        r[4] = p[4] * (q00 * n00 + q01 * n10);
+        r[5] = p[4] * (q00 * n01 + q01 * n11);
+        r[6] = p[4] * (q10 * n00 + q11 * n10);
+        r[7] = p[4] * (q10 * n01 + q11 * n11);
+    }
+};
+
+// Pin residuals s (target - x) of one handle (GaussNewtonTest.cc:62-69 with
+// weight s).  p = [target (D), s].
+template <int D>
+struct PinResidual
+{
+    template <class S>
+    __device__ void operator()(const S* x, const double* p, S* r) const
+    {
+        for (int i = 0; i < D; ++i) r[i] = p[D] * (p[i] - x[i]);
+    }
+};
+
+// Symmetric Dirichlet tet residuals: s (J (9 entries), J^-1 (9 entries)),
+// J^-1 = cof(J)^T / det J; +inf if det M <= 0.  p = [Mr^-1 (3 x 3 row-major), s].
+struct SymdirTetResidual
+{
+    template <class S>
+    __device__ void operator()(const S* x, const double* p, S* r) const
+    {
+        S m[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int c = 0; c < 3; ++c) m[i][c] = x[3 * (c + 1) + i] - x[i];
+        const S detm = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+                       m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                       m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+        if (detm <= 0.0)
+        {
+            for (int o = 0; o < 18; ++o) r[o] = S(INFINITY);
+            return;
+        }
+        S j[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int c = 0; c < 3; ++c) j[i][c] = m[i][0] * p[0 * 3 + c] + m[i][1] * p[1 * 3 + c] + m[i][2] * p[2 * 3 + c];
+        const S detj = j[0][0] * (j[1][1] * j[2][2] - j[1][2] * j[2][1]) -
+                       j[0][1] * (j[1][0] * j[2][2] - j[1][2] * j[2][0]) +
+                       j[0][2] * (j[1][0] * j[2][1] - j[1][1] * j[2][0]);
+        const S inv = 1.0 / detj;
+        for (int i = 0; i < 3; ++i)
+            for (int c = 0; c < 3; ++c)
+            {
+                r[3 * i + c] = p[9] * j[i][c];
+                // (J^-1)_{ic} = cof(J)_{ci} / det J
+                const int c1 = (c + 1) % 3, c2 = (c + 2) % 3, i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+                r[9 + 3 * i + c] = p[9] * ((j[c1][i1] * j[c2][i2] - j[c1][i2] * j[c2][i1]) * inv);
+            }
+    }
+};
+
+const tad_device_residual_ops g_res_tri = tinyad_amd::device_residual<SymdirTriResidual, 3, 2, 8, 5>();
+const tad_device_residual_ops g_res_pin2 = tinyad_amd::device_residual<PinResidual<2>, 1, 2, 2, 3>();
+const tad_device_residual_ops g_res_pin3 = tinyad_amd::device_residual<PinResidual<3>, 1, 3, 3, 4>();
+const tad_device_residual_ops g_res_tet = tinyad_amd::device_residual<SymdirTetResidual, 4, 3, 18, 10>();
+
 const tad_device_ops g_tri = tinyad_amd::device_element<SymdirTri, 3, 2, 5>();
 const tad_device_ops g_tet = tinyad_amd::device_element<SymdirTet, 4, 3, 10>();
 const tad_device_ops g_quad = tinyad_amd::device_element<Quadratic4, 2, 2, 21>();
@@ -191,4 +276,8 @@ const tad_device_ops* tad_test_powers_ops(int i) { return (i >= 0 && i < 2) ? &g
 const tad_device_ops* tad_test_sum_squares_ops(int i) { return (i >= 0 && i < 2) ? &g_sum_sq[i] : nullptr; }
 // valences 6, 7, 8, 10 (add_elements_dynamic<6, 7, 8, 10>)
 const tad_device_ops* tad_test_one_ring_ops(int i) { return (i >= 0 && i < 4) ? &g_one_ring[i] : nullptr; }
+// residual elements of vector functions
+const tad_device_residual_ops* tad_test_res_symdir_tri_ops() { return &g_res_tri; }
+const tad_device_residual_ops* tad_test_res_pin_ops(int d) { return d == 2 ? &g_res_pin2 : d == 3 ? &g_res_pin3 : nullptr; }
+const tad_device_residual_ops* tad_test_res_symdir_tet_ops() { return &g_res_tet; }
 }

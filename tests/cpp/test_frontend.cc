@@ -50,6 +50,9 @@ using tinyad_amd::VectorXd;
 extern "C" const tad_device_ops* tad_test_symdir_tri_ops();
 // ||sum of the element's handle values||^2 compiled at valences 3 (i = 0) and 1 (i = 1)
 extern "C" const tad_device_ops* tad_test_sum_squares_ops(int i);
+// residuals (tests/cpp/device_energies.hip): symmetric Dirichlet triangles (8), pins (d)
+extern "C" const tad_device_residual_ops* tad_test_res_symdir_tri_ops();
+extern "C" const tad_device_residual_ops* tad_test_res_pin_ops(int d);
 
 static std::vector<double> quad_params(const std::vector<double>& A, const std::vector<double>& b, double c)
 {
@@ -144,6 +147,73 @@ static bool test_newton(VectorXd* x_out, const tad_device_ops* ops = nullptr)
     func.x_to_data(x, [&](int v, const std::array<double, 2>& p) { V[v] = p; });
     EXPECT(V[4][0] == x[8] && V[4][1] == x[9]);
     if (x_out) *x_out = x;
+    return ok;
+}
+
+// tests/GaussNewtonTest.cc:13-162: the Newton test's energy as residuals
+// (8 per triangle scaled by 1/sqrt(F.rows()), 2 per pin), compared with the
+// scalar form at every iterate (f and g to 1e-12), 20 Gauss-Newton iterations
+// with w = 1e-12 and a line search on eval_sum_of_squares; f -> 4 +- 0.1.
+static bool test_gauss_newton()
+{
+    bool ok = true;
+    const double V_rest[6][2] = { { 0, 0 }, { 1, 0 }, { 0, 1 }, { 1, 1 }, { 0, 2 }, { 1, 2 } };
+    const int F[4][3] = { { 0, 1, 2 }, { 1, 3, 2 }, { 2, 3, 5 }, { 2, 5, 4 } };
+    const int b[2] = { 0, 4 };
+    const double bc[2][2] = { { 0.0, 0.0 }, { -2.0, 0.0 } };
+    auto conn_of = [&](int f) { return std::array<int, 3>{ F[f][0], F[f][1], F[f][2] }; };
+    auto rest_params = [&](int f, double w) {
+        const double m00 = V_rest[F[f][1]][0] - V_rest[F[f][0]][0], m10 = V_rest[F[f][1]][1] - V_rest[F[f][0]][1];
+        const double m01 = V_rest[F[f][2]][0] - V_rest[F[f][0]][0], m11 = V_rest[F[f][2]][1] - V_rest[F[f][0]][1];
+        const double det = m00 * m11 - m01 * m10;
+        return std::vector<double>{ m11 / det, -m01 / det, -m10 / det, m00 / det, w };
+    };
+    auto pin_of = [&](int i) { return std::array<int, 1>{ b[i] }; };
+    auto pin_params = [&](int i) { return std::vector<double>{ bc[i][0], bc[i][1], 1.0 }; };
+    auto func_sos = tinyad_amd::vector_function<2>(tinyad_amd::range(6));
+    func_sos.add_elements<3, 8>(tinyad_amd::range(4), *tad_test_res_symdir_tri_ops(), conn_of,
+                                [&](int f) { return rest_params(f, 1.0 / std::sqrt(4.0)); });
+    func_sos.add_elements<1, 2>(tinyad_amd::range(2), *tad_test_res_pin_ops(2), pin_of, pin_params);
+    auto func_ref = tinyad_amd::scalar_function<2>(tinyad_amd::range(6));
+    func_ref.add_elements<3>(tinyad_amd::range(4), TAD_E_SYMDIR_TRI, conn_of,
+                             [&](int f) { return rest_params(f, 1.0 / 4.0); });
+    func_ref.add_elements<1>(tinyad_amd::range(2), TAD_E_PENALTY, pin_of, pin_params);
+    EXPECT(func_sos.n_outputs == 36);
+    VectorXd x = func_sos.x_from_data([&](int v) { return std::array<double, 2>{ 0.5 * V_rest[v][0], 0.25 * V_rest[v][1] }; });
+    tinyad_amd::LinearSolver solver;
+    solver.opts.tol = 1e-13;
+    for (int i = 0; i < 20; ++i)
+    {
+        auto [f_ref, g_ref, H_ref] = func_ref.eval_with_hessian_proj(x);
+        EXPECT(std::fabs(func_sos.eval_sum_of_squares(x) - f_ref) <= 1e-12);
+        auto [f, g, r, J] = func_sos.eval_sum_of_squares_with_derivatives(x);
+        EXPECT(std::fabs(f - f_ref) <= 1e-12);
+        double dg = 0.0;
+        for (size_t k = 0; k < g.size(); ++k) dg = std::fmax(dg, std::fabs(g[k] - g_ref[k]));
+        EXPECT(dg <= 1e-12);
+        EXPECT(J.rows() == 36 && J.cols() == 12 && J.nonZeros() == 4 * 8 * 6 + 2 * 2 * 2);
+        // g = 2 J^T r from the returned J and r
+        VectorXd jtr(12, 0.0);
+        for (int64_t row = 0; row < J.rows(); ++row)
+            for (int32_t k = J.outer[row]; k < J.outer[row + 1]; ++k) jtr[J.inner[k]] += J.values[k] * r[row];
+        double dj = 0.0;
+        for (size_t k = 0; k < g.size(); ++k) dj = std::fmax(dj, std::fabs(2.0 * jtr[k] - g[k]));
+        EXPECT(dj <= 1e-12);
+        const VectorXd d = tinyad_amd::gauss_newton_direction(func_sos, x, solver, 1e-12);
+        x = tinyad_amd::line_search(x, d, f, g, [&](const VectorXd& z) { return func_sos.eval_sum_of_squares(z); });
+    }
+    auto [f, g, r, J] = func_sos.eval_sum_of_squares_with_derivatives(x);
+    double gmax = 0.0;
+    for (double gi : g) gmax = std::fmax(gmax, std::fabs(gi));
+    EXPECT(std::fabs(f - 4.0) <= 0.1 && gmax <= 0.1);
+    const VectorXd rr = func_sos(x);
+    double drr = 0.0;
+    for (size_t k = 0; k < rr.size(); ++k) drr = std::fmax(drr, std::fabs(rr[k] - r[k]));
+    EXPECT(rr.size() == 36 && drr <= 1e-14);
+    bool threw = false;
+    try { func_sos.add_elements<3, 8>(tinyad_amd::range(1), *tad_test_res_pin_ops(2), conn_of, pin_params); }
+    catch (const std::runtime_error&) { threw = true; }
+    EXPECT(threw);  // valence / output count differ from the ops
     return ok;
 }
 
@@ -477,6 +547,7 @@ int main()
         { "NewtonTest.2DDeformation", []() { return test_newton(nullptr); } },
         { "NewtonTest.Deterministic", test_newton_deterministic },
         { "NewtonTest.2DDeformation.DeviceFunctor", []() { return test_newton(nullptr, tad_test_symdir_tri_ops()); } },
+        { "GaussNewtonTest.2DDeformationGaussNewton", test_gauss_newton },
         { "ExceptionTest", test_errors_and_recovery },
         { "Batching.SegmentsAutotuneParams", test_segments_autotune_params },
         { "Solver.TwoLevel", test_two_level_solve },

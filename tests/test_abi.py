@@ -44,7 +44,7 @@ def test_struct_layouts_match_header(tmp_path):
     import subprocess
     mirrors = {"tad_solve_opts": _lib.SolveOpts, "tad_solve_stats": _lib.SolveStats,
                "tad_newton_opts": _lib.NewtonOpts, "tad_newton_stats": _lib.NewtonStats,
-               "tad_device_ops": _lib.DeviceOps}
+               "tad_device_ops": _lib.DeviceOps, "tad_device_residual_ops": _lib.DeviceResidualOps}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "tinyad_amd.h"', "int main(void) {"]
     for cname, py in mirrors.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')

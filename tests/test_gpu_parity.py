@@ -660,6 +660,41 @@ def test_autotune_keeps_bits(ctx, precision, kind):
             fg.set_launch_shape(len(P.terms) - 1, 64, 8, 0)
     assert (fg.launch_shape(0), fg.assembly_shape) == tuned
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pipelined_evaluation_keeps_bits(ctx, precision):
+    """Pipelined second-order evaluation (element chunks on the context's
+    stream, the rows each chunk completes assembled on a second stream):
+    identical f, g and H for 1 (off), 2, 3, 4, 8 and 64 chunks, every
+    assembly shape, plain and persistent element launches, projected and
+    unprojected Hessians, host and device outputs; the plan is kept until the
+    elements or the request change."""
+    import torch
+    P = configs.twist_cube(m=10, seed=3)   # 6,000 tets + pins
+    fg = tad.ScalarFunction.from_problem(P, ctx, precision=precision)
+    fg.pipeline = 1
+    assert fg.pipeline == (1, 1)
+    ref = fg.eval_with_hessian_proj(P.x0)
+    ref_d = fg.eval_with_derivatives(P.x0)
+    xt = torch.tensor(P.x0, device="cuda:0")
+    for chunks in (2, 3, 4, 8, 64):
+        fg.pipeline = chunks
+        assert fg.pipeline == (chunks, min(chunks, 6000 // 256))
+        for shape in range(4):
+            fg.assembly_shape = shape
+            for ls in [(0, 0, 0), (64, 64, 0), (128, 16, 1)]:
+                fg.set_launch_shape(0, *ls)
+                f, g, H = fg.eval_with_hessian_proj(P.x0)
+                assert f == ref[0] and np.array_equal(g, ref[1]) and np.array_equal(H.data, ref[2].data), \
+                    (chunks, shape, ls)
+        f, g, H = fg.eval_with_derivatives(P.x0)
+        assert f == ref_d[0] and np.array_equal(g, ref_d[1]) and np.array_equal(H.data, ref_d[2].data)
+        fd, gd, Hd = fg.eval_with_hessian_proj(xt)
+        assert fd == ref[0] and np.array_equal(gd.cpu().numpy(), ref[1]) and np.array_equal(Hd.cpu().numpy(), ref[2].data)
+    fg.pipeline = 0
+    assert fg.pipeline == (0, 1)  # automatic: off
+    with pytest.raises(tad.TadError):
+        fg.pipeline = 65
+
 
 # ---------------------------------------------------------------------------
 # Line search and Newton decrement (LineSearch.hh:14-65, NewtonDecrement.hh:20-26)

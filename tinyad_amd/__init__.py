@@ -429,6 +429,18 @@ class ScalarFunction:
     def assembly_shape(self, shape: int):
         check(lib().tad_fn_set_assembly_shape(self._h, int(shape)))
 
+    @property
+    def pipeline(self) -> tuple:
+        """(requested, used) element chunks of the pipelined second-order
+        evaluation (requested 0: automatic, 1: off); identical results for all."""
+        r, u = ctypes.c_int(), ctypes.c_int()
+        check(lib().tad_fn_get_pipeline(self._h, ctypes.byref(r), ctypes.byref(u)))
+        return r.value, u.value
+
+    @pipeline.setter
+    def pipeline(self, chunks: int):
+        check(lib().tad_fn_set_pipeline(self._h, int(chunks)))
+
     # -- solver ------------------------------------------------------------
     def _dev_vec(self, a, n: int, name: str):
         """a checked as a contiguous float64 CUDA tensor of n entries on the context's device."""
@@ -538,6 +550,147 @@ class ScalarFunction:
         g = self._host_vec(g, self.n_vars, "g")
         check(lib().tad_newton_decrement(self._h, d.ctypes.data, g.ctypes.data, _lib.MEM_HOST, ctypes.byref(out)))
         return out.value
+
+
+class VectorFunction(ScalarFunction):
+    """VectorFunction<d, double, int> (VectorFunction.hh:19-192) backed by the
+    HIP library.  Residual elements are device functors
+    (tinyad_amd::device_residual<F, N, d, M, NP>() in
+    include/tinyad_amd/DeviceElements.hh).  The inherited ScalarFunction
+    methods act on the sum of squares f = r . r: eval_with_gradient gives
+    g = 2 J^T r, the second-order evaluations the Gauss-Newton Hessian
+    2 J^T J, line_search searches on f and newton_step is a Gauss-Newton step
+    (solve w = 2 w_identity for the reference's w_identity)."""
+
+    def __init__(self, d: int, n_handles: int, ctx: Context | None = None, precision: str = "fp64"):
+        self.ctx = ctx or Context.default()
+        self.d = int(d)
+        self.n_handles = int(n_handles)
+        self.n_vars = self.d * self.n_handles
+        self.precision = precision
+        prec = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}[precision]
+        h = ctypes.c_void_p()
+        check(lib().tad_fn_create_vector(self.ctx._h, self.d, self.n_handles, prec, ctypes.byref(h)))
+        self._h = h
+        self.n_elements = 0
+        self._pattern = None
+        self._jpattern = None
+
+    def add_elements(self, *a, **kw):
+        raise TadError(1, "vector functions take residual elements (add_elements_device with residual ops)")
+
+    def add_elements_device(self, ops, conn, params=None):
+        """add_elements<N, M>(range, lambda) (VectorFunctionImpl.hh:58-90) for
+        residuals compiled on the device: ops is the address of a
+        tad_device_residual_ops (tinyad_amd::device_residual<F, N, d, M, NP>()),
+        conn n_elem x N handles, params n_elem x NP rows (one row is broadcast)."""
+        addr = ctypes.cast(ops, ctypes.c_void_p).value if not isinstance(ops, int) else ops
+        if not addr:
+            raise TadError(1, "null residual ops")
+        o = _lib.DeviceResidualOps.from_address(addr)
+        conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1, o.valence)
+        n = conn.shape[0]
+        pp = None
+        if o.n_params > 0:
+            params = np.ascontiguousarray(params, dtype=np.float64)
+            if params.ndim == 1:
+                params = np.ascontiguousarray(np.broadcast_to(params, (n, params.size)))
+            if params.shape != (n, o.n_params):
+                raise TadError(1, f"residual expects {o.n_params} parameters per element")
+            pp = params.ctypes.data
+        check(lib().tad_fn_add_residuals_device(self._h, addr, n, conn.ctypes.data, pp))
+        self.n_elements += n
+        self._pattern = None
+        self._jpattern = None
+
+    def add_elements_dynamic_device(self, *a, **kw):
+        raise TadError(1, "vector functions take fixed-valence residual elements")
+
+    @property
+    def n_outputs(self) -> int:
+        m = ctypes.c_int64()
+        check(lib().tad_fn_n_outputs(self._h, ctypes.byref(m)))
+        return m.value
+
+    def jacobian_pattern(self):
+        """(row_ptr, col_idx) host copies of the Jacobian CSR pattern (n_outputs x n_vars)."""
+        if self._jpattern is None:
+            nnz = ctypes.c_int64()
+            check(lib().tad_fn_jacobian_pattern(self._h, ctypes.byref(nnz), None, None, None, None))
+            rp = np.zeros(self.n_outputs + 1, dtype=np.int32)
+            ci = np.zeros(nnz.value, dtype=np.int32)
+            check(lib().tad_fn_jacobian_pattern(self._h, None, None, None, rp.ctypes.data, ci.ctypes.data))
+            self._jpattern = (rp, ci)
+        return self._jpattern
+
+    def _r_J(self, mem, with_j):
+        m = self.n_outputs
+        nnz_j = int(self.jacobian_pattern()[1].size) if with_j else 0
+        if mem == _lib.MEM_DEVICE:
+            import torch
+            dev = torch.device("cuda", self.ctx.device)
+            r = torch.empty(m, dtype=torch.float64, device=dev)
+            J = torch.empty(nnz_j, dtype=torch.float64, device=dev) if with_j else None
+            return r, J, r.data_ptr(), (J.data_ptr() if with_j else None)
+        r = np.zeros(m)
+        J = np.zeros(nnz_j) if with_j else None
+        return r, J, r.ctypes.data, (J.ctypes.data if with_j else None)
+
+    def _wrap_j(self, J, mem):
+        if mem == _lib.MEM_DEVICE:
+            return J
+        import scipy.sparse as sp
+        rp, ci = self.jacobian_pattern()
+        return sp.csr_matrix((J, ci, rp), shape=(self.n_outputs, self.n_vars))
+
+    def eval(self, x):
+        """r(x) (VectorFunctionImpl.hh:130-147)."""
+        ptr, mem = self._x(x)
+        r, _, rp, _ = self._r_J(mem, False)
+        check(lib().tad_fn_eval_residuals(self._h, ptr, rp, mem))
+        return r
+
+    __call__ = eval
+
+    def eval_with_jacobian(self, x):
+        """(r, J) (VectorFunctionImpl.hh:157-176); J is a scipy CSR matrix for
+        host x, the CUDA tensor of its values (jacobian_pattern order) for device x."""
+        ptr, mem = self._x(x)
+        r, J, rp, jp = self._r_J(mem, True)
+        check(lib().tad_fn_eval_with_jacobian(self._h, ptr, rp, jp, mem))
+        return r, self._wrap_j(J, mem)
+
+    def eval_sum_of_squares(self, x) -> float:
+        """sum_e r_e . r_e (VectorFunctionImpl.hh:228-239)."""
+        return ScalarFunction.eval(self, x)
+
+    def eval_sum_of_squares_with_derivatives(self, x):
+        """(f, g, r, J) with f = r . r and g = 2 J^T r (VectorFunctionImpl.hh:258-275)."""
+        ptr, mem = self._x(x)
+        g, _, gp, _ = self._outputs(mem, False)
+        r, J, rp, jp = self._r_J(mem, True)
+        f = ctypes.c_double()
+        check(lib().tad_fn_eval_sum_of_squares_with_derivatives(self._h, ptr, ctypes.byref(f), gp, rp, jp, mem))
+        return f.value, g, r, self._wrap_j(J, mem)
+
+    def eval_gauss_newton(self, x):
+        """(f, g, H) with g = 2 J^T r and H = 2 J^T J over the Hessian pattern."""
+        return self.eval_with_derivatives(x)
+
+    def gauss_newton_direction(self, x, w_identity: float = 0.0, **solve_opts):
+        """d with (w_identity I + J^T J) d = -J^T r at x (GaussNewtonDirection.hh:24-47):
+        the library's PCG on 2 (w_identity I + J^T J) d = -2 J^T r."""
+        _, g, H = self.eval_with_derivatives(x)
+        rhs = -g
+        sol, st = self.solve(rhs, H if _is_torch(rhs) else H.data, w_identity=2.0 * float(w_identity), **solve_opts)
+        if not st.converged:
+            raise TadError(4, "Gauss-Newton solve did not converge")
+        return sol
+
+
+def vector_function(d: int, n_handles: int, ctx: Context | None = None, precision: str = "fp64") -> VectorFunction:
+    """vector_function<d>(range(n_handles)) (VectorFunctionImpl.hh:300-325)."""
+    return VectorFunction(d, n_handles, ctx, precision)
 
 
 def scalar_function(d: int, n_handles: int, ctx: Context | None = None, precision: str = "fp64") -> ScalarFunction:

@@ -31,7 +31,10 @@ EXPORTED = [
     "tad_line_search", "tad_newton_decrement",
     "tad_fn_set_segments", "tad_fn_segment_energies", "tad_fn_autotune",
     "tad_fn_get_launch_shape", "tad_fn_set_launch_shape", "tad_fn_get_assembly_shape", "tad_fn_set_assembly_shape",
+    "tad_fn_get_pipeline", "tad_fn_set_pipeline",
     "tad_fn_set_params", "tad_fn_rest_vjp",
+    "tad_fn_create_vector", "tad_fn_add_residuals_device", "tad_fn_n_outputs", "tad_fn_jacobian_pattern",
+    "tad_fn_eval_residuals", "tad_fn_eval_with_jacobian", "tad_fn_eval_sum_of_squares_with_derivatives",
     "tad_selftest_dual",
 ]
 
@@ -56,6 +59,13 @@ class DeviceOps(ctypes.Structure):
     """tad_device_ops (read-only view; the function pointers are opaque)."""
     _fields_ = [("version", ctypes.c_int), ("valence", ctypes.c_int), ("dim", ctypes.c_int),
                 ("n_params", ctypes.c_int), ("element", ctypes.c_void_p), ("energy", ctypes.c_void_p)]
+
+
+class DeviceResidualOps(ctypes.Structure):
+    """tad_device_residual_ops (read-only view; the function pointers are opaque)."""
+    _fields_ = [("version", ctypes.c_int), ("valence", ctypes.c_int), ("dim", ctypes.c_int),
+                ("n_params", ctypes.c_int), ("n_outputs", ctypes.c_int), ("element", ctypes.c_void_p),
+                ("energy", ctypes.c_void_p), ("residuals", ctypes.c_void_p)]
 
 
 class NewtonOpts(ctypes.Structure):
@@ -119,6 +129,13 @@ def lib():
         "tad_fn_n_elements": [vp, pi64],
         "tad_fn_pattern": [vp, pi64, pvp, pvp, vp, vp],
         "tad_fn_eval": [vp, vp, i32, pdbl],
+        "tad_fn_create_vector": [vp, i32, i64, i32, vp],
+        "tad_fn_add_residuals_device": [vp, vp, i64, vp, vp],
+        "tad_fn_n_outputs": [vp, vp],
+        "tad_fn_jacobian_pattern": [vp, vp, vp, vp, vp, vp],
+        "tad_fn_eval_residuals": [vp, vp, vp, i32],
+        "tad_fn_eval_with_jacobian": [vp, vp, vp, vp, i32],
+        "tad_fn_eval_sum_of_squares_with_derivatives": [vp, vp, pdbl, vp, vp, vp, i32],
         "tad_fn_eval_with_gradient": [vp, vp, pdbl, vp, i32],
         "tad_fn_eval_with_derivatives": [vp, vp, pdbl, vp, vp, i32],
         "tad_fn_eval_with_hessian_proj": [vp, vp, dbl, pdbl, vp, vp, i32],
@@ -139,6 +156,8 @@ def lib():
         "tad_fn_set_launch_shape": [vp, i32, i32, i32, i32],
         "tad_fn_get_assembly_shape": [vp, ctypes.POINTER(ctypes.c_int)],
         "tad_fn_set_assembly_shape": [vp, i32],
+        "tad_fn_get_pipeline": [vp, vp, vp],
+        "tad_fn_set_pipeline": [vp, i32],
         "tad_fn_set_params": [vp, i32, vp, i32],
         "tad_fn_rest_vjp": [vp, i32, vp, vp, vp, dbl, vp, i32],
         "tad_selftest_dual": [vp, i32, dbl, dbl, pdbl],

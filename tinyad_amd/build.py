@@ -15,14 +15,13 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(HERE, "libtinyad_amd.so")
-SOURCES = ["tad_element.hip", "tad_energy.hip", "tad_linalg.hip", "tad_adjoint.hip", "tad_api.cpp"]
+SOURCES = ["tad_element.hip", "tad_energy.hip", "tad_linalg.hip", "tad_dense.hip", "tad_adjoint.hip", "tad_api.cpp"]
 HEADERS = ["tad_energies.hpp", "tad_elements.hpp", "tad_internal.hpp"]
 # public headers the library's sources also compile (device code shared with
 # element kernels built from user energies)
 PUBLIC_HEADERS = ["tinyad_amd.h", "tinyad_amd/device/Dual.hpp", "tinyad_amd/device/Projection.hpp",
                   "tinyad_amd/device/Scatter.hpp", "tinyad_amd/device/Modes.hpp", "tinyad_amd/device/Kernels.hpp"]
 ARCH = os.environ.get("TAD_OFFLOAD_ARCH", "gfx950")
-ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
          # packed FP32 instructions stay disabled in all device code
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -50,8 +49,7 @@ def build(force: bool = False, jobs: int = 4) -> str:
         objs = list(ex.map(lambda s: _compile(s, force), SOURCES))
     if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < max(os.path.getmtime(o) for o in objs):
         tmp = LIB + ".tmp"
-        cmd = ["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + [
-            "-L" + os.path.join(ROCM, "lib"), "-lrocsolver", "-lrocblas", "-Wl,-rpath," + os.path.join(ROCM, "lib")]
+        cmd = ["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"link failed:\n{r.stderr[-8000:]}")

@@ -17,8 +17,6 @@
 #include "tad_internal.hpp"
 
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -178,7 +176,9 @@ struct tad_ctx_s
     hipStream_t own_stream = nullptr;
     KernelTimer timer;
     double* pinned = nullptr;  // small pinned staging buffer (256 doubles)
-    rocblas_handle blas = nullptr;  // dense factorization of coarse matrices (created on first use)
+    // second stream and chunk events of pipelined evaluations (created on first use)
+    hipStream_t aux = nullptr;
+    std::vector<hipEvent_t> events;
     // one reference held by the creator plus one per live function: the
     // context is released when the last of them is gone, whatever the order
     // of tad_ctx_destroy and tad_fn_destroy calls
@@ -192,7 +192,8 @@ static void ctx_release(tad_ctx ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->timer.collect();
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
-    if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
+    for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
+    if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
 }
@@ -210,6 +211,13 @@ struct tad_fn_s
     int d = 0;
     int64_t n_handles = 0, n_vars = 0, n_elem_total = 0;
     int precision = 0;
+    // vector function (residual terms only): residual count and the Jacobian
+    // pattern (CSR, n_outputs x n_vars), built on first use
+    bool vec = false;
+    int64_t n_outputs = 0, nnz_J = 0;
+    bool jpattern_ready = false;
+    DevBuf<int32_t> jrow_ptr, jcol;
+    DevBuf<double> r_buf, J_buf;
     std::vector<TermDev> terms;
     std::vector<HostTerm> hterms;
 
@@ -228,6 +236,16 @@ struct tad_fn_s
     DevBuf<int32_t> uplace;        // 4 per upper block
     int asm_shape = 0;             // Hessian assembly launch shape (tad_fn_autotune)
     std::vector<int32_t> h_brow_ptr, h_bcol;  // host copy of the block pattern (coarse aggregation)
+    std::vector<int64_t> h_urow_ptr;          // first upper block of each block row (host)
+    // pipelined second-order evaluation: the largest catalog term's elements
+    // in pipe_used chunks on the context's stream; after chunk k the rows
+    // [pipe_rows[k], pipe_rows[k+1]) have all their contributions and are
+    // assembled on a second stream while the next chunk runs
+    int pipe_chunks = 0;       // requested chunks (0: automatic, 1: off)
+    bool pipe_ready = false;   // plan below valid for the pattern and request
+    int pipe_term = -1;
+    int pipe_used = 1;
+    std::vector<int64_t> pipe_e, pipe_rows;
 
     // evaluation buffers
     DevBuf<unsigned char> gc_buf, hc_buf;  // contributions (FP64 or FP32 by precision)
@@ -257,7 +275,7 @@ struct tad_fn_s
         int max_agg_chunks = 0;
         DevBuf<int32_t> agg_of, agg_ptr, agg_vert, pair_blocks, pair_ab, chunk_ptr, agg_chunk, pair_chunk;
         DevBuf<int64_t> gchunk_ptr;
-        DevBuf<double> dvec, Ac, rc, yc, cparts, chunk_part, gpart;
+        DevBuf<double> dvec, Ac, rc, yc, cparts, chunk_part, gpart, inv_work;
         DevBuf<int> info;
         // the inverse coarse matrix in Ac and the lever arms in dvec belong
         // together and stay usable by a later solve that asks for reuse when
@@ -270,7 +288,7 @@ struct tad_fn_s
             agg_of.release(); agg_ptr.release(); agg_vert.release(); pair_blocks.release(); pair_ab.release();
             chunk_ptr.release(); agg_chunk.release(); pair_chunk.release(); gchunk_ptr.release();
             dvec.release(); Ac.release(); rc.release(); yc.release(); cparts.release(); chunk_part.release();
-            gpart.release(); info.release();
+            gpart.release(); inv_work.release(); info.release();
             requested = target = 0;
             n_agg = n_pairs = n_chunks = n_gchunks = 0;
             max_agg_chunks = 0;
@@ -297,6 +315,8 @@ struct tad_fn_s
         a.n_upper = n_upper;
         a.uslot_ptr = uslot_ptr.p;
         a.uplace = reinterpret_cast<const int4*>(uplace.p);
+        a.brow_ptr_h = h_brow_ptr.data();
+        a.urow_ptr_h = h_urow_ptr.data();
         return a;
     }
 };
@@ -315,7 +335,8 @@ void free_pattern(tad_fn fn)
     fn->row_ptr.release(); fn->col_idx.release(); fn->hslot_ptr.release(); fn->gslot_ptr.release();
     fn->block_mirror.release(); fn->uslot_ptr.release(); fn->uplace.release();
     fn->co.release();
-    fn->h_brow_ptr.clear(); fn->h_bcol.clear();
+    fn->h_brow_ptr.clear(); fn->h_bcol.clear(); fn->h_urow_ptr.clear();
+    fn->pipe_ready = false;
     for (TermDev& t : fn->terms)
     {
         if (t.gpos) (void)hipFree(t.gpos);
@@ -323,6 +344,8 @@ void free_pattern(tad_fn fn)
         t.gpos = t.hpos = nullptr;
     }
     fn->pattern_ready = false;
+    fn->jrow_ptr.release(); fn->jcol.release();
+    fn->jpattern_ready = false;
 }
 
 // Builds the block CSR pattern, the scalar CSR pattern and the element-order
@@ -539,6 +562,13 @@ tad_status build_pattern(tad_fn fn)
     TAD_HIP(hipStreamSynchronize(s));
     fn->h_brow_ptr = std::move(brow_ptr);
     fn->h_bcol = std::move(bcol);
+    fn->h_urow_ptr.assign((size_t)nh + 1, 0);
+    for (int64_t u = 0; u < nh; ++u)
+    {
+        int64_t c = 0;
+        for (int32_t k = fn->h_brow_ptr[u]; k < fn->h_brow_ptr[u + 1]; ++k) c += fn->h_bcol[k] >= u ? 1 : 0;
+        fn->h_urow_ptr[u + 1] = fn->h_urow_ptr[u] + c;
+    }
     fn->n_hslots = hptr[nb];
     fn->n_gslots = gptr[nh];
     fn->n_blocks = nb;
@@ -699,8 +729,8 @@ constexpr int64_t kAutoCoarse = 1400;
 constexpr int64_t kAutoMinAggregate = 32;
 
 // Per solve: coarse unknowns per aggregate, lever arms, A_c = P^T (w I + H) P,
-// its Cholesky factorization and inverse (rocSOLVER potrf / potri, lower
-// triangle, rocBLAS atomics off so the result is deterministic), symmetrized.
+// its inverse (blocked Gauss-Jordan sweeps with FP64 MFMA, tad_dense.hip;
+// fixed-order sums, so the result is deterministic), symmetrized.
 // *use = false when the coarse space is off or A_c is not numerically
 // positive definite (block-Jacobi alone then).  With o.coarse_reuse the
 // inverse and lever arms of the previous compatible solve are kept (*reused).
@@ -786,37 +816,19 @@ tad_status setup_coarse(tad_fn fn, const double* H_dev, const tad_solve_opts& o,
     // directions prolong to zero)
     if (free_dev) TAD_HIP(launch_shift_diagonal(nc, c.Ac.p, 1e-10, s));
     tm.end("coarse_galerkin", s);
-    tad_ctx ctx = fn->ctx;
-    if (!ctx->blas)
-    {
-        if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
-        {
-            ctx->blas = nullptr;
-            return fail(TAD_ERR_HIP, "rocblas_create_handle failed");
-        }
-        (void)rocblas_set_atomics_mode(ctx->blas, rocblas_atomics_not_allowed);
-    }
-    if (rocblas_set_stream(ctx->blas, s) != rocblas_status_success) return fail(TAD_ERR_HIP, "rocblas_set_stream failed");
-    tm.begin("coarse_potrf", s);
-    if (rocsolver_dpotrf(ctx->blas, rocblas_fill_lower, (rocblas_int)nc, c.Ac.p, (rocblas_int)nc, c.info.p) !=
-        rocblas_status_success)
-        return fail(TAD_ERR_HIP, "rocsolver_dpotrf failed");
-    tm.end("coarse_potrf", s);
-    tm.begin("coarse_potri", s);
-    if (rocsolver_dpotri(ctx->blas, rocblas_fill_lower, (rocblas_int)nc, c.Ac.p, (rocblas_int)nc, c.info.p + 1) !=
-        rocblas_status_success)
-        return fail(TAD_ERR_HIP, "rocsolver_dpotri failed");
-    tm.end("coarse_potri", s);
-    int info[2] = { 0, 0 };
-    TAD_HIP(hipMemcpyAsync(info, c.info.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    TAD_HIP(c.inv_work.ensure((size_t)spd_inverse_work(nc)));
+    tm.begin("coarse_inverse", s);
+    TAD_HIP(launch_spd_inverse(nc, c.Ac.p, c.inv_work.p, c.info.p, s));
+    tm.end("coarse_inverse", s);
+    int* info = reinterpret_cast<int*>(fn->ctx->pinned + 96);  // past the solver scalars at 56..
+    TAD_HIP(hipMemcpyAsync(info, c.info.p, sizeof(int), hipMemcpyDeviceToHost, s));
     TAD_HIP(hipStreamSynchronize(s));
-    if (info[0] != 0 || info[1] != 0)
+    if (info[0] != 0)
     {
         tm.end("coarse_setup", s);
         cd->kc = 0;
         return TAD_OK;  // not positive definite at working precision: block-Jacobi alone
     }
-    TAD_HIP(launch_symmetrize_lower(nc, c.Ac.p, s));
     tm.end("coarse_setup", s);
     *use = true;
     c.factor_valid = !free_dev;
@@ -856,27 +868,145 @@ const double* stage_x(tad_fn fn, const double* x, tad_mem mem)
 
 // Runs element kernels + gathers.  mode: MODE_GRAD / MODE_HESS / MODE_HESS_PROJ.
 // g_dev / H_dev: device destinations.  Returns f via *f (synchronizes).
+// Most element chunks of a pipelined evaluation.  The automatic setting (0)
+// does not pipeline: on the 60^3 twisted cube (1.3 M tets) two to eight
+// chunks were 2-10 % slower end to end than one launch in FP64 and in mixed
+// precision, since every chunk drains its last waves before the next fills
+// the chip; tad_fn_autotune times 1, 2, 4 and 8 chunks and keeps the best.
+constexpr int kPipeMaxChunks = 64;
+
+// Plans the pipelined evaluation for the current pattern: the largest catalog
+// term is cut into chunks of whole tiles; a handle's stage is the last chunk
+// holding one of its elements (-1: only other terms, which are launched
+// first); row r is complete after chunk k when every row <= r has stage <= k,
+// so the rows become ready as prefixes.
+void plan_pipeline(tad_fn fn)
+{
+    fn->pipe_ready = true;
+    fn->pipe_used = 1;
+    fn->pipe_term = -1;
+    int64_t best = 0;
+    for (size_t i = 0; i < fn->terms.size(); ++i)
+        if (!fn->terms[i].dev_ops.element && fn->terms[i].n_elem > best)
+        {
+            best = fn->terms[i].n_elem;
+            fn->pipe_term = (int)i;
+        }
+    int S = fn->pipe_chunks > 0 ? fn->pipe_chunks : 1;
+    S = (int)std::min<int64_t>(S, std::max<int64_t>(1, best / 256));
+    if (fn->pipe_term < 0 || S <= 1) return;
+    const HostTerm& h = fn->hterms[fn->pipe_term];
+    const int64_t n = h.n_elem;
+    fn->pipe_e.assign((size_t)S + 1, 0);
+    for (int k = 1; k < S; ++k) fn->pipe_e[k] = (n * k / S) / 256 * 256;
+    fn->pipe_e[S] = n;
+    std::vector<int> stage((size_t)fn->n_handles, -1);
+    for (int k = 0; k < S; ++k)
+        for (int64_t e = fn->pipe_e[k]; e < fn->pipe_e[k + 1]; ++e)
+            for (int a = 0; a < h.N; ++a)
+            {
+                const int32_t v = h.conn[e * h.N + a];
+                if (v >= 0) stage[v] = std::max(stage[v], k);
+            }
+    fn->pipe_rows.assign((size_t)S + 1, 0);
+    int run = -1;
+    int64_t u = 0;
+    for (int k = 0; k < S; ++k)
+    {
+        while (u < fn->n_handles && std::max(run, stage[u]) <= k)
+        {
+            run = std::max(run, stage[u]);
+            ++u;
+        }
+        fn->pipe_rows[k + 1] = k + 1 == S ? fn->n_handles : u;
+    }
+    fn->pipe_used = S;
+}
+
+tad_status ensure_aux(tad_ctx ctx, int n_events)
+{
+    if (!ctx->aux) TAD_HIP(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
+    while ((int)ctx->events.size() < n_events)
+    {
+        hipEvent_t e;
+        TAD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        ctx->events.push_back(e);
+    }
+    return TAD_OK;
+}
+
 tad_status eval_core(tad_fn fn, const double* x_dev, int mode, double eps, double* g_dev, double* H_dev, double* f)
 {
     hipStream_t s = fn->s();
     KernelTimer& tm = fn->ctx->timer;
     TAD_HIP(hipMemsetAsync(fn->err.p, 0, sizeof(int), s));
-    tm.begin("element", s);
-    for (const TermDev& t : fn->terms)
-        TAD_HIP(launch_element(t, fn->d, fn->precision, mode, x_dev, eps, fn->elem_f.p, fn->gc_buf.p, fn->hc_buf.p,
-                               fn->err.p, s));
-    tm.end("element", s);
-    if (g_dev)
+    if (!fn->pipe_ready) plan_pipeline(fn);
+    const bool pipe = fn->pipe_used > 1 && H_dev && mode >= MODE_HESS && fn->n_blocks > 0;
+    if (pipe)
     {
-        tm.begin("assemble_gradient", s);
-        TAD_HIP(launch_assemble_gradient(fn->d, fn->precision, fn->n_handles, fn->gslot_ptr.p, fn->gc_buf.p, g_dev, s));
-        tm.end("assemble_gradient", s);
+        // other terms first, then the chunks of the pipelined term; the
+        // assembly of the rows each chunk completes runs on the second stream.
+        // Every entry's sum has the same fixed order whatever the launch order.
+        const int S = fn->pipe_used;
+        tad_ctx ctx = fn->ctx;
+        TAD_TRY(ensure_aux(ctx, S + 2));
+        hipStream_t s2 = ctx->aux;
+        const HessAsm pat = fn->hess_asm();
+        const TermDev& tp = fn->terms[fn->pipe_term];
+        TAD_HIP(hipEventRecord(ctx->events[S], s));  // contributions of the previous call are consumed
+        TAD_HIP(hipStreamWaitEvent(s2, ctx->events[S], 0));
+        tm.begin("element", s);
+        for (size_t i = 0; i < fn->terms.size(); ++i)
+            if ((int)i != fn->pipe_term)
+                TAD_HIP(launch_element(fn->terms[i], fn->d, fn->precision, mode, x_dev, eps, fn->elem_f.p,
+                                       fn->gc_buf.p, fn->hc_buf.p, fn->err.p, s));
+        for (int k = 0; k < S; ++k)
+        {
+            TAD_HIP(launch_element_range(tp, fn->d, fn->precision, mode, x_dev, eps, fn->elem_f.p, fn->gc_buf.p,
+                                         fn->hc_buf.p, fn->err.p, s, fn->pipe_e[k], fn->pipe_e[k + 1]));
+            TAD_HIP(hipEventRecord(ctx->events[k], s));
+            const int64_t r0 = fn->pipe_rows[k], r1 = fn->pipe_rows[k + 1];
+            if (r1 > r0)
+            {
+                TAD_HIP(hipStreamWaitEvent(s2, ctx->events[k], 0));
+                // one timed span per chunk: gradient and Hessian rows of the chunk
+                tm.begin("assemble_hessian", s2);
+                TAD_HIP(launch_assemble_hessian(fn->d, fn->precision, fn->asm_shape, pat, fn->hc_buf.p, H_dev, s2, r0,
+                                                r1));
+                if (g_dev)
+                    TAD_HIP(launch_assemble_gradient(fn->d, fn->precision, fn->n_handles, fn->gslot_ptr.p,
+                                                     fn->gc_buf.p, g_dev, s2, r0, r1));
+                tm.end("assemble_hessian", s2);
+            }
+        }
+        tm.end("element", s);
+        TAD_HIP(hipEventRecord(ctx->events[S + 1], s2));
+        TAD_HIP(hipStreamWaitEvent(s, ctx->events[S + 1], 0));
+    }
+    else
+    {
+        tm.begin("element", s);
+        for (const TermDev& t : fn->terms)
+            TAD_HIP(launch_element(t, fn->d, fn->precision, mode, x_dev, eps, fn->elem_f.p, fn->gc_buf.p, fn->hc_buf.p,
+                                   fn->err.p, s));
+        tm.end("element", s);
+        if (g_dev)
+        {
+            tm.begin("assemble_gradient", s);
+            TAD_HIP(launch_assemble_gradient(fn->d, fn->precision, fn->n_handles, fn->gslot_ptr.p, fn->gc_buf.p, g_dev,
+                                             s));
+            tm.end("assemble_gradient", s);
+        }
     }
     if (H_dev && mode >= MODE_HESS)
     {
-        tm.begin("assemble_hessian", s);
-        TAD_HIP(launch_assemble_hessian(fn->d, fn->precision, fn->asm_shape, fn->hess_asm(), fn->hc_buf.p, H_dev, s));
-        tm.end("assemble_hessian", s);
+        if (!pipe)
+        {
+            tm.begin("assemble_hessian", s);
+            TAD_HIP(launch_assemble_hessian(fn->d, fn->precision, fn->asm_shape, fn->hess_asm(), fn->hc_buf.p, H_dev,
+                                            s));
+            tm.end("assemble_hessian", s);
+        }
         // positions of this Hessian: lever arms of the coarse space's rotations
         TAD_HIP(fn->x_hess.ensure((size_t)fn->n_vars));
         TAD_HIP(hipMemcpyAsync(fn->x_hess.p, x_dev, fn->n_vars * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -1270,6 +1400,7 @@ tad_status tad_fn_add_elements(tad_fn fn, tad_energy energy, int valence, int64_
                                const double* params, int params_per_elem)
 {
     TAD_TRY(check_fn(fn));
+    if (fn->vec) return fail(TAD_ERR_INVALID, "vector functions take residual elements (tad_fn_add_residuals_device)");
     const int d = fn->d, N = valence;
     if (N < 1 || d * N > 12) return fail(TAD_ERR_INVALID, "element size d * valence must be in 1..12");
     if (n_elem < 0) return fail(TAD_ERR_INVALID, "negative element count");
@@ -1311,6 +1442,7 @@ tad_status tad_fn_add_elements_dynamic_device(tad_fn fn, const tad_device_ops* c
                                               const int64_t* elem_ptr, const int32_t* handles, const double* params)
 {
     TAD_TRY(check_fn(fn));
+    if (fn->vec) return fail(TAD_ERR_INVALID, "vector functions take residual elements (tad_fn_add_residuals_device)");
     if (!ops || n_ops < 1) return fail(TAD_ERR_INVALID, "no device ops");
     if (n_elem < 0) return fail(TAD_ERR_INVALID, "negative element count");
     for (int i = 0; i < n_ops; ++i)
@@ -1377,9 +1509,112 @@ tad_status tad_fn_add_elements_device(tad_fn fn, const tad_device_ops* ops, int6
                                       const double* params)
 {
     TAD_TRY(check_fn(fn));
+    if (fn->vec) return fail(TAD_ERR_INVALID, "vector functions take residual elements (tad_fn_add_residuals_device)");
     TAD_TRY(check_device_ops(fn, ops));
     if (n_elem < 0) return fail(TAD_ERR_INVALID, "negative element count");
     return add_term(fn, TAD_E_DEVICE, ops->valence, n_elem, conn, params, ops->n_params, ops);
+}
+
+tad_status tad_fn_create_vector(tad_ctx ctx, int d, int64_t n_handles, tad_precision precision, tad_fn* out)
+{
+    TAD_TRY(tad_fn_create(ctx, d, n_handles, precision, out));
+    (*out)->vec = true;
+    return TAD_OK;
+}
+
+tad_status tad_fn_add_residuals_device(tad_fn fn, const tad_device_residual_ops* ops, int64_t n_elem,
+                                       const int32_t* conn, const double* params)
+{
+    TAD_TRY(check_fn(fn));
+    if (!fn->vec) return fail(TAD_ERR_INVALID, "residual elements need a vector function (tad_fn_create_vector)");
+    if (!ops) return fail(TAD_ERR_INVALID, "null residual ops");
+    if (ops->n_outputs < 1 || ops->n_outputs > 64) return fail(TAD_ERR_INVALID, "residuals per element must be in 1..64");
+    if (!ops->residuals) return fail(TAD_ERR_INVALID, "residual ops without a residual function");
+    tad_device_ops o{};
+    o.version = ops->version;
+    o.valence = ops->valence;
+    o.dim = ops->dim;
+    o.n_params = ops->n_params;
+    o.element = ops->element;
+    o.energy = ops->energy;
+    TAD_TRY(check_device_ops(fn, &o));
+    if (n_elem < 0) return fail(TAD_ERR_INVALID, "negative element count");
+    const int64_t K = (int64_t)fn->d * ops->valence;
+    int64_t nnz_J = 0;
+    for (const TermDev& t : fn->terms) nnz_J = std::max(nnz_J, t.jv_base + t.n_elem * t.n_out * (int64_t)t.N * fn->d);
+    if ((double)(fn->n_outputs + n_elem * ops->n_outputs) >= 2147483647.0 ||
+        (double)(nnz_J + n_elem * ops->n_outputs * K) >= 2147483647.0)
+        return fail(TAD_ERR_UNSUPPORTED, "too many residuals or Jacobian entries (int32 CSR indices)");
+    TAD_TRY(add_term(fn, TAD_E_DEVICE, ops->valence, n_elem, conn, params, ops->n_params, &o));
+    TermDev& t = fn->terms.back();
+    t.n_out = ops->n_outputs;
+    t.r_base = fn->n_outputs;
+    t.jv_base = nnz_J;
+    t.residuals = ops->residuals;
+    fn->n_outputs += n_elem * ops->n_outputs;
+    return TAD_OK;
+}
+
+tad_status tad_fn_n_outputs(tad_fn fn, int64_t* n_outputs)
+{
+    TAD_TRY(check_fn(fn));
+    if (!n_outputs) return fail(TAD_ERR_INVALID, "null argument");
+    *n_outputs = fn->n_outputs;
+    return TAD_OK;
+}
+
+static tad_status ensure_jacobian_pattern(tad_fn fn)
+{
+    if (!fn->vec) return fail(TAD_ERR_INVALID, "not a vector function (tad_fn_create_vector)");
+    if (fn->jpattern_ready) return TAD_OK;
+    const int d = fn->d;
+    int64_t nnz = 0;
+    for (const TermDev& t : fn->terms) nnz += t.n_elem * t.n_out * (int64_t)t.N * d;
+    std::vector<int32_t> rp((size_t)fn->n_outputs + 1), ci((size_t)nnz);
+    rp[0] = 0;
+    for (size_t k = 0; k < fn->terms.size(); ++k)
+    {
+        const TermDev& t = fn->terms[k];
+        const HostTerm& h = fn->hterms[k];
+        const int K = t.N * d;
+        std::vector<int32_t> hs(t.N);
+        for (int64_t e = 0; e < t.n_elem; ++e)
+        {
+            hs.assign(h.conn.begin() + e * t.N, h.conn.begin() + (e + 1) * t.N);
+            std::sort(hs.begin(), hs.end());
+            for (int o = 0; o < t.n_out; ++o)
+            {
+                const int64_t row = t.r_base + e * t.n_out + o;
+                const int64_t v0 = t.jv_base + (e * t.n_out + o) * (int64_t)K;
+                rp[row + 1] = (int32_t)(v0 + K);
+                for (int a = 0; a < t.N; ++a)
+                    for (int i = 0; i < d; ++i) ci[v0 + a * d + i] = hs[a] * d + i;
+            }
+        }
+    }
+    TAD_HIP(fn->jrow_ptr.ensure(rp.size()));
+    TAD_HIP(fn->jcol.ensure(ci.size()));
+    TAD_HIP(hipMemcpy(fn->jrow_ptr.p, rp.data(), rp.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!ci.empty()) TAD_HIP(hipMemcpy(fn->jcol.p, ci.data(), ci.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    fn->nnz_J = nnz;
+    fn->jpattern_ready = true;
+    return TAD_OK;
+}
+
+tad_status tad_fn_jacobian_pattern(tad_fn fn, int64_t* nnz, const int32_t** row_ptr_dev, const int32_t** col_idx_dev,
+                                   int32_t* row_ptr_host, int32_t* col_idx_host)
+{
+    TAD_TRY(check_fn(fn));
+    TAD_HIP(hipSetDevice(fn->ctx->device));
+    TAD_TRY(ensure_jacobian_pattern(fn));
+    if (nnz) *nnz = fn->nnz_J;
+    if (row_ptr_dev) *row_ptr_dev = fn->jrow_ptr.p;
+    if (col_idx_dev) *col_idx_dev = fn->jcol.p;
+    if (row_ptr_host)
+        TAD_HIP(hipMemcpy(row_ptr_host, fn->jrow_ptr.p, (fn->n_outputs + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (col_idx_host && fn->nnz_J)
+        TAD_HIP(hipMemcpy(col_idx_host, fn->jcol.p, fn->nnz_J * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return TAD_OK;
 }
 
 tad_status tad_fn_n_terms(tad_fn fn, int* n_terms)
@@ -1495,6 +1730,91 @@ tad_status tad_fn_eval_with_hessian_proj(tad_fn fn, const double* x, double eps,
 {
     if (eps != eps) return fail(TAD_ERR_INVALID, "projection eps is NaN");
     return eval_second(fn, x, MODE_HESS_PROJ, eps, f, g, H_values, mem);
+}
+
+// r (and J values) of a vector function at device x into device r_dev / J_dev.
+static tad_status eval_residuals_dev(tad_fn fn, const double* xd, double* r_dev, double* J_dev)
+{
+    hipStream_t s = fn->s();
+    KernelTimer& tm = fn->ctx->timer;
+    tm.begin("residuals", s);
+    for (const TermDev& t : fn->terms)
+    {
+        if (t.n_elem == 0) continue;
+        tad_device_residual_args a{};
+        a.n_elem = t.n_elem;
+        a.conn = t.conn;
+        a.params = t.params;
+        a.x = xd;
+        a.r = r_dev + t.r_base;
+        a.J = J_dev ? J_dev + t.jv_base : nullptr;
+        a.stream = (void*)s;
+        TAD_HIP((hipError_t)t.residuals(&a));
+    }
+    tm.end("residuals", s);
+    return TAD_OK;
+}
+
+// Copies r and J values of a vector function at x to the caller (r, J_values
+// may be null).
+static tad_status residual_outputs(tad_fn fn, const double* xd, double* r, double* J_values, tad_mem mem)
+{
+    hipStream_t s = fn->s();
+    if (J_values) TAD_TRY(ensure_jacobian_pattern(fn));
+    double* r_dev = (mem == TAD_MEM_DEVICE && r) ? r : nullptr;
+    if (!r_dev)
+    {
+        TAD_HIP(fn->r_buf.ensure((size_t)fn->n_outputs));
+        r_dev = fn->r_buf.p;
+    }
+    double* J_dev = nullptr;
+    if (J_values)
+    {
+        J_dev = mem == TAD_MEM_DEVICE ? J_values : nullptr;
+        if (!J_dev)
+        {
+            TAD_HIP(fn->J_buf.ensure((size_t)fn->nnz_J));
+            J_dev = fn->J_buf.p;
+        }
+    }
+    TAD_TRY(eval_residuals_dev(fn, xd, r_dev, J_dev));
+    if (mem == TAD_MEM_HOST)
+    {
+        if (r) TAD_HIP(hipMemcpyAsync(r, r_dev, fn->n_outputs * sizeof(double), hipMemcpyDeviceToHost, s));
+        if (J_values && fn->nnz_J)
+            TAD_HIP(hipMemcpyAsync(J_values, J_dev, fn->nnz_J * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    TAD_HIP(hipStreamSynchronize(s));
+    return TAD_OK;
+}
+
+tad_status tad_fn_eval_residuals(tad_fn fn, const double* x, double* r, tad_mem mem)
+{
+    return tad_fn_eval_with_jacobian(fn, x, r, nullptr, mem);
+}
+
+tad_status tad_fn_eval_with_jacobian(tad_fn fn, const double* x, double* r, double* J_values, tad_mem mem)
+{
+    TAD_TRY(check_fn(fn));
+    if (!fn->vec) return fail(TAD_ERR_INVALID, "not a vector function (tad_fn_create_vector)");
+    if (!x || !r) return fail(TAD_ERR_INVALID, "null argument");
+    TAD_HIP(hipSetDevice(fn->ctx->device));
+    TAD_TRY(ensure_ready(fn));
+    const double* xd = stage_x(fn, x, mem);
+    if (!xd) return fail(TAD_ERR_HIP, "copy of x to the device failed");
+    return residual_outputs(fn, xd, r, J_values, mem);
+}
+
+tad_status tad_fn_eval_sum_of_squares_with_derivatives(tad_fn fn, const double* x, double* f, double* g, double* r,
+                                                       double* J_values, tad_mem mem)
+{
+    TAD_TRY(check_fn(fn));
+    if (!fn->vec) return fail(TAD_ERR_INVALID, "not a vector function (tad_fn_create_vector)");
+    TAD_TRY(eval_second(fn, x, MODE_GRAD, 0.0, f, g, nullptr, mem));
+    if (!r && !J_values) return TAD_OK;
+    const double* xd = stage_x(fn, x, mem);
+    if (!xd) return fail(TAD_ERR_HIP, "copy of x to the device failed");
+    return residual_outputs(fn, xd, r, J_values, mem);
 }
 
 tad_status tad_fn_device_buffers(tad_fn fn, const double** g_dev, const double** H_values_dev)
@@ -1813,6 +2133,30 @@ tad_status tad_fn_set_assembly_shape(tad_fn fn, int shape)
     return TAD_OK;
 }
 
+tad_status tad_fn_get_pipeline(tad_fn fn, int* requested, int* used)
+{
+    TAD_TRY(check_fn(fn));
+    if (!requested && !used) return fail(TAD_ERR_INVALID, "null argument");
+    if (requested) *requested = fn->pipe_chunks;
+    if (used)
+    {
+        TAD_HIP(hipSetDevice(fn->ctx->device));
+        TAD_TRY(ensure_ready(fn));
+        if (!fn->pipe_ready) plan_pipeline(fn);
+        *used = fn->pipe_used;
+    }
+    return TAD_OK;
+}
+
+tad_status tad_fn_set_pipeline(tad_fn fn, int chunks)
+{
+    TAD_TRY(check_fn(fn));
+    if (chunks < 0 || chunks > kPipeMaxChunks) return fail(TAD_ERR_INVALID, "pipeline chunks must be in 0..64");
+    fn->pipe_chunks = chunks;
+    fn->pipe_ready = false;
+    return TAD_OK;
+}
+
 tad_status tad_fn_autotune(tad_fn fn, const double* x, tad_mem mem, double eps, char* report, size_t report_len)
 {
     TAD_TRY(check_fn(fn));
@@ -1832,7 +2176,7 @@ tad_status tad_fn_autotune(tad_fn fn, const double* x, tad_mem mem, double eps, 
         tad_fn fn;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         std::vector<TermDev> saved;
-        int saved_shape = 0;
+        int saved_shape = 0, saved_chunks = 0;
         bool commit = false;
         ~Guard()
         {
@@ -1847,11 +2191,14 @@ tad_status tad_fn_autotune(tad_fn fn, const double* x, tad_mem mem, double eps, 
                     fn->terms[i].tune_map = saved[i].tune_map;
                 }
                 fn->asm_shape = saved_shape;
+                fn->pipe_chunks = saved_chunks;
+                fn->pipe_ready = false;
             }
         }
     } guard{fn};
     guard.saved = fn->terms;
     guard.saved_shape = fn->asm_shape;
+    guard.saved_chunks = fn->pipe_chunks;
     TAD_HIP(hipEventCreate(&guard.e0));
     TAD_HIP(hipEventCreate(&guard.e1));
     hipEvent_t e0 = guard.e0, e1 = guard.e1;
@@ -1939,6 +2286,39 @@ tad_status tad_fn_autotune(tad_fn fn, const double* x, tad_mem mem, double eps, 
                       "%d lanes per upper block %.4f ms, one thread per upper block %.4f ms\n",
                       fn->d * fn->d, ms_shape[0], ms_shape[1], fn->d * fn->d, ms_shape[2], ms_shape[3]);
         rep += line;
+        // pipelined evaluation: whole eval_with_hessian_proj with the chosen
+        // shapes for 1, 2, 4 and 8 element chunks (identical results)
+        int best_chunks = 1;
+        float best_ms = 1e30f;
+        std::string pl = "pipelined evaluation:";
+        for (int chunks : { 1, 2, 4, 8 })
+        {
+            fn->pipe_chunks = chunks;
+            plan_pipeline(fn);
+            if (fn->pipe_used != chunks) continue;
+            float best = 1e30f;
+            for (int rep_i = 0; rep_i < 4; ++rep_i)
+            {
+                double fv = 0.0;
+                TAD_HIP(hipEventRecord(e0, s));
+                TAD_TRY(eval_core(fn, xd, MODE_HESS_PROJ, eps, fn->g_buf.p, fn->H_buf.p, &fv));
+                TAD_HIP(hipEventRecord(e1, s));
+                TAD_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                TAD_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (rep_i > 0) best = std::min(best, ms);
+            }
+            std::snprintf(line, sizeof(line), " %d chunk%s %.4f ms;", chunks, chunks > 1 ? "s" : "", best);
+            pl += line;
+            if (best < best_ms)
+            {
+                best_ms = best;
+                best_chunks = chunks;
+            }
+        }
+        fn->pipe_chunks = best_chunks;
+        fn->pipe_ready = false;
+        rep += pl + "\n";
     }
     fn->h_valid = false;
     guard.commit = true;

@@ -30,6 +30,13 @@ struct TermDev
     int tune_map = 0;
     // caller-compiled element and energy kernels (energy == TAD_E_DEVICE)
     tad_device_ops dev_ops{};
+    // residual terms of vector functions: residuals per element, first
+    // residual row of the term, first Jacobian value of the term, and the
+    // caller-compiled residual / Jacobian launcher
+    int n_out = 0;
+    int64_t r_base = 0;
+    int64_t jv_base = 0;
+    int (*residuals)(const tad_device_residual_args*) = nullptr;
 };
 
 // Solver bookkeeping written by block 0 of the PCG kernels (see tad_linalg.hip).
@@ -55,6 +62,10 @@ constexpr int kReduceBlocks = 256;
 // contribution arrays (d and d*d values per slot).
 hipError_t launch_element(const TermDev& t, int d, int precision, int mode, const double* x, double eps,
                           double* elem_f, void* G, void* Hc, int* err, hipStream_t s);
+
+// The elements [e0, e1) of a catalog term (launch shape as for the whole term).
+hipError_t launch_element_range(const TermDev& t, int d, int precision, int mode, const double* x, double eps,
+                                double* elem_f, void* G, void* Hc, int* err, hipStream_t s, int64_t e0, int64_t e1);
 
 // Energies at x + steps[c] * dir (dir may be null: plain x) for c < ncand,
 // written to elem_f[c * stride + f_base + e].
@@ -84,6 +95,10 @@ struct HessAsm
     int64_t n_upper = 0;
     const int64_t* uslot_ptr = nullptr;
     const int4* uplace = nullptr;
+    // host copies for row ranges: first CSR block of each row (n_rows + 1) and
+    // first upper block of each row (n_rows + 1)
+    const int32_t* brow_ptr_h = nullptr;
+    const int64_t* urow_ptr_h = nullptr;
 };
 
 constexpr int kAsmShapes = 4;
@@ -93,11 +108,12 @@ constexpr int kAsmShapes = 4;
 // thread per block, 2: d*d lanes per upper block writing it and its mirror,
 // 3: one thread per upper block writing both.  Every shape sums each run in
 // slot order (identical results).
+// r0 >= 0: only the block rows (handles) [r0, r1).
 hipError_t launch_assemble_hessian(int d, int precision, int shape, const HessAsm& a, const void* Hc, double* H,
-                                   hipStream_t s);
+                                   hipStream_t s, int64_t r0 = -1, int64_t r1 = -1);
 
 hipError_t launch_assemble_gradient(int d, int precision, int64_t n_handles, const int64_t* gslot_ptr,
-                                    const void* G, double* g, hipStream_t s);
+                                    const void* G, double* g, hipStream_t s, int64_t v0 = -1, int64_t v1 = -1);
 
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
 
@@ -155,6 +171,13 @@ hipError_t launch_coarse_galerkin(int d, int64_t n_pairs, int64_t n_gchunks, con
                                   const int32_t* block_row, const int32_t* brow_ptr, const int32_t* bcol,
                                   const int32_t* row_ptr, const double* H, double w, const CoarseDev& c,
                                   double* gpart, double* Ac, hipStream_t s);
+
+// A (n x n, column-major, symmetric positive definite) <- A^-1 (symmetric),
+// by blocked Gauss-Jordan sweeps with FP64 MFMA products.  work:
+// spd_inverse_work(n) doubles; info[0] = 1 when a pivot block is not
+// positive definite at working precision (A is then garbage).
+int64_t spd_inverse_work(int64_t n);
+hipError_t launch_spd_inverse(int64_t n, double* A, double* work, int* info, hipStream_t s);
 
 // A_jj <- A_jj (1 + rel), 1 where A_jj = 0 (column-major n x n).
 hipError_t launch_shift_diagonal(int64_t n, double* A, double rel, hipStream_t s);

@@ -180,7 +180,7 @@ __device__ __forceinline__ double run_sum(const T* __restrict__ src, int64_t c0,
 // the run), so every contribution is read as one contiguous D*D-value segment
 // by consecutive lanes; a 64-lane wave serves floor(64 / D^2) blocks.
 template <class T, int D>
-__global__ __launch_bounds__(kBlock) void k_assemble_hessian(int64_t nb, const int32_t* __restrict__ block_row,
+__global__ __launch_bounds__(kBlock) void k_assemble_hessian(int64_t b_begin, int64_t nb, const int32_t* __restrict__ block_row,
                                                              const int32_t* __restrict__ brow_ptr,
                                                              const int64_t* __restrict__ hslot_ptr,
                                                              const int32_t* __restrict__ mirror,
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_hessian(int64_t nb, const i
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (lane >= BPW * DD) return;
-    const int64_t b = wave * BPW + lane / DD;
+    const int64_t b = b_begin + wave * BPW + lane / DD;
     if (b >= nb) return;
     const int m = lane % DD;
     const int u = block_row[b];
@@ -213,7 +213,8 @@ __global__ __launch_bounds__(kBlock) void k_assemble_hessian(int64_t nb, const i
 // faster depends on d, the precision and the run lengths (the autotuner times
 // both).
 template <class T, int D>
-__global__ __launch_bounds__(kBlock) void k_assemble_hessian_tpb(int64_t nb, const int32_t* __restrict__ block_row,
+__global__ __launch_bounds__(kBlock) void k_assemble_hessian_tpb(int64_t b_begin, int64_t nb,
+                                                                 const int32_t* __restrict__ block_row,
                                                                  const int32_t* __restrict__ brow_ptr,
                                                                  const int64_t* __restrict__ hslot_ptr,
                                                                  const int32_t* __restrict__ mirror,
@@ -222,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_hessian_tpb(int64_t nb, con
                                                                  double* __restrict__ H)
 {
     constexpr int DD = D * D;
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t b = b_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
     const int u = block_row[b];
     const int b0 = brow_ptr[u];
@@ -269,7 +270,8 @@ __global__ __launch_bounds__(kBlock) void k_assemble_hessian_tpb(int64_t nb, con
 // once (shapes 0 / 1 read the off-diagonal runs twice).  D*D lanes per upper
 // block, lane m = (i, j) writes H_upper(i, j) and H_lower(j, i).
 template <class T, int D>
-__global__ __launch_bounds__(kBlock) void k_assemble_hessian_upper(int64_t nu, const int64_t* __restrict__ uslot_ptr,
+__global__ __launch_bounds__(kBlock) void k_assemble_hessian_upper(int64_t b_begin, int64_t nu,
+                                                                   const int64_t* __restrict__ uslot_ptr,
                                                                    const int4* __restrict__ uplace,
                                                                    const T* __restrict__ Hc, double* __restrict__ H)
 {
@@ -278,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_hessian_upper(int64_t nu, c
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (lane >= BPW * DD) return;
-    const int64_t b = wave * BPW + lane / DD;
+    const int64_t b = b_begin + wave * BPW + lane / DD;
     if (b >= nu) return;
     const int m = lane % DD;
     const int i = m / D, j = m % D;
@@ -290,14 +292,14 @@ __global__ __launch_bounds__(kBlock) void k_assemble_hessian_upper(int64_t nu, c
 
 // One thread per upper block, two contributions in flight.
 template <class T, int D>
-__global__ __launch_bounds__(kBlock) void k_assemble_hessian_upper_tpb(int64_t nu,
+__global__ __launch_bounds__(kBlock) void k_assemble_hessian_upper_tpb(int64_t b_begin, int64_t nu,
                                                                        const int64_t* __restrict__ uslot_ptr,
                                                                        const int4* __restrict__ uplace,
                                                                        const T* __restrict__ Hc,
                                                                        double* __restrict__ H)
 {
     constexpr int DD = D * D;
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t b = b_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nu) return;
     double acc[DD];
 #pragma unroll
@@ -341,75 +343,98 @@ __global__ __launch_bounds__(kBlock) void k_assemble_hessian_upper_tpb(int64_t n
 
 // D lanes per handle, floor(64 / D) handles per wave.
 template <class T, int D>
-__global__ __launch_bounds__(kBlock) void k_assemble_gradient(int64_t nh, const int64_t* __restrict__ gslot_ptr,
+__global__ __launch_bounds__(kBlock) void k_assemble_gradient(int64_t v_begin, int64_t nh,
+                                                              const int64_t* __restrict__ gslot_ptr,
                                                               const T* __restrict__ G, double* __restrict__ g)
 {
     constexpr int HPW = 64 / D;
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (lane >= HPW * D) return;
-    const int64_t v = wave * HPW + lane / D;
+    const int64_t v = v_begin + wave * HPW + lane / D;
     if (v >= nh) return;
     const int i = lane % D;
     g[v * D + i] = run_sum<D>(G + i, gslot_ptr[v], gslot_ptr[v + 1]);
 }
 
+// Blocks [b0, b1) of the shape's index space: CSR blocks (shapes 0, 1) or
+// upper blocks in CSR order (shapes 2, 3).
 template <class T, int D>
-static void launch_assemble_hessian_td(int shape, const HessAsm& a, const T* Hc, double* H, hipStream_t s)
+static void launch_assemble_hessian_td(int shape, const HessAsm& a, int64_t b0, int64_t b1, const T* Hc, double* H,
+                                       hipStream_t s)
 {
     constexpr int bpw = 64 / (D * D);
-    const int64_t n = shape >= 2 ? a.n_upper : a.n_blocks;
+    const int64_t n = b1 - b0;
+    if (n <= 0) return;
     const unsigned gt = (unsigned)((n + kBlock - 1) / kBlock);
     const unsigned gl = (unsigned)((((n + bpw - 1) / bpw) * 64 + kBlock - 1) / kBlock);
     if (shape == 0)
-        k_assemble_hessian<T, D><<<gl, kBlock, 0, s>>>(a.n_blocks, a.block_row, a.brow_ptr, a.hslot_ptr, a.mirror, Hc,
+        k_assemble_hessian<T, D><<<gl, kBlock, 0, s>>>(b0, b1, a.block_row, a.brow_ptr, a.hslot_ptr, a.mirror, Hc,
                                                        a.row_ptr, H);
     else if (shape == 1)
-        k_assemble_hessian_tpb<T, D><<<gt, kBlock, 0, s>>>(a.n_blocks, a.block_row, a.brow_ptr, a.hslot_ptr, a.mirror,
+        k_assemble_hessian_tpb<T, D><<<gt, kBlock, 0, s>>>(b0, b1, a.block_row, a.brow_ptr, a.hslot_ptr, a.mirror,
                                                            Hc, a.row_ptr, H);
     else if (shape == 2)
-        k_assemble_hessian_upper<T, D><<<gl, kBlock, 0, s>>>(a.n_upper, a.uslot_ptr, a.uplace, Hc, H);
+        k_assemble_hessian_upper<T, D><<<gl, kBlock, 0, s>>>(b0, b1, a.uslot_ptr, a.uplace, Hc, H);
     else
-        k_assemble_hessian_upper_tpb<T, D><<<gt, kBlock, 0, s>>>(a.n_upper, a.uslot_ptr, a.uplace, Hc, H);
+        k_assemble_hessian_upper_tpb<T, D><<<gt, kBlock, 0, s>>>(b0, b1, a.uslot_ptr, a.uplace, Hc, H);
 }
 
 template <class T>
-static hipError_t launch_assemble_hessian_t(int d, int shape, const HessAsm& a, const T* Hc, double* H, hipStream_t s)
+static hipError_t launch_assemble_hessian_t(int d, int shape, const HessAsm& a, int64_t r0, int64_t r1, const T* Hc,
+                                            double* H, hipStream_t s)
 {
     if (a.n_blocks == 0) return hipSuccess;
     if (shape < 0 || shape >= kAsmShapes) return hipErrorInvalidValue;
-    if (d == 1) launch_assemble_hessian_td<T, 1>(shape, a, Hc, H, s);
-    else if (d == 2) launch_assemble_hessian_td<T, 2>(shape, a, Hc, H, s);
-    else launch_assemble_hessian_td<T, 3>(shape, a, Hc, H, s);
+    int64_t b0, b1;
+    if (shape >= 2)
+    {
+        b0 = r0 < 0 ? 0 : a.urow_ptr_h[r0];
+        b1 = r0 < 0 ? a.n_upper : a.urow_ptr_h[r1];
+    }
+    else
+    {
+        b0 = r0 < 0 ? 0 : a.brow_ptr_h[r0];
+        b1 = r0 < 0 ? a.n_blocks : a.brow_ptr_h[r1];
+    }
+    if (d == 1) launch_assemble_hessian_td<T, 1>(shape, a, b0, b1, Hc, H, s);
+    else if (d == 2) launch_assemble_hessian_td<T, 2>(shape, a, b0, b1, Hc, H, s);
+    else launch_assemble_hessian_td<T, 3>(shape, a, b0, b1, Hc, H, s);
     return hipGetLastError();
 }
 
 hipError_t launch_assemble_hessian(int d, int precision, int shape, const HessAsm& a, const void* Hc, double* H,
-                                   hipStream_t s)
+                                   hipStream_t s, int64_t r0, int64_t r1)
 {
-    if (precision == 0) return launch_assemble_hessian_t<double>(d, shape, a, (const double*)Hc, H, s);
-    return launch_assemble_hessian_t<float>(d, shape, a, (const float*)Hc, H, s);
+    if (precision == 0) return launch_assemble_hessian_t<double>(d, shape, a, r0, r1, (const double*)Hc, H, s);
+    return launch_assemble_hessian_t<float>(d, shape, a, r0, r1, (const float*)Hc, H, s);
 }
 
 template <class T>
-static hipError_t launch_assemble_gradient_t(int d, int64_t nh, const int64_t* gslot_ptr, const T* G, double* g,
-                                             hipStream_t s)
+static hipError_t launch_assemble_gradient_t(int d, int64_t v0, int64_t v1, const int64_t* gslot_ptr, const T* G,
+                                             double* g, hipStream_t s)
 {
-    if (nh == 0) return hipSuccess;
+    const int64_t nh = v1 - v0;
+    if (nh <= 0) return hipSuccess;
     const int hpw = 64 / d;
     const int64_t waves = (nh + hpw - 1) / hpw;
     const unsigned grid = (unsigned)((waves * 64 + kBlock - 1) / kBlock);
-    if (d == 1) k_assemble_gradient<T, 1><<<grid, kBlock, 0, s>>>(nh, gslot_ptr, G, g);
-    else if (d == 2) k_assemble_gradient<T, 2><<<grid, kBlock, 0, s>>>(nh, gslot_ptr, G, g);
-    else k_assemble_gradient<T, 3><<<grid, kBlock, 0, s>>>(nh, gslot_ptr, G, g);
+    if (d == 1) k_assemble_gradient<T, 1><<<grid, kBlock, 0, s>>>(v0, v1, gslot_ptr, G, g);
+    else if (d == 2) k_assemble_gradient<T, 2><<<grid, kBlock, 0, s>>>(v0, v1, gslot_ptr, G, g);
+    else k_assemble_gradient<T, 3><<<grid, kBlock, 0, s>>>(v0, v1, gslot_ptr, G, g);
     return hipGetLastError();
 }
 
 hipError_t launch_assemble_gradient(int d, int precision, int64_t nh, const int64_t* gslot_ptr, const void* G,
-                                    double* g, hipStream_t s)
+                                    double* g, hipStream_t s, int64_t v0, int64_t v1)
 {
-    if (precision == 0) return launch_assemble_gradient_t<double>(d, nh, gslot_ptr, (const double*)G, g, s);
-    return launch_assemble_gradient_t<float>(d, nh, gslot_ptr, (const float*)G, g, s);
+    if (v0 < 0)
+    {
+        v0 = 0;
+        v1 = nh;
+    }
+    if (precision == 0) return launch_assemble_gradient_t<double>(d, v0, v1, gslot_ptr, (const double*)G, g, s);
+    return launch_assemble_gradient_t<float>(d, v0, v1, gslot_ptr, (const float*)G, g, s);
 }
 
 // ---------------------------------------------------------------------------
