@@ -304,6 +304,19 @@ tad_status tad_fn_pattern(tad_fn fn, int64_t* nnz, const int32_t** row_ptr_dev, 
 /* eval(x) (ScalarFunctionImpl.hh:256-282): passive energy, FP64 sum. */
 tad_status tad_fn_eval(tad_fn fn, const double* x, tad_mem mem, double* f);
 
+/* Like tad_fn_eval with x and f in device memory, but the library allocates
+ * nothing, not even on the first call: no counterpart in the reference; for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops).  The per-workgroup partial sums go to a static device array of
+ * 65,536 doubles (512 KiB per device, held until the process ends), so the
+ * function's catalog terms may hold at most 16,777,216 elements in total
+ * (larger functions fail with TAD_ERR_UNSUPPORTED, as do functions with
+ * device-functor terms).  Asynchronous: f_dev[0] is written in the order of
+ * the context's stream and the call does not synchronize.  The sum order
+ * differs from tad_fn_eval's (equal to rounding).  Calls that overlap on one
+ * device share the array and race. */
+tad_status tad_fn_eval_noalloc(tad_fn fn, const double* x_dev, double* f_dev);
+
 /* eval_with_gradient(x, f, g) (ScalarFunctionImpl.hh:284-314). g: n_vars. */
 tad_status tad_fn_eval_with_gradient(tad_fn fn, const double* x, double* f, double* g, tad_mem mem);
 

@@ -308,6 +308,19 @@ class ScalarFunction:
 
     __call__ = eval
 
+    def eval_noalloc(self, x, f_out) -> None:
+        """tad_fn_eval_noalloc: the energy at device x into the device scalar
+        f_out (a float64 CUDA tensor with at least one entry), with no
+        allocation by the library.  Asynchronous on the context's stream."""
+        import torch
+        ptr, mem = self._x(x)
+        if mem != _lib.MEM_DEVICE:
+            raise TadError(1, "eval_noalloc needs x on the device")
+        if (not isinstance(f_out, torch.Tensor) or f_out.device.type != "cuda" or f_out.dtype != torch.float64
+                or f_out.numel() < 1):
+            raise TadError(1, "f_out must be a float64 CUDA tensor with at least one entry")
+        check(lib().tad_fn_eval_noalloc(self._h, ptr, f_out.data_ptr()))
+
     def _outputs(self, mem, with_h):
         if mem == _lib.MEM_DEVICE:
             import torch

@@ -25,7 +25,7 @@ EXPORTED = [
     "tad_fn_create", "tad_fn_destroy", "tad_fn_add_elements", "tad_fn_add_elements_device",
     "tad_fn_add_elements_dynamic_device", "tad_fn_n_terms", "tad_fn_term_info", "tad_fn_n_vars",
     "tad_fn_n_elements",
-    "tad_fn_pattern", "tad_fn_eval", "tad_fn_eval_with_gradient", "tad_fn_eval_with_derivatives",
+    "tad_fn_pattern", "tad_fn_eval", "tad_fn_eval_noalloc", "tad_fn_eval_with_gradient", "tad_fn_eval_with_derivatives",
     "tad_fn_eval_with_hessian_proj", "tad_fn_device_buffers",
     "tad_solve_default_opts", "tad_fn_solve", "tad_fn_solve_reduced", "tad_newton_default_opts", "tad_newton_step",
     "tad_line_search", "tad_newton_decrement",
@@ -129,6 +129,7 @@ def lib():
         "tad_fn_n_elements": [vp, pi64],
         "tad_fn_pattern": [vp, pi64, pvp, pvp, vp, vp],
         "tad_fn_eval": [vp, vp, i32, pdbl],
+        "tad_fn_eval_noalloc": [vp, vp, vp],
         "tad_fn_create_vector": [vp, i32, i64, i32, vp],
         "tad_fn_add_residuals_device": [vp, vp, i64, vp, vp],
         "tad_fn_n_outputs": [vp, vp],

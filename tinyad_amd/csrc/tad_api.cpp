@@ -176,6 +176,7 @@ struct tad_ctx_s
     hipStream_t own_stream = nullptr;
     KernelTimer timer;
     double* pinned = nullptr;  // small pinned staging buffer (256 doubles)
+    double* noalloc = nullptr;  // this device's static scratch array (tad_fn_eval_noalloc)
     // second stream and chunk events of pipelined evaluations (created on first use)
     hipStream_t aux = nullptr;
     std::vector<hipEvent_t> events;
@@ -1235,6 +1236,7 @@ tad_status tad_ctx_create(int device, tad_ctx* out)
     TAD_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
     c->stream = c->own_stream;
     TAD_HIP(hipHostMalloc((void**)&c->pinned, 256 * sizeof(double), hipHostMallocDefault));
+    TAD_HIP(noalloc_scratch(device, &c->noalloc));
     *out = c.release();
     return TAD_OK;
 }
@@ -1677,6 +1679,25 @@ tad_status tad_fn_eval(tad_fn fn, const double* x, tad_mem mem, double* f)
     double fv = 0.0;
     TAD_TRY(eval_energy_batch(fn, xd, nullptr, nullptr, 1, &fv));
     *f = fv;
+    return TAD_OK;
+}
+
+tad_status tad_fn_eval_noalloc(tad_fn fn, const double* x_dev, double* f_dev)
+{
+    TAD_TRY(check_fn(fn));
+    if (!x_dev || !f_dev) return fail(TAD_ERR_INVALID, "null argument");
+    for (const TermDev& t : fn->terms)
+        if (t.dev_ops.element)
+            return fail(TAD_ERR_UNSUPPORTED, "tad_fn_eval_noalloc: device-functor terms are not supported");
+    const int64_t parts = noalloc_partials(fn->terms.data(), (int)fn->terms.size());
+    if (parts > kNoAllocPartials)
+        return fail(TAD_ERR_UNSUPPORTED, "tad_fn_eval_noalloc: " + std::to_string(fn->n_elem_total) +
+                                             " elements need " + std::to_string(parts) +
+                                             " partial sums, the static scratch holds " +
+                                             std::to_string(kNoAllocPartials));
+    TAD_HIP(hipSetDevice(fn->ctx->device));
+    TAD_HIP(launch_energy_noalloc(fn->terms.data(), (int)fn->terms.size(), fn->d, x_dev, fn->ctx->noalloc, f_dev,
+                                  fn->s()));
     return TAD_OK;
 }
 

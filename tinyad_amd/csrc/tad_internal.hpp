@@ -72,6 +72,18 @@ hipError_t launch_element_range(const TermDev& t, int d, int precision, int mode
 hipError_t launch_energy(const TermDev& t, int d, const double* x, const double* dir, const double* steps,
                          int ncand, double* elem_f, int64_t stride, hipStream_t s);
 
+// Energy without library allocations: per-workgroup partial sums of the
+// passive energies of every (catalog) term into the static scratch array of
+// the device (at most kNoAllocPartials of them), then their fixed-order sum
+// into f_out[0] (device).  noalloc_scratch gives the array's address on
+// `device` (which must be current).
+constexpr int64_t kNoAllocPartials = 65536;
+constexpr int kMaxDevices = 64;
+hipError_t noalloc_scratch(int device, double** ptr);
+int64_t noalloc_partials(const TermDev* terms, int n_terms);
+hipError_t launch_energy_noalloc(const TermDev* terms, int n_terms, int d, const double* x, double* scratch_ptr,
+                                 double* f_out, hipStream_t s);
+
 // Deterministic sum of ncand rows of n doubles (row stride `stride`) into out[c].
 hipError_t launch_sum(const double* in, int64_t n, int ncand, int64_t stride, double* partials, double* out,
                       hipStream_t s);
