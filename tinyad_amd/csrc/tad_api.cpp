@@ -276,7 +276,7 @@ struct tad_fn_s
         int max_agg_chunks = 0;
         DevBuf<int32_t> agg_of, agg_ptr, agg_vert, pair_blocks, pair_ab, chunk_ptr, agg_chunk, pair_chunk;
         DevBuf<int64_t> gchunk_ptr;
-        DevBuf<double> dvec, Ac, rc, yc, cparts, chunk_part, gpart, inv_work;
+        DevBuf<double> dvec, Ac, rc, cparts, chunk_part, gpart, inv_work;
         DevBuf<int> info;
         // the inverse coarse matrix in Ac and the lever arms in dvec belong
         // together and stay usable by a later solve that asks for reuse when
@@ -288,7 +288,7 @@ struct tad_fn_s
         {
             agg_of.release(); agg_ptr.release(); agg_vert.release(); pair_blocks.release(); pair_ab.release();
             chunk_ptr.release(); agg_chunk.release(); pair_chunk.release(); gchunk_ptr.release();
-            dvec.release(); Ac.release(); rc.release(); yc.release(); cparts.release(); chunk_part.release();
+            dvec.release(); Ac.release(); rc.release(); cparts.release(); chunk_part.release();
             gpart.release(); inv_work.release(); info.release();
             requested = target = 0;
             n_agg = n_pairs = n_chunks = n_gchunks = 0;
@@ -773,8 +773,7 @@ tad_status setup_coarse(tad_fn fn, const double* H_dev, const tad_solve_opts& o,
     TAD_HIP(c.dvec.ensure((size_t)nh * d));
     TAD_HIP(c.Ac.ensure((size_t)(nc * nc)));
     TAD_HIP(c.rc.ensure((size_t)nc));
-    TAD_HIP(c.yc.ensure((size_t)nc));
-    const int Gc = coarse_apply_groups(nc);
+    const int Gc = coarse_apply_groups(c.n_agg);
     TAD_HIP(c.cparts.ensure((size_t)2 * Gc));
     TAD_HIP(c.info.ensure(2));
     TAD_HIP(c.chunk_part.ensure((size_t)c.n_chunks * kc));
@@ -793,7 +792,6 @@ tad_status setup_coarse(tad_fn fn, const double* H_dev, const tad_solve_opts& o,
     cd->dvec = c.dvec.p;
     cd->Ainv = c.Ac.p;
     cd->rc = c.rc.p;
-    cd->yc = c.yc.p;
     cd->cparts = c.cparts.p;
     cd->Gc = Gc;
     cd->free_vars = free_dev;
@@ -1093,7 +1091,7 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
     const int64_t nh = fn->n_handles, n = fn->n_vars;
     hipStream_t s = fn->s();
     KernelTimer& tm = fn->ctx->timer;
-    TAD_HIP(fn->Dinv.ensure((size_t)nh * d * d));
+    TAD_HIP(fn->Dinv.ensure((size_t)nh * d * (d + 1) / 2));  // packed symmetric blocks
     TAD_HIP(fn->r.ensure(n));
     TAD_HIP(fn->z.ensure(n));
     TAD_HIP(fn->p.ensure(2 * n));

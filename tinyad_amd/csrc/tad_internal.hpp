@@ -159,16 +159,16 @@ struct CoarseDev
     const double* dvec = nullptr;    // [n_handles * d]
     const double* Ainv = nullptr;    // [nc * nc] inverse Galerkin matrix (symmetric, both triangles)
     double* rc = nullptr;            // [nc] restricted residual
-    double* yc = nullptr;            // [nc] coarse correction
-    double* cparts = nullptr;        // [2 * Gc] partials of rc . yc, slot k & 1
+    double* cparts = nullptr;        // [2 * Gc] partials of r_c . A_c^-1 r_c, slot k & 1
     int Gc = 0;                      // workgroups of the coarse product
     const unsigned char* free_vars = nullptr;  // [n_vars] reduced basis (null: every variable free)
 };
 
 constexpr int kCoarseChunk = 256;  // handles per restriction chunk, fine blocks per Galerkin chunk
 
-// Workgroups of the coarse product for nc coarse unknowns.
-int coarse_apply_groups(int64_t nc);
+// Workgroups of the coarse product (one per aggregate), i.e. the r_c . y_c
+// partials it writes per CG iteration.
+int coarse_apply_groups(int64_t n_agg);
 
 // Lever arms dvec[v] = x_v - x_(first member of v's aggregate).
 hipError_t launch_coarse_dvec(int d, int64_t nh, const double* x, const CoarseDev& c, double* dvec, hipStream_t s);
@@ -197,14 +197,17 @@ hipError_t launch_shift_diagonal(int64_t n, double* A, double rel, hipStream_t s
 // Copies the lower triangle of the column-major n x n matrix A to the upper.
 hipError_t launch_symmetrize_lower(int64_t n, double* A, hipStream_t s);
 
-// z += P A_c^-1 P^T r (restriction, coarse product with partials of rc . yc
-// into slot `slot`, prolongation); every kernel returns at once once the
-// solve has terminated.
+// z += P A_c^-1 P^T r (the restriction r_c = P^T r, then the coarse product
+// and the prolongation in one launch, with the partials of r_c . A_c^-1 r_c
+// into slot `slot`); every kernel returns at once once the solve has
+// terminated.
 hipError_t launch_coarse_correction(int d, int64_t nh, const CoarseDev& c, const double* r, double* z, int slot,
                                     PcgScalars* sc, hipStream_t s, Timer* timer);
 // free_vars (optional, n bytes, 0 = fixed): CG on the free variables only, the
 // reduced basis of newton_direction_reduced_basis (NewtonDirection.hh:76-105)
 // for a basis B that selects variables; fixed variables of the solution are 0.
+// Dinv: the block-Jacobi inverse, per handle the upper triangle of its
+// symmetric d x d block (d (d + 1) / 2 doubles, row by row).
 hipError_t launch_pcg_setup(int d, int64_t n_handles, const int32_t* brow_ptr, const int32_t* diag_block,
                             const int32_t* row_ptr, const double* H, double w, const unsigned char* free_vars,
                             double* Dinv, hipStream_t s);

@@ -480,6 +480,18 @@ __device__ __forceinline__ double sum_parts(const double* __restrict__ parts, in
     return block_sum_n<NT>(acc, sh);
 }
 
+// The block-Jacobi inverse keeps the upper triangle of each symmetric D x D
+// block, row by row: (0,0) (0,1) .. (0,D-1) (1,1) .. (D-1,D-1).
+template <int D>
+struct PackedBlock
+{
+    static constexpr int n = D * (D + 1) / 2;
+    __device__ static constexpr int at(int i, int j)
+    {
+        return i <= j ? i * D - (i * (i - 1)) / 2 + (j - i) : j * D - (j * (j - 1)) / 2 + (i - j);
+    }
+};
+
 template <int D>
 __device__ __forceinline__ void invert_block(const double* B, double* o)
 {
@@ -530,8 +542,11 @@ __global__ __launch_bounds__(kBlock) void k_pcg_setup(int64_t nh, const int32_t*
     {
         // handle used by no element: its rows are empty, (w I + H) is w I there
 #pragma unroll
-        for (int m = 0; m < D * D; ++m)
-            Dinv[u * D * D + m] = (m % (D + 1) == 0 && fr[m / D]) ? (w > 0.0 ? 1.0 / w : 1.0) : 0.0;
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = i; j < D; ++j)
+                Dinv[u * PackedBlock<D>::n + PackedBlock<D>::at(i, j)] =
+                    (i == j && fr[i]) ? (w > 0.0 ? 1.0 / w : 1.0) : 0.0;
         return;
     }
 #pragma unroll
@@ -546,8 +561,13 @@ __global__ __launch_bounds__(kBlock) void k_pcg_setup(int64_t nh, const int32_t*
 #pragma unroll
         for (int j = 0; j < D; ++j)
             if (!(fr[i] && fr[j])) o[i * D + j] = 0.0;
+    // the upper triangle: the inverse of a symmetric block is symmetric (the
+    // cofactor products above commute), and the stored preconditioner is
+    // symmetric exactly even where H's block is so only up to rounding
 #pragma unroll
-    for (int m = 0; m < D * D; ++m) Dinv[u * D * D + m] = o[m];
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = i; j < D; ++j) Dinv[u * PackedBlock<D>::n + PackedBlock<D>::at(i, j)] = o[i * D + j];
 }
 
 // x = 0, r = b, z = M^-1 r; partials rho_0 = r.z (slot 1) and ||b||^2 (rr0)
@@ -575,7 +595,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t nh, const double* _
         {
             double zi = 0.0;
 #pragma unroll
-            for (int j = 0; j < D; ++j) zi += Dinv[u * D * D + i * D + j] * rv[j];
+            for (int j = 0; j < D; ++j) zi += Dinv[u * PackedBlock<D>::n + PackedBlock<D>::at(i, j)] * rv[j];
             z[u * D + i] = zi;
             rz += rv[i] * zi;
             rr += rv[i] * rv[i];
@@ -606,7 +626,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t nh, const double* _
 // so the SpMV gathers only z, which is complete when it runs.
 
 // CG iteration k, first half: termination test on rr_k = ||r_k||^2 (k = 0:
-// ||b||^2, recorded), then s = (w I + H) z and partial z.s.
+// ||b||^2, recorded), then s = (w I + H) z and partial z.s.  The termination
+// flag, the residual partials and the first row group's block-row data are
+// all requested before the test, so their memory latencies overlap.
 template <int D, int LPB>
 __global__ __launch_bounds__(kBlock) void k_spmv(int64_t nh, const int32_t* __restrict__ brow_ptr,
                                                  const int32_t* __restrict__ bcol, const int32_t* __restrict__ row_ptr,
@@ -614,17 +636,60 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t nh, const int32_t* __re
                                                  double* __restrict__ sv, double* __restrict__ parts, int Gs, int Gv,
                                                  int k, PcgScalars* sc)
 {
-    if (block_done(sc)) return;
+    static_assert(kBlock * 4 >= 1024, "the residual partials (at most 1024) are read four per thread");
     __shared__ double sh[kBlock];
+    __shared__ int flag;
+    const int done_v = threadIdx.x == 0 ? sc->done : 0;
+    const double* rparts = k == 0 ? parts + Gs + 4 * Gv : parts + Gs + 2 * Gv + ((k + 1) & 1) * Gv;
+    double pa[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+    {
+        const int i = threadIdx.x + q * kBlock;
+        pa[q] = i < Gv ? rparts[i] : 0.0;
+    }
+    const int lane = threadIdx.x % LPB;
+    constexpr int rows_per_block = kBlock / LPB;
+    int64_t u0 = (int64_t)blockIdx.x * rows_per_block;
+    int f_nbu = 0;
+    int64_t f_base = 0, f_v = 0;
+    double f_h[D * D];
+    {
+        const int64_t u = u0 + threadIdx.x / LPB;
+        if (u < nh)
+        {
+            const int b0 = brow_ptr[u];
+            f_nbu = brow_ptr[u + 1] - b0;
+            f_base = row_ptr[D * u];
+            if (lane < f_nbu)
+            {
+                f_v = bcol[b0 + lane];
+#pragma unroll
+                for (int i = 0; i < D; ++i)
+#pragma unroll
+                    for (int j = 0; j < D; ++j) f_h[i * D + j] = H[f_base + (int64_t)i * D * f_nbu + D * lane + j];
+            }
+        }
+    }
+    if (threadIdx.x == 0) flag = done_v;
+    __syncthreads();
+    if (flag) return;
+    // the partials in index order per thread, then the fixed block tree (the
+    // order of sum_parts)
+    double racc = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if ((int)threadIdx.x + q * kBlock < Gv) racc += pa[q];
+    const double rsum = block_sum_n<kBlock>(racc, sh);
     double rr, bb;
     if (k == 0)
     {
-        bb = sum_parts(parts + Gs + 4 * Gv, Gv, sh);
+        bb = rsum;
         rr = bb;
     }
     else
     {
-        rr = sum_parts(parts + Gs + 2 * Gv + ((k + 1) & 1) * Gv, Gv, sh);
+        rr = rsum;
         bb = sc->bb;
     }
     const bool stop = rr <= sc->tol2 * bb || k >= sc->max_iters;
@@ -636,10 +701,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t nh, const int32_t* __re
         if (stop) sc->done = 1;
     }
     if (stop) return;
-    const int lane = threadIdx.x % LPB;
-    constexpr int rows_per_block = kBlock / LPB;
     double zs = 0.0;
-    for (int64_t u0 = (int64_t)blockIdx.x * rows_per_block; u0 < nh; u0 += (int64_t)gridDim.x * rows_per_block)
+    bool first = true;
+    for (; u0 < nh; u0 += (int64_t)gridDim.x * rows_per_block)
     {
         const int64_t u = u0 + threadIdx.x / LPB;
         double acc[D];
@@ -647,10 +711,34 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t nh, const int32_t* __re
         for (int i = 0; i < D; ++i) acc[i] = 0.0;
         if (u < nh)
         {
-            const int b0 = brow_ptr[u];
-            const int nbu = brow_ptr[u + 1] - b0;
-            const int64_t base = row_ptr[D * u];
-            for (int kk = lane; kk < nbu; kk += LPB)
+            int b0, nbu, kk0;
+            int64_t base;
+            if (first)
+            {
+                nbu = f_nbu;
+                base = f_base;
+                b0 = 0;
+                if (lane < nbu)
+                {
+                    double zv[D];
+#pragma unroll
+                    for (int j = 0; j < D; ++j) zv[j] = z[f_v * D + j];
+#pragma unroll
+                    for (int i = 0; i < D; ++i)
+#pragma unroll
+                        for (int j = 0; j < D; ++j) acc[i] += f_h[i * D + j] * zv[j];
+                }
+                kk0 = lane + LPB;
+                if (kk0 < nbu) b0 = brow_ptr[u];
+            }
+            else
+            {
+                b0 = brow_ptr[u];
+                nbu = brow_ptr[u + 1] - b0;
+                base = row_ptr[D * u];
+                kk0 = lane;
+            }
+            for (int kk = kk0; kk < nbu; kk += LPB)
             {
                 const int64_t v = bcol[b0 + kk];
                 double zv[D];
@@ -662,6 +750,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t nh, const int32_t* __re
                     for (int j = 0; j < D; ++j) acc[i] += H[base + (int64_t)i * D * nbu + D * kk + j] * zv[j];
             }
         }
+        first = false;
 #pragma unroll
         for (int off = LPB / 2; off > 0; off >>= 1)
 #pragma unroll
@@ -703,17 +792,19 @@ __global__ __launch_bounds__(kUpdVerts * D) void k_pcg_update(int64_t nh, const 
                                                               PcgScalars* sc)
 {
     constexpr int NT = kUpdVerts * D;
+    constexpr int PB = PackedBlock<D>::n;
+    constexpr int ND = (kUpdVerts * PB + NT - 1) / NT;  // packed block entries per thread and chunk
     if (block_done(sc)) return;
     __shared__ double sh[NT];
-    __shared__ double sD[kUpdVerts * D * D];
+    __shared__ double sD[ND * NT];
     __shared__ double sR[NT];
     const int64_t n = nh * D;
-    const int64_t dn = nh * D * D;
+    const int64_t dn = nh * PB;
     const int t = threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * kUpdVerts;
-    // operands of one chunk: this thread's variable and its third of the
-    // chunk's D x D preconditioner blocks
-    double zc = 0.0, pc = 0.0, qc = 0.0, sc_ = 0.0, xc = 0.0, rc = 0.0, dc[D];
+    // operands of one chunk: this thread's variable and its share of the
+    // chunk's packed preconditioner blocks
+    double zc = 0.0, pc = 0.0, qc = 0.0, sc_ = 0.0, xc = 0.0, rc = 0.0, dc[ND];
     auto load = [&](int64_t c0) {
         const int64_t iv = c0 * D + t;
         if (iv < n)
@@ -726,10 +817,10 @@ __global__ __launch_bounds__(kUpdVerts * D) void k_pcg_update(int64_t nh, const 
             rc = r[iv];
         }
 #pragma unroll
-        for (int m = 0; m < D; ++m)
+        for (int m = 0; m < ND; ++m)
         {
-            const int64_t id = c0 * D * D + t + (int64_t)m * NT;
-            dc[m] = id < dn ? Dinv[id] : 0.0;
+            const int64_t id = c0 * PB + t + (int64_t)m * NT;
+            dc[m] = (t + m * NT < kUpdVerts * PB && id < dn) ? Dinv[id] : 0.0;
         }
     };
     int64_t c0 = (int64_t)blockIdx.x * kUpdVerts;
@@ -790,7 +881,7 @@ __global__ __launch_bounds__(kUpdVerts * D) void k_pcg_update(int64_t nh, const 
         __syncthreads();  // the previous chunk's LDS reads are complete
         sR[t] = rv;
 #pragma unroll
-        for (int m = 0; m < D; ++m) sD[t + m * NT] = dc[m];
+        for (int m = 0; m < ND; ++m) sD[t + m * NT] = dc[m];
         __syncthreads();
         if (c0 + stride < nh) load(c0 + stride);  // next chunk in flight during this one's z
         if (iv < n)
@@ -798,7 +889,7 @@ __global__ __launch_bounds__(kUpdVerts * D) void k_pcg_update(int64_t nh, const 
             const int vl = t / D, i = t - vl * D;
             double zi = 0.0;
 #pragma unroll
-            for (int j = 0; j < D; ++j) zi += sD[vl * D * D + i * D + j] * sR[vl * D + j];
+            for (int j = 0; j < D; ++j) zi += sD[vl * PB + PackedBlock<D>::at(i, j)] * sR[vl * D + j];
             z[iv] = zi;
             rz += rv * zi;
             rr += rv * rv;
@@ -1037,15 +1128,16 @@ __global__ __launch_bounds__(kBlock) void k_coarse_restrict_sum(int64_t n_agg, c
     rc[id] = t;
 }
 
-// Both restriction stages in one launch: one workgroup per aggregate, its
-// waves take the aggregate's chunks in turn (each chunk reduced exactly as by
-// k_coarse_restrict_chunk), the chunk partials go to LDS and are summed in
-// chunk order as by k_coarse_restrict_sum, so r_c has the same bits.  For
-// aggregates of at most kRestrictMaxChunks chunks.
+// Both restriction stages in one launch: one workgroup of kCoarseBlock
+// threads per aggregate, its waves take the aggregate's chunks in turn (each
+// chunk reduced exactly as by k_coarse_restrict_chunk), the chunk partials go
+// to LDS and are summed in chunk order as by k_coarse_restrict_sum, so r_c
+// has the same bits.  For aggregates of at most kRestrictMaxChunks chunks.
 constexpr int kRestrictMaxChunks = 64;
+constexpr int kCoarseBlock = 1024;
 
 template <int D, int KC>
-__global__ __launch_bounds__(kBlock) void k_coarse_restrict(const int32_t* __restrict__ agg_chunk,
+__global__ __launch_bounds__(kCoarseBlock) void k_coarse_restrict(const int32_t* __restrict__ agg_chunk,
                                                             const int32_t* __restrict__ chunk_ptr,
                                                             const int32_t* __restrict__ agg_vert,
                                                             const double* __restrict__ dvec,
@@ -1057,7 +1149,7 @@ __global__ __launch_bounds__(kBlock) void k_coarse_restrict(const int32_t* __res
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t a = blockIdx.x;
     const int32_t c0 = agg_chunk[a], c1 = agg_chunk[a + 1];
-    for (int32_t ch = c0 + wv; ch < c1; ch += kBlock / 64)
+    for (int32_t ch = c0 + wv; ch < c1; ch += kCoarseBlock / 64)
     {
         double acc[KC];
 #pragma unroll
@@ -1090,78 +1182,80 @@ __global__ __launch_bounds__(kBlock) void k_coarse_restrict(const int32_t* __res
     }
 }
 
-// y_c = A_c^-1 r_c, one wave per row (row i of the symmetric inverse is its
-// contiguous column i); one partial of r_c . y_c per workgroup
-constexpr int kCoarseRows = kBlock / 64;
-
-__global__ __launch_bounds__(kBlock) void k_coarse_apply(int64_t nc, const double* __restrict__ Ainv,
-                                                         const double* __restrict__ rc, double* __restrict__ yc,
-                                                         double* __restrict__ cparts, PcgScalars* sc)
+// Coarse product and prolongation in one launch, one workgroup of
+// kCoarseBlock threads per aggregate a: y_a = rows a KC .. a KC + KC - 1 of
+// A_c^-1 r_c (row i of the symmetric inverse is its contiguous column i;
+// thread t takes the entries t, t + kCoarseBlock, ..., then a fixed wave tree
+// and the waves' partials in order), one partial r_c[a] . y_a, then
+// z_v += R_v y_a for every member v of the aggregate.
+template <int D, int KC>
+__global__ __launch_bounds__(kCoarseBlock) void k_coarse_apply_prolong(int64_t nc, const double* __restrict__ Ainv,
+                                                                       const double* __restrict__ rc,
+                                                                       const int32_t* __restrict__ agg_ptr,
+                                                                       const int32_t* __restrict__ agg_vert,
+                                                                       const double* __restrict__ dvec,
+                                                                       const unsigned char* __restrict__ free_vars,
+                                                                       double* __restrict__ z,
+                                                                       double* __restrict__ cparts, PcgScalars* sc)
 {
     if (block_done(sc)) return;
-    __shared__ double wpart[kCoarseRows];
+    constexpr int NW = kCoarseBlock / 64;
+    __shared__ double wpart[NW][KC];
+    __shared__ double y[KC];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int64_t i = (int64_t)blockIdx.x * kCoarseRows + wv;
-    double prod = 0.0;
-    if (i < nc)
+    const int64_t a = blockIdx.x;
+    const double* __restrict__ col = Ainv + a * KC * nc;
+    double acc[KC];
+#pragma unroll
+    for (int c = 0; c < KC; ++c) acc[c] = 0.0;
+    for (int64_t j = threadIdx.x; j < nc; j += kCoarseBlock)
     {
-        const double* __restrict__ col = Ainv + i * nc;
-        double acc[1] = { 0.0 };
-        int64_t j = lane;
-        for (; j + 3 * 64 < nc; j += 4 * 64)
-        {
-            const double a0 = col[j], a1 = col[j + 64], a2 = col[j + 128], a3 = col[j + 192];
-            const double r0 = rc[j], r1 = rc[j + 64], r2 = rc[j + 128], r3 = rc[j + 192];
-            acc[0] += a0 * r0;
-            acc[0] += a1 * r1;
-            acc[0] += a2 * r2;
-            acc[0] += a3 * r3;
-        }
-        for (; j < nc; j += 64) acc[0] += col[j] * rc[j];
-        wave_sum<1>(acc);
-        if (lane == 0) yc[i] = acc[0];
-        prod = rc[i] * acc[0];
+        const double rj = rc[j];
+#pragma unroll
+        for (int c = 0; c < KC; ++c) acc[c] += col[c * nc + j] * rj;
     }
-    if (lane == 0) wpart[wv] = prod;
+    wave_sum<KC>(acc);
+#pragma unroll
+    for (int c = 0; c < KC; ++c)
+        if (lane == c) wpart[wv][c] = acc[c];
+    __syncthreads();
+    if (threadIdx.x < KC)
+    {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) t += wpart[k][threadIdx.x];
+        y[threadIdx.x] = t;
+    }
     __syncthreads();
     if (threadIdx.x == 0)
     {
         double t = 0.0;
 #pragma unroll
-        for (int k = 0; k < kCoarseRows; ++k) t += wpart[k];
-        cparts[blockIdx.x] = t;
+        for (int c = 0; c < KC; ++c) t += rc[a * KC + c] * y[c];
+        cparts[a] = t;
     }
-}
-
-// z_v += R_v y_c[a(v)]
-template <int D, int KC>
-__global__ __launch_bounds__(kBlock) void k_coarse_prolong(int64_t nh, const int32_t* __restrict__ agg_of,
-                                                           const double* __restrict__ dvec,
-                                                           const double* __restrict__ yc,
-                                                           const unsigned char* __restrict__ free_vars,
-                                                           double* __restrict__ z, PcgScalars* sc)
-{
-    if (block_done(sc)) return;
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nh) return;
-    const int64_t a = agg_of[v];
-    double R[D][KC];
-    coarse_rows<D, KC>(dvec + v * D, R);
-    double y[KC];
+    double ya[KC];
 #pragma unroll
-    for (int c = 0; c < KC; ++c) y[c] = yc[a * KC + c];
-#pragma unroll
-    for (int i = 0; i < D; ++i)
+    for (int c = 0; c < KC; ++c) ya[c] = y[c];
+    const int32_t m1 = agg_ptr[a + 1];
+    for (int32_t m = agg_ptr[a] + threadIdx.x; m < m1; m += kCoarseBlock)
     {
-        if (free_vars && !free_vars[v * D + i]) continue;
-        double t = z[v * D + i];
+        const int64_t v = agg_vert[m];
+        double R[D][KC];
+        coarse_rows<D, KC>(dvec + v * D, R);
 #pragma unroll
-        for (int c = 0; c < KC; ++c) t += R[i][c] * y[c];
-        z[v * D + i] = t;
+        for (int i = 0; i < D; ++i)
+        {
+            if (free_vars && !free_vars[v * D + i]) continue;
+            double t = z[v * D + i];
+#pragma unroll
+            for (int c = 0; c < KC; ++c) t += R[i][c] * ya[c];
+            z[v * D + i] = t;
+        }
     }
 }
 
-int coarse_apply_groups(int64_t nc) { return (int)std::max<int64_t>(1, (nc + kCoarseRows - 1) / kCoarseRows); }
+int coarse_apply_groups(int64_t n_agg) { return (int)std::max<int64_t>(1, n_agg); }
 
 hipError_t launch_coarse_dvec(int d, int64_t nh, const double* x, const CoarseDev& c, double* dvec, hipStream_t s)
 {
@@ -1225,8 +1319,8 @@ static void coarse_correction_t(int64_t nh, const CoarseDev& c, const double* r,
 {
     if (c.max_agg_chunks <= kRestrictMaxChunks)
     {
-        k_coarse_restrict<D, KC><<<(unsigned)c.n_agg, kBlock, 0, s>>>(c.agg_chunk, c.chunk_ptr, c.agg_vert, c.dvec, r,
-                                                                      c.rc, sc);
+        k_coarse_restrict<D, KC><<<(unsigned)c.n_agg, kCoarseBlock, 0, s>>>(c.agg_chunk, c.chunk_ptr, c.agg_vert,
+                                                                            c.dvec, r, c.rc, sc);
     }
     else
     {
@@ -1236,9 +1330,8 @@ static void coarse_correction_t(int64_t nh, const CoarseDev& c, const double* r,
         const unsigned gs = (unsigned)((c.n_agg * KC + kBlock - 1) / kBlock);
         k_coarse_restrict_sum<KC><<<gs, kBlock, 0, s>>>(c.n_agg, c.agg_chunk, c.chunk_part, c.rc, sc);
     }
-    k_coarse_apply<<<c.Gc, kBlock, 0, s>>>(c.nc, c.Ainv, c.rc, c.yc, c.cparts + (int64_t)slot * c.Gc, sc);
-    k_coarse_prolong<D, KC><<<(unsigned)((nh + kBlock - 1) / kBlock), kBlock, 0, s>>>(nh, c.agg_of, c.dvec, c.yc,
-                                                                                       c.free_vars, z, sc);
+    k_coarse_apply_prolong<D, KC><<<(unsigned)c.n_agg, kCoarseBlock, 0, s>>>(
+        c.nc, c.Ainv, c.rc, c.agg_ptr, c.agg_vert, c.dvec, c.free_vars, z, c.cparts + (int64_t)slot * c.Gc, sc);
 }
 
 hipError_t launch_coarse_correction(int d, int64_t nh, const CoarseDev& c, const double* r, double* z, int slot,
