@@ -82,7 +82,7 @@ def pmc_traffic(tag, kernel):
 
 
 def pmc_valu(tag, kernel):
-    """(FP operations per launch, VALU busy fraction) of `kernel` from
+    """(FP operations per launch, VALU issue utilization) of `kernel` from
     profiles/valu_<tag>.md, or (None, None)."""
     rows = profile_table(f"valu_{tag}.md")
     for cells in rows or []:
@@ -496,12 +496,12 @@ def main():
     e_avg_s = ems / max(ecnt, 1) / 1e3
     Nv = P.terms[0].valence
     ekernel = f"k_element<SymdirElem<{P.d}>,{'double' if args.precision == 'fp64' else 'float'}>"
-    flops, busy = pmc_valu(tag, ekernel)
+    flops, valu_util = pmc_valu(tag, ekernel)
     peak_tf = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
     roof_el = {"kernel": ekernel, "bound": "valu", "unit": "TFLOP/s", "peak": peak_tf,
                "avg_launch_ms": e_avg_s * 1e3, "launches": ecnt,
                "achieved": (flops / e_avg_s / 1e12) if flops else None,
-               "flops_per_launch": flops, "valu_busy": busy,
+               "flops_per_launch": flops, "valu_issue_utilization": valu_util,
                "source": f"profiles/valu_{tag}.md (SQ_INSTS_VALU_{{ADD,MUL,FMA,TRANS}}_F{{64,32}} x 64 lanes, "
                          "FMA counted twice)"}
     roof_el["frac"] = roof_el["achieved"] / peak_tf if roof_el["achieved"] else None

@@ -452,6 +452,21 @@ tad_status tad_fn_solve(tad_fn fn, const double* H_values, const double* rhs, do
 tad_status tad_fn_solve_reduced(tad_fn fn, const double* H_values, const double* rhs, double* sol, tad_mem mem,
                                 const unsigned char* free_vars, const tad_solve_opts* opts, tad_solve_stats* stats);
 
+/* The solve in a general subspace (newton_direction_reduced_basis,
+ * NewtonDirection.hh:76-105, for any sparse basis): sol = B y with
+ * (B^T H B + w I_m) y = B^T rhs, B an n_vars x m CSR matrix (b_row_ptr:
+ * n_vars + 1 offsets from 0, b_col: column indices in 0 .. m-1, b_val:
+ * finite values; all in the memory space `mem` like the vectors).  Conjugate
+ * gradients on y with the operator applied as products with B, H and B^T
+ * and the diagonal of B^T H B + w I as preconditioner (no coarse space);
+ * tol, max_iters and w_identity of `opts` apply.  TAD_ERR_SOLVE when a
+ * diagonal entry or a CG step shows B^T H B + w I not positive definite
+ * (the reference's factorization fails there); stats->converged = 0 when
+ * max_iters ran out. */
+tad_status tad_fn_solve_basis(tad_fn fn, const double* H_values, int64_t m, const int32_t* b_row_ptr,
+                              const int32_t* b_col, const double* b_val, const double* rhs, double* sol, tad_mem mem,
+                              const tad_solve_opts* opts, tad_solve_stats* stats);
+
 typedef struct tad_newton_opts
 {
     double projection_eps;  /* eval_with_hessian_proj eps (default 1e-9); NaN: unprojected */

@@ -801,37 +801,46 @@ VectorXd gauss_newton_direction(const VectorFunction<dd, H>& func, const VectorX
 }
 
 // newton_direction_reduced_basis(g, H_proj, B, solver, w) (NewtonDirection.hh:76-105):
-// d = B (B^T H B + w I)^-1 (-B^T g) for an n x m basis B that selects
-// variables (every column one entry 1, distinct rows: the pinning case);
-// other bases throw std::runtime_error.  d is 0 on the variables B omits.
+// d = B (B^T H B + w I)^-1 (-B^T g) for an n x m basis B.  A basis that
+// selects variables (every column one entry 1, distinct rows: the pinning
+// case) runs the two-level preconditioned solve on the free variables (d is 0
+// on the variables B omits); any other basis runs conjugate gradients on the
+// reduced system with products by B, H and B^T (tad_fn_solve_basis).
 template <int dd, class H>
 VectorXd newton_direction_reduced_basis(const ScalarFunction<dd, H>& func, const VectorXd& g,
                                         const SparseMatrix& H_proj, const SparseMatrix& B, LinearSolver& solver,
                                         double w_identity = 0.0)
 {
     const int64_t n = (int64_t)g.size();
-    if (B.rows() != n || H_proj.rows() != n || H_proj.cols() != n)
+    if (B.rows() != n || H_proj.rows() != n || H_proj.cols() != n || B.cols() < 1)
         throw std::runtime_error("[tinyad_amd] newton_direction_reduced_basis: size mismatch");
     std::vector<unsigned char> free_vars((size_t)n, 0);
     std::vector<int> per_col((size_t)B.cols(), 0);
-    for (int64_t r = 0; r < B.rows(); ++r)
+    bool selects = true;
+    for (int64_t r = 0; r < B.rows() && selects; ++r)
         for (int32_t k = B.outer[r]; k < B.outer[r + 1]; ++k)
         {
             if (B.values[k] == 0.0) continue;
             if (B.values[k] != 1.0 || free_vars[r])
-                throw std::runtime_error("[tinyad_amd] newton_direction_reduced_basis: B must select variables");
+            {
+                selects = false;
+                break;
+            }
             free_vars[r] = 1;
             per_col[B.inner[k]]++;
         }
-    for (int c : per_col)
-        if (c != 1) throw std::runtime_error("[tinyad_amd] newton_direction_reduced_basis: B must select variables");
+    for (int c : per_col) selects = selects && c == 1;
     VectorXd minus_g(g.size());
     for (size_t i = 0; i < g.size(); ++i) minus_g[i] = -g[i];
     VectorXd dvec(g.size());
     tad_solve_opts o = solver.opts;
     o.w_identity = w_identity;
-    check(tad_fn_solve_reduced(func.handle(), H_proj.values.data(), minus_g.data(), dvec.data(), TAD_MEM_HOST,
-                               free_vars.data(), &o, &solver.last_stats));
+    if (selects)
+        check(tad_fn_solve_reduced(func.handle(), H_proj.values.data(), minus_g.data(), dvec.data(), TAD_MEM_HOST,
+                                   free_vars.data(), &o, &solver.last_stats));
+    else
+        check(tad_fn_solve_basis(func.handle(), H_proj.values.data(), B.cols(), B.outer.data(), B.inner.data(),
+                                 B.values.data(), minus_g.data(), dvec.data(), TAD_MEM_HOST, &o, &solver.last_stats));
     if (!solver.last_stats.converged) throw std::runtime_error("[tinyad_amd] Linear solve failed.");
     return dvec;
 }

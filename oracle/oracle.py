@@ -269,6 +269,17 @@ def gauss_newton_direction(r, J, w_identity: float = 0.0):
     return spla.spsolve(A, -(J.T @ r))
 
 
+def newton_direction_reduced_basis(g, H, B, w_identity: float = 0.0):
+    """d = B y with (B^T H B + w I_m) y = -B^T g, by a direct sparse solve
+    (NewtonDirection.hh:76-105 factorizes H_reduced with SimplicialLDLT)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    B = sp.csr_matrix(B)
+    Hr = (B.T @ H @ B + w_identity * sp.identity(B.shape[1])).tocsc()
+    y = spla.spsolve(Hr, -(B.T @ g))
+    return B @ np.atleast_1d(y)
+
+
 def pcg(H, b, w: float = 0.0, block: int = 1, tol: float = 1e-10, max_iters: int = 10000, n_threads: int = 0):
     """Block-Jacobi PCG on (w I + H) x = b with x0 = 0 (the GPU solve's
     recurrences without its coarse correction).  n_threads: OpenMP threads

@@ -395,7 +395,9 @@ static bool test_two_level_solve()
 
 // SwitchSolverTest (SwitchSolverTest.cc:16-47): x^2 + y^2 at (1, 1), the Newton
 // direction and the reduced-basis one with B = I are (-1, -1); a basis that
-// keeps only x gives (-1, 0); a basis that does not select variables throws.
+// keeps only x gives (-1, 0) whatever its scale; the basis (1, 1)^T gives
+// (-1, -1), and with w = 1 (B^T H B + w = 5, B^T g = 4) (-0.8, -0.8); a basis
+// with an empty column (B^T H B + w I singular) throws.
 static bool test_reduced_basis()
 {
     bool ok = true;
@@ -421,8 +423,22 @@ static bool test_reduced_basis()
     d = tinyad_amd::newton_direction_reduced_basis(func, g, H, Bx);
     EXPECT(std::fabs(d[0] + 1.0) <= 1e-12 && d[1] == 0.0);
     Bx.values = { 2.0 };
+    d = tinyad_amd::newton_direction_reduced_basis(func, g, H, Bx, solver);
+    EXPECT(std::fabs(d[0] + 1.0) <= 1e-12 && d[1] == 0.0);
+    tinyad_amd::SparseMatrix B1;
+    B1.n_rows = 2;
+    B1.n_cols = 1;
+    B1.outer = { 0, 1, 2 };
+    B1.inner = { 0, 0 };
+    B1.values = { 1.0, 1.0 };
+    d = tinyad_amd::newton_direction_reduced_basis(func, g, H, B1, solver);
+    EXPECT(std::fabs(d[0] + 1.0) <= 1e-12 && std::fabs(d[1] + 1.0) <= 1e-12);
+    d = tinyad_amd::newton_direction_reduced_basis(func, g, H, B1, solver, 1.0);
+    EXPECT(std::fabs(d[0] + 0.8) <= 1e-12 && std::fabs(d[1] + 0.8) <= 1e-12);
+    tinyad_amd::SparseMatrix B2 = B1;
+    B2.n_cols = 2;  // column 1 has no entry
     bool threw = false;
-    try { tinyad_amd::newton_direction_reduced_basis(func, g, H, Bx, solver); }
+    try { tinyad_amd::newton_direction_reduced_basis(func, g, H, B2, solver); }
     catch (const std::runtime_error&) { threw = true; }
     EXPECT(threw);
     return ok;

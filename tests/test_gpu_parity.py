@@ -392,6 +392,51 @@ def test_reduced_basis_solve(ctx):
         fg.solve(-g, free=free[:-1])
 
 
+def test_general_basis_solve(ctx):
+    """newton_direction_reduced_basis with a general sparse basis B
+    (NewtonDirection.hh:76-105): sol = B (B^T H B + w I)^-1 B^T rhs against the
+    oracle's direct solve, for a random sparse B (one to three entries per
+    column, plus a unit entry so it has full column rank), a selection basis
+    scaled column by column (the same subspace as the selection solve), B = I
+    (the full solve), host and device, and the errors."""
+    import torch
+    P = configs.twist_cube(m=6, seed=6)
+    fg = tad.ScalarFunction.from_problem(P, ctx)
+    f, g, H = fg.eval_with_hessian_proj(P.x0)
+    n = g.size
+    rng = np.random.default_rng(12)
+    m = n // 3
+    rows = np.concatenate([rng.choice(n, m, replace=False), rng.integers(0, n, 2 * m)])
+    cols = np.concatenate([np.arange(m), rng.integers(0, m, 2 * m)])
+    vals = np.concatenate([np.ones(m), rng.uniform(-1.0, 1.0, 2 * m)])
+    B = sp.csr_matrix((vals, (rows, cols)), shape=(n, m))
+    for w in [0.0, 0.25]:
+        ref = oracle.newton_direction_reduced_basis(g, H, B, w)
+        d, st = fg.solve_basis(-g, B, tol=1e-13, w_identity=w, max_iters=20000)
+        print(f"general basis n={n} m={m} w={w}: {st.iters} iterations, rel residual {st.rel_residual:.2e}")
+        assert st.converged and st.coarse_unknowns == 0
+        assert nrel(d, ref) <= 1e-8
+        d2, _ = fg.solve_basis(-g, B, H=H, tol=1e-13, w_identity=w, max_iters=20000)
+        assert np.array_equal(d, d2)  # deterministic, and H=None is the last evaluated Hessian
+    free = pinned_cube_mask(P)
+    fi = np.nonzero(free)[0]
+    S = sp.csr_matrix((rng.uniform(0.5, 2.0, fi.size), (fi, np.arange(fi.size))), shape=(n, fi.size))
+    ds, st = fg.solve_basis(-g, S, tol=1e-13, max_iters=20000)
+    dsel, _ = fg.solve(-g, free=free, tol=1e-13)
+    assert st.converged and nrel(ds, dsel) <= 1e-8 and np.all(ds[free == 0] == 0.0)
+    dI, st = fg.solve_basis(-g, sp.identity(n, format="csr"), tol=1e-13, max_iters=20000)
+    assert st.converged and nrel(dI, fg.solve(-g, tol=1e-13)[0]) <= 1e-8
+    dd, st = fg.solve_basis(torch.tensor(-g, device="cuda:0"), B, tol=1e-13, max_iters=20000)
+    assert st.converged and np.array_equal(dd.cpu().numpy(), fg.solve_basis(-g, B, tol=1e-13, max_iters=20000)[0])
+    with pytest.raises(tad.TadError):
+        fg.solve_basis(-g, B[:-1])
+    Bz = sp.hstack([B, sp.csr_matrix((n, 1))]).tocsr()  # an empty column: B^T H B + w I is singular
+    with pytest.raises(tad.TadError):
+        fg.solve_basis(-g, Bz)
+    _, st = fg.solve_basis(-g, B, tol=1e-13, max_iters=3)
+    assert not st.converged and st.iters == 3
+
+
 def test_reduced_basis_newton_keeps_pins(ctx):
     """Projected Newton with hard pins (reduced-basis directions and the
     library's line search): pinned coordinates never move, the energy

@@ -585,3 +585,39 @@ def test_vector_inverted_and_pins_only():
     assert np.array_equal(J.toarray(), -2.0 * np.eye(6))
     d = oracle.gauss_newton_direction(r, J, 0.0)
     assert np.abs(x0 + d - tgt.ravel()).max() <= 1e-15
+
+
+def test_reduced_basis_direction_pins():
+    """newton_direction_reduced_basis (NewtonDirection.hh:76-105) restated:
+    the reference's SwitchSolverTest answer (H = 2I, g = 2, B = I gives
+    (-1, -1), SwitchSolverTest.cc:20-36); B = I equals the full Newton
+    direction with w; a selection basis equals the solve on the kept rows and
+    columns with zeros elsewhere; a rescaled basis B D gives the same d (the
+    subspace is the same); and d is the minimizer of the quadratic model over
+    the subspace (its reduced gradient B^T (H d + g) vanishes)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    H = 2.0 * sp.identity(2, format="csr")
+    g = np.array([2.0, 2.0])
+    assert np.allclose(oracle.newton_direction_reduced_basis(g, H, sp.identity(2)), [-1.0, -1.0], atol=1e-14)
+    rng = np.random.default_rng(4)
+    n = 40
+    A = sp.random(n, n, density=0.1, random_state=5)
+    H = (A @ A.T + 0.5 * sp.identity(n)).tocsr()
+    g = rng.standard_normal(n)
+    w = 0.3
+    full = spla.spsolve((H + w * sp.identity(n)).tocsc(), -g)
+    assert np.allclose(oracle.newton_direction_reduced_basis(g, H, sp.identity(n), w), full, rtol=1e-12, atol=1e-12)
+    keep = np.sort(rng.choice(n, 25, replace=False))
+    S = sp.csr_matrix((np.ones(keep.size), (keep, np.arange(keep.size))), shape=(n, keep.size))
+    d = oracle.newton_direction_reduced_basis(g, H, S, w)
+    Hk = H[keep][:, keep] + w * sp.identity(keep.size)
+    ref = np.zeros(n)
+    ref[keep] = spla.spsolve(Hk.tocsc(), -g[keep])
+    assert np.allclose(d, ref, rtol=1e-12, atol=1e-12)
+    B = sp.random(n, 12, density=0.15, random_state=6) + sp.eye(n, 12)
+    d1 = oracle.newton_direction_reduced_basis(g, H, B, 0.0)
+    D = sp.diags(rng.uniform(0.5, 2.0, 12))
+    d2 = oracle.newton_direction_reduced_basis(g, H, B @ D, 0.0)
+    assert np.allclose(d1, d2, rtol=1e-10, atol=1e-12)
+    assert np.abs(B.T @ (H @ d1 + g)).max() <= 1e-10 * np.abs(B.T @ g).max()

@@ -517,6 +517,48 @@ class ScalarFunction:
                                              fm.ctypes.data, ctypes.byref(o), ctypes.byref(st)))
         return sol, st
 
+    def solve_basis(self, rhs, B, H=None, **opts):
+        """sol = B y with (B^T H B + w I_m) y = B^T rhs for a general sparse basis B (scipy
+        matrix, n_vars x m): newton_direction_reduced_basis (NewtonDirection.hh:76-105) with
+        rhs = -g.  rhs numpy (H then numpy or None) or a float64 CUDA tensor (H then a CUDA
+        tensor of the nnz values or None); H=None uses the last evaluated Hessian."""
+        import scipy.sparse as sp
+        Bc = sp.csr_matrix(B)
+        Bc.sort_indices()
+        if Bc.shape[0] != self.n_vars or Bc.shape[1] < 1:
+            raise TadError(1, f"B must be {self.n_vars} x m with m >= 1, got {Bc.shape}")
+        m = Bc.shape[1]
+        rp = np.ascontiguousarray(Bc.indptr, dtype=np.int32)
+        ci = np.ascontiguousarray(Bc.indices, dtype=np.int32)
+        bv = np.ascontiguousarray(Bc.data, dtype=np.float64)
+        o = solve_default_opts(**opts)
+        st = SolveStats()
+        if _is_torch(rhs):
+            import torch
+            self._dev_vec(rhs, self.n_vars, "rhs")
+            hp = self._dev_vec(H, self.nnz, "H").data_ptr() if H is not None else None
+            dev = rhs.device
+            rpt = torch.from_numpy(rp).to(dev)
+            cit = torch.from_numpy(ci).to(dev) if ci.size else torch.zeros(1, dtype=torch.int32, device=dev)
+            bvt = torch.from_numpy(bv).to(dev) if bv.size else torch.zeros(1, dtype=torch.float64, device=dev)
+            _order_after_torch(self.ctx, dev)
+            sol = torch.empty_like(rhs)
+            check(lib().tad_fn_solve_basis(self._h, hp, m, rpt.data_ptr(), cit.data_ptr(), bvt.data_ptr(),
+                                           rhs.data_ptr(), sol.data_ptr(), _lib.MEM_DEVICE, ctypes.byref(o),
+                                           ctypes.byref(st)))
+            self.ctx.synchronize()
+            return sol, st
+        rhs = self._host_vec(rhs, self.n_vars, "rhs")
+        sol = np.zeros_like(rhs)
+        hv = self._host_vec(H, self.nnz, "H") if H is not None else None
+        hptr = hv.ctypes.data if hv is not None else None
+        ci_ = ci if ci.size else np.zeros(1, dtype=np.int32)
+        bv_ = bv if bv.size else np.zeros(1, dtype=np.float64)
+        check(lib().tad_fn_solve_basis(self._h, hptr, m, rp.ctypes.data, ci_.ctypes.data, bv_.ctypes.data,
+                                       rhs.ctypes.data, sol.ctypes.data, _lib.MEM_HOST, ctypes.byref(o),
+                                       ctypes.byref(st)))
+        return sol, st
+
     def newton_step(self, x, opts: NewtonOpts | None = None) -> dict:
         """One projected-Newton iteration in place on x (NewtonTest.cc:69-75 loop body)."""
         o = opts or newton_default_opts()

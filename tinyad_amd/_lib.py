@@ -27,7 +27,7 @@ EXPORTED = [
     "tad_fn_n_elements",
     "tad_fn_pattern", "tad_fn_eval", "tad_fn_eval_noalloc", "tad_fn_eval_with_gradient", "tad_fn_eval_with_derivatives",
     "tad_fn_eval_with_hessian_proj", "tad_fn_device_buffers",
-    "tad_solve_default_opts", "tad_fn_solve", "tad_fn_solve_reduced", "tad_newton_default_opts", "tad_newton_step",
+    "tad_solve_default_opts", "tad_fn_solve", "tad_fn_solve_reduced", "tad_fn_solve_basis", "tad_newton_default_opts", "tad_newton_step",
     "tad_line_search", "tad_newton_decrement",
     "tad_fn_set_segments", "tad_fn_segment_energies", "tad_fn_autotune",
     "tad_fn_get_launch_shape", "tad_fn_set_launch_shape", "tad_fn_get_assembly_shape", "tad_fn_set_assembly_shape",
@@ -144,6 +144,8 @@ def lib():
         "tad_solve_default_opts": [ctypes.POINTER(SolveOpts)],
         "tad_fn_solve": [vp, vp, vp, vp, i32, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats)],
         "tad_fn_solve_reduced": [vp, vp, vp, vp, i32, vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats)],
+        "tad_fn_solve_basis": [vp, vp, i64, vp, vp, vp, vp, vp, i32, ctypes.POINTER(SolveOpts),
+                               ctypes.POINTER(SolveStats)],
         "tad_newton_default_opts": [ctypes.POINTER(NewtonOpts)],
         "tad_newton_step": [vp, vp, i32, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonStats)],
         "tad_line_search": [vp, vp, vp, dbl, vp, vp, i32, ctypes.POINTER(NewtonOpts), pdbl, pdbl,

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(HERE, "libtinyad_amd.so")
-SOURCES = ["tad_element.hip", "tad_energy.hip", "tad_linalg.hip", "tad_dense.hip", "tad_adjoint.hip", "tad_api.cpp"]
+SOURCES = ["tad_element.hip", "tad_energy.hip", "tad_linalg.hip", "tad_dense.hip", "tad_basis.hip", "tad_adjoint.hip", "tad_api.cpp"]
 HEADERS = ["tad_energies.hpp", "tad_elements.hpp", "tad_internal.hpp"]
 # public headers the library's sources also compile (device code shared with
 # element kernels built from user energies)

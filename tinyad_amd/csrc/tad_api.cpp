@@ -299,6 +299,10 @@ struct tad_fn_s
     // positions of the last second-order evaluation (rotation modes of the coarse space)
     DevBuf<double> x_hess;
     DevBuf<unsigned char> free_buf;  // staged free-variable mask of a reduced-basis solve
+    // general reduced basis (tad_fn_solve_basis): B (n x m CSR), B^T, and the
+    // vectors y r z p q dinv (m each) and t s rhs out (n each)
+    DevBuf<int32_t> bas_rp, bas_ci, bas_tp, bas_tc;
+    DevBuf<double> bas_v, bas_tv, bas_vec;
     bool x_hess_valid = false;
     int last_solve_iters = 0;  // iterations of the previous solve (first poll of the next one)
     int solves_since_rebuild = 0;  // solves that reused the coarse space since it was built
@@ -1330,6 +1334,8 @@ tad_status tad_fn_destroy(tad_fn fn)
     fn->scal.release(); fn->pcg_parts.release(); fn->seg_ranges.release(); fn->seg_f.release();
     fn->adj_g.release(); fn->adj_rest.release(); fn->adj_lam.release();
     fn->co.release(); fn->x_hess.release(); fn->free_buf.release();
+    fn->bas_rp.release(); fn->bas_ci.release(); fn->bas_tp.release(); fn->bas_tc.release();
+    fn->bas_v.release(); fn->bas_tv.release(); fn->bas_vec.release();
     tad_ctx ctx = fn->ctx;
     delete fn;
     ctx_release(ctx);
@@ -1920,6 +1926,199 @@ tad_status tad_fn_solve_reduced(tad_fn fn, const double* H_values, const double*
 {
     if (!free_vars) return fail(TAD_ERR_INVALID, "null free-variable mask");
     return solve_entry(fn, H_values, rhs, sol, mem, free_vars, opts, stats);
+}
+
+// Length-n dot products a_i . b_i (fixed order, as launch_dot) with one host
+// synchronization.
+static tad_status dots_n(tad_fn fn, int64_t n, int count, const double* const* a, const double* const* b,
+                         double* out)
+{
+    hipStream_t s = fn->s();
+    for (int i = 0; i < count; ++i)
+        TAD_HIP(launch_dot(a[i], b[i], n, fn->partials.p, fn->results.p + 10 + i, s));
+    double* pin = fn->ctx->pinned;
+    TAD_HIP(hipMemcpyAsync(pin + 50, fn->results.p + 10, count * sizeof(double), hipMemcpyDeviceToHost, s));
+    TAD_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < count; ++i) out[i] = pin[50 + i];
+    return TAD_OK;
+}
+
+tad_status tad_fn_solve_basis(tad_fn fn, const double* H_values, int64_t m, const int32_t* b_row_ptr,
+                              const int32_t* b_col, const double* b_val, const double* rhs, double* sol, tad_mem mem,
+                              const tad_solve_opts* opts, tad_solve_stats* stats)
+{
+    TAD_TRY(check_fn(fn));
+    if (!b_row_ptr || !b_col || !b_val || !rhs || !sol) return fail(TAD_ERR_INVALID, "null argument");
+    if (m < 1 || m > INT32_MAX) return fail(TAD_ERR_INVALID, "the basis needs 1 .. 2^31 - 1 columns");
+    TAD_HIP(hipSetDevice(fn->ctx->device));
+    TAD_TRY(ensure_ready(fn));
+    tad_solve_opts o;
+    tad_solve_default_opts(&o);
+    if (opts) o = *opts;
+    if (!(o.tol > 0.0) || o.max_iters < 1) return fail(TAD_ERR_INVALID, "tol must be positive, max_iters >= 1");
+    hipStream_t s = fn->s();
+    const int64_t n = fn->n_vars;
+    const double* Hd;
+    if (!H_values)
+    {
+        if (!fn->h_valid) return fail(TAD_ERR_INVALID, "no Hessian evaluated yet");
+        Hd = fn->H_buf.p;
+    }
+    else if (mem == TAD_MEM_DEVICE)
+        Hd = H_values;
+    else
+    {
+        TAD_HIP(hipMemcpyAsync(fn->H_buf.p, H_values, fn->nnz * sizeof(double), hipMemcpyHostToDevice, s));
+        fn->h_valid = true;
+        Hd = fn->H_buf.p;
+    }
+    // B on the host, checked, and its transpose (rows of B^T = columns of B,
+    // entries in row order of B)
+    std::vector<int32_t> rp((size_t)n + 1);
+    auto fetch = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+        if (bytes == 0) return hipSuccess;
+        if (mem == TAD_MEM_HOST)
+        {
+            std::memcpy(dst, src, bytes);
+            return hipSuccess;
+        }
+        hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
+        return e != hipSuccess ? e : hipStreamSynchronize(s);
+    };
+    TAD_HIP(fetch(rp.data(), b_row_ptr, rp.size() * sizeof(int32_t)));
+    if (rp[0] != 0) return fail(TAD_ERR_INVALID, "basis row offsets must start at 0");
+    for (int64_t i = 0; i < n; ++i)
+        if (rp[i + 1] < rp[i]) return fail(TAD_ERR_INVALID, "basis row offsets must not decrease");
+    const int64_t nb = rp[n];
+    std::vector<int32_t> ci((size_t)nb);
+    std::vector<double> bv((size_t)nb);
+    TAD_HIP(fetch(ci.data(), b_col, ci.size() * sizeof(int32_t)));
+    TAD_HIP(fetch(bv.data(), b_val, bv.size() * sizeof(double)));
+    std::vector<int32_t> tp((size_t)m + 1, 0);
+    for (int64_t k = 0; k < nb; ++k)
+    {
+        if (ci[k] < 0 || ci[k] >= m) return fail(TAD_ERR_INVALID, "basis column index out of range");
+        if (!std::isfinite(bv[k])) return fail(TAD_ERR_INVALID, "basis entries must be finite");
+        tp[ci[k] + 1]++;
+    }
+    for (int64_t j = 0; j < m; ++j) tp[j + 1] += tp[j];
+    std::vector<int32_t> tc((size_t)nb);
+    std::vector<double> tv((size_t)nb);
+    {
+        std::vector<int32_t> next(tp.begin(), tp.end() - 1);
+        for (int64_t r = 0; r < n; ++r)
+            for (int32_t k = rp[r]; k < rp[r + 1]; ++k)
+            {
+                const int32_t q = next[ci[k]]++;
+                tc[q] = (int32_t)r;
+                tv[q] = bv[k];
+            }
+    }
+    auto up = [&](auto& buf, const auto& vec) -> hipError_t {
+        hipError_t e = buf.ensure(vec.size());
+        if (e != hipSuccess || vec.empty()) return e;
+        return hipMemcpyAsync(buf.p, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice, s);
+    };
+    TAD_HIP(up(fn->bas_rp, rp));
+    TAD_HIP(up(fn->bas_ci, ci));
+    TAD_HIP(up(fn->bas_v, bv));
+    TAD_HIP(up(fn->bas_tp, tp));
+    TAD_HIP(up(fn->bas_tc, tc));
+    TAD_HIP(up(fn->bas_tv, tv));
+    TAD_HIP(fn->bas_vec.ensure((size_t)(6 * m + 4 * n)));
+    double* y = fn->bas_vec.p;
+    double* r = y + m;
+    double* z = r + m;
+    double* p = z + m;
+    double* q = p + m;
+    double* dinv = q + m;
+    double* t = dinv + m;
+    double* sv = t + n;
+    double* bd = sv + n;
+    double* out = bd + n;
+    if (mem == TAD_MEM_HOST)
+        TAD_HIP(hipMemcpyAsync(bd, rhs, n * sizeof(double), hipMemcpyHostToDevice, s));
+    else
+        TAD_HIP(hipMemcpyAsync(bd, rhs, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    const int32_t* brp = fn->bas_rp.p;
+    const int32_t* bci = fn->bas_ci.p;
+    const double* bva = fn->bas_v.p;
+    const int32_t* btp = fn->bas_tp.p;
+    const int32_t* btc = fn->bas_tc.p;
+    const double* btv = fn->bas_tv.p;
+    // r = B^T rhs, the Jacobi preconditioner of B^T H B + w I, y = 0
+    TAD_HIP(launch_csr_spmv(m, btp, btc, btv, bd, 0.0, nullptr, r, s));
+    TAD_HIP(hipMemsetAsync(fn->err.p, 0, sizeof(int), s));
+    TAD_HIP(launch_basis_diag(m, btp, btc, btv, fn->row_ptr.p, fn->col_idx.p, Hd, o.w_identity, dinv, fn->err.p, s));
+    TAD_HIP(launch_fill(y, m, 0.0, s));
+    int bad = 0;
+    TAD_HIP(hipMemcpyAsync(fn->ctx->pinned + 2, fn->err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    TAD_HIP(hipStreamSynchronize(s));
+    std::memcpy(&bad, fn->ctx->pinned + 2, sizeof(int));
+    if (bad) return fail(TAD_ERR_SOLVE, "B^T H B + w I has a diagonal entry that is not positive");
+    TAD_HIP(launch_basis_scale(m, r, dinv, z, s));
+    TAD_HIP(hipMemcpyAsync(p, z, m * sizeof(double), hipMemcpyDeviceToDevice, s));
+    double d2[2];
+    {
+        const double* a[2] = { r, r };
+        const double* b[2] = { r, z };
+        TAD_TRY(dots_n(fn, m, 2, a, b, d2));
+    }
+    const double bb = d2[0];
+    double rho = d2[1], rr = bb;
+    int it = 0;
+    bool converged = bb == 0.0;
+    const double tol2 = o.tol * o.tol;
+    while (!converged && it < o.max_iters)
+    {
+        // q = B^T H B p + w p
+        TAD_HIP(launch_csr_spmv(n, brp, bci, bva, p, 0.0, nullptr, t, s));
+        TAD_HIP(launch_csr_spmv(n, fn->row_ptr.p, fn->col_idx.p, Hd, t, 0.0, nullptr, sv, s));
+        TAD_HIP(launch_csr_spmv(m, btp, btc, btv, sv, o.w_identity, p, q, s));
+        double pq;
+        {
+            const double* a[1] = { p };
+            const double* b[1] = { q };
+            TAD_TRY(dots_n(fn, m, 1, a, b, &pq));
+        }
+        if (!(pq > 0.0)) return fail(TAD_ERR_SOLVE, "B^T H B + w I is not positive definite (CG breakdown)");
+        const double alpha = rho / pq;
+        TAD_HIP(launch_basis_step(m, alpha, p, q, y, r, s));
+        TAD_HIP(launch_basis_scale(m, r, dinv, z, s));
+        ++it;
+        {
+            const double* a[2] = { r, r };
+            const double* b[2] = { r, z };
+            TAD_TRY(dots_n(fn, m, 2, a, b, d2));
+        }
+        rr = d2[0];
+        if (rr <= tol2 * bb)
+        {
+            converged = true;
+            break;
+        }
+        const double beta = d2[1] / rho;
+        rho = d2[1];
+        TAD_HIP(launch_basis_direction(m, z, beta, p, s));
+    }
+    // sol = B y
+    TAD_HIP(launch_csr_spmv(n, brp, bci, bva, y, 0.0, nullptr, out, s));
+    if (mem == TAD_MEM_HOST)
+    {
+        TAD_HIP(hipMemcpyAsync(sol, out, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        TAD_HIP(hipStreamSynchronize(s));
+    }
+    else
+        TAD_HIP(hipMemcpyAsync(sol, out, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (stats)
+    {
+        stats->iters = it;
+        stats->converged = converged ? 1 : 0;
+        stats->rel_residual = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+        stats->coarse_unknowns = 0;
+        stats->coarse_reused = 0;
+    }
+    return TAD_OK;
 }
 
 tad_status tad_newton_default_opts(tad_newton_opts* o)

@@ -226,6 +226,20 @@ hipError_t launch_pcg_iteration(int d, int64_t n_handles, const int32_t* brow_pt
                                 bool with_update, const unsigned char* free_vars, const CoarseDev* coarse,
                                 hipStream_t s, Timer* timer);
 
+// General reduced-basis solves (tad_basis.hip): y = A x (+ w add) for a CSR
+// matrix of nrows rows; dinv = 1 / diag(B^T H B + w I) with B^T given as CSR
+// (flag[0] = 1 where that diagonal is not positive); z = dinv r; y += alpha p,
+// r -= alpha q; p = z + beta p.
+hipError_t launch_csr_spmv(int64_t nrows, const int32_t* row_ptr, const int32_t* col, const double* val,
+                           const double* x, double w, const double* add, double* y, hipStream_t s);
+hipError_t launch_basis_diag(int64_t m, const int32_t* bt_ptr, const int32_t* bt_col, const double* bt_val,
+                             const int32_t* h_row_ptr, const int32_t* h_col, const double* H, double w, double* dinv,
+                             int* flag, hipStream_t s);
+hipError_t launch_basis_scale(int64_t m, const double* r, const double* dinv, double* z, hipStream_t s);
+hipError_t launch_basis_step(int64_t m, double alpha, const double* p, const double* q, double* y, double* r,
+                             hipStream_t s);
+hipError_t launch_basis_direction(int64_t m, const double* z, double beta, double* p, hipStream_t s);
+
 // Size (doubles) of the partial-sum buffer the PCG kernels need.
 int64_t pcg_parts_size(int d, int64_t n_handles);
 
