@@ -206,6 +206,44 @@ def test_inverted_elements(ctx):
     assert nrel(H2.toarray(), H1.toarray()) <= FP64_TOL
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("kind", ["tets", "triangles"])
+def test_grouped_and_per_element_sums_agree(ctx, precision, kind):
+    """Hessian blocks summed per wave of 64 consecutive elements first (the
+    elements in mesh order: few distinct blocks per group) and one slot per
+    element block (the same elements shuffled: too many distinct blocks per
+    group, so the term keeps per-element slots) give the same f, g and H up to
+    summation order, and both match the oracle (ScalarFunctionImpl.hh:378-416)."""
+    import dataclasses
+    if kind == "tets":
+        P = configs.twist_cube(m=8, seed=5)
+    else:
+        P, _ = configs.concat(configs.disk_batch(count=8, seed=5, n_min=20, n_max=40))
+    rng = np.random.default_rng(3)
+    perm = rng.permutation(P.terms[0].conn.shape[0])
+    t0 = P.terms[0]
+    Q = dataclasses.replace(P, terms=[dataclasses.replace(t0, conn=np.ascontiguousarray(t0.conn[perm]),
+                                                          params=np.ascontiguousarray(t0.params[perm]))]
+                            + list(P.terms[1:]))
+    fa = tad.ScalarFunction.from_problem(P, ctx, precision=precision)
+    fb = tad.ScalarFunction.from_problem(Q, ctx, precision=precision)
+    ra = fa.eval_with_hessian_proj(P.x0)
+    rb = fb.eval_with_hessian_proj(P.x0)
+    tol = 1e-12 if precision == "fp64" else 2e-6
+    assert abs(ra[0] - rb[0]) <= 1e-12 * abs(ra[0])
+    assert nrel(ra[1], rb[1]) <= tol and nrel(ra[2].data, rb[2].data) <= tol
+    fo = oracle.Function.from_problem(P)
+    fo_, go_, Ho_ = fo.eval_with_hessian_proj(P.x0)
+    bar = FP64_TOL if precision == "fp64" else 2e-5
+    for r in (ra, rb):
+        dH = abs(r[2] - Ho_).max() / abs(Ho_).max()
+        assert nrel(r[1], go_) <= bar and dH <= bar, (dH, nrel(r[1], go_))
+    for shape in range(4):  # every assembly shape sums the group partials in the same order
+        fa.assembly_shape = shape
+        f2, g2, H2 = fa.eval_with_hessian_proj(P.x0)
+        assert f2 == ra[0] and np.array_equal(g2, ra[1]) and np.array_equal(H2.data, ra[2].data)
+
+
 def test_deterministic_bits(ctx):
     P = perturbed_cube(4, 0.2, 21)
     fg = tad.ScalarFunction.from_problem(P, ctx)

@@ -347,6 +347,12 @@ void free_pattern(tad_fn fn)
         if (t.gpos) (void)hipFree(t.gpos);
         if (t.hpos) (void)hipFree(t.hpos);
         t.gpos = t.hpos = nullptr;
+        if (t.gpass) (void)hipFree(t.gpass);
+        if (t.gout_ptr) (void)hipFree(t.gout_ptr);
+        if (t.gout) (void)hipFree(t.gout);
+        t.gpass = nullptr;
+        t.gout_ptr = t.gout = nullptr;
+        t.grouped = t.group_max = 0;
     }
     fn->pattern_ready = false;
     fn->jrow_ptr.release(); fn->jcol.release();
@@ -413,15 +419,105 @@ tad_status build_pattern(tad_fn fn)
     //    the transpose), and a lower block is assembled from its mirror's run.
     //    Gradient runs per handle.  Per element the slot of each block and
     //    gradient part (SoA, coalesced in the element kernel).
+    //    Catalog terms are summed per group of kElemGroup consecutive
+    //    elements first (in LDS, by the element kernel): such a term has one
+    //    slot per distinct upper block of each group.  Within a group the
+    //    blocks are added round by round (local block (a, b) in order) and
+    //    within a round in passes, so that no two lanes of the wave add to
+    //    the same accumulator at once; the plan below numbers each group's
+    //    blocks and each contribution's pass.  A term whose groups touch more
+    //    than kGroupMaxBlocks blocks keeps one slot per element block.
+    const size_t nterms = fn->hterms.size();
+    std::vector<char> grouped(nterms, 0);
+    std::vector<int> gmax(nterms, 0);
+    std::vector<std::vector<uint32_t>> hcode(nterms), gofs(nterms);
+    std::vector<std::vector<uint8_t>> gpass(nterms);
+    std::vector<std::vector<int32_t>> gblock(nterms);  // per group slot: its upper CSR block
+    {
+        std::vector<int64_t> stamp(nb, -1), rstamp(nb, -1);
+        std::vector<int32_t> gsof(nb, 0), rcnt(nb, 0);
+        int64_t gid = 0;
+        for (size_t ti = 0; ti < nterms; ++ti)
+        {
+            const HostTerm& t = fn->hterms[ti];
+            if (fn->terms[ti].dev_ops.element || t.N < 2 || t.n_elem == 0) continue;
+            // FP32 contributions of tets keep one slot per element block: the
+            // pass loops of the grouped sums raise the FP32 tet kernel from
+            // 152 to 191 registers (3 -> 2 waves per SIMD), which costs more
+            // (element kernel 0.51 -> 0.73 ms on the 60^3 cube) than the
+            // shorter assembly saves; FP64 tets (256 registers either way) and
+            // FP32 triangles gain
+            if (fn->precision != TAD_PREC_FP64 && t.N * fn->d >= 12) continue;
+            const int64_t ne = t.n_elem;
+            const int NB = t.N * (t.N + 1) / 2;
+            const int64_t ng = (ne + kElemGroup - 1) / kElemGroup;
+            std::vector<uint32_t> code((size_t)ne * NB);
+            std::vector<uint8_t> np_((size_t)ng * NB, 0);
+            std::vector<uint32_t> go((size_t)ng + 1, 0);
+            std::vector<int32_t> gb;
+            bool ok = true;
+            int gm = 0;
+            for (int64_t q = 0; q < ng && ok; ++q, ++gid)
+            {
+                const int64_t e0 = q * kElemGroup, e1 = std::min<int64_t>(ne, e0 + kElemGroup);
+                const size_t g0 = gb.size();
+                int bi = 0;
+                for (int a = 0; a < t.N; ++a)
+                    for (int b = a; b < t.N; ++b, ++bi)
+                    {
+                        const int64_t rkey = gid * 16 + bi;
+                        int maxp = -1;
+                        for (int64_t e = e0; e < e1; ++e)
+                        {
+                            const int32_t u = t.conn[e * t.N + a], v = t.conn[e * t.N + b];
+                            const int64_t kb = find_block(std::min(u, v), std::max(u, v));
+                            if (stamp[kb] != gid)
+                            {
+                                stamp[kb] = gid;
+                                gsof[kb] = (int32_t)(gb.size() - g0);
+                                gb.push_back((int32_t)kb);
+                            }
+                            if (rstamp[kb] != rkey)
+                            {
+                                rstamp[kb] = rkey;
+                                rcnt[kb] = 0;
+                            }
+                            const int pass = rcnt[kb]++;
+                            maxp = std::max(maxp, pass);
+                            code[(size_t)bi * ne + e] =
+                                (uint32_t)gsof[kb] | ((uint32_t)(pass & 0xff) << 16) | (u > v ? 0x80000000u : 0u);
+                        }
+                        if (maxp > 255) ok = false;
+                        np_[(size_t)q * NB + bi] = (uint8_t)(maxp + 1);
+                    }
+                const int G = (int)(gb.size() - g0);
+                if (G > kGroupMaxBlocks) ok = false;
+                gm = std::max(gm, G);
+                go[q + 1] = (uint32_t)gb.size();
+            }
+            if (!ok) continue;
+            grouped[ti] = 1;
+            gmax[ti] = gm;
+            hcode[ti] = std::move(code);
+            gpass[ti] = std::move(np_);
+            gofs[ti] = std::move(go);
+            gblock[ti] = std::move(gb);
+        }
+    }
     std::vector<int64_t> hptr(nb + 1, 0);
     std::vector<int64_t> gptr(nh + 1, 0);
-    for (const HostTerm& t : fn->hterms)
+    for (size_t ti = 0; ti < nterms; ++ti)
+    {
+        const HostTerm& t = fn->hterms[ti];
+        if (grouped[ti])
+            for (int32_t kb : gblock[ti]) hptr[kb + 1] += 1;
         for (int64_t e = 0; e < t.n_elem; ++e)
             for (int a = 0; a < t.N; ++a)
             {
                 const int32_t u = t.conn[e * t.N + a];
                 if (u < 0) continue;
                 gptr[u + 1] += 1;
+                if (grouped[ti]) continue;
                 for (int b = a; b < t.N; ++b)
                 {
                     const int32_t v = t.conn[e * t.N + b];
@@ -429,6 +525,7 @@ tad_status build_pattern(tad_fn fn)
                     hptr[find_block(std::min(u, v), std::max(u, v)) + 1] += 1;
                 }
             }
+    }
     for (int64_t k = 0; k < nb; ++k) hptr[k + 1] += hptr[k];
     for (int64_t u = 0; u < nh; ++u) gptr[u + 1] += gptr[u];
     // slot 2^31 - 1 (with the transpose flag: kNoSlotHost) stays unused
@@ -438,21 +535,30 @@ tad_status build_pattern(tad_fn fn)
     for (int64_t u = 0; u < nh; ++u)
         for (int32_t k = brow_ptr[u]; k < brow_ptr[u + 1]; ++k)
             mirror[k] = bcol[k] >= u ? k : (int32_t)find_block(bcol[k], (int32_t)u);
-    std::vector<std::vector<uint32_t>> gpos(fn->hterms.size()), hpos(fn->hterms.size());
+    std::vector<std::vector<uint32_t>> gpos(nterms), hpos(nterms), gout(nterms);
     {
         std::vector<int64_t> hc(hptr.begin(), hptr.end() - 1), gc(gptr.begin(), gptr.end() - 1);
-        for (size_t ti = 0; ti < fn->hterms.size(); ++ti)
+        for (size_t ti = 0; ti < nterms; ++ti)
         {
             const HostTerm& t = fn->hterms[ti];
             const int64_t ne = t.n_elem;
             const int NB = t.N * (t.N + 1) / 2;
             gpos[ti].resize((size_t)ne * t.N);
-            hpos[ti].resize((size_t)ne * NB);
+            if (grouped[ti])
+            {
+                // group slots in group order, each group's in its own order
+                gout[ti].resize(gblock[ti].size());
+                for (size_t i = 0; i < gblock[ti].size(); ++i) gout[ti][i] = (uint32_t)hc[gblock[ti][i]]++;
+                hpos[ti] = std::move(hcode[ti]);
+            }
+            else
+                hpos[ti].resize((size_t)ne * NB);
             for (int64_t e = 0; e < ne; ++e)
                 for (int a = 0; a < t.N; ++a)
                 {
                     const int32_t u = t.conn[e * t.N + a];
                     gpos[ti][a * ne + e] = u >= 0 ? (uint32_t)gc[u]++ : kNoSlotHost;
+                    if (grouped[ti]) continue;
                     for (int b = a; b < t.N; ++b)
                     {
                         const int32_t v = t.conn[e * t.N + b];
@@ -563,6 +669,19 @@ tad_status build_pattern(tad_fn fn)
         TAD_HIP(hipMalloc((void**)&td.hpos, hpos[ti].size() * sizeof(uint32_t)));
         TAD_HIP(hipMemcpyAsync(td.gpos, gpos[ti].data(), gpos[ti].size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
         TAD_HIP(hipMemcpyAsync(td.hpos, hpos[ti].data(), hpos[ti].size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        td.grouped = grouped[ti];
+        td.group_max = gmax[ti];
+        if (grouped[ti])
+        {
+            TAD_HIP(hipMalloc((void**)&td.gpass, gpass[ti].size()));
+            TAD_HIP(hipMalloc((void**)&td.gout_ptr, gofs[ti].size() * sizeof(uint32_t)));
+            TAD_HIP(hipMalloc((void**)&td.gout, std::max<size_t>(1, gout[ti].size()) * sizeof(uint32_t)));
+            TAD_HIP(hipMemcpyAsync(td.gpass, gpass[ti].data(), gpass[ti].size(), hipMemcpyHostToDevice, s));
+            TAD_HIP(hipMemcpyAsync(td.gout_ptr, gofs[ti].data(), gofs[ti].size() * sizeof(uint32_t),
+                                   hipMemcpyHostToDevice, s));
+            TAD_HIP(hipMemcpyAsync(td.gout, gout[ti].data(), gout[ti].size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                   s));
+        }
     }
     TAD_HIP(hipStreamSynchronize(s));
     fn->h_brow_ptr = std::move(brow_ptr);

@@ -21,6 +21,19 @@ struct TermDev
     uint32_t* hpos = nullptr;  // SoA [N(N+1)/2][n_elem]: slot of upper block (a <= b) of element e;
                                // top bit: the slot's block is (b, a), write the transpose
     int64_t f_base = 0;        // first entry of this term in the element-energy array
+    // Hessian blocks summed per group of kElemGroup consecutive elements (one
+    // wave of the element kernel) in LDS before they are written (grouped
+    // catalog terms): hpos then holds, per (block a <= b, element), the
+    // group slot (bits 0-15), the pass of the sum it joins (bits 16-23) and
+    // the transpose flag (bit 31); gpass[q * N(N+1)/2 + block] the passes of
+    // group q for that block; group q's slots are gout[gout_ptr[q] ..
+    // gout_ptr[q+1]) (contribution slots in Hc); group_max the most slots of
+    // one group.  Gradient parts keep one slot per element (gpos).
+    int grouped = 0;
+    int group_max = 0;
+    uint8_t* gpass = nullptr;
+    uint32_t* gout_ptr = nullptr;
+    uint32_t* gout = nullptr;
     // launch shape of the projected-Hessian element kernel chosen by the
     // autotuner: tune_block = 0 is the plain one-tile-per-workgroup grid,
     // otherwise a persistent grid of tune_grid workgroups of tune_block lanes
@@ -38,6 +51,12 @@ struct TermDev
     int64_t jv_base = 0;
     int (*residuals)(const tad_device_residual_args*) = nullptr;
 };
+
+// Elements per group of a grouped term (one wave), and the most distinct
+// upper blocks a group may have (its LDS accumulators); terms with a larger
+// group keep one slot per element block.
+constexpr int kElemGroup = 64;
+constexpr int kGroupMaxBlocks = 224;  // 4 waves x 224 x 9 doubles < 64 KiB of LDS
 
 // Solver bookkeeping written by block 0 of the PCG kernels (see tad_linalg.hip).
 struct PcgScalars
