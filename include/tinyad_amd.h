@@ -123,8 +123,13 @@ tad_status tad_fn_destroy(tad_fn fn);
 /* add_elements<N>(range(n_elem), energy) (ScalarFunctionImpl.hh:63-100).
  * conn: n_elem x N handle indices (row-major, host memory), local variable order
  * = connectivity order (first-access order of Element::variables, Element.hh:208-260).
- * params: n_elem x params_per_elem (row-major, host memory).  Elements may not
- * reference the same handle twice.  Invalidates the Hessian pattern. */
+ * params: n_elem x params_per_elem (row-major, host memory).  A quadratic
+ * element may list a handle more than once: as Element::variables gives a
+ * handle met again the local variables of its first access (Element.hh:221-243),
+ * the repeated entries share that handle's variables, their derivatives add up
+ * and the merged element Hessian is what gets projected.  Symmetric Dirichlet
+ * elements with a repeated vertex (degenerate simplices) are rejected.
+ * Invalidates the Hessian pattern. */
 tad_status tad_fn_add_elements(tad_fn fn, tad_energy energy, int valence, int64_t n_elem,
                                const int32_t* conn, const double* params, int params_per_elem);
 
@@ -139,7 +144,7 @@ tad_status tad_fn_add_elements(tad_fn fn, tad_energy energy, int valence, int64_
  * tad_device_ops with host functions that launch them.  The library owns the
  * pattern, the scatter maps, assembly, solve and line search as for its own
  * energies, and calls these functions with the arguments below. */
-#define TAD_DEVICE_OPS_VERSION 1
+#define TAD_DEVICE_OPS_VERSION 2
 
 typedef struct tad_device_element_args
 {
@@ -189,7 +194,10 @@ typedef struct tad_device_ops
  * valence) and params (n_elem x n_params), row-major host memory, as for
  * tad_fn_add_elements.  ops is copied; its functions must stay loaded while the
  * function exists.  Terms added this way take the plain one-lane-per-element
- * launch (no autotuned shapes, no rest-shape products). */
+ * launch (no autotuned shapes, no rest-shape products).  An element may list a
+ * handle more than once (merged as for quadratic elements): the slot maps give
+ * such a repeated entry no slot (tad::kNoSlot), and the element kernels of
+ * DeviceElements.hh fold its derivatives into the first occurrence. */
 tad_status tad_fn_add_elements_device(tad_fn fn, const tad_device_ops* ops, int64_t n_elem, const int32_t* conn,
                                       const double* params);
 
@@ -257,7 +265,8 @@ tad_status tad_fn_create_vector(tad_ctx ctx, int d, int64_t n_handles, tad_preci
 
 /* add_elements<N, M>(range(n_elem), lambda) (VectorFunctionImpl.hh:58-90):
  * conn (n_elem x valence) and params (n_elem x n_params), row-major host
- * memory; the element's residuals follow those of the terms added before. */
+ * memory; the element's residuals follow those of the terms added before.
+ * Residual elements must list distinct handles (one Jacobian column per entry). */
 tad_status tad_fn_add_residuals_device(tad_fn fn, const tad_device_residual_ops* ops, int64_t n_elem,
                                        const int32_t* conn, const double* params);
 

@@ -59,6 +59,10 @@ struct DeviceElement
     // elements of dynamic-valence terms end in padding slots (handle -1):
     // their coordinates read as 0 and their derivatives are not stored
     static constexpr bool padded = true;
+    // an element may list a handle more than once: its entries are merged
+    // into the first occurrence (one set of variables per handle, as the
+    // reference's Element::variables gives) before the projection
+    static constexpr bool merges = true;
     static_assert(NN >= 1 && DD >= 1 && DD <= 3 && K <= 12, "valence * d must be in 1..12, d in 1..3");
     static_assert(NPAR >= 0, "negative parameter count");
 

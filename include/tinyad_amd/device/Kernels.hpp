@@ -31,6 +31,18 @@ struct padded_slots<E, std::void_t<decltype(E::padded)>> : std::bool_constant<E:
 {
 };
 
+// E::merges (optional, default false): an element may list a handle more than
+// once; its entries are merged into the first occurrence before projection
+// (dense records only) and only first occurrences get slots.
+template <class E, class = void>
+struct merged_slots : std::false_type
+{
+};
+template <class E>
+struct merged_slots<E, std::void_t<decltype(E::merges)>> : std::bool_constant<E::merges>
+{
+};
+
 // ---------------------------------------------------------------------------
 // Element kernel
 // ---------------------------------------------------------------------------
@@ -50,12 +62,15 @@ __device__ __forceinline__ void element_body(int64_t e, int64_t n, const int32_t
                                              double eps, double* __restrict__ elem_f, const ElemOut<T>& o,
                                              int* __restrict__ err)
 {
+    constexpr bool kMerge = merged_slots<E>::value && E::N > 1;
     double X[E::N * E::D];
     int n_active = 0;  // handles before the padding (padded terms)
+    int32_t h[E::N];
 #pragma unroll
     for (int a = 0; a < E::N; ++a)
     {
         const int64_t v = conn[a * n + e];
+        h[a] = (int32_t)v;
         if constexpr (padded_slots<E>::value)
         {
             n_active += v >= 0 ? 1 : 0;
@@ -72,7 +87,24 @@ __device__ __forceinline__ void element_body(int64_t e, int64_t n, const int32_t
 #pragma unroll
     for (int k = 0; k < E::NP; ++k) p[k] = params[k * n + e];
     bool nf = false;
-    const ScatterSink<E::N, E::D, T, padded_slots<E>::value> out{o.G, o.Hc, o.gpos, o.hpos, e, n};
+    ScatterSink<E::N, E::D, T, padded_slots<E>::value || kMerge> out{o.G, o.Hc, o.gpos, o.hpos, e, n};
+    if constexpr (kMerge)
+    {
+        // entry a repeats entry r < a (the first with its handle; padding
+        // entries repeat nothing)
+#pragma unroll
+        for (int a = 1; a < E::N; ++a)
+        {
+            int r = a;
+#pragma unroll
+            for (int b = a - 1; b >= 0; --b) r = (h[b] == h[a] && h[a] >= 0) ? b : r;
+            if (r != a)
+            {
+                out.dups |= 1u << a;
+                out.reps |= (uint64_t)r << (4 * a);
+            }
+        }
+    }
     if constexpr (padded_slots<E>::value)
         elem_f[e] = E::template eval<T, MODE>(X, p, T(eps), out, nf, n_active);
     else

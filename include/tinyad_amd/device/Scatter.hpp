@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 #include "Dual.hpp"
 #include "Modes.hpp"
 #include "Projection.hpp"
@@ -59,9 +60,15 @@ __device__ __forceinline__ void store_run(float* __restrict__ dst, const float* 
 // block) and, within a destination, by term and element, so the assembly sums
 // contiguous runs in the reference's element order.
 // Slot of a padding entry (handle -1) of an element of a dynamic-valence
-// term: nothing is written for it.
+// term, and of every entry of a handle the element lists again after its
+// first occurrence: nothing is written for it.
 constexpr uint32_t kNoSlot = 0xffffffffu;
 
+// dups: bit a set when entry a of the element repeats the handle of an
+// earlier entry; reps: 4 bits per entry, the first entry with that handle.
+// write_dense_record merges such entries into their first occurrence before
+// projecting (the reference gives a handle one set of local variables however
+// often the element asks for it, Element.hh:208-243).
 template <int N, int Dm, class T, bool Padded = false>
 struct ScatterSink
 {
@@ -70,6 +77,8 @@ struct ScatterSink
     const uint32_t* __restrict__ gpos;
     const uint32_t* __restrict__ hpos;
     int64_t e, n;
+    uint32_t dups = 0;
+    uint64_t reps = 0;
 
     __device__ __forceinline__ void grad(int v, const T* g) const
     {
@@ -162,12 +171,85 @@ __device__ __forceinline__ void write_blocks(const T* Hp, const Sink& out, bool&
         }
 }
 
+template <class S, class = void>
+struct sink_merges : std::false_type
+{
+};
+template <class S>
+struct sink_merges<S, std::void_t<decltype(S::dups)>> : std::true_type
+{
+};
+
+// Merges entry a into entry b (b < a, same handle) in place: g_b += g_a and,
+// with HESS, the congruence with the matrix that adds variable block a to
+// block b (rows and columns of b gain those of a, the diagonal block gains
+// H_ba + H_ab + H_aa); entry a is left with zero gradient, rows and columns.
+template <int N, int Dm, class T, bool HESS, int a, int b>
+__device__ __forceinline__ void merge_entry(T* g, T* Hp)
+{
+    constexpr int K = N * Dm;
+#pragma unroll
+    for (int i = 0; i < Dm; ++i)
+    {
+        g[b * Dm + i] += g[a * Dm + i];
+        g[a * Dm + i] = T(0);
+    }
+    if constexpr (HESS)
+    {
+#pragma unroll
+        for (int i = 0; i < Dm; ++i)
+#pragma unroll
+            for (int k = i; k < Dm; ++k)
+                Hp[Sym<K>::idx(b * Dm + i, b * Dm + k)] += Hp[Sym<K>::at(b * Dm + i, a * Dm + k)] +
+                                                           Hp[Sym<K>::at(a * Dm + i, b * Dm + k)] +
+                                                           Hp[Sym<K>::at(a * Dm + i, a * Dm + k)];
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+        {
+            if (j / Dm == a || j / Dm == b) continue;
+#pragma unroll
+            for (int i = 0; i < Dm; ++i) Hp[Sym<K>::at(b * Dm + i, j)] += Hp[Sym<K>::at(a * Dm + i, j)];
+        }
+#pragma unroll
+        for (int i = 0; i < Dm; ++i)
+#pragma unroll
+            for (int j = 0; j < K; ++j) Hp[Sym<K>::at(a * Dm + i, j)] = T(0);
+    }
+}
+
+template <int N, int Dm, class T, bool HESS, int a, int b>
+__device__ __forceinline__ void merge_pairs(T* g, T* Hp, uint32_t dups, uint64_t reps)
+{
+    if constexpr (a < N)
+    {
+        if constexpr (b < a)
+        {
+            if (((dups >> a) & 1u) && (int)((reps >> (4 * a)) & 15u) == b) merge_entry<N, Dm, T, HESS, a, b>(g, Hp);
+            merge_pairs<N, Dm, T, HESS, a, b + 1>(g, Hp, dups, reps);
+        }
+        else
+            merge_pairs<N, Dm, T, HESS, a + 1, 0>(g, Hp, dups, reps);
+    }
+}
+
+// Repeated entries folded into their first occurrence, in entry order, so the
+// record is the reference's element record up to the order of its local
+// variables (the repeated entries' rows and columns stay as zeros, as the
+// reference's unused trailing local variables do).
+template <int N, int Dm, class T, bool HESS>
+__device__ __forceinline__ void merge_repeated(T* g, T* Hp, uint32_t dups, uint64_t reps)
+{
+    merge_pairs<N, Dm, T, HESS, 1, 0>(g, Hp, dups, reps);
+}
+
 // Generic small dense terms (penalty, quadratic): full K x K Hessian in
 // packed form, projected without deflation.
 template <int N, int Dm, class T, int MODE, class Sink>
-__device__ __forceinline__ void write_dense_record(const T* g, T* Hp, T eps, const Sink& out, bool& nonfinite)
+__device__ __forceinline__ void write_dense_record(T* g, T* Hp, T eps, const Sink& out, bool& nonfinite)
 {
     constexpr int K = N * Dm;
+    if constexpr (N > 1 && sink_merges<Sink>::value)
+        if (out.dups) merge_repeated<N, Dm, T, (MODE >= MODE_HESS)>(g, Hp, out.dups, out.reps);
 #pragma unroll
     for (int i = 0; i < K; ++i) nonfinite = nonfinite || !finite_val(g[i]);
 #pragma unroll

@@ -439,21 +439,38 @@ struct ElementResult
     double grad[12];
     double hess[144];
     int idx[12];
+    int n_loc;  // local variables in use (idx_local_to_global.size())
 };
 
+// Element::variables (Element.hh:208-260): the element's handles in access
+// (= connectivity) order; a handle met again reuses the local variables of
+// its first access, so its derivatives add up there, and local variables
+// past the last new handle stay unused (zero rows and columns of the k x k
+// Hessian, which the projection sees, ScalarObjectiveTerm.hh:242-249; only
+// the used ones are scattered, :259-276).
 template <int K>
 void element_second_order(const Function& fn, const Term& t, long e, const double* x,
                           bool project, double eps, ElementResult& out)
 {
     const int d = fn.d;
     Dual<K> v[K];
+    int n_loc = 0;
     for (int a = 0; a < t.N; ++a)
-        for (int i = 0; i < d; ++i)
+    {
+        const long g0 = (long)d * t.conn[e * t.N + a];
+        int start = -1;  // local variables already given to this handle
+        for (int c = 0; c < n_loc && start < 0; c += d)
+            if (out.idx[c] == g0) start = c;
+        if (start < 0)
         {
-            const long gi = (long)d * t.conn[e * t.N + a] + i;
-            v[a * d + i] = Dual<K>(x[gi], a * d + i);
-            out.idx[a * d + i] = (int)gi;
+            start = n_loc;
+            for (int i = 0; i < d; ++i) out.idx[n_loc + i] = (int)(g0 + i);
+            n_loc += d;
         }
+        for (int i = 0; i < d; ++i) v[a * d + i] = Dual<K>(x[g0 + i], start + i);
+    }
+    out.n_loc = n_loc;
+    for (int i = n_loc; i < K; ++i) out.idx[i] = -1;  // unused local variables
     const Dual<K> r = eval_energy<Dual<K>>(t.energy, v, &t.params[e * t.n_params], std::min(d, K), t.N);
     out.val = r.val;
     for (int i = 0; i < K; ++i) out.grad[i] = r.grad[i];
@@ -539,10 +556,10 @@ inline void eval_second_order(const Function& fn, const double* x, bool project,
         {
             const ElementResult& r = res[e];
             f += r.val;
-            for (int i = 0; i < t.K; ++i) g[r.idx[i]] += r.grad[i];
+            for (int i = 0; i < r.n_loc; ++i) g[r.idx[i]] += r.grad[i];
             if (want_hessian)
-                for (int i = 0; i < t.K; ++i)
-                    for (int j = 0; j < t.K; ++j)
+                for (int i = 0; i < r.n_loc; ++i)
+                    for (int j = 0; j < r.n_loc; ++j)
                         trip.push_back({ r.idx[i], r.idx[j], r.hess[i * t.K + j] });
         }
     }
@@ -1010,7 +1027,8 @@ void or_get_csr(void* fnp, int* row_ptr, int* col_idx, double* values)
     std::copy(fn.values.begin(), fn.values.end(), values);
 }
 
-// Dense per-element result: val, grad[K], hess[K*K], global idx[K].
+// Dense per-element result: val, grad[K], hess[K*K], global idx[K] (-1 for
+// local variables the element leaves unused).
 int or_element(void* fnp, int term, long e, const double* x, int project, double eps,
                double* val, double* grad, double* hess, int* idx)
 {

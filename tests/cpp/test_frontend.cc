@@ -102,6 +102,26 @@ static bool test_2d_nonconvex()
     return ok;
 }
 
+// ScalarFunctionTest.cc:153-178: an element that asks for the same handle twice
+// gets one set of variables for it.  Here a valence-2 quadratic element lists
+// handle 0 twice: E = (|x_0|^2 + |x_0|^2) / 2, so f = |x_0|^2, g_0 = 2 x_0,
+// H_00 = 2 I, and handle 1 stays out of the pattern.
+static bool test_element_variables()
+{
+    bool ok = true;
+    auto func = tinyad_amd::scalar_function<2>(tinyad_amd::range(2));
+    func.add_elements<2>(tinyad_amd::range(1), TAD_E_QUADRATIC, [](int) { return std::array<int, 2>{ 0, 0 }; },
+                         [](int) {
+                             return quad_params({ 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1 }, { 0, 0, 0, 0 }, 0);
+                         });
+    VectorXd x = { 1.0, 2.0, 0.0, 0.0 };
+    auto [f, g, H] = func.eval_with_hessian_proj(x);
+    EXPECT(f == 5.0 && g[0] == 2.0 && g[1] == 4.0 && g[2] == 0.0 && g[3] == 0.0);
+    EXPECT(H.nonZeros() == 4 && H.coeff(0, 0) == 2.0 && H.coeff(1, 1) == 2.0 && H.coeff(0, 1) == 0.0);
+    EXPECT(func.eval(x) == 5.0);
+    return ok;
+}
+
 // tests/NewtonTest.cc:12-95 on the 6-vertex planar fixture (tests/Meshes.hh:12-42)
 // NewtonTest.cc:14-80; the triangle energy from the catalog, or with ops
 // given, from a device functor written like the reference's lambda
@@ -560,6 +580,7 @@ int main()
     const Case cases[] = {
         { "ScalarFunctionTest.Basic+Move", test_basic },
         { "ScalarFunctionTest.2DNonConvex", test_2d_nonconvex },
+        { "ScalarFunctionTest.ElementVariables", test_element_variables },
         { "NewtonTest.2DDeformation", []() { return test_newton(nullptr); } },
         { "NewtonTest.Deterministic", test_newton_deterministic },
         { "NewtonTest.2DDeformation.DeviceFunctor", []() { return test_newton(nullptr, tad_test_symdir_tri_ops()); } },

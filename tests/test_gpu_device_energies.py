@@ -17,7 +17,10 @@ tests/cpp/bin/libtad_test_device_energies.so.  Pins:
     both derivative paths, against their closed forms;
   * mixed precision within the tolerance of the catalog's mixed mode, also
     for meshes translated far from the origin;
-  * a Newton trajectory equal to the catalog's to the converged-energy bar.
+  * a Newton trajectory equal to the catalog's to the converged-energy bar;
+  * elements that list a handle more than once (merged into one set of
+    variables before the projection, Element.hh:208-243) against the oracle,
+    closed forms and an explicit merge of distinct-handle derivatives.
 """
 import ctypes
 import os
@@ -327,8 +330,8 @@ def test_device_ops_validation(ctx, plugin):
     fn2 = tad.ScalarFunction(2, 5, ctx)
     with pytest.raises(tad.TadError):  # parameter row of the wrong length
         fn2.add_elements_device(plugin["symdir_tri"], [[0, 1, 2]], np.zeros(4))
-    with pytest.raises(tad.TadError):  # repeated handle
-        fn2.add_elements_device(plugin["symdir_tri"], [[0, 1, 1]], np.zeros(5))
+    with pytest.raises(tad.TadError, match="twice"):  # degenerate catalog simplex
+        fn2.add_elements(3, tad.SYMDIR_TRI, [[0, 1, 1]], np.zeros(5))
     bad = _lib.DeviceOps.from_buffer_copy(_lib.DeviceOps.from_address(plugin["symdir_tri"]))
     bad.version = 999
     with pytest.raises(tad.TadError, match="TAD_DEVICE_OPS_VERSION"):
@@ -460,8 +463,6 @@ def test_dynamic_validation(ctx, plugin):
         fn.add_elements_dynamic_device([ops[0], ops[0]], [[0, 1]])
     with pytest.raises(tad.TadError, match="out of range"):
         fn.add_elements_dynamic_device(ops, [[0, -1]])
-    with pytest.raises(tad.TadError, match="twice"):
-        fn.add_elements_dynamic_device(ops, [[1], [2, 2]])
     assert fn.terms() == []  # failed calls add nothing
 
 
@@ -518,3 +519,148 @@ def test_device_tet_mixed_precision(ctx, plugin):
     fa, ga, Ha = fc.eval_with_derivatives(P.x0)
     fb, gb, Hb = fm.eval_with_derivatives(P.x0)
     assert nrel(gb, ga) <= 2e-5 and nrel(Hb.data, Ha.data) <= 2e-5
+
+
+# ---------------------------------------------------------------------------
+# Repeated handles within an element (Element.hh:208-243)
+# ---------------------------------------------------------------------------
+
+def repeated_quad_problem(seed, N, n_handles, n_elem):
+    """Quadratic elements (non-symmetric, indefinite A) of valence N, about half
+    of them listing some handle more than once (adjacent and not, all equal)."""
+    rng = np.random.default_rng(seed)
+    conn = []
+    for e in range(n_elem):
+        c = rng.choice(n_handles, N, replace=False)
+        if e % 2 == 0:
+            a, b = sorted(rng.choice(N, 2, replace=False))
+            c[b] = c[a]
+        if e % 7 == 3:
+            c[:] = c[0]
+        conn.append(c)
+    conn = np.array(conn, dtype=np.int32)
+    K = 2 * N
+    A = rng.standard_normal((n_elem, K, K))
+    b = rng.standard_normal((n_elem, K))
+    c = rng.standard_normal((n_elem, 1))
+    params = np.concatenate([A.reshape(n_elem, K * K), b, c], axis=1)
+    return conn, params, rng.standard_normal(2 * n_handles)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("eps", [1e-9, -1.0, 0.3])
+def test_repeated_handles_match_oracle(ctx, plugin, eps, precision):
+    """Catalog quadratics of valence 3 and device quadratics of valence 2 with
+    repeated handles: the entries of a repeated handle share its variables, the
+    merged element Hessian is projected, and only the distinct handles' blocks
+    are assembled, against the oracle (which restates Element::variables).
+    Same sparsity pattern; FP64 1e-10, FP32 contributions 2e-5 normwise."""
+    conn3, params3, x = repeated_quad_problem(21, 3, 8, 24)
+    conn2, params2, _ = repeated_quad_problem(22, 2, 8, 16)
+    assert any(len(set(r)) < 3 for r in conn3) and any(len(set(r)) == 1 for r in conn2)
+    fn = tad.ScalarFunction(2, 8, ctx, precision)
+    fn.add_elements(3, tad.QUADRATIC, conn3, params3)
+    fn.add_elements_device(plugin["quadratic4"], conn2, params2)
+    fo = oracle.Function(2, 8)
+    fo.add_elements(tad.QUADRATIC, conn3, params3)
+    fo.add_elements(tad.QUADRATIC, conn2, params2)
+    tol = FP64_TOL if precision == "fp64" else 2e-5
+    f1, g1, H1 = fo.eval_with_hessian_proj(x, eps)
+    f2, g2, H2 = fn.eval_with_hessian_proj(x, eps)
+    assert np.array_equal(H2.indptr, H1.indptr) and np.array_equal(H2.indices, H1.indices)
+    assert abs(f2 - f1) <= 1e-12 * abs(f1)
+    assert nrel(g2, g1) <= tol and nrel(H2.data, H1.data) <= tol
+    fa, ga = fn.eval_with_gradient(x)
+    fb, gb, Hb = fn.eval_with_derivatives(x)
+    fo2, go2, Ho2 = fo.eval_with_derivatives(x)
+    assert fa == f2 and fb == f2 and abs(fn.eval(x) - f1) <= 1e-12 * abs(f1)
+    assert nrel(ga, g1) <= tol and nrel(gb, go2) <= tol
+    assert np.array_equal(Hb.indices, Ho2.indices) and nrel(Hb.data, Ho2.data) <= tol
+
+
+def test_repeated_handles_hyper_dual_element(ctx, plugin):
+    """A K = 8 device element (hyper-dual passes) listing handle 0 twice,
+    against the same energy on four distinct handles with handle 3 placed at
+    handle 0 and then merged (M = S^T H S, gradient S^T g); the projection of
+    M padded with two zero rows and columns (the reference's unused local
+    variables) against numpy.  FP64 bars 1e-12."""
+    rng = np.random.default_rng(23)
+    p = np.array([0.7, 0.4])
+    x = rng.uniform(-1.0, 1.0, 6)
+    fr = tad.ScalarFunction(2, 3, ctx)
+    fr.add_elements_device(plugin["smooth8"], [[0, 1, 0, 2]], p)
+    fs = tad.ScalarFunction(2, 4, ctx)
+    fs.add_elements_device(plugin["smooth8"], [[0, 1, 3, 2]], p)
+    xs = np.concatenate([x, x[:2]])
+    _, gs, Hs = fs.eval_with_derivatives(xs)
+    S = np.zeros((8, 6))
+    S[:6, :6] = np.eye(6)
+    S[6:8, 0:2] = np.eye(2)
+    M = S.T @ Hs.toarray() @ S
+    f, g, H = fr.eval_with_derivatives(x)
+    assert abs(f - smooth8_energy(np.concatenate([x[:4], x[:2], x[4:]]), p)) <= 1e-14 * abs(f)
+    assert nrel(g, S.T @ gs) <= 1e-12 and nrel(H.toarray(), M) <= 1e-12
+    for eps in [1e-9, -1.0, 0.5]:
+        _, gp, Hp = fr.eval_with_hessian_proj(x, eps)
+        pad = np.zeros((8, 8))
+        pad[:6, :6] = H.toarray()
+        assert np.array_equal(gp, g)
+        assert nrel(Hp.toarray(), reference_projection(pad, eps)[:6, :6]) <= 1e-12, eps
+
+
+def test_repeated_handles_dynamic_and_newton(ctx, plugin):
+    """Dynamic-valence elements with repeated handles (sum of squares of the
+    listed 2D values, DynamicElementsTest.cc:9-33): [2, 2] is ||2 x_2||^2 and
+    [1, 2, 1] is ||2 x_1 + x_2||^2, in closed form.  Then a Newton step on
+    repeated-handle quadratics that are convex after merging lands on the
+    minimizer of the merged quadratic."""
+    fn = tad.ScalarFunction(2, 3, ctx)
+    fn.add_elements_dynamic_device(dyn_ops(plugin, "sum_squares", 2), [[2, 2], [1, 2, 1]])
+    x = np.array([0.3, -0.2, 0.5, 0.25, -0.4, 0.7])
+    X = x.reshape(3, 2)
+    f, g, H = fn.eval_with_derivatives(x)
+    s = 2 * X[1] + X[2]
+    assert abs(f - (4 * X[2] @ X[2] + s @ s)) <= 1e-15
+    gref = np.zeros((3, 2))
+    gref[2] = 8 * X[2] + 2 * s
+    gref[1] = 4 * s
+    assert np.abs(g - gref.ravel()).max() <= 1e-15
+    Href = np.zeros((6, 6))
+    I2 = np.eye(2)
+    Href[4:6, 4:6] = 8 * I2 + 2 * I2
+    Href[2:4, 2:4] = 8 * I2
+    Href[2:4, 4:6] = Href[4:6, 2:4] = 4 * I2
+    assert np.array_equal(H.toarray(), Href)
+    # convex merged quadratics: A = L L^T + I on distinct-handle blocks of the
+    # merged variables, written on elements that list a handle twice
+    rng = np.random.default_rng(29)
+    n = 6
+    conn, params = [], []
+    for e in range(10):
+        h = rng.choice(n, 2, replace=False)
+        c = np.array([h[0], h[1], h[0]], dtype=np.int32)
+        L = rng.standard_normal((4, 4))
+        Mz = L @ L.T + np.eye(4)  # merged Hessian over (h0, h1)
+        S = np.zeros((6, 4))
+        S[0:2, 0:2] = S[4:6, 0:2] = np.eye(2)
+        S[2:4, 2:4] = np.eye(2)
+        A = np.zeros((6, 6))
+        A[:4, :4] = Mz  # y^T A y = z^T M z when entry 2 repeats entry 0
+        conn.append(c)
+        params.append(np.concatenate([A.ravel(), rng.standard_normal(6), [0.0]]))
+    conn, params = np.array(conn), np.array(params)
+    fq = tad.ScalarFunction(2, n, ctx)
+    fq.add_elements(3, tad.QUADRATIC, conn, params)
+    fo = oracle.Function(2, n)
+    fo.add_elements(tad.QUADRATIC, conn, params)
+    x0 = np.zeros(2 * n)
+    _, g0, H0 = fo.eval_with_derivatives(x0)
+    used = np.unique(conn)
+    cols = np.concatenate([np.arange(2 * v, 2 * v + 2) for v in used])
+    xs = np.zeros(2 * n)
+    xs[cols] = np.linalg.solve(H0.toarray()[np.ix_(cols, cols)], -g0[cols])
+    import torch
+    xt = torch.zeros(2 * n, dtype=torch.float64, device="cuda:0")
+    st = fq.newton_step(xt, tad.newton_default_opts(projection_eps=float("nan"), solve_tol=1e-13))
+    assert st["step"] == 1.0 and st["ls_failed"] == 0
+    assert np.abs(xt.cpu().numpy() - xs).max() <= 1e-8 * max(1.0, np.abs(xs).max())

@@ -621,3 +621,83 @@ def test_reduced_basis_direction_pins():
     d2 = oracle.newton_direction_reduced_basis(g, H, B @ D, 0.0)
     assert np.allclose(d1, d2, rtol=1e-10, atol=1e-12)
     assert np.abs(B.T @ (H @ d1 + g)).max() <= 1e-10 * np.abs(B.T @ g).max()
+
+
+# ---------------------------------------------------------------------------
+# Repeated handles within an element (Element.hh:208-243)
+# ---------------------------------------------------------------------------
+
+def reference_projection(H, eps):
+    """project_positive_definite (HessianProjection.hh:48-101) with numpy:
+    the positive-diagonal-dominance early out, then V clamp(lambda) V^T
+    unless nothing was clamped."""
+    off = np.abs(H).sum(axis=1) - np.abs(np.diag(H))
+    if not np.any(np.diag(H) < off + eps):
+        return H
+    lam, V = np.linalg.eigh(H)
+    lc = np.maximum(lam, eps) if eps >= 0 else np.abs(lam)
+    return H if np.array_equal(lc, lam) else (V * lc) @ V.T
+
+
+def merge_matrix(conn, d):
+    """S with y = S z: element coordinates y (one block per entry) from the
+    element's local variables z (one block per distinct handle, in first-access
+    order); returns S and the distinct handles in that order."""
+    order = list(dict.fromkeys(int(h) for h in conn))
+    S = np.zeros((len(conn) * d, len(order) * d))
+    for a, h in enumerate(conn):
+        j = order.index(int(h))
+        S[a * d:(a + 1) * d, j * d:(j + 1) * d] = np.eye(d)
+    return S, order
+
+
+@pytest.mark.parametrize("eps", [1e-9, -1.0, 0.3])
+def test_repeated_handles_merge_before_projection(eps):
+    """An element that lists a handle twice gives that handle one set of local
+    variables (first-access order), so for a quadratic element E(y) = y^T A y / 2
+    + b^T y + c with y = S z its gradient is S^T (A y + b), its Hessian
+    M = S^T A S, and the projection acts on M padded with zero rows and columns
+    for the unused local variables (ScalarObjectiveTerm.hh:242-276); only the
+    used ones are scattered.  Projecting A first and merging after gives a
+    different matrix, which this case tells apart."""
+    rng = np.random.default_rng(17)
+    d, k = 2, 6
+    Q, _ = np.linalg.qr(rng.standard_normal((k, k)))
+    A = (Q * np.array([-2.0, -0.7, 0.2, 0.9, 1.6, 3.1])) @ Q.T
+    A = 0.5 * (A + A.T)
+    b = rng.standard_normal(k)
+    conn = [2, 0, 2]
+    x = rng.standard_normal(4 * d)
+    S, order = merge_matrix(conn, d)
+    cols = np.concatenate([np.arange(h * d, (h + 1) * d) for h in order])
+    y = np.concatenate([x[h * d:(h + 1) * d] for h in conn])
+    M = S.T @ A @ S
+    pad = np.zeros((k, k))
+    pad[:M.shape[0], :M.shape[0]] = M
+    P_ref = reference_projection(pad, eps)[:M.shape[0], :M.shape[0]]
+    assert np.abs(S.T @ reference_projection(A, eps) @ S - P_ref).max() > 1e-3
+    fo = oracle.Function(d, 4)
+    fo.add_elements(4, [conn], quad_params(A, b, 0.25))
+    f, g, H = fo.eval_with_derivatives(x)
+    assert abs(f - (0.5 * y @ A @ y + b @ y + 0.25)) <= 1e-13
+    g_ref = np.zeros(4 * d)
+    g_ref[cols] = S.T @ (A @ y + b)
+    assert np.abs(g - g_ref).max() <= 1e-13
+    Hd = H.toarray()
+    assert np.abs(Hd[np.ix_(cols, cols)] - M).max() <= 1e-13
+    assert H.nnz == M.size  # only the used local variables are scattered
+    _, gp, Hp = fo.eval_with_hessian_proj(x, eps)
+    assert np.array_equal(gp, g)
+    assert np.abs(Hp.toarray()[np.ix_(cols, cols)] - P_ref).max() <= 1e-12 * np.abs(P_ref).max()
+    _, _, He, idx = fo.element(0, 0, x, k, project=True, eps=eps)
+    assert list(idx) == list(cols) + [-1] * (k - cols.size)
+    # the same function written as one valence-2 element over the distinct
+    # handles: identical unprojected derivatives, equal projection (the
+    # padded rows only add eigenvalues that are dropped)
+    f2 = oracle.Function(d, 4)
+    f2.add_elements(4, [order], quad_params(M, S.T @ b, 0.25))
+    fb, gb, Hb = f2.eval_with_derivatives(x)
+    assert abs(fb - f) <= 1e-13 and np.abs(gb - g).max() <= 1e-13
+    assert np.abs(Hb.toarray() - Hd).max() <= 1e-13
+    _, _, Hbp = f2.eval_with_hessian_proj(x, eps)
+    assert np.abs(Hbp.toarray() - Hp.toarray()).max() <= 1e-12 * np.abs(P_ref).max()

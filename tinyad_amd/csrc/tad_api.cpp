@@ -53,7 +53,7 @@ tad_status fail(tad_status st, const std::string& msg)
         if (s_ != TAD_OK) return s_;                                                               \
     } while (0)
 
-constexpr uint32_t kNoSlotHost = 0xffffffffu;  // slot of a padding entry (tad::kNoSlot)
+constexpr uint32_t kNoSlotHost = 0xffffffffu;  // slot of a padding or repeated entry (tad::kNoSlot)
 
 int energy_param_count(int energy, int d, int N)
 {
@@ -204,6 +204,18 @@ struct HostTerm
     int energy, N, n_params;
     int64_t n_elem;
     std::vector<int32_t> conn;  // row-major n_elem x N
+    bool repeats = false;       // some element lists a handle more than once
+
+    // entry a of element e repeats the handle of an earlier entry (it gets no
+    // slot: the element kernel merges it into the first occurrence)
+    bool repeated(int64_t e, int a) const
+    {
+        if (!repeats) return false;
+        const int32_t* c = conn.data() + e * N;
+        for (int b = 0; b < a; ++b)
+            if (c[b] == c[a]) return true;
+        return false;
+    }
 };
 
 struct tad_fn_s
@@ -440,7 +452,7 @@ tad_status build_pattern(tad_fn fn)
         for (size_t ti = 0; ti < nterms; ++ti)
         {
             const HostTerm& t = fn->hterms[ti];
-            if (fn->terms[ti].dev_ops.element || t.N < 2 || t.n_elem == 0) continue;
+            if (fn->terms[ti].dev_ops.element || t.N < 2 || t.n_elem == 0 || t.repeats) continue;
             // FP32 contributions of tets keep one slot per element block: the
             // pass loops of the grouped sums raise the FP32 tet kernel from
             // 152 to 191 registers (3 -> 2 waves per SIMD), which costs more
@@ -515,13 +527,13 @@ tad_status build_pattern(tad_fn fn)
             for (int a = 0; a < t.N; ++a)
             {
                 const int32_t u = t.conn[e * t.N + a];
-                if (u < 0) continue;
+                if (u < 0 || t.repeated(e, a)) continue;
                 gptr[u + 1] += 1;
                 if (grouped[ti]) continue;
                 for (int b = a; b < t.N; ++b)
                 {
                     const int32_t v = t.conn[e * t.N + b];
-                    if (v < 0) continue;
+                    if (v < 0 || t.repeated(e, b)) continue;
                     hptr[find_block(std::min(u, v), std::max(u, v)) + 1] += 1;
                 }
             }
@@ -557,13 +569,14 @@ tad_status build_pattern(tad_fn fn)
                 for (int a = 0; a < t.N; ++a)
                 {
                     const int32_t u = t.conn[e * t.N + a];
-                    gpos[ti][a * ne + e] = u >= 0 ? (uint32_t)gc[u]++ : kNoSlotHost;
+                    const bool skip_a = u < 0 || t.repeated(e, a);
+                    gpos[ti][a * ne + e] = skip_a ? kNoSlotHost : (uint32_t)gc[u]++;
                     if (grouped[ti]) continue;
                     for (int b = a; b < t.N; ++b)
                     {
                         const int32_t v = t.conn[e * t.N + b];
                         const int bi = a * t.N - (a * (a - 1)) / 2 + (b - a);
-                        if (u < 0 || v < 0)
+                        if (skip_a || v < 0 || t.repeated(e, b))
                         {
                             hpos[ti][(size_t)bi * ne + e] = kNoSlotHost;
                             continue;
@@ -1462,13 +1475,17 @@ tad_status tad_fn_destroy(tad_fn fn)
 }
 
 // Validates and uploads one term (shared by the catalog and device-energy
-// entry points): handle range, no repeated handle, finite parameters.
+// entry points): handle range, repeated handles, finite parameters.
 // allow_pad: elements may end in padding entries -1 (dynamic valences).
+// allow_repeats: an element may list a handle more than once (its entries are
+// merged, as the reference's Element::variables does, Element.hh:208-243);
+// otherwise that is an error.
 static tad_status add_term(tad_fn fn, int energy, int N, int64_t n_elem, const int32_t* conn, const double* params,
-                           int np, const tad_device_ops* ops, bool allow_pad = false)
+                           int np, const tad_device_ops* ops, bool allow_pad, bool allow_repeats)
 {
     if (n_elem > 0 && (!conn || (np > 0 && !params)))
         return fail(TAD_ERR_INVALID, "null connectivity or parameters");
+    bool repeats = false;
     for (int64_t e = 0; e < n_elem; ++e)
         for (int a = 0; a < N; ++a)
         {
@@ -1478,7 +1495,13 @@ static tad_status add_term(tad_fn fn, int energy, int N, int64_t n_elem, const i
                 return fail(TAD_ERR_INVALID, "element " + std::to_string(e) + " references handle out of range");
             for (int b = 0; b < a; ++b)
                 if (conn[e * N + b] == v)
-                    return fail(TAD_ERR_INVALID, "element " + std::to_string(e) + " references a handle twice");
+                {
+                    if (!allow_repeats)
+                        return fail(TAD_ERR_INVALID, "element " + std::to_string(e) +
+                                                         " references a handle twice (symmetric Dirichlet and "
+                                                         "residual elements need distinct handles)");
+                    repeats = true;
+                }
         }
     for (int64_t i = 0; i < n_elem * (int64_t)np; ++i)
         if (!std::isfinite(params[i])) return fail(TAD_ERR_INVALID, "non-finite element parameter");
@@ -1512,6 +1535,7 @@ static tad_status add_term(tad_fn fn, int energy, int N, int64_t n_elem, const i
     h.n_params = np;
     h.n_elem = n_elem;
     h.conn.assign(conn, conn + n_elem * N);
+    h.repeats = repeats;
     fn->terms.push_back(t);
     fn->hterms.push_back(std::move(h));
     fn->n_elem_total += n_elem;
@@ -1543,7 +1567,9 @@ tad_status tad_fn_add_elements(tad_fn fn, tad_energy energy, int valence, int64_
     if (np != params_per_elem)
         return fail(TAD_ERR_INVALID, "energy expects " + std::to_string(np) + " parameters per element, got " +
                                          std::to_string(params_per_elem));
-    return add_term(fn, energy, N, n_elem, conn, params, np, nullptr);
+    // a symmetric Dirichlet simplex with a repeated vertex is degenerate
+    return add_term(fn, energy, N, n_elem, conn, params, np, nullptr, false,
+                    energy == TAD_E_QUADRATIC || energy == TAD_E_PENALTY);
 }
 
 static tad_status check_device_ops(tad_fn fn, const tad_device_ops* ops)
@@ -1598,16 +1624,12 @@ tad_status tad_fn_add_elements_dynamic_device(tad_fn fn, const tad_device_ops* c
     }
     if (n_elem > 0 && elem_ptr[n_elem] > 0 && !handles) return fail(TAD_ERR_INVALID, "null handles");
     if (n_elem > 0 && np > 0 && !params) return fail(TAD_ERR_INVALID, "null parameters");
-    // validate everything before the first term is added
+    // validate everything before the first term is added (a handle may
+    // appear more than once in an element)
     for (int64_t e = 0; e < n_elem; ++e)
         for (int64_t a = elem_ptr[e]; a < elem_ptr[e + 1]; ++a)
-        {
             if (handles[a] < 0 || handles[a] >= fn->n_handles)
                 return fail(TAD_ERR_INVALID, "element " + std::to_string(e) + " references handle out of range");
-            for (int64_t b = elem_ptr[e]; b < a; ++b)
-                if (handles[b] == handles[a])
-                    return fail(TAD_ERR_INVALID, "element " + std::to_string(e) + " references a handle twice");
-        }
     for (int64_t i = 0; i < n_elem * (int64_t)np; ++i)
         if (!std::isfinite(params[i])) return fail(TAD_ERR_INVALID, "non-finite element parameter");
     for (int i = 0; i < n_ops; ++i)
@@ -1625,7 +1647,7 @@ tad_status tad_fn_add_elements_dynamic_device(tad_fn fn, const tad_device_ops* c
             ++ng;
         }
         if (ng == 0) continue;
-        TAD_TRY(add_term(fn, TAD_E_DEVICE, N, ng, conn.data(), ps.data(), np, ops[i], true));
+        TAD_TRY(add_term(fn, TAD_E_DEVICE, N, ng, conn.data(), ps.data(), np, ops[i], true, true));
     }
     return TAD_OK;
 }
@@ -1637,7 +1659,7 @@ tad_status tad_fn_add_elements_device(tad_fn fn, const tad_device_ops* ops, int6
     if (fn->vec) return fail(TAD_ERR_INVALID, "vector functions take residual elements (tad_fn_add_residuals_device)");
     TAD_TRY(check_device_ops(fn, ops));
     if (n_elem < 0) return fail(TAD_ERR_INVALID, "negative element count");
-    return add_term(fn, TAD_E_DEVICE, ops->valence, n_elem, conn, params, ops->n_params, ops);
+    return add_term(fn, TAD_E_DEVICE, ops->valence, n_elem, conn, params, ops->n_params, ops, false, true);
 }
 
 tad_status tad_fn_create_vector(tad_ctx ctx, int d, int64_t n_handles, tad_precision precision, tad_fn* out)
@@ -1670,7 +1692,8 @@ tad_status tad_fn_add_residuals_device(tad_fn fn, const tad_device_residual_ops*
     if ((double)(fn->n_outputs + n_elem * ops->n_outputs) >= 2147483647.0 ||
         (double)(nnz_J + n_elem * ops->n_outputs * K) >= 2147483647.0)
         return fail(TAD_ERR_UNSUPPORTED, "too many residuals or Jacobian entries (int32 CSR indices)");
-    TAD_TRY(add_term(fn, TAD_E_DEVICE, ops->valence, n_elem, conn, params, ops->n_params, &o));
+    // residual Jacobian rows keep one column per element entry
+    TAD_TRY(add_term(fn, TAD_E_DEVICE, ops->valence, n_elem, conn, params, ops->n_params, &o, false, false));
     TermDev& t = fn->terms.back();
     t.n_out = ops->n_outputs;
     t.r_base = fn->n_outputs;

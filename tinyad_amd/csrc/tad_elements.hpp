@@ -43,6 +43,7 @@ struct QuadElem
 {
     static constexpr int N = NN, D = Dm, K = NN * Dm, NP = K * K + K + 1;
     static constexpr bool tunable = false;
+    static constexpr bool merges = true;  // a handle may appear more than once
     template <class T, int MODE, class Sink>
     __device__ static double eval(const double* X, const double* p, T eps, const Sink& out, bool& nf)
     {
