@@ -2563,10 +2563,32 @@ tad_status tad_fn_autotune(tad_fn fn, const double* x, tad_mem mem, double eps, 
     TAD_HIP(hipEventCreate(&guard.e1));
     hipEvent_t e0 = guard.e0, e1 = guard.e1;
     std::string rep;
-    for (TermDev& t : fn->terms)
+    // whole evaluations, three in a row per timing (as a caller's loop runs
+    // them): a launch shape's time in isolation, after an idle device, can
+    // rank shapes differently from their time between the other kernels of
+    // an evaluation
+    auto time_evals = [&](float* out) -> tad_status {
+        float best = 1e30f;
+        for (int rep_i = 0; rep_i < 4; ++rep_i)
+        {
+            double fv = 0.0;
+            TAD_HIP(hipEventRecord(e0, s));
+            for (int b = 0; b < 3; ++b) TAD_TRY(eval_core(fn, xd, MODE_HESS_PROJ, eps, fn->g_buf.p, fn->H_buf.p, &fv));
+            TAD_HIP(hipEventRecord(e1, s));
+            TAD_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            TAD_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (rep_i > 0) best = std::min(best, ms / 3.f);
+        }
+        *out = best;
+        return TAD_OK;
+    };
+    struct Cand { int block, grid, map; double ms; };
+    std::vector<std::vector<Cand>> shortlist(fn->terms.size());
+    for (size_t ti = 0; ti < fn->terms.size(); ++ti)
     {
+        TermDev& t = fn->terms[ti];
         if (!(t.energy == TAD_E_SYMDIR_TRI || t.energy == TAD_E_SYMDIR_TET) || t.n_elem == 0) continue;
-        struct Cand { int block, grid, map; double ms; };
         std::vector<Cand> cands;
         cands.push_back({ 0, 0, 0, 0.0 });
         for (int b : { 64, 128, 256 })
@@ -2604,6 +2626,12 @@ tad_status tad_fn_autotune(tad_fn fn, const double* x, tad_mem mem, double eps, 
         t.tune_block = bestc->block;
         t.tune_grid = bestc->grid;
         t.tune_map = bestc->map;
+        // finalists for the whole-evaluation round: the plain grid and the
+        // two fastest persistent shapes
+        std::vector<Cand> pers(cands.begin() + 1, cands.end());
+        std::sort(pers.begin(), pers.end(), [](const Cand& a, const Cand& b) { return a.ms < b.ms; });
+        shortlist[ti].push_back(cands[0]);
+        for (size_t i = 0; i < pers.size() && i < 2; ++i) shortlist[ti].push_back(pers[i]);
         char line[256];
         std::snprintf(line, sizeof(line), "term energy %d (%lld elements): plain grid %.4f ms; best %s block %d grid %d map %s %.4f ms\n",
                       t.energy, (long long)t.n_elem, cands[0].ms, bestc->block ? "persistent" : "plain", bestc->block,
@@ -2646,6 +2674,37 @@ tad_status tad_fn_autotune(tad_fn fn, const double* x, tad_mem mem, double eps, 
                       "%d lanes per upper block %.4f ms, one thread per upper block %.4f ms\n",
                       fn->d * fn->d, ms_shape[0], ms_shape[1], fn->d * fn->d, ms_shape[2], ms_shape[3]);
         rep += line;
+        // element launch shape by whole evaluations among the finalists
+        fn->pipe_chunks = 1;
+        fn->pipe_ready = false;
+        for (size_t ti = 0; ti < fn->terms.size(); ++ti)
+        {
+            if (shortlist[ti].size() < 2) continue;
+            TermDev& t = fn->terms[ti];
+            const Cand* bestc = nullptr;
+            float best_ms = 1e30f;
+            std::string fl = "whole evaluations:";
+            for (const Cand& c : shortlist[ti])
+            {
+                t.tune_block = c.block;
+                t.tune_grid = c.grid;
+                t.tune_map = c.map;
+                float ms = 0.f;
+                TAD_TRY(time_evals(&ms));
+                std::snprintf(line, sizeof(line), " %s block %d grid %d %.4f ms;", c.block ? "persistent" : "plain",
+                              c.block, c.grid, ms);
+                fl += line;
+                if (ms < best_ms)
+                {
+                    best_ms = ms;
+                    bestc = &c;
+                }
+            }
+            t.tune_block = bestc->block;
+            t.tune_grid = bestc->grid;
+            t.tune_map = bestc->map;
+            rep += fl + "\n";
+        }
         // pipelined evaluation: whole eval_with_hessian_proj with the chosen
         // shapes for 1, 2, 4 and 8 element chunks (identical results)
         int best_chunks = 1;
@@ -2657,17 +2716,7 @@ tad_status tad_fn_autotune(tad_fn fn, const double* x, tad_mem mem, double eps, 
             plan_pipeline(fn);
             if (fn->pipe_used != chunks) continue;
             float best = 1e30f;
-            for (int rep_i = 0; rep_i < 4; ++rep_i)
-            {
-                double fv = 0.0;
-                TAD_HIP(hipEventRecord(e0, s));
-                TAD_TRY(eval_core(fn, xd, MODE_HESS_PROJ, eps, fn->g_buf.p, fn->H_buf.p, &fv));
-                TAD_HIP(hipEventRecord(e1, s));
-                TAD_HIP(hipEventSynchronize(e1));
-                float ms = 0.f;
-                TAD_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (rep_i > 0) best = std::min(best, ms);
-            }
+            TAD_TRY(time_evals(&best));
             std::snprintf(line, sizeof(line), " %d chunk%s %.4f ms;", chunks, chunks > 1 ? "s" : "", best);
             pl += line;
             if (best < best_ms)
