@@ -86,6 +86,10 @@ template <> struct QlTraits<float>
 // eigenvectors as columns and lam the eigenvalues.  Every register index is a
 // compile-time constant: data-dependent loop bounds of the QL sweep are
 // applied as per-step activity masks, so nothing goes to scratch.
+// Entry of e holding coupling i of the reversed tridiagonal matrix (e[n-1]
+// is the zero sentinel).
+__host__ __device__ constexpr int ql_rev_e(int n, int i) { return i <= n - 2 ? n - 2 - i : n - 1; }
+
 template <int n, class T>
 __device__ __forceinline__ void tridiag_ql_eigen(T* A, T* Z, T* lam)
 {
@@ -157,7 +161,15 @@ __device__ __forceinline__ void tridiag_ql_eigen(T* A, T* Z, T* lam)
             for (int i = 0; i < L; ++i) Z[(k + 1 + i) * n + j] -= s * A[Sym<n>::idx(k, k + 1 + i)];
         }
     }
-    // 3. implicit QL on (lam, e); e[i] couples lam[i] and lam[i+1]
+    // 3. implicit QL on the tridiagonal matrix in reverse order, J T J with
+    //    J the reversal (lr[i] = lam[n-1-i], er[i] couples lr[i] and lr[i+1]):
+    //    the reduction above runs from the first row down, and deflating the
+    //    reversed matrix from the top needs fewer sweeps (element Hessians of
+    //    the 60^3 twisted cube: 16.6 instead of 19.0 QL iterations per 9 x 9
+    //    matrix, 18 instead of 22 for the slowest lane of a wave).  Column i
+    //    of the reversed problem is column n-1-i of Z.
+    // (index maps only: lr(i) is lam[n-1-i], er(i) is e[n-2-i] and er(n-1)
+    // the zero sentinel e[n-1], so the sweep works on the same registers)
     const T eps = QlTraits<T>::eps;
 #pragma unroll
     for (int l = 0; l + 1 < n; ++l)
@@ -167,15 +179,15 @@ __device__ __forceinline__ void tridiag_ql_eigen(T* A, T* Z, T* lam)
             int m = n - 1;
 #pragma unroll
             for (int j = n - 2; j >= l; --j)
-                if (fabs(e[j]) <= eps * (fabs(lam[j]) + fabs(lam[j + 1]))) m = j;
+                if (fabs(e[ql_rev_e(n, j)]) <= eps * (fabs(lam[n - 1 - (j)]) + fabs(lam[n - 1 - (j + 1)]))) m = j;
             if (m == l) break;
-            T dm = lam[n - 1];
+            T dm = lam[n - 1 - (n - 1)];
 #pragma unroll
             for (int j = l; j + 1 < n; ++j)
-                if (j == m) dm = lam[j];
-            T g = (lam[l + 1] - lam[l]) / (T(2) * e[l]);
+                if (j == m) dm = lam[n - 1 - (j)];
+            T g = (lam[n - 1 - (l + 1)] - lam[n - 1 - (l)]) / (T(2) * e[ql_rev_e(n, l)]);
             T r = sqrt(g * g + T(1));
-            g = dm - lam[l] + e[l] / (g + (g >= T(0) ? r : -r));
+            g = dm - lam[n - 1 - (l)] + e[ql_rev_e(n, l)] / (g + (g >= T(0) ? r : -r));
             T s = T(1), c = T(1), p = T(0);
             bool alive = true;  // false once a rotation underflowed (r == 0)
 #pragma unroll
@@ -183,47 +195,47 @@ __device__ __forceinline__ void tridiag_ql_eigen(T* A, T* Z, T* lam)
             {
                 if (alive && i < m)
                 {
-                    const T f = s * e[i];
-                    const T b = c * e[i];
+                    const T f = s * e[ql_rev_e(n, i)];
+                    const T b = c * e[ql_rev_e(n, i)];
                     // r = |(f, g)| and 1/r from one refined reciprocal square root
                     const T qq = f * f + g * g;
                     const T ir = qq > T(0) ? ql_rsqrt(qq) : T(0);
                     r = qq * ir;
-                    e[i + 1] = r;
+                    e[ql_rev_e(n, i + 1)] = r;
                     if (r == T(0))
                     {
-                        lam[i + 1] -= p;
+                        lam[n - 1 - (i + 1)] -= p;
 #pragma unroll
                         for (int j = l; j + 1 < n; ++j)
-                            if (j == m) e[j] = T(0);
+                            if (j == m) e[ql_rev_e(n, j)] = T(0);
                         alive = false;
                     }
                     else
                     {
                         s = f * ir;
                         c = g * ir;
-                        g = lam[i + 1] - p;
-                        r = (lam[i] - g) * s + T(2) * c * b;
+                        g = lam[n - 1 - (i + 1)] - p;
+                        r = (lam[n - 1 - (i)] - g) * s + T(2) * c * b;
                         p = s * r;
-                        lam[i + 1] = g + p;
+                        lam[n - 1 - (i + 1)] = g + p;
                         g = c * r - b;
 #pragma unroll
                         for (int k = 0; k < n; ++k)
                         {
-                            const T zi = Z[k * n + i], zi1 = Z[k * n + i + 1];
-                            Z[k * n + i + 1] = s * zi + c * zi1;
-                            Z[k * n + i] = c * zi - s * zi1;
+                            const T zi = Z[k * n + (n - 1 - i)], zi1 = Z[k * n + (n - 2 - i)];
+                            Z[k * n + (n - 2 - i)] = s * zi + c * zi1;
+                            Z[k * n + (n - 1 - i)] = c * zi - s * zi1;
                         }
                     }
                 }
             }
             if (alive)
             {
-                lam[l] -= p;
-                e[l] = g;
+                lam[n - 1 - (l)] -= p;
+                e[ql_rev_e(n, l)] = g;
 #pragma unroll
                 for (int j = l; j + 1 < n; ++j)
-                    if (j == m) e[j] = T(0);
+                    if (j == m) e[ql_rev_e(n, j)] = T(0);
             }
         }
     }
