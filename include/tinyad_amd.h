@@ -418,8 +418,9 @@ typedef struct tad_solve_opts
     int check_every;    /* host polls convergence every this many iterations             */
     int coarse_aggregate; /* two-level preconditioner: handles per aggregate of the      */
                         /* coarse space; 0: block-Jacobi only; < 0 (default): automatic, */
-                        /* about 1,400 coarse unknowns with at least 32 handles per      */
-                        /* aggregate                                                     */
+                        /* about 1,400 coarse unknowns up to 30K handles, growing with   */
+                        /* the cube root of the handle count to at most 2,800, with at   */
+                        /* least 32 handles per aggregate                                */
     int coarse_reuse;   /* 1: keep the coarse space and inverse coarse matrix of the     */
                         /* function's previous solve when it is compatible (same         */
                         /* aggregates, unknowns per aggregate and w, neither solve       */

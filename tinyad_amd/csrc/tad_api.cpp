@@ -861,8 +861,11 @@ tad_status build_coarse(tad_fn fn, int target)
 
 // Largest coarse matrix the two-level preconditioner builds (dense FP64 inverse).
 constexpr int64_t kMaxCoarse = 8192;
-// Automatic aggregate size (tad_solve_opts.coarse_aggregate < 0).
+// Automatic aggregate size (tad_solve_opts.coarse_aggregate < 0): about
+// kAutoCoarse coarse unknowns up to kAutoHandles handles, growing with the
+// cube root of the handle count above, to at most 2 kAutoCoarse.
 constexpr int64_t kAutoCoarse = 1400;
+constexpr double kAutoHandles = 30000.0;
 constexpr int64_t kAutoMinAggregate = 32;
 
 // Per solve: coarse unknowns per aggregate, lever arms, A_c = P^T (w I + H) P,
@@ -881,10 +884,15 @@ tad_status setup_coarse(tad_fn fn, const double* H_dev, const tad_solve_opts& o,
     if (o.coarse_aggregate == 0 || nh == 0 || fn->n_blocks == 0) return TAD_OK;
     const bool rot = fn->x_hess_valid && d >= 2;
     const int kc = rot ? (d == 3 ? 6 : 3) : d;
-    // automatic size: about kAutoCoarse coarse unknowns (measured optimum on
-    // 2D grids and tet cubes of 10K to 227K handles), at least 32 handles
-    const int64_t agg = o.coarse_aggregate > 0 ? o.coarse_aggregate
-                                               : std::max<int64_t>(kAutoMinAggregate, (nh * kc + kAutoCoarse - 1) / kAutoCoarse);
+    // automatic size (measured optimum, ms per Newton iteration with the
+    // coarse inverse rebuilt every 5th: 30^3 cube 1,404 unknowns 3.95 against
+    // 2,790 4.47; 40^3 1,380 7.25 against 2,778 7.41; 60^3 1,404 20.7,
+    // 2,526 19.4, 2,808 19.0, 3,186 21.0), at least 32 handles per aggregate
+    const double nc_auto = std::min(2.0 * kAutoCoarse,
+                                    kAutoCoarse * std::cbrt(std::max(1.0, (double)nh / kAutoHandles)));
+    const int64_t agg = o.coarse_aggregate > 0
+                            ? o.coarse_aggregate
+                            : std::max<int64_t>(kAutoMinAggregate, (int64_t)std::ceil((double)(nh * kc) / nc_auto));
     // keep the dense coarse matrix bounded: larger aggregates for large meshes
     // (grown further while breadth-first aggregates come out too small)
     const int want = (int)std::max<int64_t>(agg, (nh * kc + kMaxCoarse - 1) / kMaxCoarse);
