@@ -222,7 +222,9 @@ def run_device_functors(args):
         sys.exit(f"{plugin} is missing: run python -m tinyad_amd.build first")
     L = ctypes.CDLL(plugin)
     ops = {}
-    for name, energy in [("symdir_tri", tad.SYMDIR_TRI), ("symdir_tet", tad.SYMDIR_TET)]:
+    # the functors declaring translation invariance (deflated eigensolve, as a
+    # geometry energy would declare it)
+    for name, energy in [("symdir_tri_inv", tad.SYMDIR_TRI), ("symdir_tet_inv", tad.SYMDIR_TET)]:
         f = getattr(L, f"tad_test_{name}_ops")
         f.restype = ctypes.c_void_p
         ops[energy] = f()

@@ -29,6 +29,10 @@
 // valences; an element with fewer handles than its group's valence sees 0 in
 // the unused trailing coordinates (pass the handle count in p when the energy
 // must know it).
+// A functor whose energy depends on the coordinates only through their
+// differences may say so with `static constexpr bool translation_invariant =
+// true;`: its projected Hessians then take the smaller eigensolve without the
+// d translations (for tets 9 x 9 instead of 12 x 12).
 // Return S(INFINITY) (or INFINITY) for an infeasible element, which then
 // contributes a zero gradient and, projected, eps I (as an infinite Double<k>
 // does in the reference).  The functor must be stateless (it is
@@ -51,6 +55,22 @@
 #include <hip/hip_runtime.h>
 
 namespace tinyad_amd {
+
+// F::translation_invariant (optional, default false): the energy depends on
+// the element's coordinates only through their differences.  The projected
+// Hessian then goes through the eigensolve of its (K - d) x (K - d) part
+// without the d translations (as the library's own simplices do: 9 x 9
+// instead of 12 x 12 for tets); an element whose Hessian is not translation
+// invariant to rounding is reported as non-finite (TAD_ERR_NONFINITE).
+template <class F, class = void>
+struct declared_translation_invariant : std::false_type
+{
+};
+template <class F>
+struct declared_translation_invariant<F, std::void_t<decltype(F::translation_invariant)>>
+    : std::bool_constant<F::translation_invariant>
+{
+};
 
 template <class F, int NN, int DD, int NPAR>
 struct DeviceElement
@@ -151,7 +171,8 @@ struct DeviceElement
             if constexpr (MODE >= tad::MODE_HESS) hyper_passes<T, true, second_dirs<T>()>(X, p, na, g, Hp);
             else hyper_passes<T, false, 1>(X, p, na, g, Hp);
         }
-        tad::write_dense_record<NN, DD, T, MODE>(g, Hp, eps, out, nonfinite);
+        tad::write_dense_record<NN, DD, T, MODE, declared_translation_invariant<F>::value>(g, Hp, eps, out,
+                                                                                          nonfinite);
         return f;
     }
 

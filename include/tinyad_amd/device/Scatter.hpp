@@ -242,9 +242,134 @@ __device__ __forceinline__ void merge_repeated(T* g, T* Hp, uint32_t dups, uint6
     merge_pairs<N, Dm, T, HESS, 1, 0>(g, Hp, dups, reps);
 }
 
-// Generic small dense terms (penalty, quadratic): full K x K Hessian in
-// packed form, projected without deflation.
-template <int N, int Dm, class T, int MODE, class Sink>
+// Entry v of column a (0 <= a < N-1) of the Helmert basis of
+// {w in R^N : sum w = 0}: 1/sqrt((a+1)(a+2)) for v <= a, -(a+1)/sqrt((a+1)(a+2))
+// for v = a+1, 0 below (orthonormal columns; every index is a compile-time
+// constant where it is used, so the values fold to constants).
+__device__ __forceinline__ double helmert(int v, int a)
+{
+    const double s = 1.0 / __builtin_sqrt((double)((a + 1) * (a + 2)));
+    return v <= a ? s : (v == a + 1 ? -(double)(a + 1) * s : 0.0);
+}
+
+// Translating every vertex of the element leaves its energy unchanged up to
+// rounding: for every row (u,i) and coordinate j, sum_v H[(u,i),(v,j)] is
+// below 64 ulp of the largest |H| entry.
+template <int N, int Dm, class T>
+__device__ __forceinline__ bool translation_invariant(const T* Hp)
+{
+    constexpr int K = N * Dm;
+    T hmax = T(0);
+#pragma unroll
+    for (int q = 0; q < Sym<K>::n; ++q) hmax = fmax(hmax, fabs(Hp[q]));
+    const T tol = T(64) * QlTraits<T>::eps * hmax;
+    bool ok = true;
+#pragma unroll
+    for (int r = 0; r < K; ++r)
+#pragma unroll
+        for (int j = 0; j < Dm; ++j)
+        {
+            T sum = T(0);
+#pragma unroll
+            for (int v = 0; v < N; ++v) sum += Hp[Sym<K>::at(r, v * Dm + j)];
+            ok = ok && (fabs(sum) <= tol);
+        }
+    return ok;
+}
+
+// Projection of a translation-invariant element Hessian on its (N-1) d
+// dimensional part, as the catalog's simplices do: with Q = U (x) I_d (U the
+// Helmert basis) and K = Q^T H Q,
+//   P_eps(H) = Q P_eps(K) Q^T + max(eps, 0) (11^T / N) (x) I_d,
+// the reference's projection of H (HessianProjection.hh:48-101) whose d
+// translations are null vectors (clamped to eps, or left at 0 for eps < 0).
+// Writes the blocks of the result; Hp is consumed.
+template <int N, int Dm, class T, class Sink>
+__device__ __forceinline__ void write_deflated_projection(const T* Hp, T eps, const Sink& out, bool& nonfinite)
+{
+    constexpr int K = N * Dm;
+    constexpr int KD = (N - 1) * Dm;
+    T Km[Sym<KD>::n];
+    // one column (b, j') of K at a time: T1[(u,i)] = sum_v H[(u,i),(v,j')] U_vb
+#pragma unroll
+    for (int b = 0; b + 1 < N; ++b)
+#pragma unroll
+        for (int jp = 0; jp < Dm; ++jp)
+        {
+            const int c2 = b * Dm + jp;
+            T T1[K];
+#pragma unroll
+            for (int r = 0; r < K; ++r)
+            {
+                T acc = T(0);
+#pragma unroll
+                for (int v = 0; v < N; ++v)
+                    if (helmert(v, b) != 0.0) acc += T(helmert(v, b)) * Hp[Sym<K>::at(r, v * Dm + jp)];
+                T1[r] = acc;
+            }
+#pragma unroll
+            for (int a = 0; a + 1 < N; ++a)
+#pragma unroll
+                for (int i = 0; i < Dm; ++i)
+                {
+                    const int c1 = a * Dm + i;
+                    if (c1 > c2) continue;
+                    T acc = T(0);
+#pragma unroll
+                    for (int u = 0; u < N; ++u)
+                        if (helmert(u, a) != 0.0) acc += T(helmert(u, a)) * T1[u * Dm + i];
+                    Km[Sym<KD>::idx(c1, c2)] = acc;
+                }
+        }
+    project_psd<KD, T, false>(Km, eps);
+    const T ct = eps > T(0) ? eps / T(N) : T(0);
+    // H[(v,i),(u,i')] = sum_{a,b} U_va U_ub P[(a,i),(b,i')] + ct delta_ii', one
+    // vertex row at a time
+#pragma unroll
+    for (int v = 0; v < N; ++v)
+    {
+        T S1[Dm][KD];
+#pragma unroll
+        for (int i = 0; i < Dm; ++i)
+#pragma unroll
+            for (int c2 = 0; c2 < KD; ++c2)
+            {
+                T acc = T(0);
+#pragma unroll
+                for (int a = 0; a + 1 < N; ++a)
+                    if (helmert(v, a) != 0.0) acc += T(helmert(v, a)) * Km[Sym<KD>::at(a * Dm + i, c2)];
+                S1[i][c2] = acc;
+            }
+#pragma unroll
+        for (int u = v; u < N; ++u)
+        {
+            T blk[Dm * Dm];
+#pragma unroll
+            for (int i = 0; i < Dm; ++i)
+#pragma unroll
+                for (int ip = 0; ip < Dm; ++ip)
+                {
+                    T acc = (i == ip) ? ct : T(0);
+#pragma unroll
+                    for (int b = 0; b + 1 < N; ++b)
+                        if (helmert(u, b) != 0.0) acc += T(helmert(u, b)) * S1[i][b * Dm + ip];
+                    blk[i * Dm + ip] = acc;
+                    nonfinite = nonfinite || !finite_val(acc);
+                }
+            out.block(v, u, blk);
+        }
+    }
+}
+
+// Generic small dense terms (penalty, quadratic, device functors): full
+// K x K Hessian in packed form.  Deflate: the energy is translation invariant
+// (a function of vertex position differences, as geometry energies are), so
+// its projected Hessian goes through the (K - d) x (K - d) eigensolve; an
+// element whose Hessian rows do not sum to zero is reported as non-finite.
+// Otherwise the Hessian is projected whole.  (The two paths are separate
+// instantiations: one kernel holding both needs far more registers than
+// either.)
+template <int N, int Dm, class T, int MODE, bool Deflate = false, class Sink>
 __device__ __forceinline__ void write_dense_record(T* g, T* Hp, T eps, const Sink& out, bool& nonfinite)
 {
     constexpr int K = N * Dm;
@@ -257,8 +382,19 @@ __device__ __forceinline__ void write_dense_record(T* g, T* Hp, T eps, const Sin
     if constexpr (MODE >= MODE_HESS)
     {
         if constexpr (MODE == MODE_HESS_PROJ)
+        {
             if (!positive_diagonally_dominant<K, T>(Hp, eps))
-                project_psd<K, T>(Hp, eps);
+            {
+                if constexpr (Deflate && N > 1)
+                {
+                    nonfinite = nonfinite || !translation_invariant<N, Dm, T>(Hp);
+                    write_deflated_projection<N, Dm, T>(Hp, eps, out, nonfinite);
+                    return;
+                }
+                else
+                    project_psd<K, T>(Hp, eps);
+            }
+        }
         write_blocks<N, Dm, T>(Hp, out, nonfinite);
     }
 }

@@ -60,6 +60,29 @@ struct SymdirTet
     }
 };
 
+// The same two energies declaring that they depend on coordinate differences
+// only (projected through the eigensolve without the translations).
+struct SymdirTriInv : SymdirTri
+{
+    static constexpr bool translation_invariant = true;
+};
+struct SymdirTetInv : SymdirTet
+{
+    static constexpr bool translation_invariant = true;
+};
+
+// |x_0|^2 + |x_1 - x_0|^2 over two 2D handles, wrongly declared translation
+// invariant (the library must report it rather than project it).
+struct FalseInvariant
+{
+    static constexpr bool translation_invariant = true;
+    template <class S>
+    __device__ S operator()(const S* x, const double*) const
+    {
+        return tad::sqr(x[0]) + tad::sqr(x[1]) + tad::sqr(x[2] - x[0]) + tad::sqr(x[3] - x[1]);
+    }
+};
+
 // 0.5 x^T A x + b^T x + c over two 2D handles (K = 4): p = [A (4 x 4), b (4), c],
 // the parameter row of the catalog's quadratic element.
 struct Quadratic4
@@ -250,6 +273,9 @@ const tad_device_residual_ops g_res_tet = tinyad_amd::device_residual<SymdirTetR
 
 const tad_device_ops g_tri = tinyad_amd::device_element<SymdirTri, 3, 2, 5>();
 const tad_device_ops g_tet = tinyad_amd::device_element<SymdirTet, 4, 3, 10>();
+const tad_device_ops g_tri_inv = tinyad_amd::device_element<SymdirTriInv, 3, 2, 5>();
+const tad_device_ops g_tet_inv = tinyad_amd::device_element<SymdirTetInv, 4, 3, 10>();
+const tad_device_ops g_false_inv = tinyad_amd::device_element<FalseInvariant, 2, 2, 0>();
 const tad_device_ops g_quad = tinyad_amd::device_element<Quadratic4, 2, 2, 21>();
 const tad_device_ops g_smooth = tinyad_amd::device_element<Smooth4, 2, 2, 2>();
 const tad_device_ops g_smooth8 = tinyad_amd::device_element<Smooth8, 4, 2, 2>();
@@ -267,6 +293,9 @@ const tad_device_ops g_one_ring[4] = { tinyad_amd::device_element<OneRing::At<6>
 extern "C" {
 const tad_device_ops* tad_test_symdir_tri_ops() { return &g_tri; }
 const tad_device_ops* tad_test_symdir_tet_ops() { return &g_tet; }
+const tad_device_ops* tad_test_symdir_tri_inv_ops() { return &g_tri_inv; }
+const tad_device_ops* tad_test_symdir_tet_inv_ops() { return &g_tet_inv; }
+const tad_device_ops* tad_test_false_invariant_ops() { return &g_false_inv; }
 const tad_device_ops* tad_test_quadratic4_ops() { return &g_quad; }
 const tad_device_ops* tad_test_smooth4_ops() { return &g_smooth; }
 const tad_device_ops* tad_test_smooth8_ops() { return &g_smooth8; }

@@ -50,7 +50,8 @@ def plugin():
         raise RuntimeError(f"{PLUGIN} is missing: run python -m tinyad_amd.build first")
     L = ctypes.CDLL(PLUGIN)
     ops = {}
-    for name in ["symdir_tri", "symdir_tet", "quadratic4", "smooth4", "smooth8"]:
+    for name in ["symdir_tri", "symdir_tet", "symdir_tri_inv", "symdir_tet_inv", "false_invariant", "quadratic4",
+                 "smooth4", "smooth8"]:
         f = getattr(L, f"tad_test_{name}_ops")
         f.restype = ctypes.c_void_p
         ops[name] = f()
@@ -58,14 +59,15 @@ def plugin():
     return ops
 
 
-def device_function(P, plugin, ctx, precision="fp64"):
-    """P with its symmetric-Dirichlet terms as device functors, other terms from the catalog."""
+def device_function(P, plugin, ctx, precision="fp64", suffix=""):
+    """P with its symmetric-Dirichlet terms as device functors, other terms from
+    the catalog; suffix "_inv": the functors that declare translation invariance."""
     fn = tad.ScalarFunction(P.d, P.n_handles, ctx, precision)
     for t in P.terms:
         if t.energy == tad.SYMDIR_TRI:
-            fn.add_elements_device(plugin["symdir_tri"], t.conn, t.params)
+            fn.add_elements_device(plugin["symdir_tri" + suffix], t.conn, t.params)
         elif t.energy == tad.SYMDIR_TET:
-            fn.add_elements_device(plugin["symdir_tet"], t.conn, t.params)
+            fn.add_elements_device(plugin["symdir_tet" + suffix], t.conn, t.params)
         else:
             fn.add_elements(t.valence, t.energy, t.conn, t.params)
     return fn
@@ -86,13 +88,16 @@ CASES = {
 }
 
 
+@pytest.mark.parametrize("suffix", ["", "_inv"])
 @pytest.mark.parametrize("name", list(CASES))
 @pytest.mark.parametrize("eps", [1e-9, -1.0])
-def test_device_symdir_matches_oracle_and_catalog(ctx, plugin, name, eps):
+def test_device_symdir_matches_oracle_and_catalog(ctx, plugin, name, eps, suffix):
+    """Projected through the whole K x K eigensolve, or (suffix _inv, the
+    functors declaring translation invariance) through the deflated one."""
     P = CASES[name]()
     fo = oracle.Function.from_problem(P)
     fc = tad.ScalarFunction.from_problem(P, ctx)
-    fd = device_function(P, plugin, ctx)
+    fd = device_function(P, plugin, ctx, suffix=suffix)
     f1, g1, H1 = fo.eval_with_hessian_proj(P.x0, eps)
     f2, g2, H2 = fd.eval_with_hessian_proj(P.x0, eps)
     f3, g3, H3 = fc.eval_with_hessian_proj(P.x0, eps)
@@ -109,7 +114,8 @@ def test_device_symdir_matches_oracle_and_catalog(ctx, plugin, name, eps):
     assert nrel(Hb.toarray(), Ho2.toarray()) <= FP64_TOL
 
 
-def test_device_inverted_element(ctx, plugin):
+@pytest.mark.parametrize("suffix", ["", "_inv"])
+def test_device_inverted_element(ctx, plugin, suffix):
     """+inf energy, zero gradient and eps I for an inverted tet (NewtonTest.cc:40-41)."""
     P = configs.twist_cube(m=2, seed=3)
     X = P.x0.reshape(-1, 3).copy()
@@ -117,11 +123,25 @@ def test_device_inverted_element(ctx, plugin):
     X[[a, b]] = X[[b, a]]
     x = X.ravel()
     fo = oracle.Function.from_problem(P)
-    fd = device_function(P, plugin, ctx)
+    fd = device_function(P, plugin, ctx, suffix=suffix)
     f1, g1, H1 = fo.eval_with_hessian_proj(x)
     f2, g2, H2 = fd.eval_with_hessian_proj(x)
     assert f1 == np.inf and f2 == np.inf and fd.eval(x) == np.inf
     assert nrel(g2, g1) <= FP64_TOL and nrel(H2.toarray(), H1.toarray()) <= FP64_TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_device_declared_invariance_is_checked(ctx, plugin, precision):
+    """A functor that declares translation invariance but is not (|x_0|^2 +
+    |x_1 - x_0|^2) is reported as non-finite by the projected evaluation, not
+    projected on the wrong subspace; unprojected evaluations are unaffected."""
+    fn = tad.ScalarFunction(2, 2, ctx, precision)
+    fn.add_elements_device(plugin["false_invariant"], [[0, 1]], np.zeros(0))
+    x = np.array([0.3, -0.2, 0.7, 0.4])
+    f, g, H = fn.eval_with_derivatives(x)
+    assert abs(f - (x[0] ** 2 + x[1] ** 2 + (x[2] - x[0]) ** 2 + (x[3] - x[1]) ** 2)) <= 1e-15
+    with pytest.raises(tad.TadError, match="NONFINITE"):
+        fn.eval_with_hessian_proj(x, 0.5)  # H = [4I -2I; -2I 2I] is not diagonally dominant by 0.5
 
 
 def quad_problem(seed, n_handles=7, n_elem=12):
@@ -277,8 +297,9 @@ def translated(P, offset):
     return Q
 
 
+@pytest.mark.parametrize("suffix", ["", "_inv"])
 @pytest.mark.parametrize("kind", ["tets", "triangles"])
-def test_device_mixed_precision_far_from_origin(ctx, plugin, kind):
+def test_device_mixed_precision_far_from_origin(ctx, plugin, kind, suffix):
     """Device functors with FP32 derivatives on a mesh translated by 1e3
     (|x| / h up to 4e3 for tets, 1.1e4 for triangles) against the FP64
     catalog: the functor's values stay FP64, so the mixed bar 2e-5 holds as
@@ -287,7 +308,7 @@ def test_device_mixed_precision_far_from_origin(ctx, plugin, kind):
     Q = translated(P, 1000.0)
     fc = tad.ScalarFunction.from_problem(Q, ctx)
     fa, ga, Ha = fc.eval_with_hessian_proj(Q.x0)
-    for fm in (device_function(Q, plugin, ctx, precision="mixed"),
+    for fm in (device_function(Q, plugin, ctx, precision="mixed", suffix=suffix),
                tad.ScalarFunction.from_problem(Q, ctx, precision="mixed")):
         fb, gb, Hb = fm.eval_with_hessian_proj(Q.x0)
         assert abs(fb - fa) <= 1e-12 * abs(fa)
@@ -486,14 +507,15 @@ def test_device_empty_terms(ctx, plugin):
     assert np.array_equal(H.toarray()[4:, 4:], 2 * np.eye(2)) and H.nnz == 4
 
 
-def test_device_tet_newton_matches_catalog(ctx, plugin):
+@pytest.mark.parametrize("suffix", ["", "_inv"])
+def test_device_tet_newton_matches_catalog(ctx, plugin, suffix):
     """Projected Newton on a twisted cube with the tet energy as a device
     functor (K = 12: hyper-dual passes) and from the catalog: same energies
     along the way and at the end (converged-energy bar 1e-8)."""
     import torch
     P = configs.twist_cube(m=8, seed=2)
     fc = tad.ScalarFunction.from_problem(P, ctx)
-    fd = device_function(P, plugin, ctx)
+    fd = device_function(P, plugin, ctx, suffix=suffix)
     xc = torch.tensor(P.x0, device="cuda:0")
     xd = xc.clone()
     for it in range(12):
@@ -505,13 +527,14 @@ def test_device_tet_newton_matches_catalog(ctx, plugin):
     assert abs(fd_ - fc_) <= 1e-8 * abs(fc_)
 
 
-def test_device_tet_mixed_precision(ctx, plugin):
+@pytest.mark.parametrize("suffix", ["", "_inv"])
+def test_device_tet_mixed_precision(ctx, plugin, suffix):
     """Tets as device functors with FP32 contributions (hyper-dual passes in
     FP32) against the FP64 catalog: f identical (passive FP64 value), g and
     H_proj within the mixed-mode bar 2e-5, the Hessian unprojected too."""
     P = perturbed_cube(4, 0.1, 17)
     fc = tad.ScalarFunction.from_problem(P, ctx)
-    fm = device_function(P, plugin, ctx, precision="mixed")
+    fm = device_function(P, plugin, ctx, precision="mixed", suffix=suffix)
     fa, ga, Ha = fc.eval_with_hessian_proj(P.x0)
     fb, gb, Hb = fm.eval_with_hessian_proj(P.x0)
     assert fa == fb
