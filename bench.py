@@ -354,6 +354,8 @@ def main():
     ap.add_argument("--coarse-refresh", type=int, default=5,
                     help="rebuild the coarse space of the direction solve every this many Newton iterations "
                          "(reusing the previous inverse coarse matrix in between; 1: every iteration)")
+    ap.add_argument("--solve-graphs", type=int, default=0, choices=[0, 1],
+                    help="CG iterations replayed from a recorded HIP graph (1) or launched directly (0)")
     ap.add_argument("--cpu-sample", type=int, default=400000, help="tets in the CPU baseline sample (0: skip)")
     ap.add_argument("--no-newton", action="store_true")
     ap.add_argument("--extras", action=argparse.BooleanOptionalAction, default=True,
@@ -407,6 +409,7 @@ def main():
     log(f"[bench] {P.name}: {n_tets} elements, {P.n_handles} vertices, precision {args.precision}")
     ctx = tad.Context(0)
     fn = tad.ScalarFunction.from_problem(P, ctx, precision=args.precision)
+    fn.set_solve_graphs(args.solve_graphs)
     if seg_off is not None:
         fn.set_segments(seg_off)
     nnz = fn.nnz
@@ -519,6 +522,8 @@ def main():
     if extras or (args.precision == "mixed" and args.workload == "twist_cube"):
         f64, fmx = (fn, tad.ScalarFunction.from_problem(P, ctx, precision="mixed")) if args.precision == "fp64" \
             else (tad.ScalarFunction.from_problem(P, ctx, precision="fp64"), fn)
+        f64.set_solve_graphs(args.solve_graphs)
+        fmx.set_solve_graphs(args.solve_graphs)
         if args.precision == "fp64":
             if args.autotune:
                 fmx.autotune(x0)
@@ -613,6 +618,7 @@ def main():
                                         else (args.coarse_aggregate if args.coarse_aggregate is not None
                                               else tad.solve_default_opts().coarse_aggregate)),
                    "coarse_refresh": args.coarse_refresh,
+                   "solve_graphs": args.solve_graphs,
                    "autotuned": bool(args.autotune),
                    "launch_shape": list(fn.launch_shape(0)), "assembly_shape": fn.assembly_shape},
         "roofline": roof,

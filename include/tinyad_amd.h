@@ -404,6 +404,15 @@ tad_status tad_fn_set_assembly_shape(tad_fn fn, int shape);
 tad_status tad_fn_get_pipeline(tad_fn fn, int* requested, int* used);
 tad_status tad_fn_set_pipeline(tad_fn fn, int chunks);
 
+/* CG iterations of the direction solves replayed from a HIP graph (no
+ * counterpart in the reference): the launches of 8 iterations (and a kernel
+ * that advances the iteration index, which the kernels then read on the
+ * device) are recorded once per operand set and replayed.  Identical results
+ * either way.  mode: 1 on, 0 off (the default: on MI355X direct launches are
+ * as fast at 3K handles and faster above).  Solves with kernel timing enabled
+ * always launch directly. */
+tad_status tad_fn_set_solve_graphs(tad_fn fn, int mode);
+
 /* Device pointers of the function's own buffers holding the gradient and CSR
  * values of the last second-order evaluation (valid until the next call). */
 tad_status tad_fn_device_buffers(tad_fn fn, const double** g_dev, const double** H_values_dev);

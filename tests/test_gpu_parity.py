@@ -578,6 +578,50 @@ def test_two_level_coarse_size_cap(ctx):
     assert 6 * 1000 <= st.coarse_unknowns <= 8192 and st.coarse_unknowns % 6 == 0
 
 
+def test_solve_graph_replays_match_direct_launches(ctx):
+    """CG iterations replayed from the recorded HIP graph give the same bits
+    as direct launches: block-Jacobi and two-level, with w, with a reduced
+    basis, on a tet cube and a triangle grid, solved repeatedly with eager work
+    and synchronizations in between; operands that change (another H,
+    another coarse space) are recorded again, and Newton trajectories agree
+    to the bit."""
+    import torch
+    for P in [configs.twist_cube(m=6, seed=3), configs.grid_param(20, seed=2)]:
+        fa = tad.ScalarFunction.from_problem(P, ctx)
+        fb = tad.ScalarFunction.from_problem(P, ctx)
+        fa.set_solve_graphs(0)
+        fb.set_solve_graphs(1)
+        f, g, H = fa.eval_with_hessian_proj(P.x0)
+        fb.eval_with_hessian_proj(P.x0)  # the positions of the coarse space's rotations
+        free = np.ones(P.d * P.n_handles, dtype=np.uint8)
+        free[: 2 * P.d] = 0
+        cases = [dict(coarse_aggregate=0), dict(coarse_aggregate=16), dict(coarse_aggregate=16, w_identity=0.3),
+                 dict(coarse_aggregate=-1, free=free)]
+        for rep in range(3):
+            for kw in cases:
+                da, sa = fa.solve(g, H=H, tol=1e-11, **kw)
+                db, sb = fb.solve(g, H=H, tol=1e-11, **kw)
+                assert sa.converged and sb.converged
+                assert sa.iters == sb.iters and np.array_equal(da, db), (P.name, rep, kw)
+                # eager work on the device between replays
+                fa.eval(P.x0)
+                torch.cuda.synchronize()
+        fa.close()
+        fb.close()
+    P = configs.twist_cube(m=5, seed=4)
+    fa = tad.ScalarFunction.from_problem(P, ctx)
+    fb = tad.ScalarFunction.from_problem(P, ctx)
+    fa.set_solve_graphs(0)
+    fb.set_solve_graphs(1)
+    xa = torch.tensor(P.x0, device="cuda:0")
+    xb = xa.clone()
+    for it in range(6):
+        sa = fa.newton_step(xa)
+        sb = fb.newton_step(xb)
+        assert sa["cg_iters"] == sb["cg_iters"] and sa["f_new"] == sb["f_new"], it
+        assert torch.equal(xa, xb), it
+
+
 def test_two_level_preconditioner_2d_and_batches(ctx):
     """Triangle meshes (2 translations + 1 rotation per aggregate) and a batch
     of independent disks (aggregates never span two meshes).  Without pinned

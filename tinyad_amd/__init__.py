@@ -454,6 +454,11 @@ class ScalarFunction:
     def pipeline(self, chunks: int):
         check(lib().tad_fn_set_pipeline(self._h, int(chunks)))
 
+    def set_solve_graphs(self, mode: int):
+        """CG iterations replayed from a recorded HIP graph: 1 on, 0 off (the
+        default); identical results."""
+        check(lib().tad_fn_set_solve_graphs(self._h, int(mode)))
+
     # -- solver ------------------------------------------------------------
     def _dev_vec(self, a, n: int, name: str):
         """a checked as a contiguous float64 CUDA tensor of n entries on the context's device."""

@@ -31,7 +31,7 @@ EXPORTED = [
     "tad_line_search", "tad_newton_decrement",
     "tad_fn_set_segments", "tad_fn_segment_energies", "tad_fn_autotune",
     "tad_fn_get_launch_shape", "tad_fn_set_launch_shape", "tad_fn_get_assembly_shape", "tad_fn_set_assembly_shape",
-    "tad_fn_get_pipeline", "tad_fn_set_pipeline",
+    "tad_fn_get_pipeline", "tad_fn_set_pipeline", "tad_fn_set_solve_graphs",
     "tad_fn_set_params", "tad_fn_rest_vjp",
     "tad_fn_create_vector", "tad_fn_add_residuals_device", "tad_fn_n_outputs", "tad_fn_jacobian_pattern",
     "tad_fn_eval_residuals", "tad_fn_eval_with_jacobian", "tad_fn_eval_sum_of_squares_with_derivatives",
@@ -161,6 +161,7 @@ def lib():
         "tad_fn_set_assembly_shape": [vp, i32],
         "tad_fn_get_pipeline": [vp, vp, vp],
         "tad_fn_set_pipeline": [vp, i32],
+        "tad_fn_set_solve_graphs": [vp, i32],
         "tad_fn_set_params": [vp, i32, vp, i32],
         "tad_fn_rest_vjp": [vp, i32, vp, vp, vp, dbl, vp, i32],
         "tad_selftest_dual": [vp, i32, dbl, dbl, pdbl],

@@ -317,6 +317,12 @@ struct tad_fn_s
     DevBuf<double> bas_v, bas_tv, bas_vec;
     bool x_hess_valid = false;
     int last_solve_iters = 0;  // iterations of the previous solve (first poll of the next one)
+    // CG iterations recorded as a HIP graph (tad_fn_set_solve_graphs): the
+    // executable graph and the operands it was recorded with
+    int solve_graphs = 0;      // 1 on, 0 off
+    hipGraphExec_t pcg_graph = nullptr;
+    std::vector<uint64_t> pcg_graph_key;
+    bool pcg_graph_failed = false;  // recording failed once: launch directly
     int solves_since_rebuild = 0;  // solves that reused the coarse space since it was built
 
     hipStream_t s() const { return ctx->stream; }
@@ -1228,6 +1234,32 @@ tad_status dot2_dev(tad_fn fn, const double* a1, const double* b1, const double*
     return TAD_OK;
 }
 
+// Recorded CG launches: iterations per graph.
+constexpr int kGraphIters = 8;
+
+// Every operand the recorded CG launches hold: the graph is recorded again
+// when any of them changes.
+std::vector<uint64_t> pcg_graph_key(tad_fn fn, const double* H_dev, const double* sol_dev,
+                                    const unsigned char* free_dev, const CoarseDev* co, double w, hipStream_t s)
+{
+    auto u = [](const void* p) { return (uint64_t)(uintptr_t)p; };
+    uint64_t wbits;
+    std::memcpy(&wbits, &w, sizeof(w));
+    std::vector<uint64_t> k = { (uint64_t)fn->d, (uint64_t)fn->n_handles, u(fn->brow_ptr.p), u(fn->bcol.p),
+                                u(fn->row_ptr.p), u(H_dev), wbits, u(fn->Dinv.p), u(sol_dev), u(fn->r.p),
+                                u(fn->z.p), u(fn->p.p), u(fn->q.p), u(fn->pcg_parts.p), u(fn->scal.p), u(free_dev),
+                                u(s) };
+    if (co)
+    {
+        const uint64_t more[] = { (uint64_t)co->kc, (uint64_t)co->n_agg, (uint64_t)co->nc, u(co->agg_ptr),
+                                  u(co->agg_vert), (uint64_t)co->n_chunks, u(co->chunk_ptr), u(co->agg_chunk),
+                                  (uint64_t)co->max_agg_chunks, u(co->chunk_part), u(co->dvec), u(co->Ainv),
+                                  u(co->rc), u(co->cparts), (uint64_t)co->Gc, u(co->free_vars) };
+        k.insert(k.end(), std::begin(more), std::end(more));
+    }
+    return k;
+}
+
 tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, double* sol_dev,
                       const tad_solve_opts& o, tad_solve_stats* st, const unsigned char* free_dev = nullptr)
 {
@@ -1273,12 +1305,72 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
     // each poll stalls the device for a host round trip.
     int k = 0;
     int chunk = std::max(every, fn->last_solve_iters * 3 / 4);
+    // recorded launches: kGraphIters iterations, each followed by the
+    // device-side iteration counter's advance, replayed until termination
+    // (every kernel returns at once after it, and iteration max_iters stops
+    // in the SpMV's termination test)
+    const bool graphs = !timer && !fn->pcg_graph_failed && fn->solve_graphs == 1;
+    hipGraphExec_t gexec = nullptr;
+    if (graphs)
+    {
+        auto launch_chunk = [&]() -> hipError_t {
+            for (int c = 0; c < kGraphIters; ++c)
+            {
+                hipError_t e = launch_pcg_iteration(d, nh, fn->brow_ptr.p, fn->bcol.p, fn->row_ptr.p, H_dev,
+                                                    o.w_identity, fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->p.p,
+                                                    fn->q.p, fn->pcg_parts.p, fn->scal.p, 0, true, free_dev, co, s,
+                                                    nullptr, true);
+                if (e == hipSuccess) e = launch_pcg_advance(fn->scal.p, s);
+                if (e != hipSuccess) return e;
+            }
+            return hipSuccess;
+        };
+        std::vector<uint64_t> key = pcg_graph_key(fn, H_dev, sol_dev, free_dev, co, o.w_identity, s);
+        if (fn->pcg_graph && fn->pcg_graph_key == key)
+            gexec = fn->pcg_graph;
+        else
+        {
+            if (fn->pcg_graph) TAD_HIP(hipGraphExecDestroy(fn->pcg_graph));
+            fn->pcg_graph = nullptr;
+            fn->pcg_graph_key.clear();
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess)
+            {
+                const hipError_t el = launch_chunk();
+                e = hipStreamEndCapture(s, &g);
+                if (el != hipSuccess) e = el;
+            }
+            if (e == hipSuccess) e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+            if (g) (void)hipGraphDestroy(g);
+            if (e == hipSuccess)
+            {
+                fn->pcg_graph = gexec;
+                fn->pcg_graph_key = std::move(key);
+            }
+            else
+            {
+                // e.g. a caller's stream that cannot be captured: direct launches
+                (void)hipGetLastError();
+                gexec = nullptr;
+                fn->pcg_graph_failed = true;
+            }
+        }
+    }
     while (k <= o.max_iters)
     {
-        for (int c = 0; c < chunk && k <= o.max_iters; ++c, ++k)
-            TAD_HIP(launch_pcg_iteration(d, nh, fn->brow_ptr.p, fn->bcol.p, fn->row_ptr.p, H_dev, o.w_identity,
-                                         fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->p.p, fn->q.p, fn->pcg_parts.p,
-                                         fn->scal.p, k, k < o.max_iters, free_dev, co, s, timer));
+        if (gexec)
+        {
+            for (int c = 0; c < chunk && k <= o.max_iters; c += kGraphIters, k += kGraphIters)
+                TAD_HIP(hipGraphLaunch(gexec, s));
+        }
+        else
+        {
+            for (int c = 0; c < chunk && k <= o.max_iters; ++c, ++k)
+                TAD_HIP(launch_pcg_iteration(d, nh, fn->brow_ptr.p, fn->bcol.p, fn->row_ptr.p, H_dev, o.w_identity,
+                                             fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->p.p, fn->q.p, fn->pcg_parts.p,
+                                             fn->scal.p, k, k < o.max_iters, free_dev, co, s, timer));
+        }
         TAD_HIP(hipMemcpyAsync(pin, fn->scal.p, sizeof(hs), hipMemcpyDeviceToHost, s));
         TAD_HIP(hipStreamSynchronize(s));
         if (pin->done) break;
@@ -1461,6 +1553,7 @@ tad_status tad_fn_destroy(tad_fn fn)
     if (!fn) return TAD_OK;
     (void)hipSetDevice(fn->ctx->device);
     (void)hipStreamSynchronize(fn->s());
+    if (fn->pcg_graph) (void)hipGraphExecDestroy(fn->pcg_graph);
     for (TermDev& t : fn->terms)
     {
         if (t.conn) (void)hipFree(t.conn);
@@ -2513,6 +2606,14 @@ tad_status tad_fn_get_pipeline(tad_fn fn, int* requested, int* used)
         if (!fn->pipe_ready) plan_pipeline(fn);
         *used = fn->pipe_used;
     }
+    return TAD_OK;
+}
+
+tad_status tad_fn_set_solve_graphs(tad_fn fn, int mode)
+{
+    TAD_TRY(check_fn(fn));
+    if (mode < 0 || mode > 1) return fail(TAD_ERR_INVALID, "solve graph mode must be 0 or 1");
+    fn->solve_graphs = mode;
     return TAD_OK;
 }
 

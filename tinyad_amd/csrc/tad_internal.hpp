@@ -64,6 +64,7 @@ struct PcgScalars
     double bb, rr, tol2;
     double rho_hist[2], alpha_hist[2];  // rho_k and alpha_k, slot k & 1
     int iters, max_iters, done, breakdown;
+    int k;  // iteration index for launches that read it from here (graph replays)
 };
 
 // Kernel timing hook (set by the host library when timing is enabled).
@@ -220,8 +221,9 @@ hipError_t launch_symmetrize_lower(int64_t n, double* A, hipStream_t s);
 // and the prolongation in one launch, with the partials of r_c . A_c^-1 r_c
 // into slot `slot`); every kernel returns at once once the solve has
 // terminated.
+// kdev (optional): the slot is (*kdev) & 1, read on the device.
 hipError_t launch_coarse_correction(int d, int64_t nh, const CoarseDev& c, const double* r, double* z, int slot,
-                                    PcgScalars* sc, hipStream_t s, Timer* timer);
+                                    PcgScalars* sc, hipStream_t s, Timer* timer, const int* kdev = nullptr);
 // free_vars (optional, n bytes, 0 = fixed): CG on the free variables only, the
 // reduced basis of newton_direction_reduced_basis (NewtonDirection.hh:76-105)
 // for a basis B that selects variables; fixed variables of the solution are 0.
@@ -238,12 +240,16 @@ hipError_t launch_pcg_init(int d, int64_t n_handles, const double* b, const unsi
 // One CG iteration k: SpMV on z with the termination test, then (with_update)
 // the scalar recurrences and the p / q / x / r / z update, and the coarse
 // correction of z when `coarse` has kc > 0.  pbuf holds 2 * n values (p, then
-// s = (w I + H) z).
+// s = (w I + H) z).  read_k: the kernels take k from sc->k on the device
+// instead of the argument (launches recorded once and replayed, with
+// launch_pcg_advance after each iteration).
 hipError_t launch_pcg_iteration(int d, int64_t n_handles, const int32_t* brow_ptr, const int32_t* bcol,
                                 const int32_t* row_ptr, const double* H, double w, const double* Dinv, double* x,
                                 double* r, double* z, double* pbuf, double* q, double* partials, PcgScalars* sc, int k,
                                 bool with_update, const unsigned char* free_vars, const CoarseDev* coarse,
-                                hipStream_t s, Timer* timer);
+                                hipStream_t s, Timer* timer, bool read_k = false);
+// sc->k += 1 (one lane)
+hipError_t launch_pcg_advance(PcgScalars* sc, hipStream_t s);
 
 // General reduced-basis solves (tad_basis.hip): y = A x (+ w add) for a CSR
 // matrix of nrows rows; dinv = 1 / diag(B^T H B + w I) with B^T given as CSR

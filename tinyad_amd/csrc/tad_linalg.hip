@@ -634,9 +634,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t nh, const int32_t* __re
                                                  const int32_t* __restrict__ bcol, const int32_t* __restrict__ row_ptr,
                                                  const double* __restrict__ H, double w, const double* __restrict__ z,
                                                  double* __restrict__ sv, double* __restrict__ parts, int Gs, int Gv,
-                                                 int k, PcgScalars* sc)
+                                                 int k, const int* __restrict__ kdev, PcgScalars* sc)
 {
     static_assert(kBlock * 4 >= 1024, "the residual partials (at most 1024) are read four per thread");
+    if (kdev) k = *kdev;
     __shared__ double sh[kBlock];
     __shared__ int flag;
     const int done_v = threadIdx.x == 0 ? sc->done : 0;
@@ -787,10 +788,12 @@ __global__ __launch_bounds__(kUpdVerts * D) void k_pcg_update(int64_t nh, const 
                                                               double* __restrict__ q,
                                                               const double* __restrict__ sv,
                                                               double* __restrict__ parts, int Gs, int Gv, int k,
+                                                              const int* __restrict__ kdev,
                                                               const double* __restrict__ cparts, int Gc,
                                                               const unsigned char* __restrict__ free_vars,
                                                               PcgScalars* sc)
 {
+    if (kdev) k = *kdev;
     constexpr int NT = kUpdVerts * D;
     constexpr int PB = PackedBlock<D>::n;
     constexpr int ND = (kUpdVerts * PB + NT - 1) / NT;  // packed block entries per thread and chunk
@@ -1196,9 +1199,12 @@ __global__ __launch_bounds__(kCoarseBlock) void k_coarse_apply_prolong(int64_t n
                                                                        const double* __restrict__ dvec,
                                                                        const unsigned char* __restrict__ free_vars,
                                                                        double* __restrict__ z,
-                                                                       double* __restrict__ cparts, PcgScalars* sc)
+                                                                       double* __restrict__ cparts_all, int slot,
+                                                                       int Gc, const int* __restrict__ kdev,
+                                                                       PcgScalars* sc)
 {
     if (block_done(sc)) return;
+    double* __restrict__ cparts = cparts_all + (int64_t)(kdev ? (*kdev & 1) : slot) * Gc;
     constexpr int NW = kCoarseBlock / 64;
     __shared__ double wpart[NW][KC];
     __shared__ double y[KC];
@@ -1315,7 +1321,7 @@ hipError_t launch_symmetrize_lower(int64_t n, double* A, hipStream_t s)
 
 template <int D, int KC>
 static void coarse_correction_t(int64_t nh, const CoarseDev& c, const double* r, double* z, int slot, PcgScalars* sc,
-                                hipStream_t s)
+                                hipStream_t s, const int* kdev)
 {
     if (c.max_agg_chunks <= kRestrictMaxChunks)
     {
@@ -1331,19 +1337,19 @@ static void coarse_correction_t(int64_t nh, const CoarseDev& c, const double* r,
         k_coarse_restrict_sum<KC><<<gs, kBlock, 0, s>>>(c.n_agg, c.agg_chunk, c.chunk_part, c.rc, sc);
     }
     k_coarse_apply_prolong<D, KC><<<(unsigned)c.n_agg, kCoarseBlock, 0, s>>>(
-        c.nc, c.Ainv, c.rc, c.agg_ptr, c.agg_vert, c.dvec, c.free_vars, z, c.cparts + (int64_t)slot * c.Gc, sc);
+        c.nc, c.Ainv, c.rc, c.agg_ptr, c.agg_vert, c.dvec, c.free_vars, z, c.cparts, slot, c.Gc, kdev, sc);
 }
 
 hipError_t launch_coarse_correction(int d, int64_t nh, const CoarseDev& c, const double* r, double* z, int slot,
-                                    PcgScalars* sc, hipStream_t s, Timer* timer)
+                                    PcgScalars* sc, hipStream_t s, Timer* timer, const int* kdev)
 {
     if (c.kc == 0 || c.n_agg == 0) return hipSuccess;
     if (timer) timer->begin("pcg_coarse", s);
-    if (d == 3 && c.kc == 6) coarse_correction_t<3, 6>(nh, c, r, z, slot, sc, s);
-    else if (d == 3 && c.kc == 3) coarse_correction_t<3, 3>(nh, c, r, z, slot, sc, s);
-    else if (d == 2 && c.kc == 3) coarse_correction_t<2, 3>(nh, c, r, z, slot, sc, s);
-    else if (d == 2 && c.kc == 2) coarse_correction_t<2, 2>(nh, c, r, z, slot, sc, s);
-    else if (d == 1 && c.kc == 1) coarse_correction_t<1, 1>(nh, c, r, z, slot, sc, s);
+    if (d == 3 && c.kc == 6) coarse_correction_t<3, 6>(nh, c, r, z, slot, sc, s, kdev);
+    else if (d == 3 && c.kc == 3) coarse_correction_t<3, 3>(nh, c, r, z, slot, sc, s, kdev);
+    else if (d == 2 && c.kc == 3) coarse_correction_t<2, 3>(nh, c, r, z, slot, sc, s, kdev);
+    else if (d == 2 && c.kc == 2) coarse_correction_t<2, 2>(nh, c, r, z, slot, sc, s, kdev);
+    else if (d == 1 && c.kc == 1) coarse_correction_t<1, 1>(nh, c, r, z, slot, sc, s, kdev);
     else return hipErrorInvalidValue;
     if (timer) timer->end("pcg_coarse", s);
     return hipGetLastError();
@@ -1400,8 +1406,9 @@ hipError_t launch_pcg_iteration(int d, int64_t nh, const int32_t* brow_ptr, cons
                                 const int32_t* row_ptr, const double* H, double w, const double* Dinv, double* x,
                                 double* r, double* z, double* pbuf, double* q, double* parts, PcgScalars* sc, int k,
                                 bool with_update, const unsigned char* free_vars, const CoarseDev* co, hipStream_t s,
-                                Timer* timer)
+                                Timer* timer, bool read_k)
 {
+    const int* kdev = read_k ? &sc->k : nullptr;
     const bool coarse = co && co->kc > 0 && co->n_agg > 0;
     const double* cparts = coarse ? co->cparts : nullptr;
     const int Gc = coarse ? co->Gc : 0;
@@ -1410,20 +1417,20 @@ hipError_t launch_pcg_iteration(int d, int64_t nh, const int32_t* brow_ptr, cons
     double* p = pbuf;
     double* sv = pbuf + nh * d;
     if (timer) timer->begin("spmv", s);
-    if (d == 1) k_spmv<1, 8><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, sv, parts, Gs, Gv, k, sc);
-    else if (d == 2) k_spmv<2, 8><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, sv, parts, Gs, Gv, k, sc);
-    else k_spmv<3, 16><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, sv, parts, Gs, Gv, k, sc);
+    if (d == 1) k_spmv<1, 8><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, sv, parts, Gs, Gv, k, kdev, sc);
+    else if (d == 2) k_spmv<2, 8><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, sv, parts, Gs, Gv, k, kdev, sc);
+    else k_spmv<3, 16><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, sv, parts, Gs, Gv, k, kdev, sc);
     if (timer) timer->end("spmv", s);
     if (!with_update) return hipGetLastError();
     if (timer) timer->begin("pcg_update", s);
     if (d == 1)
-        k_pcg_update<1><<<Gv, kUpdVerts, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, cparts, Gc,
+        k_pcg_update<1><<<Gv, kUpdVerts, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, kdev, cparts, Gc,
                                                   free_vars, sc);
     else if (d == 2)
-        k_pcg_update<2><<<Gv, 2 * kUpdVerts, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, cparts, Gc,
+        k_pcg_update<2><<<Gv, 2 * kUpdVerts, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, kdev, cparts, Gc,
                                                       free_vars, sc);
     else
-        k_pcg_update<3><<<Gv, 3 * kUpdVerts, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, cparts, Gc,
+        k_pcg_update<3><<<Gv, 3 * kUpdVerts, 0, s>>>(nh, Dinv, x, r, z, p, q, sv, parts, Gs, Gv, k, kdev, cparts, Gc,
                                                       free_vars, sc);
     if (timer) timer->end("pcg_update", s);
     // z of iteration k + 1 gets the coarse correction; its r.z partials go to
@@ -1432,8 +1439,19 @@ hipError_t launch_pcg_iteration(int d, int64_t nh, const int32_t* brow_ptr, cons
     {
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        return launch_coarse_correction(d, nh, *co, r, z, k & 1, sc, s, timer);
+        return launch_coarse_correction(d, nh, *co, r, z, k & 1, sc, s, timer, kdev);
     }
+    return hipGetLastError();
+}
+
+__global__ void k_pcg_advance(PcgScalars* sc)
+{
+    if (threadIdx.x == 0) sc->k += 1;
+}
+
+hipError_t launch_pcg_advance(PcgScalars* sc, hipStream_t s)
+{
+    k_pcg_advance<<<1, 64, 0, s>>>(sc);
     return hipGetLastError();
 }
 
