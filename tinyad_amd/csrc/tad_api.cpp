@@ -459,13 +459,6 @@ tad_status build_pattern(tad_fn fn)
         {
             const HostTerm& t = fn->hterms[ti];
             if (fn->terms[ti].dev_ops.element || t.N < 2 || t.n_elem == 0 || t.repeats) continue;
-            // FP32 contributions of tets keep one slot per element block: the
-            // pass loops of the grouped sums raise the FP32 tet kernel from
-            // 152 to 191 registers (3 -> 2 waves per SIMD), which costs more
-            // (element kernel 0.51 -> 0.73 ms on the 60^3 cube) than the
-            // shorter assembly saves; FP64 tets (256 registers either way) and
-            // FP32 triangles gain
-            if (fn->precision != TAD_PREC_FP64 && t.N * fn->d >= 12) continue;
             const int64_t ne = t.n_elem;
             const int NB = t.N * (t.N + 1) / 2;
             const int64_t ng = (ne + kElemGroup - 1) / kElemGroup;
