@@ -510,6 +510,12 @@ __device__ __forceinline__ double symdir_element(const double* X, const double* 
 #pragma unroll
         for (int v = 0; v < N; ++v)
         {
+            // form this vertex row's products here: scheduled ahead of the
+            // previous row's block writes (the grouped sums' pass loops) they
+            // were spilled to scratch memory (FP64 plain grid: 328 -> 220
+            // bytes per lane, same time)
+#pragma unroll
+            for (int q = 0; q < Sym<KJ>::n; ++q) asm volatile("" : "+v"(Km[q]));
             T S1[Dm][KJ];
 #pragma unroll
             for (int i = 0; i < Dm; ++i)
