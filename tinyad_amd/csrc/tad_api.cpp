@@ -247,6 +247,14 @@ struct tad_fn_s
     int64_t n_upper = 0;
     DevBuf<int64_t> uslot_ptr;
     DevBuf<int32_t> uplace;        // 4 per upper block
+    // the solver's copy of H: the lower triangle's blocks (diagonal included)
+    // in CSR order, D*D contiguous values each, the triangle the reference's
+    // SimplicialLDLT reads (LinearSolver.hh:14); per CSR block the stored
+    // block it reads (its own ordinal, or ~ordinal of its mirrored lower block)
+    DevBuf<int32_t> sym_full;      // per stored block: its CSR block
+    DevBuf<int32_t> sym_src;       // per CSR block
+    DevBuf<double> sym_H;          // n_sym * D * D
+    int64_t n_sym = 0;
     int asm_shape = 0;             // Hessian assembly launch shape (tad_fn_autotune)
     std::vector<int32_t> h_brow_ptr, h_bcol;  // host copy of the block pattern (coarse aggregation)
     std::vector<int64_t> h_urow_ptr;          // first upper block of each block row (host)
@@ -357,6 +365,7 @@ void free_pattern(tad_fn fn)
     fn->brow_ptr.release(); fn->bcol.release(); fn->block_row.release(); fn->diag_block.release();
     fn->row_ptr.release(); fn->col_idx.release(); fn->hslot_ptr.release(); fn->gslot_ptr.release();
     fn->block_mirror.release(); fn->uslot_ptr.release(); fn->uplace.release();
+    fn->sym_full.release(); fn->sym_src.release(); fn->sym_H.release();
     fn->co.release();
     fn->h_brow_ptr.clear(); fn->h_bcol.clear(); fn->h_urow_ptr.clear();
     fn->pipe_ready = false;
@@ -546,6 +555,38 @@ tad_status build_pattern(tad_fn fn)
     for (int64_t u = 0; u < nh; ++u)
         for (int32_t k = brow_ptr[u]; k < brow_ptr[u + 1]; ++k)
             mirror[k] = bcol[k] >= u ? k : (int32_t)find_block(bcol[k], (int32_t)u);
+    // the solver's lower-triangle copy: ordinals of the lower blocks, and per
+    // block the stored block it reads, checked against the pattern (mirror
+    // maps each lower block to its upper block; lower_of inverts it)
+    std::vector<int32_t> sym_full, sym_src(nb);
+    {
+        std::vector<int32_t> ord(nb, -1), lower_of(nb, -1);
+        for (int64_t u = 0; u < nh; ++u)
+            for (int32_t k = brow_ptr[u]; k < brow_ptr[u + 1]; ++k)
+                if (bcol[k] <= u)
+                {
+                    ord[k] = (int32_t)sym_full.size();
+                    sym_full.push_back(k);
+                    const int32_t m = mirror[k];
+                    const bool ok = bcol[k] == u ? (m == k)
+                                                 : (m >= 0 && m < nb && block_row[m] == bcol[k] &&
+                                                    bcol[m] == (int32_t)u && lower_of[m] < 0);
+                    if (!ok) return fail(TAD_ERR_UNSUPPORTED, "Hessian mirror map inconsistent");
+                    lower_of[m] = k;
+                }
+        for (int64_t u = 0; u < nh; ++u)
+            for (int32_t k = brow_ptr[u]; k < brow_ptr[u + 1]; ++k)
+            {
+                if (bcol[k] <= u)
+                {
+                    sym_src[k] = ord[k];
+                    continue;
+                }
+                const int32_t l = lower_of[k];
+                if (l < 0 || ord[l] < 0) return fail(TAD_ERR_UNSUPPORTED, "Hessian pattern not symmetric");
+                sym_src[k] = ~ord[l];
+            }
+    }
     std::vector<std::vector<uint32_t>> gpos(nterms), hpos(nterms), gout(nterms);
     {
         std::vector<int64_t> hc(hptr.begin(), hptr.end() - 1), gc(gptr.begin(), gptr.end() - 1);
@@ -673,6 +714,8 @@ tad_status build_pattern(tad_fn fn)
     TAD_HIP(up(fn->block_mirror, mirror));
     TAD_HIP(up(fn->uslot_ptr, uptr));
     TAD_HIP(up(fn->uplace, uplace));
+    TAD_HIP(up(fn->sym_full, sym_full));
+    TAD_HIP(up(fn->sym_src, sym_src));
     for (size_t ti = 0; ti < fn->terms.size(); ++ti)
     {
         TermDev& td = fn->terms[ti];
@@ -709,6 +752,7 @@ tad_status build_pattern(tad_fn fn)
     fn->n_gslots = gptr[nh];
     fn->n_blocks = nb;
     fn->n_upper = (int64_t)uptr.size() - 1;
+    fn->n_sym = (int64_t)sym_full.size();
     fn->nnz = nnz;
     fn->pattern_ready = true;
     return TAD_OK;
@@ -1241,7 +1285,7 @@ std::vector<uint64_t> pcg_graph_key(tad_fn fn, const double* H_dev, const double
     std::vector<uint64_t> k = { (uint64_t)fn->d, (uint64_t)fn->n_handles, u(fn->brow_ptr.p), u(fn->bcol.p),
                                 u(fn->row_ptr.p), u(H_dev), wbits, u(fn->Dinv.p), u(sol_dev), u(fn->r.p),
                                 u(fn->z.p), u(fn->p.p), u(fn->q.p), u(fn->pcg_parts.p), u(fn->scal.p), u(free_dev),
-                                u(s) };
+                                u(s), u(fn->sym_src.p), u(fn->sym_H.p) };
     if (co)
     {
         const uint64_t more[] = { (uint64_t)co->kc, (uint64_t)co->n_agg, (uint64_t)co->nc, u(co->agg_ptr),
@@ -1272,6 +1316,11 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
     TAD_HIP(launch_pcg_setup(d, nh, fn->brow_ptr.p, fn->diag_block.p, fn->row_ptr.p, H_dev, o.w_identity, free_dev,
                              fn->Dinv.p, s));
     tm.end("pcg_setup", s);
+    TAD_HIP(fn->sym_H.ensure((size_t)fn->n_sym * d * d));
+    tm.begin("sym_pack", s);
+    TAD_HIP(launch_sym_pack(d, fn->n_sym, fn->sym_full.p, fn->block_row.p, fn->brow_ptr.p, fn->row_ptr.p, H_dev,
+                            fn->sym_H.p, s));
+    tm.end("sym_pack", s);
     CoarseDev cd;
     bool use_coarse = false, coarse_reused = false;
     TAD_TRY(setup_coarse(fn, H_dev, o, free_dev, &cd, &use_coarse, &coarse_reused));
@@ -1310,9 +1359,9 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
             for (int c = 0; c < kGraphIters; ++c)
             {
                 hipError_t e = launch_pcg_iteration(d, nh, fn->brow_ptr.p, fn->bcol.p, fn->row_ptr.p, H_dev,
-                                                    o.w_identity, fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->p.p,
-                                                    fn->q.p, fn->pcg_parts.p, fn->scal.p, 0, true, free_dev, co, s,
-                                                    nullptr, true);
+                                                    fn->sym_src.p, fn->sym_H.p, o.w_identity, fn->Dinv.p, sol_dev,
+                                                    fn->r.p, fn->z.p, fn->p.p, fn->q.p, fn->pcg_parts.p, fn->scal.p,
+                                                    0, true, free_dev, co, s, nullptr, true);
                 if (e == hipSuccess) e = launch_pcg_advance(fn->scal.p, s);
                 if (e != hipSuccess) return e;
             }
@@ -1360,7 +1409,8 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
         else
         {
             for (int c = 0; c < chunk && k <= o.max_iters; ++c, ++k)
-                TAD_HIP(launch_pcg_iteration(d, nh, fn->brow_ptr.p, fn->bcol.p, fn->row_ptr.p, H_dev, o.w_identity,
+                TAD_HIP(launch_pcg_iteration(d, nh, fn->brow_ptr.p, fn->bcol.p, fn->row_ptr.p, H_dev, fn->sym_src.p,
+                                             fn->sym_H.p, o.w_identity,
                                              fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->p.p, fn->q.p, fn->pcg_parts.p,
                                              fn->scal.p, k, k < o.max_iters, free_dev, co, s, timer));
         }

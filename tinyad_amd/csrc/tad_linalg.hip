@@ -772,6 +772,201 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t nh, const int32_t* __re
     if (threadIdx.x == 0) parts[blockIdx.x] = tot;
 }
 
+// The solver's copy of H (the lower triangle's blocks, diagonal included, in
+// CSR order; D*D contiguous row-major values each), one thread per value.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_sym_pack(int64_t n_sym, const int32_t* __restrict__ sym_full,
+                                                     const int32_t* __restrict__ block_row,
+                                                     const int32_t* __restrict__ brow_ptr,
+                                                     const int32_t* __restrict__ row_ptr,
+                                                     const double* __restrict__ H, double* __restrict__ Hs)
+{
+    constexpr int DD = D * D;
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n_sym * DD) return;
+    const int64_t l = id / DD;
+    const int m = (int)(id - l * DD);
+    const int i = m / D, j = m - i * D;
+    const int32_t b = sym_full[l];
+    const int32_t u = block_row[b];
+    const int32_t b0 = brow_ptr[u];
+    const int64_t nbu = brow_ptr[u + 1] - b0;
+    Hs[id] = H[(int64_t)row_ptr[D * u] + i * D * nbu + (int64_t)D * (b - b0) + j];
+}
+
+// k_spmv's product from the solver's copy of H: block b of row u is stored
+// block src[b], or the transpose of stored block ~src[b] (the lower block
+// (v, u) it mirrors), so every stored off-diagonal block is read twice per
+// product.  For a symmetric H every product term and its order within a row
+// are k_spmv's.  XCD: the workgroups of one XCD (blockIdx % 8, the dispatch
+// order across the eight XCDs) take a contiguous eighth of the row groups in
+// order, so the block an upper block reads belongs to a row that the same
+// XCD streams shortly after, and the second read can hit in that XCD's L2.
+// Requires gridDim.x % 8 == 0 for XCD.
+template <int D, int LPB, bool XCD>
+__global__ __launch_bounds__(kBlock) void k_spmv_sym(int64_t nh, const int32_t* __restrict__ brow_ptr,
+                                                     const int32_t* __restrict__ bcol,
+                                                     const int32_t* __restrict__ src, const double* __restrict__ Hs,
+                                                     double w, const double* __restrict__ z,
+                                                     double* __restrict__ sv, double* __restrict__ parts, int Gs,
+                                                     int Gv, int k, const int* __restrict__ kdev, PcgScalars* sc)
+{
+    static_assert(kBlock * 4 >= 1024, "the residual partials (at most 1024) are read four per thread");
+    constexpr int DD = D * D;
+    if (kdev) k = *kdev;
+    __shared__ double sh[kBlock];
+    __shared__ int flag;
+    const int done_v = threadIdx.x == 0 ? sc->done : 0;
+    const double* rparts = k == 0 ? parts + Gs + 4 * Gv : parts + Gs + 2 * Gv + ((k + 1) & 1) * Gv;
+    double pa[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+    {
+        const int i = threadIdx.x + q * kBlock;
+        pa[q] = i < Gv ? rparts[i] : 0.0;
+    }
+    const int lane = threadIdx.x % LPB;
+    constexpr int rows_per_block = kBlock / LPB;
+    const int64_t n_groups = (nh + rows_per_block - 1) / rows_per_block;
+    int64_t g0, gstep, gend;
+    if constexpr (XCD)
+    {
+        const int64_t per = (n_groups + 7) / 8;
+        const int64_t x = blockIdx.x & 7;
+        g0 = x * per + (blockIdx.x >> 3);
+        gstep = gridDim.x >> 3;
+        gend = min(n_groups, (x + 1) * per);
+    }
+    else
+    {
+        g0 = blockIdx.x;
+        gstep = gridDim.x;
+        gend = n_groups;
+    }
+    auto load_block = [&](int64_t b, double* h) {
+        const int32_t si = src[b];
+        const double* __restrict__ hb = Hs + (int64_t)(si < 0 ? ~si : si) * DD;
+        double t[DD];
+#pragma unroll
+        for (int m = 0; m < DD; ++m) t[m] = hb[m];
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j) h[i * D + j] = si < 0 ? t[j * D + i] : t[i * D + j];
+    };
+    int f_nbu = 0, f_b0 = 0;
+    int64_t f_v = 0;
+    double f_h[DD];
+    if (g0 < gend)
+    {
+        const int64_t u = g0 * rows_per_block + threadIdx.x / LPB;
+        if (u < nh)
+        {
+            f_b0 = brow_ptr[u];
+            f_nbu = brow_ptr[u + 1] - f_b0;
+            if (lane < f_nbu)
+            {
+                f_v = bcol[f_b0 + lane];
+                load_block(f_b0 + lane, f_h);
+            }
+        }
+    }
+    if (threadIdx.x == 0) flag = done_v;
+    __syncthreads();
+    if (flag) return;
+    double racc = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if ((int)threadIdx.x + q * kBlock < Gv) racc += pa[q];
+    const double rsum = block_sum_n<kBlock>(racc, sh);
+    double rr, bb;
+    if (k == 0)
+    {
+        bb = rsum;
+        rr = bb;
+    }
+    else
+    {
+        rr = rsum;
+        bb = sc->bb;
+    }
+    const bool stop = rr <= sc->tol2 * bb || k >= sc->max_iters;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        if (k == 0) sc->bb = bb;
+        sc->rr = rr;
+        sc->iters = k;
+        if (stop) sc->done = 1;
+    }
+    if (stop) return;
+    double zs = 0.0;
+    bool first = true;
+    for (int64_t g = g0; g < gend; g += gstep)
+    {
+        const int64_t u = g * rows_per_block + threadIdx.x / LPB;
+        double acc[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) acc[i] = 0.0;
+        if (u < nh)
+        {
+            int b0, nbu, kk0;
+            if (first)
+            {
+                b0 = f_b0;
+                nbu = f_nbu;
+                if (lane < nbu)
+                {
+                    double zv[D];
+#pragma unroll
+                    for (int j = 0; j < D; ++j) zv[j] = z[f_v * D + j];
+#pragma unroll
+                    for (int i = 0; i < D; ++i)
+#pragma unroll
+                        for (int j = 0; j < D; ++j) acc[i] += f_h[i * D + j] * zv[j];
+                }
+                kk0 = lane + LPB;
+            }
+            else
+            {
+                b0 = brow_ptr[u];
+                nbu = brow_ptr[u + 1] - b0;
+                kk0 = lane;
+            }
+            for (int kk = kk0; kk < nbu; kk += LPB)
+            {
+                const int64_t v = bcol[b0 + kk];
+                double h[DD];
+                load_block(b0 + kk, h);
+                double zv[D];
+#pragma unroll
+                for (int j = 0; j < D; ++j) zv[j] = z[v * D + j];
+#pragma unroll
+                for (int i = 0; i < D; ++i)
+#pragma unroll
+                    for (int j = 0; j < D; ++j) acc[i] += h[i * D + j] * zv[j];
+            }
+        }
+        first = false;
+#pragma unroll
+        for (int off = LPB / 2; off > 0; off >>= 1)
+#pragma unroll
+            for (int i = 0; i < D; ++i) acc[i] += __shfl_xor(acc[i], off, LPB);
+        if (u < nh && lane == 0)
+        {
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+            {
+                const double zi = z[u * D + i];
+                const double si = acc[i] + w * zi;
+                sv[u * D + i] = si;
+                zs += zi * si;
+            }
+        }
+    }
+    const double tot = block_sum(zs, sh);
+    if (threadIdx.x == 0) parts[blockIdx.x] = tot;
+}
+
 // CG iteration k, second half: scalars, direction recurrences, x / r / z
 // update; partials of rho_{k+1} = r.z and ||r||^2 into slot k & 1.  alpha and
 // rho of iteration k are kept (slot k & 1 of sc) for iteration k + 1.
@@ -1402,8 +1597,34 @@ hipError_t launch_pcg_init(int d, int64_t nh, const double* b, const unsigned ch
     return hipGetLastError();
 }
 
+hipError_t launch_sym_pack(int d, int64_t n_sym, const int32_t* sym_full, const int32_t* block_row,
+                           const int32_t* brow_ptr, const int32_t* row_ptr, const double* H, double* sym_H,
+                           hipStream_t s)
+{
+    if (n_sym == 0) return hipSuccess;
+    const unsigned grid = (unsigned)((n_sym * d * d + kBlock - 1) / kBlock);
+    if (d == 1) k_sym_pack<1><<<grid, kBlock, 0, s>>>(n_sym, sym_full, block_row, brow_ptr, row_ptr, H, sym_H);
+    else if (d == 2) k_sym_pack<2><<<grid, kBlock, 0, s>>>(n_sym, sym_full, block_row, brow_ptr, row_ptr, H, sym_H);
+    else k_sym_pack<3><<<grid, kBlock, 0, s>>>(n_sym, sym_full, block_row, brow_ptr, row_ptr, H, sym_H);
+    return hipGetLastError();
+}
+
+template <int D, int LPB>
+static void spmv_sym_t(int64_t nh, const int32_t* brow_ptr, const int32_t* bcol, const int32_t* src,
+                       const double* Hs, double w, const double* z, double* sv, double* parts, int Gs, int Gv, int k,
+                       const int* kdev, PcgScalars* sc, hipStream_t s)
+{
+    if (Gs % 8 == 0)
+        k_spmv_sym<D, LPB, true><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, src, Hs, w, z, sv, parts, Gs, Gv, k,
+                                                        kdev, sc);
+    else
+        k_spmv_sym<D, LPB, false><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, src, Hs, w, z, sv, parts, Gs, Gv, k,
+                                                         kdev, sc);
+}
+
 hipError_t launch_pcg_iteration(int d, int64_t nh, const int32_t* brow_ptr, const int32_t* bcol,
-                                const int32_t* row_ptr, const double* H, double w, const double* Dinv, double* x,
+                                const int32_t* row_ptr, const double* H, const int32_t* sym_src,
+                                const double* sym_H, double w, const double* Dinv, double* x,
                                 double* r, double* z, double* pbuf, double* q, double* parts, PcgScalars* sc, int k,
                                 bool with_update, const unsigned char* free_vars, const CoarseDev* co, hipStream_t s,
                                 Timer* timer, bool read_k)
@@ -1417,7 +1638,13 @@ hipError_t launch_pcg_iteration(int d, int64_t nh, const int32_t* brow_ptr, cons
     double* p = pbuf;
     double* sv = pbuf + nh * d;
     if (timer) timer->begin("spmv", s);
-    if (d == 1) k_spmv<1, 8><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, sv, parts, Gs, Gv, k, kdev, sc);
+    if (sym_H)
+    {
+        if (d == 1) spmv_sym_t<1, 8>(nh, brow_ptr, bcol, sym_src, sym_H, w, z, sv, parts, Gs, Gv, k, kdev, sc, s);
+        else if (d == 2) spmv_sym_t<2, 8>(nh, brow_ptr, bcol, sym_src, sym_H, w, z, sv, parts, Gs, Gv, k, kdev, sc, s);
+        else spmv_sym_t<3, 16>(nh, brow_ptr, bcol, sym_src, sym_H, w, z, sv, parts, Gs, Gv, k, kdev, sc, s);
+    }
+    else if (d == 1) k_spmv<1, 8><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, sv, parts, Gs, Gv, k, kdev, sc);
     else if (d == 2) k_spmv<2, 8><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, sv, parts, Gs, Gv, k, kdev, sc);
     else k_spmv<3, 16><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, sv, parts, Gs, Gv, k, kdev, sc);
     if (timer) timer->end("spmv", s);
