@@ -468,11 +468,15 @@ def main():
     prec_tag = args.precision
     tag = f"{args.workload}_{args.m if args.workload == 'twist_cube' else args.disks}_{prec_tag}"
     if dom == "spmv":
-        # s = (w I + H) z: CSR values, block columns, block-row offsets, the
-        # first scalar-row offset of each block row, z read once, s written
-        traffic_alg = 8 * nnz + 4 * nb + 4 * (P.n_handles + 1) + 4 * P.n_handles + 2 * 8 * n
-        roof = {"kernel": "k_spmv<3,16>", "bound": "hbm", "achieved": traffic_alg / avg_s / 1e9, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "bytes_per_launch": traffic_alg, "pmc_name": "tad::k_spmv<3, 16>"}
+        # s = (w I + H) z from the solver's packed lower triangle: its blocks
+        # once (every vertex has its diagonal block here, so n_b + n_v blocks
+        # make both triangles), per CSR block its column and its stored-block
+        # index, block-row offsets, z read once, s written
+        n_sym = (nb + P.n_handles) // 2
+        traffic_alg = 72 * n_sym + 8 * nb + 4 * (P.n_handles + 1) + 2 * 8 * n
+        roof = {"kernel": "k_spmv_sym<3,8,true>", "bound": "hbm", "achieved": traffic_alg / avg_s / 1e9,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "bytes_per_launch": traffic_alg,
+                "pmc_name": "tad::k_spmv_sym<3, 8, true>"}
     elif dom == "assemble_hessian":
         tsz = 8 if args.precision == "fp64" else 4
         d = P.d

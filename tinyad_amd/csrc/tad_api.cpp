@@ -1358,8 +1358,8 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
         auto launch_chunk = [&]() -> hipError_t {
             for (int c = 0; c < kGraphIters; ++c)
             {
-                hipError_t e = launch_pcg_iteration(d, nh, fn->brow_ptr.p, fn->bcol.p, fn->row_ptr.p, H_dev,
-                                                    fn->sym_src.p, fn->sym_H.p, o.w_identity, fn->Dinv.p, sol_dev,
+                hipError_t e = launch_pcg_iteration(d, nh, fn->brow_ptr.p, fn->bcol.p, fn->sym_src.p,
+                                                    fn->sym_H.p, o.w_identity, fn->Dinv.p, sol_dev,
                                                     fn->r.p, fn->z.p, fn->p.p, fn->q.p, fn->pcg_parts.p, fn->scal.p,
                                                     0, true, free_dev, co, s, nullptr, true);
                 if (e == hipSuccess) e = launch_pcg_advance(fn->scal.p, s);
@@ -1409,8 +1409,8 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
         else
         {
             for (int c = 0; c < chunk && k <= o.max_iters; ++c, ++k)
-                TAD_HIP(launch_pcg_iteration(d, nh, fn->brow_ptr.p, fn->bcol.p, fn->row_ptr.p, H_dev, fn->sym_src.p,
-                                             fn->sym_H.p, o.w_identity,
+                TAD_HIP(launch_pcg_iteration(d, nh, fn->brow_ptr.p, fn->bcol.p, fn->sym_src.p, fn->sym_H.p,
+                                             o.w_identity,
                                              fn->Dinv.p, sol_dev, fn->r.p, fn->z.p, fn->p.p, fn->q.p, fn->pcg_parts.p,
                                              fn->scal.p, k, k < o.max_iters, free_dev, co, s, timer));
         }

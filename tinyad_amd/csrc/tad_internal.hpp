@@ -249,12 +249,11 @@ hipError_t launch_sym_pack(int d, int64_t n_sym, const int32_t* sym_full, const 
 // correction of z when `coarse` has kc > 0.  pbuf holds 2 * n values (p, then
 // s = (w I + H) z).  read_k: the kernels take k from sc->k on the device
 // instead of the argument (launches recorded once and replayed, with
-// launch_pcg_advance after each iteration).  With sym_H (packed by
-// launch_sym_pack) the SpMV reads the lower triangle only: CSR block b reads
-// stored block sym_src[b], or the transpose of ~sym_src[b] when negative.
+// launch_pcg_advance after each iteration).  The SpMV reads H from sym_H
+// (packed by launch_sym_pack): CSR block b reads stored block sym_src[b], or
+// the transpose of ~sym_src[b] when negative.
 hipError_t launch_pcg_iteration(int d, int64_t n_handles, const int32_t* brow_ptr, const int32_t* bcol,
-                                const int32_t* row_ptr, const double* H, const int32_t* sym_src,
-                                const double* sym_H, double w, const double* Dinv, double* x,
+                                const int32_t* sym_src, const double* sym_H, double w, const double* Dinv, double* x,
                                 double* r, double* z, double* pbuf, double* q, double* partials, PcgScalars* sc, int k,
                                 bool with_update, const unsigned char* free_vars, const CoarseDev* coarse,
                                 hipStream_t s, Timer* timer, bool read_k = false);

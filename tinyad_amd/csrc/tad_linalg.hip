@@ -6,7 +6,7 @@
 //                      through the precomputed maps (replaces the serial
 //                      triplet loop ScalarObjectiveTerm.hh:257-277 and
 //                      setFromTriplets, ScalarFunctionImpl.hh:398)
-//   k_pcg_*, k_spmv    block-Jacobi PCG on the block-structured CSR
+//   k_pcg_*, k_spmv_sym  block-Jacobi PCG on the block-structured CSR
 //                      (replaces SimplicialLDLT, NewtonDirection.hh:25-48)
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -625,153 +625,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t nh, const double* _
 //   x += alpha p,  r -= alpha q,  z = M^-1 r
 // so the SpMV gathers only z, which is complete when it runs.
 
-// CG iteration k, first half: termination test on rr_k = ||r_k||^2 (k = 0:
-// ||b||^2, recorded), then s = (w I + H) z and partial z.s.  The termination
-// flag, the residual partials and the first row group's block-row data are
-// all requested before the test, so their memory latencies overlap.
-template <int D, int LPB>
-__global__ __launch_bounds__(kBlock) void k_spmv(int64_t nh, const int32_t* __restrict__ brow_ptr,
-                                                 const int32_t* __restrict__ bcol, const int32_t* __restrict__ row_ptr,
-                                                 const double* __restrict__ H, double w, const double* __restrict__ z,
-                                                 double* __restrict__ sv, double* __restrict__ parts, int Gs, int Gv,
-                                                 int k, const int* __restrict__ kdev, PcgScalars* sc)
-{
-    static_assert(kBlock * 4 >= 1024, "the residual partials (at most 1024) are read four per thread");
-    if (kdev) k = *kdev;
-    __shared__ double sh[kBlock];
-    __shared__ int flag;
-    const int done_v = threadIdx.x == 0 ? sc->done : 0;
-    const double* rparts = k == 0 ? parts + Gs + 4 * Gv : parts + Gs + 2 * Gv + ((k + 1) & 1) * Gv;
-    double pa[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-    {
-        const int i = threadIdx.x + q * kBlock;
-        pa[q] = i < Gv ? rparts[i] : 0.0;
-    }
-    const int lane = threadIdx.x % LPB;
-    constexpr int rows_per_block = kBlock / LPB;
-    int64_t u0 = (int64_t)blockIdx.x * rows_per_block;
-    int f_nbu = 0;
-    int64_t f_base = 0, f_v = 0;
-    double f_h[D * D];
-    {
-        const int64_t u = u0 + threadIdx.x / LPB;
-        if (u < nh)
-        {
-            const int b0 = brow_ptr[u];
-            f_nbu = brow_ptr[u + 1] - b0;
-            f_base = row_ptr[D * u];
-            if (lane < f_nbu)
-            {
-                f_v = bcol[b0 + lane];
-#pragma unroll
-                for (int i = 0; i < D; ++i)
-#pragma unroll
-                    for (int j = 0; j < D; ++j) f_h[i * D + j] = H[f_base + (int64_t)i * D * f_nbu + D * lane + j];
-            }
-        }
-    }
-    if (threadIdx.x == 0) flag = done_v;
-    __syncthreads();
-    if (flag) return;
-    // the partials in index order per thread, then the fixed block tree (the
-    // order of sum_parts)
-    double racc = 0.0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-        if ((int)threadIdx.x + q * kBlock < Gv) racc += pa[q];
-    const double rsum = block_sum_n<kBlock>(racc, sh);
-    double rr, bb;
-    if (k == 0)
-    {
-        bb = rsum;
-        rr = bb;
-    }
-    else
-    {
-        rr = rsum;
-        bb = sc->bb;
-    }
-    const bool stop = rr <= sc->tol2 * bb || k >= sc->max_iters;
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-    {
-        if (k == 0) sc->bb = bb;
-        sc->rr = rr;
-        sc->iters = k;
-        if (stop) sc->done = 1;
-    }
-    if (stop) return;
-    double zs = 0.0;
-    bool first = true;
-    for (; u0 < nh; u0 += (int64_t)gridDim.x * rows_per_block)
-    {
-        const int64_t u = u0 + threadIdx.x / LPB;
-        double acc[D];
-#pragma unroll
-        for (int i = 0; i < D; ++i) acc[i] = 0.0;
-        if (u < nh)
-        {
-            int b0, nbu, kk0;
-            int64_t base;
-            if (first)
-            {
-                nbu = f_nbu;
-                base = f_base;
-                b0 = 0;
-                if (lane < nbu)
-                {
-                    double zv[D];
-#pragma unroll
-                    for (int j = 0; j < D; ++j) zv[j] = z[f_v * D + j];
-#pragma unroll
-                    for (int i = 0; i < D; ++i)
-#pragma unroll
-                        for (int j = 0; j < D; ++j) acc[i] += f_h[i * D + j] * zv[j];
-                }
-                kk0 = lane + LPB;
-                if (kk0 < nbu) b0 = brow_ptr[u];
-            }
-            else
-            {
-                b0 = brow_ptr[u];
-                nbu = brow_ptr[u + 1] - b0;
-                base = row_ptr[D * u];
-                kk0 = lane;
-            }
-            for (int kk = kk0; kk < nbu; kk += LPB)
-            {
-                const int64_t v = bcol[b0 + kk];
-                double zv[D];
-#pragma unroll
-                for (int j = 0; j < D; ++j) zv[j] = z[v * D + j];
-#pragma unroll
-                for (int i = 0; i < D; ++i)
-#pragma unroll
-                    for (int j = 0; j < D; ++j) acc[i] += H[base + (int64_t)i * D * nbu + D * kk + j] * zv[j];
-            }
-        }
-        first = false;
-#pragma unroll
-        for (int off = LPB / 2; off > 0; off >>= 1)
-#pragma unroll
-            for (int i = 0; i < D; ++i) acc[i] += __shfl_xor(acc[i], off, LPB);
-        if (u < nh && lane == 0)
-        {
-#pragma unroll
-            for (int i = 0; i < D; ++i)
-            {
-                const double zi = z[u * D + i];
-                const double si = acc[i] + w * zi;
-                sv[u * D + i] = si;
-                zs += zi * si;
-            }
-        }
-    }
-    const double tot = block_sum(zs, sh);
-    if (threadIdx.x == 0) parts[blockIdx.x] = tot;
-}
-
 // The solver's copy of H (the lower triangle's blocks, diagonal included, in
 // CSR order; D*D contiguous row-major values each), one thread per value.
 template <int D>
@@ -794,12 +647,16 @@ __global__ __launch_bounds__(kBlock) void k_sym_pack(int64_t n_sym, const int32_
     Hs[id] = H[(int64_t)row_ptr[D * u] + i * D * nbu + (int64_t)D * (b - b0) + j];
 }
 
-// k_spmv's product from the solver's copy of H: block b of row u is stored
+// CG iteration k, first half: termination test on rr_k = ||r_k||^2 (k = 0:
+// ||b||^2, recorded), then s = (w I + H) z and partial z.s.  The termination
+// flag, the residual partials and the first row group's block data are all
+// requested before the test, so their memory latencies overlap.  LPB lanes
+// per block row, lane k takes blocks k, k + LPB, ... of the row, then a
+// shuffle tree.  H comes from the solver's copy: block b of row u is stored
 // block src[b], or the transpose of stored block ~src[b] (the lower block
 // (v, u) it mirrors), so every stored off-diagonal block is read twice per
-// product.  For a symmetric H every product term and its order within a row
-// are k_spmv's.  XCD: the workgroups of one XCD (blockIdx % 8, the dispatch
-// order across the eight XCDs) take a contiguous eighth of the row groups in
+// product.  XCD: the workgroups of one XCD (blockIdx % 8, the dispatch order
+// across the eight XCDs) take a contiguous eighth of the row groups in
 // order, so the block an upper block reads belongs to a row that the same
 // XCD streams shortly after, and the second read can hit in that XCD's L2.
 // Requires gridDim.x % 8 == 0 for XCD.
@@ -1550,13 +1407,19 @@ hipError_t launch_coarse_correction(int d, int64_t nh, const CoarseDev& c, const
     return hipGetLastError();
 }
 
+constexpr int kSpmvLanes = 8;
+
 void pcg_grids(int d, int64_t nh, int* Gs, int* Gv)
 {
-    const int lpb = d == 3 ? 16 : 8;
-    const int64_t rows_per_block = kBlock / lpb;
+    // SpMV: kSpmvLanes lanes per block row, at most 1,024 workgroups (60^3
+    // tet cube, 227K block rows: 8 lanes and 1,024 workgroups 40.7 us per
+    // product against 42.1 with 768, 49.4 with 512, 43.1 with 1,536 and 44.8
+    // with 2,048; 16 lanes with 2,048 workgroups 47.3, 4 lanes with 1,536 50.1)
+    (void)d;
+    const int64_t rows_per_block = kBlock / kSpmvLanes;
     const int64_t gs = (nh + rows_per_block - 1) / rows_per_block;
     const int64_t gv = (nh + kUpdVerts - 1) / kUpdVerts;
-    *Gs = (int)std::max<int64_t>(1, std::min<int64_t>(gs, 2048));
+    *Gs = (int)std::max<int64_t>(1, std::min<int64_t>(gs, 1024));
     *Gv = (int)std::max<int64_t>(1, std::min<int64_t>(gv, 1024));
 }
 
@@ -1623,8 +1486,7 @@ static void spmv_sym_t(int64_t nh, const int32_t* brow_ptr, const int32_t* bcol,
 }
 
 hipError_t launch_pcg_iteration(int d, int64_t nh, const int32_t* brow_ptr, const int32_t* bcol,
-                                const int32_t* row_ptr, const double* H, const int32_t* sym_src,
-                                const double* sym_H, double w, const double* Dinv, double* x,
+                                const int32_t* sym_src, const double* sym_H, double w, const double* Dinv, double* x,
                                 double* r, double* z, double* pbuf, double* q, double* parts, PcgScalars* sc, int k,
                                 bool with_update, const unsigned char* free_vars, const CoarseDev* co, hipStream_t s,
                                 Timer* timer, bool read_k)
@@ -1637,16 +1499,14 @@ hipError_t launch_pcg_iteration(int d, int64_t nh, const int32_t* brow_ptr, cons
     pcg_grids(d, nh, &Gs, &Gv);
     double* p = pbuf;
     double* sv = pbuf + nh * d;
+    if (!sym_H || !sym_src) return hipErrorInvalidValue;
     if (timer) timer->begin("spmv", s);
-    if (sym_H)
-    {
-        if (d == 1) spmv_sym_t<1, 8>(nh, brow_ptr, bcol, sym_src, sym_H, w, z, sv, parts, Gs, Gv, k, kdev, sc, s);
-        else if (d == 2) spmv_sym_t<2, 8>(nh, brow_ptr, bcol, sym_src, sym_H, w, z, sv, parts, Gs, Gv, k, kdev, sc, s);
-        else spmv_sym_t<3, 16>(nh, brow_ptr, bcol, sym_src, sym_H, w, z, sv, parts, Gs, Gv, k, kdev, sc, s);
-    }
-    else if (d == 1) k_spmv<1, 8><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, sv, parts, Gs, Gv, k, kdev, sc);
-    else if (d == 2) k_spmv<2, 8><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, sv, parts, Gs, Gv, k, kdev, sc);
-    else k_spmv<3, 16><<<Gs, kBlock, 0, s>>>(nh, brow_ptr, bcol, row_ptr, H, w, z, sv, parts, Gs, Gv, k, kdev, sc);
+    if (d == 1)
+        spmv_sym_t<1, kSpmvLanes>(nh, brow_ptr, bcol, sym_src, sym_H, w, z, sv, parts, Gs, Gv, k, kdev, sc, s);
+    else if (d == 2)
+        spmv_sym_t<2, kSpmvLanes>(nh, brow_ptr, bcol, sym_src, sym_H, w, z, sv, parts, Gs, Gv, k, kdev, sc, s);
+    else
+        spmv_sym_t<3, kSpmvLanes>(nh, brow_ptr, bcol, sym_src, sym_H, w, z, sv, parts, Gs, Gv, k, kdev, sc, s);
     if (timer) timer->end("spmv", s);
     if (!with_update) return hipGetLastError();
     if (timer) timer->begin("pcg_update", s);
