@@ -296,7 +296,9 @@ struct tad_fn_s
         int max_agg_chunks = 0;
         DevBuf<int32_t> agg_of, agg_ptr, agg_vert, pair_blocks, pair_ab, chunk_ptr, agg_chunk, pair_chunk;
         DevBuf<int64_t> gchunk_ptr;
-        DevBuf<double> dvec, Ac, rc, cparts, chunk_part, gpart, inv_work;
+        DevBuf<double> dvec, Ac, rc, cparts, chunk_part, gpart, inv_work, rowsum;
+        DevBuf<float> Ainv32;  // the inverse in FP32 with a diagonal shift, read by every CG iteration
+        bool use32 = false;    // Ainv32 belongs to the current factor
         DevBuf<int> info;
         // the inverse coarse matrix in Ac and the lever arms in dvec belong
         // together and stay usable by a later solve that asks for reuse when
@@ -309,7 +311,7 @@ struct tad_fn_s
             agg_of.release(); agg_ptr.release(); agg_vert.release(); pair_blocks.release(); pair_ab.release();
             chunk_ptr.release(); agg_chunk.release(); pair_chunk.release(); gchunk_ptr.release();
             dvec.release(); Ac.release(); rc.release(); cparts.release(); chunk_part.release();
-            gpart.release(); inv_work.release(); info.release();
+            gpart.release(); inv_work.release(); rowsum.release(); Ainv32.release(); info.release();
             requested = target = 0;
             n_agg = n_pairs = n_chunks = n_gchunks = 0;
             max_agg_chunks = 0;
@@ -910,6 +912,8 @@ constexpr int64_t kMaxCoarse = 8192;
 constexpr int64_t kAutoCoarse = 1400;
 constexpr double kAutoHandles = 30000.0;
 constexpr int64_t kAutoMinAggregate = 32;
+// Largest relative change of the coarse correction accepted from its FP32 copy.
+constexpr double kCoarse32Change = 0.1;
 
 // Per solve: coarse unknowns per aggregate, lever arms, A_c = P^T (w I + H) P,
 // its inverse (blocked Gauss-Jordan sweeps with FP64 MFMA, tad_dense.hip;
@@ -978,6 +982,7 @@ tad_status setup_coarse(tad_fn fn, const double* H_dev, const tad_solve_opts& o,
     cd->chunk_part = c.chunk_part.p;
     cd->dvec = c.dvec.p;
     cd->Ainv = c.Ac.p;
+    cd->Ainv32 = c.use32 ? c.Ainv32.p : nullptr;
     cd->rc = c.rc.p;
     cd->cparts = c.cparts.p;
     cd->Gc = Gc;
@@ -989,6 +994,8 @@ tad_status setup_coarse(tad_fn fn, const double* H_dev, const tad_solve_opts& o,
         return TAD_OK;
     }
     c.factor_valid = false;
+    c.use32 = false;
+    cd->Ainv32 = nullptr;
     tm.begin("coarse_setup", s);
     tm.begin("coarse_galerkin", s);
     if (rot) TAD_HIP(launch_coarse_dvec(d, nh, fn->x_hess.p, *cd, c.dvec.p, s));
@@ -1003,17 +1010,37 @@ tad_status setup_coarse(tad_fn fn, const double* H_dev, const tad_solve_opts& o,
     if (free_dev) TAD_HIP(launch_shift_diagonal(nc, c.Ac.p, 1e-10, s));
     tm.end("coarse_galerkin", s);
     TAD_HIP(c.inv_work.ensure((size_t)spd_inverse_work(nc)));
+    TAD_HIP(c.rowsum.ensure((size_t)nc + 2));  // row sums, then ||A_c||_inf, ||A_c^-1||_inf
+    TAD_HIP(launch_abs_rowmax(nc, c.Ac.p, c.rowsum.p, c.rowsum.p + nc, s));
     tm.begin("coarse_inverse", s);
     TAD_HIP(launch_spd_inverse(nc, c.Ac.p, c.inv_work.p, c.info.p, s));
     tm.end("coarse_inverse", s);
+    TAD_HIP(launch_abs_rowmax(nc, c.Ac.p, c.rowsum.p, c.rowsum.p + nc + 1, s));
     int* info = reinterpret_cast<int*>(fn->ctx->pinned + 96);  // past the solver scalars at 56..
+    double* norms = reinterpret_cast<double*>(fn->ctx->pinned + 104);
     TAD_HIP(hipMemcpyAsync(info, c.info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    TAD_HIP(hipMemcpyAsync(norms, c.rowsum.p + nc, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     TAD_HIP(hipStreamSynchronize(s));
     if (info[0] != 0)
     {
         tm.end("coarse_setup", s);
         cd->kc = 0;
         return TAD_OK;  // not positive definite at working precision: block-Jacobi alone
+    }
+    // The CG iterations read an FP32 copy with the diagonal shift
+    // delta = 2^-23 ||A_c^-1||_inf that keeps it positive definite
+    // (launch_coarse_to32) when the shift is small against the smallest
+    // eigenvalue of A_c^-1, 1 / lambda_max(A_c) >= 1 / ||A_c||_inf:
+    // 2^-22 kappa_inf <= 0.1 bounds the relative change of the coarse
+    // correction by 10 %.  Otherwise (e.g. a Hessian that is singular up to
+    // the projection's eps along rigid motions) they read the FP64 inverse.
+    const double kappa = norms[0] * norms[1];
+    if (std::isfinite(kappa) && std::ldexp(kappa, -22) <= kCoarse32Change)
+    {
+        TAD_HIP(c.Ainv32.ensure((size_t)(nc * nc)));
+        TAD_HIP(launch_coarse_to32(nc, c.Ac.p, std::ldexp(norms[1], -23), c.Ainv32.p, s));
+        c.use32 = true;
+        cd->Ainv32 = c.Ainv32.p;
     }
     tm.end("coarse_setup", s);
     *use = true;
@@ -1290,7 +1317,7 @@ std::vector<uint64_t> pcg_graph_key(tad_fn fn, const double* H_dev, const double
     {
         const uint64_t more[] = { (uint64_t)co->kc, (uint64_t)co->n_agg, (uint64_t)co->nc, u(co->agg_ptr),
                                   u(co->agg_vert), (uint64_t)co->n_chunks, u(co->chunk_ptr), u(co->agg_chunk),
-                                  (uint64_t)co->max_agg_chunks, u(co->chunk_part), u(co->dvec), u(co->Ainv),
+                                  (uint64_t)co->max_agg_chunks, u(co->chunk_part), u(co->dvec), u(co->Ainv32),
                                   u(co->rc), u(co->cparts), (uint64_t)co->Gc, u(co->free_vars) };
         k.insert(k.end(), std::begin(more), std::end(more));
     }

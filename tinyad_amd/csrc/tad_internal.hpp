@@ -178,6 +178,7 @@ struct CoarseDev
     double* chunk_part = nullptr;        // [n_chunks * kc] restriction partials
     const double* dvec = nullptr;    // [n_handles * d]
     const double* Ainv = nullptr;    // [nc * nc] inverse Galerkin matrix (symmetric, both triangles)
+    const float* Ainv32 = nullptr;   // [nc * nc] its FP32 copy for the CG iterations (launch_coarse_to32), or null
     double* rc = nullptr;            // [nc] restricted residual
     double* cparts = nullptr;        // [2 * Gc] partials of r_c . A_c^-1 r_c, slot k & 1
     int Gc = 0;                      // workgroups of the coarse product
@@ -213,6 +214,14 @@ hipError_t launch_spd_inverse(int64_t n, double* A, double* work, int* info, hip
 
 // A_jj <- A_jj (1 + rel), 1 where A_jj = 0 (column-major n x n).
 hipError_t launch_shift_diagonal(int64_t n, double* A, double rel, hipStream_t s);
+// *out = ||A||_inf = max_i sum_j |A_ij| of the n x n matrix A (rowsum: n
+// doubles of work space); fixed-order sums.
+hipError_t launch_abs_rowmax(int64_t n, const double* A, double* rowsum, double* out, hipStream_t s);
+// A32 = fl32(A + delta I).  With delta = 2^-23 ||A||_inf the rounding error E
+// has ||E||_2 <= ||E||_inf <= 2^-24 (||A||_inf + delta) < delta, so for a
+// symmetric positive definite A the difference A32 - A = delta I + E is
+// positive definite and so is A32.
+hipError_t launch_coarse_to32(int64_t n, const double* A, double delta, float* A32, hipStream_t s);
 
 // Copies the lower triangle of the column-major n x n matrix A to the upper.
 hipError_t launch_symmetrize_lower(int64_t n, double* A, hipStream_t s);

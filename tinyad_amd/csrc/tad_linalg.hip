@@ -1243,8 +1243,8 @@ __global__ __launch_bounds__(kCoarseBlock) void k_coarse_restrict(const int32_t*
 // thread t takes the entries t, t + kCoarseBlock, ..., then a fixed wave tree
 // and the waves' partials in order), one partial r_c[a] . y_a, then
 // z_v += R_v y_a for every member v of the aggregate.
-template <int D, int KC>
-__global__ __launch_bounds__(kCoarseBlock) void k_coarse_apply_prolong(int64_t nc, const double* __restrict__ Ainv,
+template <int D, int KC, class TA>
+__global__ __launch_bounds__(kCoarseBlock) void k_coarse_apply_prolong(int64_t nc, const TA* __restrict__ Ainv,
                                                                        const double* __restrict__ rc,
                                                                        const int32_t* __restrict__ agg_ptr,
                                                                        const int32_t* __restrict__ agg_vert,
@@ -1262,7 +1262,7 @@ __global__ __launch_bounds__(kCoarseBlock) void k_coarse_apply_prolong(int64_t n
     __shared__ double y[KC];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t a = blockIdx.x;
-    const double* __restrict__ col = Ainv + a * KC * nc;
+    const TA* __restrict__ col = Ainv + a * KC * nc;
     double acc[KC];
 #pragma unroll
     for (int c = 0; c < KC; ++c) acc[c] = 0.0;
@@ -1270,7 +1270,7 @@ __global__ __launch_bounds__(kCoarseBlock) void k_coarse_apply_prolong(int64_t n
     {
         const double rj = rc[j];
 #pragma unroll
-        for (int c = 0; c < KC; ++c) acc[c] += col[c * nc + j] * rj;
+        for (int c = 0; c < KC; ++c) acc[c] += (double)col[c * nc + j] * rj;
     }
     wave_sum<KC>(acc);
 #pragma unroll
@@ -1364,6 +1364,60 @@ hipError_t launch_shift_diagonal(int64_t n, double* A, double rel, hipStream_t s
     return hipGetLastError();
 }
 
+// |A| row sums, one wave per row: lane-strided partial sums, then a fixed
+// wave tree; k_rowmax then takes their maximum (||A||_inf).
+__global__ __launch_bounds__(kBlock) void k_abs_rowsum(int64_t n, const double* __restrict__ A,
+                                                       double* __restrict__ rowsum)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (i >= n) return;
+    double t = 0.0;
+    for (int64_t j = lane; j < n; j += 64) t += fabs(A[i * n + j]);
+    wave_sum<1>(&t);
+    if (lane == 0) rowsum[i] = t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_rowmax(int64_t n, const double* __restrict__ rowsum,
+                                                   double* __restrict__ out)
+{
+    __shared__ double sh[kBlock];
+    double m = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += kBlock) m = fmax(m, rowsum[i]);
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1)
+    {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_coarse_to32(int64_t n, const double* __restrict__ A, double delta,
+                                                        float* __restrict__ A32)
+{
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n * n) return;
+    const int64_t i = id / n, j = id - i * n;
+    A32[id] = (float)(A[id] + (i == j ? delta : 0.0));
+}
+
+hipError_t launch_abs_rowmax(int64_t n, const double* A, double* rowsum, double* out, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    k_abs_rowsum<<<(unsigned)((n * 64 + kBlock - 1) / kBlock), kBlock, 0, s>>>(n, A, rowsum);
+    k_rowmax<<<1, kBlock, 0, s>>>(n, rowsum, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_coarse_to32(int64_t n, const double* A, double delta, float* A32, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    k_coarse_to32<<<(unsigned)((n * n + kBlock - 1) / kBlock), kBlock, 0, s>>>(n, A, delta, A32);
+    return hipGetLastError();
+}
+
 hipError_t launch_symmetrize_lower(int64_t n, double* A, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
@@ -1388,8 +1442,12 @@ static void coarse_correction_t(int64_t nh, const CoarseDev& c, const double* r,
         const unsigned gs = (unsigned)((c.n_agg * KC + kBlock - 1) / kBlock);
         k_coarse_restrict_sum<KC><<<gs, kBlock, 0, s>>>(c.n_agg, c.agg_chunk, c.chunk_part, c.rc, sc);
     }
-    k_coarse_apply_prolong<D, KC><<<(unsigned)c.n_agg, kCoarseBlock, 0, s>>>(
-        c.nc, c.Ainv, c.rc, c.agg_ptr, c.agg_vert, c.dvec, c.free_vars, z, c.cparts, slot, c.Gc, kdev, sc);
+    if (c.Ainv32)
+        k_coarse_apply_prolong<D, KC, float><<<(unsigned)c.n_agg, kCoarseBlock, 0, s>>>(
+            c.nc, c.Ainv32, c.rc, c.agg_ptr, c.agg_vert, c.dvec, c.free_vars, z, c.cparts, slot, c.Gc, kdev, sc);
+    else
+        k_coarse_apply_prolong<D, KC, double><<<(unsigned)c.n_agg, kCoarseBlock, 0, s>>>(
+            c.nc, c.Ainv, c.rc, c.agg_ptr, c.agg_vert, c.dvec, c.free_vars, z, c.cparts, slot, c.Gc, kdev, sc);
 }
 
 hipError_t launch_coarse_correction(int d, int64_t nh, const CoarseDev& c, const double* r, double* z, int slot,
