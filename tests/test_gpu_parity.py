@@ -365,6 +365,33 @@ def test_pcg_matches_direct_solve(ctx):
     assert nrel(d3, ref3) <= 1e-9
 
 
+def test_solve_reads_lower_triangle(ctx):
+    """The solve uses H's scalar lower triangle, as the reference's
+    SimplicialLDLT (LinearSolver.hh:14, Eigen's default Lower): scaling the
+    strictly upper entries of a symmetric H leaves the solution unchanged;
+    doubling the lower triangle and the diagonal only solves with 2 H
+    (block-Jacobi alone: the coarse space's Galerkin sums read both
+    triangles)."""
+    import scipy.sparse.linalg as spla
+    P = configs.twist_cube(m=4, seed=3)
+    fg = tad.ScalarFunction.from_problem(P, ctx)
+    f, g, H = fg.eval_with_hessian_proj(P.x0)
+    H = H.tocsr()
+    H.sort_indices()
+    rows = np.repeat(np.arange(H.shape[0]), np.diff(H.indptr))
+    upper = H.indices > rows
+    d_ref, st = fg.solve(g, H=H.data.copy(), tol=1e-13, coarse_aggregate=0)
+    assert st.converged and nrel(d_ref, spla.spsolve(H.tocsc(), g)) <= 1e-9
+    Hu = H.data.copy()
+    Hu[upper] *= 1.7
+    d_u, st_u = fg.solve(g, H=Hu, tol=1e-13, coarse_aggregate=0)
+    assert st_u.converged and nrel(d_u, d_ref) <= 1e-9
+    Hl = H.data.copy()
+    Hl[~upper] *= 2.0
+    d_l, st_l = fg.solve(g, H=Hl, tol=1e-13, coarse_aggregate=0)
+    assert st_l.converged and nrel(d_l, 0.5 * d_ref) <= 1e-9
+
+
 def test_two_level_preconditioner(ctx):
     """Block-Jacobi plus the aggregate coarse correction: the direct solution,
     at least 2x fewer iterations than block-Jacobi alone, bitwise

@@ -1345,8 +1345,8 @@ tad_status solve_core(tad_fn fn, const double* H_dev, const double* rhs_dev, dou
     tm.end("pcg_setup", s);
     TAD_HIP(fn->sym_H.ensure((size_t)fn->n_sym * d * d));
     tm.begin("sym_pack", s);
-    TAD_HIP(launch_sym_pack(d, fn->n_sym, fn->sym_full.p, fn->block_row.p, fn->brow_ptr.p, fn->row_ptr.p, H_dev,
-                            fn->sym_H.p, s));
+    TAD_HIP(launch_sym_pack(d, fn->n_sym, fn->sym_full.p, fn->block_row.p, fn->brow_ptr.p, fn->bcol.p,
+                            fn->row_ptr.p, H_dev, fn->sym_H.p, s));
     tm.end("sym_pack", s);
     CoarseDev cd;
     bool use_coarse = false, coarse_reused = false;

@@ -248,10 +248,11 @@ hipError_t launch_pcg_init(int d, int64_t n_handles, const double* b, const unsi
 
 // The solver's copy of H: the n_sym blocks of its lower triangle (diagonal
 // included) in CSR order, d*d contiguous row-major values each (sym_full: CSR
-// block of each stored block).
+// block of each stored block), diagonal blocks symmetrized from their lower
+// entries: the solve uses H's scalar lower triangle, as SimplicialLDLT does.
 hipError_t launch_sym_pack(int d, int64_t n_sym, const int32_t* sym_full, const int32_t* block_row,
-                           const int32_t* brow_ptr, const int32_t* row_ptr, const double* H, double* sym_H,
-                           hipStream_t s);
+                           const int32_t* brow_ptr, const int32_t* bcol, const int32_t* row_ptr, const double* H,
+                           double* sym_H, hipStream_t s);
 
 // One CG iteration k: SpMV on z with the termination test, then (with_update)
 // the scalar recurrences and the p / q / x / r / z update, and the coarse

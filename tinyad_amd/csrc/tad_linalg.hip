@@ -549,12 +549,16 @@ __global__ __launch_bounds__(kBlock) void k_pcg_setup(int64_t nh, const int32_t*
                     (i == j && fr[i]) ? (w > 0.0 ? 1.0 / w : 1.0) : 0.0;
         return;
     }
+    // the block's lower entries (the solve uses H's scalar lower triangle)
 #pragma unroll
     for (int i = 0; i < D; ++i)
 #pragma unroll
         for (int j = 0; j < D; ++j)
-            B[i * D + j] = (fr[i] && fr[j]) ? H[base + (int64_t)i * D * nbu + D * k + j] + (i == j ? w : 0.0)
+        {
+            const int il = i > j ? i : j, jl = i > j ? j : i;
+            B[i * D + j] = (fr[i] && fr[j]) ? H[base + (int64_t)il * D * nbu + D * k + jl] + (i == j ? w : 0.0)
                                             : (i == j ? 1.0 : 0.0);
+        }
     invert_block<D>(B, o);
 #pragma unroll
     for (int i = 0; i < D; ++i)
@@ -627,10 +631,13 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t nh, const double* _
 
 // The solver's copy of H (the lower triangle's blocks, diagonal included, in
 // CSR order; D*D contiguous row-major values each), one thread per value.
+// A diagonal block takes its upper entries from its lower ones, so the copy
+// is the symmetric matrix of H's scalar lower triangle.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_sym_pack(int64_t n_sym, const int32_t* __restrict__ sym_full,
                                                      const int32_t* __restrict__ block_row,
                                                      const int32_t* __restrict__ brow_ptr,
+                                                     const int32_t* __restrict__ bcol,
                                                      const int32_t* __restrict__ row_ptr,
                                                      const double* __restrict__ H, double* __restrict__ Hs)
 {
@@ -639,9 +646,15 @@ __global__ __launch_bounds__(kBlock) void k_sym_pack(int64_t n_sym, const int32_
     if (id >= n_sym * DD) return;
     const int64_t l = id / DD;
     const int m = (int)(id - l * DD);
-    const int i = m / D, j = m - i * D;
+    int i = m / D, j = m - i * D;
     const int32_t b = sym_full[l];
     const int32_t u = block_row[b];
+    if (bcol[b] == u && i < j)
+    {
+        const int t = i;
+        i = j;
+        j = t;
+    }
     const int32_t b0 = brow_ptr[u];
     const int64_t nbu = brow_ptr[u + 1] - b0;
     Hs[id] = H[(int64_t)row_ptr[D * u] + i * D * nbu + (int64_t)D * (b - b0) + j];
@@ -1519,14 +1532,17 @@ hipError_t launch_pcg_init(int d, int64_t nh, const double* b, const unsigned ch
 }
 
 hipError_t launch_sym_pack(int d, int64_t n_sym, const int32_t* sym_full, const int32_t* block_row,
-                           const int32_t* brow_ptr, const int32_t* row_ptr, const double* H, double* sym_H,
-                           hipStream_t s)
+                           const int32_t* brow_ptr, const int32_t* bcol, const int32_t* row_ptr, const double* H,
+                           double* sym_H, hipStream_t s)
 {
     if (n_sym == 0) return hipSuccess;
     const unsigned grid = (unsigned)((n_sym * d * d + kBlock - 1) / kBlock);
-    if (d == 1) k_sym_pack<1><<<grid, kBlock, 0, s>>>(n_sym, sym_full, block_row, brow_ptr, row_ptr, H, sym_H);
-    else if (d == 2) k_sym_pack<2><<<grid, kBlock, 0, s>>>(n_sym, sym_full, block_row, brow_ptr, row_ptr, H, sym_H);
-    else k_sym_pack<3><<<grid, kBlock, 0, s>>>(n_sym, sym_full, block_row, brow_ptr, row_ptr, H, sym_H);
+    if (d == 1)
+        k_sym_pack<1><<<grid, kBlock, 0, s>>>(n_sym, sym_full, block_row, brow_ptr, bcol, row_ptr, H, sym_H);
+    else if (d == 2)
+        k_sym_pack<2><<<grid, kBlock, 0, s>>>(n_sym, sym_full, block_row, brow_ptr, bcol, row_ptr, H, sym_H);
+    else
+        k_sym_pack<3><<<grid, kBlock, 0, s>>>(n_sym, sym_full, block_row, brow_ptr, bcol, row_ptr, H, sym_H);
     return hipGetLastError();
 }
 
