@@ -456,7 +456,11 @@ tad_status tad_solve_default_opts(tad_solve_opts* opts);
  * block-Jacobi (d x d diagonal blocks) plus, unless coarse_aggregate is 0, an
  * additive coarse correction on aggregates of about that many handles
  * (translations, and rotations about the positions of the last second-order
- * evaluation when there was one; Galerkin coarse matrix inverted in FP64).
+ * evaluation when there was one; Galerkin coarse matrix inverted in FP64; the
+ * iterations read an FP32 copy of the inverse, shifted to stay positive
+ * definite, when its condition estimate bounds the change of the correction
+ * by 10 %).  Like SimplicialLDLT, the solve reads H's lower triangle (and
+ * diagonal) only: the matrix solved with is that triangle mirrored.
  * H_values == NULL uses the function's buffer from the last second-order eval. */
 tad_status tad_fn_solve(tad_fn fn, const double* H_values, const double* rhs, double* sol, tad_mem mem,
                         const tad_solve_opts* opts, tad_solve_stats* stats);
