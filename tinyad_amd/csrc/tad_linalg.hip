@@ -1202,7 +1202,10 @@ __global__ __launch_bounds__(kBlock) void k_coarse_restrict_sum(int64_t n_agg, c
 // to LDS and are summed in chunk order as by k_coarse_restrict_sum, so r_c
 // has the same bits.  For aggregates of at most kRestrictMaxChunks chunks.
 constexpr int kRestrictMaxChunks = 64;
-constexpr int kCoarseBlock = 1024;
+// lanes per aggregate in the restriction and the coarse product (60^3 tet
+// cube, 459 aggregates: 512 and 256 22.4 us per CG iteration for both
+// kernels, 1,024 24.2 us)
+constexpr int kCoarseBlock = 512;
 
 template <int D, int KC>
 __global__ __launch_bounds__(kCoarseBlock) void k_coarse_restrict(const int32_t* __restrict__ agg_chunk,
